@@ -1,0 +1,72 @@
+"""ctypes binding of libcostuum_amd.so (the C ABI in include/costuum_amd.h).
+
+The library must be built (costuum_amd.build) before use; there is no
+fallback implementation: a missing library raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcostuum_amd.so")
+
+CTM_OK = 0
+CTM_CONVERGED = 0
+CTM_NOT_CONVERGED = 1
+CTM_PRECISION_FP64 = 0
+CTM_PRECISION_MIXED = 1
+
+# Every entry point declared in include/costuum_amd.h.
+EXPORTS = (
+    "ctm_create", "ctm_destroy", "ctm_last_error", "ctm_stream",
+    "ctm_synchronize", "ctm_tmatrix_batch", "ctm_tmatrix_fixed",
+    "ctm_tmatrix_size", "ctm_tmatrix_get", "ctm_batch_orders",
+)
+
+
+class CtmConvergence(ctypes.Structure):
+    _fields_ = [("min_order", ctypes.c_uint32),
+                ("max_order", ctypes.c_uint32),
+                ("glfac", ctypes.c_uint32),
+                ("precision", ctypes.c_uint32),
+                ("tolerance", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libcostuum_amd.so is not built; run python -m costuum_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    dp = ctypes.c_void_p  # device or host double pointers
+    L.ctm_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+    L.ctm_create.restype = ctypes.c_int
+    L.ctm_destroy.argtypes = [vp]
+    L.ctm_destroy.restype = ctypes.c_int
+    L.ctm_last_error.argtypes = [vp]
+    L.ctm_last_error.restype = ctypes.c_char_p
+    L.ctm_stream.argtypes = [vp]
+    L.ctm_stream.restype = vp
+    L.ctm_synchronize.argtypes = [vp]
+    L.ctm_synchronize.restype = ctypes.c_int
+    L.ctm_tmatrix_batch.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp, dp,
+                                    ctypes.POINTER(CtmConvergence), dp, dp,
+                                    dp, dp]
+    L.ctm_tmatrix_batch.restype = ctypes.c_int
+    L.ctm_tmatrix_fixed.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp, dp,
+                                    dp, dp, ctypes.c_uint32, dp, dp]
+    L.ctm_tmatrix_fixed.restype = ctypes.c_int
+    L.ctm_tmatrix_size.argtypes = [vp, ctypes.c_int]
+    L.ctm_tmatrix_size.restype = ctypes.c_size_t
+    L.ctm_tmatrix_get.argtypes = [vp, ctypes.c_int, dp, ctypes.c_size_t]
+    L.ctm_tmatrix_get.restype = ctypes.c_int
+    L.ctm_batch_orders.argtypes = [vp, dp, dp, dp]
+    L.ctm_batch_orders.restype = ctypes.c_int
+    _lib = L
+    return L

@@ -1,0 +1,649 @@
+// ctm_api.hip -- host orchestration and C ABI of the batched T-matrix path.
+//
+// The reference runs one dependency chain per grain on CPU threads
+// (InteractionTask -> TMatrixM0Task per order until converged, then
+// TMatrixMAllTask per m; TaskManager.hpp:1146-1197,1314-1327).  Here a batch
+// of grains runs as a few wide launches:
+//   phase 1 (orders): for every unconverged grain, all orders of a window
+//     [lo, hi] are independent m = 0 problems and run in parallel; a
+//     per-grain scan then applies the reference's sequential convergence
+//     test (TMatrixResource.hpp:743-775) in order.  Grains that did not
+//     converge get the next window.
+//   phase 2 (all m): every (grain, m) block at the converged order.
+// Grain-independent tables (Gauss-Legendre nodes per G, Wigner d per (N, m))
+// are computed once per batch and shared, like the reference's
+// GaussBasedResources and WignerDResources.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "../../include/costuum_amd.h"
+#include "ctm_device.hpp"
+#include "ctm_kernels.hpp"
+
+using namespace ctm;
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    size_t want = std::max(bytes, (size_t)256);
+    want = want + want / 4;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) return e;
+    cap = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// Grain-independent table caches (per batch).
+struct SharedTables {
+  std::map<int, int> gl_index;                 // G -> key index
+  std::map<std::tuple<int, int, int>, int> wig_index;  // (N, m, G) -> key
+  std::vector<GlKey> gl_keys;
+  std::vector<WigKey> wig_keys;
+  int64_t gl_size = 0, wig_size = 0;
+  size_t gl_done = 0, wig_done = 0;  // keys already computed on device
+  void clear() {
+    gl_index.clear();
+    wig_index.clear();
+    gl_keys.clear();
+    wig_keys.clear();
+    gl_size = wig_size = 0;
+    gl_done = wig_done = 0;
+  }
+  int gl(int G) {
+    auto f = gl_index.find(G);
+    if (f != gl_index.end()) return f->second;
+    GlKey k{};
+    k.G = G;
+    k.off = gl_size;
+    gl_size += 2 * (int64_t)G;
+    gl_keys.push_back(k);
+    const int idx = (int)gl_keys.size() - 1;
+    gl_index[G] = idx;
+    return idx;
+  }
+  int wig(int N, int m, int G) {
+    auto f = wig_index.find(std::make_tuple(N, m, G));
+    if (f != wig_index.end()) return f->second;
+    const int g = gl(G);
+    WigKey k{};
+    k.N = N;
+    k.m = m;
+    k.G = G;
+    k.gl_off = gl_keys[g].off;
+    k.off = wig_size;
+    const int nmin = m > 0 ? m : 1;
+    wig_size += 2 * (int64_t)(N + 1 - nmin) * G;
+    wig_keys.push_back(k);
+    const int idx = (int)wig_keys.size() - 1;
+    wig_index[std::make_tuple(N, m, G)] = idx;
+    return idx;
+  }
+};
+
+// One set of work lists: table sets, items, tiles, solve tasks.
+struct WorkList {
+  std::vector<TableSet> sets;
+  std::vector<Item> items;
+  std::vector<TileTask> tiles;
+  std::vector<int32_t> solve;  // item * 2 + class
+  int64_t table_size = 0, q_size = 0, t_size = 0;
+  void clear() {
+    sets.clear();
+    items.clear();
+    tiles.clear();
+    solve.clear();
+    table_size = q_size = t_size = 0;
+  }
+  int add_set(int grain, int N, int G, int64_t gl_off) {
+    TableSet s{};
+    s.grain = grain;
+    s.N = N;
+    s.G = G;
+    s.gl_off = gl_off;
+    s.off = table_size;
+    table_size += (int64_t)kNodeScalars * G + (int64_t)kBesselArrays * N * G;
+    sets.push_back(s);
+    return (int)sets.size() - 1;
+  }
+  int add_item(int grain, int N, int m, int G, int set, int wig,
+               bool want_t) {
+    Item it{};
+    it.grain = grain;
+    it.N = N;
+    it.m = m;
+    it.G = G;
+    it.set = set;
+    it.wig = wig;
+    const int nmin = m > 0 ? m : 1;
+    const int64_t nm = N + 1 - nmin;
+    it.q_off = q_size;
+    q_size += 4 * nm * nm;
+    it.t_off = want_t ? t_size : -1;
+    if (want_t) t_size += 2 * nm * nm;
+    items.push_back(it);
+    return (int)items.size() - 1;
+  }
+};
+
+}  // namespace
+
+struct ctm_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  SharedTables shared;
+  WorkList work;
+  DevBuf d_gl, d_glkeys, d_wig, d_wigkeys, d_tables, d_sets, d_items,
+      d_tiles, d_solve, d_q, d_t, d_qparts, d_state, d_active, d_winoff,
+      d_winitems, d_gioff, d_gitems;
+  // Results of the last batch (host).
+  int n_grains = 0;
+  std::vector<uint32_t> nmax, ngauss;
+  std::vector<int32_t> status;
+  std::vector<Item> final_items;          // phase-2 items
+  std::vector<int32_t> grain_items_off;   // per grain into final_items
+  std::vector<int32_t> grain_items;
+};
+
+#define CTM_HIP(expr)                                                   \
+  do {                                                                  \
+    hipError_t _e = (expr);                                             \
+    if (_e != hipSuccess) {                                             \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);     \
+      return CTM_ERR_HIP;                                               \
+    }                                                                   \
+  } while (0)
+
+namespace {
+
+template <typename T>
+hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+  hipError_t e = b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (v.empty()) return hipSuccess;
+  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T),
+                        hipMemcpyHostToDevice, s);
+}
+
+// Computes the grain-independent tables whose keys were added since the
+// last call.
+int flush_shared(ctm_context *ctx) {
+  SharedTables &sh = ctx->shared;
+  const size_t ngl = sh.gl_keys.size(), nwig = sh.wig_keys.size();
+  if (sh.gl_done == ngl && sh.wig_done == nwig) return CTM_OK;
+  // The caches are append-only within a batch; grow by copying.
+  auto grow = [&](DevBuf &b, size_t bytes) -> hipError_t {
+    if (bytes <= b.cap) return hipSuccess;
+    DevBuf nb;
+    hipError_t e = nb.ensure(bytes * 2);
+    if (e != hipSuccess) return e;
+    if (b.p) {
+      e = hipMemcpyAsync(nb.p, b.p, b.cap, hipMemcpyDeviceToDevice,
+                         ctx->stream);
+      if (e != hipSuccess) return e;
+      e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return e;
+      b.release();
+    }
+    b = nb;
+    return hipSuccess;
+  };
+  CTM_HIP(grow(ctx->d_gl, sh.gl_size * sizeof(double)));
+  CTM_HIP(grow(ctx->d_wig, std::max<int64_t>(sh.wig_size, 1) * sizeof(double)));
+  if (sh.gl_done < ngl) {
+    std::vector<GlKey> fresh(sh.gl_keys.begin() + sh.gl_done, sh.gl_keys.end());
+    CTM_HIP(upload(ctx->d_glkeys, fresh, ctx->stream));
+    CTM_HIP(launch_gauss_legendre(ctx->d_glkeys.as<GlKey>(), (int)fresh.size(),
+                                  ctx->d_gl.as<double>(), ctx->stream));
+    // the key buffer is rewritten below; wait for the launch to read it
+    CTM_HIP(hipStreamSynchronize(ctx->stream));
+    sh.gl_done = ngl;
+  }
+  if (sh.wig_done < nwig) {
+    std::vector<WigKey> fresh(sh.wig_keys.begin() + sh.wig_done,
+                              sh.wig_keys.end());
+    CTM_HIP(upload(ctx->d_glkeys, fresh, ctx->stream));
+    CTM_HIP(launch_wigner(ctx->d_glkeys.as<WigKey>(), (int)fresh.size(),
+                          ctx->d_gl.as<double>(), ctx->d_wig.as<double>(),
+                          ctx->stream));
+    CTM_HIP(hipStreamSynchronize(ctx->stream));
+    sh.wig_done = nwig;
+  }
+  // all Wigner keys on device for the assembly kernel
+  CTM_HIP(upload(ctx->d_wigkeys, sh.wig_keys, ctx->stream));
+  return CTM_OK;
+}
+
+// Builds tile and solve task lists of the work list, sorted by cost so that
+// the largest blocks are dispatched first.
+void build_tasks(WorkList &w) {
+  w.tiles.clear();
+  w.solve.clear();
+  for (int i = 0; i < (int)w.items.size(); ++i) {
+    const Item &it = w.items[i];
+    const int nmin = it.m > 0 ? it.m : 1;
+    const int nm = it.N + 1 - nmin;
+    const int nt = (nm + 15) / 16;
+    for (int a = 0; a < nt; ++a)
+      for (int b = 0; b < nt; ++b) w.tiles.push_back({i, a, b, 0});
+    w.solve.push_back(2 * i);
+    w.solve.push_back(2 * i + 1);
+  }
+  std::stable_sort(w.tiles.begin(), w.tiles.end(),
+                   [&](const TileTask &x, const TileTask &y) {
+                     return w.items[x.item].G > w.items[y.item].G;
+                   });
+  auto nm_of = [&](int task) {
+    const Item &it = w.items[task >> 1];
+    const int nmin = it.m > 0 ? it.m : 1;
+    return it.N + 1 - nmin;
+  };
+  std::stable_sort(w.solve.begin(), w.solve.end(),
+                   [&](int x, int y) { return nm_of(x) > nm_of(y); });
+}
+
+// Runs tables + assembly + inversion/solve for the current work list.
+int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
+  WorkList &w = ctx->work;
+  if (w.items.empty()) return CTM_OK;
+  int rc = flush_shared(ctx);
+  if (rc != CTM_OK) return rc;
+  build_tasks(w);
+  hipStream_t s = ctx->stream;
+  CTM_HIP(upload(ctx->d_sets, w.sets, s));
+  CTM_HIP(upload(ctx->d_items, w.items, s));
+  CTM_HIP(upload(ctx->d_tiles, w.tiles, s));
+  CTM_HIP(upload(ctx->d_solve, w.solve, s));
+  CTM_HIP(ctx->d_tables.ensure(std::max<int64_t>(w.table_size, 1) *
+                               sizeof(double)));
+  CTM_HIP(ctx->d_q.ensure(std::max<int64_t>(w.q_size, 1) * sizeof(cdbl)));
+  if (want_t)
+    CTM_HIP(ctx->d_t.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
+  CTM_HIP(ctx->d_qparts.ensure(std::max<size_t>(w.items.size(), 1) * 2 *
+                               kQParts * sizeof(double)));
+  CTM_HIP(launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(), gp,
+                        ctx->d_gl.as<double>(), ctx->d_tables.as<double>(), s));
+  CTM_HIP(launch_assemble(ctx->d_tiles.as<TileTask>(), (int)w.tiles.size(),
+                          ctx->d_items.as<Item>(), ctx->d_sets.as<TableSet>(),
+                          ctx->d_wigkeys.as<WigKey>(), gp,
+                          ctx->d_tables.as<double>(), ctx->d_wig.as<double>(),
+                          ctx->d_q.as<cdbl>(), s));
+  // solve tasks in LDS-size buckets, largest first
+  const int bounds[] = {kMaxLdsOrder, 72, 48, 24};
+  size_t pos = 0;
+  const int32_t *dsolve = ctx->d_solve.as<int32_t>();
+  for (int b = 0; b < 4 && pos < w.solve.size(); ++b) {
+    const int hi = bounds[b];
+    const int lo = (b < 3) ? bounds[b + 1] : 0;
+    size_t end = pos;
+    int nm_max = 0;
+    while (end < w.solve.size()) {
+      const Item &it = w.items[w.solve[end] >> 1];
+      const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+      if (nm <= lo) break;
+      if (nm > hi) {
+        ctx->err = "block order exceeds the LDS-resident solver";
+        return CTM_ERR_ARGUMENT;
+      }
+      nm_max = std::max(nm_max, nm);
+      ++end;
+    }
+    if (end > pos) {
+      CTM_HIP(launch_invert_solve(dsolve + pos, (int)(end - pos), nm_max,
+                                  ctx->d_items.as<Item>(), ctx->d_q.as<cdbl>(),
+                                  want_t ? ctx->d_t.as<cdbl>() : nullptr,
+                                  ctx->d_qparts.as<double>(), s));
+    }
+    pos = end;
+  }
+  return CTM_OK;
+}
+
+// Rough order estimate from the largest size parameter of the spheroid
+// (TMatrixCalculator.hpp:83-85 uses x + 4.05 x^(1/3) of the equal-volume
+// sphere); only used to size the first window of orders.
+int order_estimate(double rv, double d, double lam) {
+  const double rmax = rv * std::max(std::cbrt(d), 1. / std::cbrt(d * d));
+  const double x = 2. * M_PI * rmax / lam;
+  return (int)(x + 4.05 * std::cbrt(x)) + 2;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int ctm_create(ctm_context **out, int device) {
+  if (!out) return CTM_ERR_ARGUMENT;
+  *out = nullptr;
+  ctm_context *ctx = new ctm_context();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return CTM_ERR_HIP;
+  }
+  *out = ctx;
+  return CTM_OK;
+}
+
+int ctm_destroy(ctm_context *ctx) {
+  if (!ctx) return CTM_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&ctx->d_gl,     &ctx->d_glkeys,  &ctx->d_wig,
+                    &ctx->d_wigkeys, &ctx->d_tables, &ctx->d_sets,
+                    &ctx->d_items,  &ctx->d_tiles,   &ctx->d_solve,
+                    &ctx->d_q,      &ctx->d_t,       &ctx->d_qparts,
+                    &ctx->d_state,  &ctx->d_active,  &ctx->d_winoff,
+                    &ctx->d_winitems, &ctx->d_gioff, &ctx->d_gitems};
+  for (DevBuf *b : bufs) b->release();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CTM_OK;
+}
+
+const char *ctm_last_error(const ctm_context *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+void *ctm_stream(ctm_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int ctm_synchronize(ctm_context *ctx) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  return CTM_OK;
+}
+
+static int check_inputs(ctm_context *ctx, int n, const double *a,
+                        const double *b, const double *c, const double *d,
+                        const double *e) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (n < 0 || (n > 0 && (!a || !b || !c || !d || !e))) {
+    ctx->err = "invalid grain arrays";
+    return CTM_ERR_ARGUMENT;
+  }
+  return CTM_OK;
+}
+
+// Phase 2: all m blocks at the given per-grain (N, G).
+static int run_all_m(ctm_context *ctx, const GrainParams &gp, int n,
+                     const std::vector<uint32_t> &N,
+                     const std::vector<uint32_t> &G, double *qsca_out,
+                     double *qext_out) {
+  WorkList &w = ctx->work;
+  w.clear();
+  SharedTables &sh = ctx->shared;
+  ctx->grain_items_off.assign(n + 1, 0);
+  for (int g = 0; g < n; ++g) {
+    const int Ng = (int)N[g], Gg = (int)G[g];
+    const int set = w.add_set(g, Ng, Gg, sh.gl_keys[sh.gl(Gg)].off);
+    ctx->grain_items_off[g] = (int)w.items.size();
+    for (int m = 0; m <= Ng; ++m) {
+      const int wk = sh.wig(Ng, m, Gg);
+      w.add_item(g, Ng, m, Gg, set, wk, true);
+    }
+  }
+  ctx->grain_items_off[n] = (int)w.items.size();
+  int rc = run_work(ctx, gp, true);
+  if (rc != CTM_OK) return rc;
+  ctx->grain_items.resize(w.items.size());
+  for (size_t i = 0; i < w.items.size(); ++i) ctx->grain_items[i] = (int32_t)i;
+  CTM_HIP(upload(ctx->d_gioff, ctx->grain_items_off, ctx->stream));
+  CTM_HIP(upload(ctx->d_gitems, ctx->grain_items, ctx->stream));
+  CTM_HIP(launch_full_sums(n, ctx->d_gioff.as<int32_t>(),
+                           ctx->d_gitems.as<int32_t>(),
+                           ctx->d_qparts.as<double>(), qsca_out, qext_out,
+                           ctx->stream));
+  ctx->final_items = w.items;
+  return CTM_OK;
+}
+
+int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
+                      const double *axis_ratio, const double *wavelength,
+                      const double *mr_re, const double *mr_im,
+                      const ctm_convergence *conv, uint32_t *nmax_out,
+                      int32_t *status_out, double *qsca_out,
+                      double *qext_out) {
+  int rc = check_inputs(ctx, n, radius, axis_ratio, wavelength, mr_re, mr_im);
+  if (rc != CTM_OK) return rc;
+  if (!conv || conv->min_order < 1 || conv->max_order <= conv->min_order + 1 ||
+      conv->glfac < 1 || !(conv->tolerance > 0.)) {
+    ctx->err = "invalid convergence parameters";
+    return CTM_ERR_ARGUMENT;
+  }
+  if ((int)conv->max_order - 1 > kMaxLdsOrder) {
+    ctx->err = "maximum order above 100 is not supported by this build";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  // host work lists of the previous call may still feed pending copies
+  CTM_HIP(hipStreamSynchronize(s));
+  ctx->err.clear();
+  ctx->shared.clear();
+  ctx->n_grains = n;
+  ctx->final_items.clear();
+  if (n == 0) return CTM_OK;
+  GrainParams gp{radius, axis_ratio, wavelength, mr_re, mr_im};
+  const int min_order = (int)conv->min_order;
+  const int last_order = (int)conv->max_order - 1;
+  const int glfac = (int)conv->glfac;
+
+  // host copies of the sizes (to size the windows)
+  std::vector<double> h_rv(n), h_d(n), h_lam(n);
+  CTM_HIP(hipMemcpyAsync(h_rv.data(), radius, n * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  CTM_HIP(hipMemcpyAsync(h_d.data(), axis_ratio, n * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  CTM_HIP(hipMemcpyAsync(h_lam.data(), wavelength, n * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  CTM_HIP(hipStreamSynchronize(s));
+
+  std::vector<ConvState> st(n);
+  for (int g = 0; g < n; ++g) {
+    ConvState c{};
+    c.old_s = c.old_e = 0.;
+    c.dqs = c.dqe = -2.;
+    c.first = 1;
+    c.done = 0;
+    c.nmax = min_order;
+    c.item = -1;
+    c.status = 1;
+    st[g] = c;
+  }
+  CTM_HIP(upload(ctx->d_state, st, s));
+
+  std::vector<int> lo(n, min_order), hi(n);
+  for (int g = 0; g < n; ++g) {
+    const int est = order_estimate(h_rv[g], h_d[g], h_lam[g]);
+    hi[g] = std::min(last_order, std::max(min_order + 1, est));
+  }
+  std::vector<int32_t> active;
+  for (int g = 0; g < n; ++g) active.push_back(g);
+
+  // phase 1: windows of orders until every grain has converged
+  while (!active.empty()) {
+    WorkList &w = ctx->work;
+    w.clear();
+    std::vector<int32_t> win_off(1, 0), win_items;
+    for (int g : active) {
+      for (int N = lo[g]; N <= hi[g]; ++N) {
+        const int G = glfac * N;
+        const int gk = ctx->shared.gl(G);
+        const int set = w.add_set(g, N, G, ctx->shared.gl_keys[gk].off);
+        const int wk = ctx->shared.wig(N, 0, G);
+        win_items.push_back(w.add_item(g, N, 0, G, set, wk, false));
+      }
+      win_off.push_back((int32_t)win_items.size());
+    }
+    rc = run_work(ctx, gp, false);
+    if (rc != CTM_OK) return rc;
+    CTM_HIP(upload(ctx->d_active, active, s));
+    CTM_HIP(upload(ctx->d_winoff, win_off, s));
+    CTM_HIP(upload(ctx->d_winitems, win_items, s));
+    CTM_HIP(launch_converge(ctx->d_active.as<int32_t>(), (int)active.size(),
+                            ctx->d_winoff.as<int32_t>(),
+                            ctx->d_winitems.as<int32_t>(),
+                            ctx->d_items.as<Item>(),
+                            ctx->d_qparts.as<double>(), conv->tolerance,
+                            last_order, ctx->d_state.as<ConvState>(), s));
+    CTM_HIP(hipMemcpyAsync(st.data(), ctx->d_state.p, n * sizeof(ConvState),
+                           hipMemcpyDeviceToHost, s));
+    CTM_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> next;
+    for (int g : active) {
+      if (st[g].done) continue;
+      const int width = std::max(4, (hi[g] + 3) / 4);
+      lo[g] = hi[g] + 1;
+      hi[g] = std::min(last_order, hi[g] + width);
+      next.push_back(g);
+    }
+    active.swap(next);
+  }
+
+  ctx->nmax.assign(n, 0);
+  ctx->ngauss.assign(n, 0);
+  ctx->status.assign(n, 0);
+  for (int g = 0; g < n; ++g) {
+    ctx->nmax[g] = (uint32_t)st[g].nmax;
+    ctx->ngauss[g] = (uint32_t)(glfac * st[g].nmax);
+    ctx->status[g] = st[g].status;
+  }
+  rc = run_all_m(ctx, gp, n, ctx->nmax, ctx->ngauss, qsca_out, qext_out);
+  if (rc != CTM_OK) return rc;
+  CTM_HIP(launch_write_orders(n, ctx->d_state.as<ConvState>(), nmax_out,
+                              status_out, s));
+  return CTM_OK;
+}
+
+int ctm_tmatrix_fixed(ctm_context *ctx, int n, const double *radius,
+                      const double *axis_ratio, const double *wavelength,
+                      const double *mr_re, const double *mr_im,
+                      const uint32_t *nmax_host, const uint32_t *ngauss_host,
+                      uint32_t precision, double *qsca_out, double *qext_out) {
+  int rc = check_inputs(ctx, n, radius, axis_ratio, wavelength, mr_re, mr_im);
+  if (rc != CTM_OK) return rc;
+  if (n > 0 && (!nmax_host || !ngauss_host)) {
+    ctx->err = "missing forced orders";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (precision != CTM_PRECISION_FP64) {
+    ctx->err = "only CTM_PRECISION_FP64 is available for forced orders";
+    return CTM_ERR_ARGUMENT;
+  }
+  for (int g = 0; g < n; ++g) {
+    if (nmax_host[g] < 1 || (int)nmax_host[g] > kMaxLdsOrder ||
+        ngauss_host[g] < 1) {
+      ctx->err = "forced order out of range (1..99) or ngauss < 1";
+      return CTM_ERR_ARGUMENT;
+    }
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->err.clear();
+  ctx->shared.clear();
+  ctx->n_grains = n;
+  ctx->final_items.clear();
+  ctx->nmax.assign(nmax_host, nmax_host + n);
+  ctx->ngauss.assign(ngauss_host, ngauss_host + n);
+  ctx->status.assign(n, 0);
+  if (n == 0) return CTM_OK;
+  GrainParams gp{radius, axis_ratio, wavelength, mr_re, mr_im};
+  return run_all_m(ctx, gp, n, ctx->nmax, ctx->ngauss, qsca_out, qext_out);
+}
+
+size_t ctm_tmatrix_size(ctm_context *ctx, int g) {
+  if (!ctx || g < 0 || g >= ctx->n_grains || ctx->final_items.empty()) return 0;
+  const size_t N = ctx->nmax[g];
+  size_t total = 0;
+  for (size_t m = 0; m <= N; ++m) {
+    const size_t nm = (m == 0) ? N : N + 1 - m;
+    total += 4 * nm * nm;
+  }
+  return total;
+}
+
+int ctm_tmatrix_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  const size_t need = ctm_tmatrix_size(ctx, g);
+  if (need == 0 || !T_host || capacity < need) {
+    ctx->err = "invalid grain index or output capacity";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  const int N = (int)ctx->nmax[g];
+  size_t out_off = 0;
+  std::vector<cdbl> blk;
+  cdbl *T = reinterpret_cast<cdbl *>(T_host);
+  for (int m = 0; m <= N; ++m) {
+    const Item &it = ctx->final_items[ctx->grain_items_off[g] + m];
+    const int nmin = m > 0 ? m : 1;
+    const int nm = N + 1 - nmin;
+    const size_t nm2 = (size_t)nm * nm;
+    blk.resize(2 * nm2);
+    CTM_HIP(hipMemcpyAsync(blk.data(), ctx->d_t.as<cdbl>() + it.t_off,
+                           2 * nm2 * sizeof(cdbl), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CTM_HIP(hipStreamSynchronize(ctx->stream));
+    const int n2m = 2 * nm;
+    cdbl *out = T + out_off;
+    for (size_t e = 0; e < (size_t)n2m * n2m; ++e) out[e] = cmk(0., 0.);
+    for (int c = 0; c < 2; ++c) {
+      for (int p1 = 0; p1 < nm; ++p1) {
+        const int h1 = (c + nmin + p1) & 1;
+        for (int p2 = 0; p2 < nm; ++p2) {
+          const int h2 = (c + nmin + p2) & 1;
+          out[(size_t)(h1 * nm + p1) * n2m + h2 * nm + p2] =
+              blk[c * nm2 + (size_t)p1 * nm + p2];
+        }
+      }
+    }
+    out_off += (size_t)n2m * n2m;
+  }
+  return CTM_OK;
+}
+
+int ctm_batch_orders(ctm_context *ctx, uint32_t *nmax_host,
+                     uint32_t *ngauss_host, int32_t *status_host) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  for (int g = 0; g < ctx->n_grains; ++g) {
+    if (nmax_host) nmax_host[g] = ctx->nmax[g];
+    if (ngauss_host) ngauss_host[g] = ctx->ngauss[g];
+    if (status_host) status_host[g] = ctx->status[g];
+  }
+  return CTM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// ctm_device.hpp -- shared device-side types of the gfx950 T-matrix path.
+//
+// Data layout in HBM (all offsets in doubles unless stated):
+//   GL cache     per distinct G: cos(theta)[G], weight[G] of the first G
+//                nodes (theta > pi/2) of the 2G-point Gauss-Legendre rule.
+//   Wigner cache per distinct (N, m): d^n_{0m}[nm][G], d'[nm][G] for
+//                n = max(m,1)..N at those nodes (grain-independent).
+//   Table set    per (grain, order N): node scalars [7][G] (w r^2, dr/r,
+//                1/sin, 1/sin^2, 1/kr, 1/(m kr) re, im) followed by Bessel
+//                tables [8][N][G] (j, [kr j]'/kr, y, [kr y]'/kr at kr; re/im
+//                of j and [z j]'/z at z = m kr).
+//   Q area       per work item (grain, N, m): four nm x nm complex blocks
+//                Q_0, Q_1, RgQ_0, RgQ_1 (parity classes 0 and 1).
+//   T area       per work item: T_0, T_1 (nm x nm complex each).
+//
+// Parity classes: for m > 0 the EBCM Q-matrix row (half h, order n) couples
+// only to columns (h', n') with (n + h) = (n' + h') mod 2, so Q splits into
+// two independent nm x nm blocks; class c holds the rows with
+// (n + h) mod 2 == c, indexed by p = n - max(m, 1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ctm {
+
+struct cdbl {
+  double x, y;
+};
+
+__host__ __device__ inline cdbl cmk(double a, double b) { return {a, b}; }
+__host__ __device__ inline cdbl cadd(cdbl a, cdbl b) {
+  return {a.x + b.x, a.y + b.y};
+}
+__host__ __device__ inline cdbl csub(cdbl a, cdbl b) {
+  return {a.x - b.x, a.y - b.y};
+}
+__host__ __device__ inline cdbl cmul(cdbl a, cdbl b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__host__ __device__ inline cdbl cscale(cdbl a, double s) {
+  return {a.x * s, a.y * s};
+}
+__host__ __device__ inline cdbl crcp(cdbl a) {
+  const double d = a.x * a.x + a.y * a.y;
+  return {a.x / d, -a.y / d};
+}
+// acc += a * b (complex)
+__host__ __device__ inline void cfma(cdbl &acc, cdbl a, cdbl b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+// acc += s * b (real times complex)
+__host__ __device__ inline void rfma(cdbl &acc, double s, cdbl b) {
+  acc.x = fma(s, b.x, acc.x);
+  acc.y = fma(s, b.y, acc.y);
+}
+
+// Number of node scalars and Bessel arrays in a table set.
+constexpr int kNodeScalars = 7;
+constexpr int kBesselArrays = 8;
+
+struct GlKey {
+  int32_t G;
+  int32_t pad;
+  int64_t off;
+};
+
+struct WigKey {
+  int32_t N, m, G, pad;
+  int64_t gl_off;
+  int64_t off;
+};
+
+struct TableSet {
+  int32_t grain, N, G, pad;
+  int64_t gl_off;
+  int64_t off;
+};
+
+// One (grain, order N, m) T-matrix block computation.
+struct Item {
+  int32_t grain, N, m, G;
+  int32_t set;   // table set index
+  int32_t wig;   // Wigner key index
+  int64_t q_off; // complex offset of the Q area
+  int64_t t_off; // complex offset of the T area
+};
+
+struct TileTask {
+  int32_t item, tr, tc, pad;
+};
+
+// Per-(item, class) partial sums written by the solve kernel.
+//   [0] sum (2n+1) |T(p,p)|^2   [1] sum (2n+1) Re T(p,p)   (m = 0 test)
+//   [2] sum |T|^2 over the block [3] sum Re T(p,p)          (full-T sums)
+constexpr int kQParts = 4;
+
+// Per-grain convergence state (TMatrixResource.hpp:112-124,743-775).
+struct ConvState {
+  double old_s, old_e, dqs, dqe;
+  int32_t first;     // 1 until the first order was evaluated
+  int32_t done;      // 1 once converged or out of orders
+  int32_t nmax;      // converged (or last) order
+  int32_t item;      // phase-1 item holding the m = 0 block at nmax
+  int32_t status;    // CTM_CONVERGED / CTM_NOT_CONVERGED
+  int32_t pad[3];
+};
+
+}  // namespace ctm

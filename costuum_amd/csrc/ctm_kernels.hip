@@ -1,0 +1,767 @@
+// ctm_kernels.hip -- gfx950 kernels of the batched T-matrix path (FP64).
+//
+// K0 k_gauss_legendre  Gauss-Legendre nodes per distinct G
+//                      (SpecialFunctions.hpp:647-715)
+// K2 k_wigner          d^n_{0m}, d/dtheta per distinct (N, m)
+//                      (SpecialFunctions.hpp:303-371, WignerDResources.hpp)
+// K1 k_tables          per (grain, N): node scalars and Bessel tables
+//                      (InteractionResource.hpp:478-527,
+//                      SpecialFunctions.hpp:107-240,873-893)
+// K3 k_assemble        Q/RgQ class blocks, one 16x16 tile of (n1, n2)
+//                      pairs per workgroup (TMatrixResource.hpp:601-709,
+//                      927-1100)
+// K4 k_invert_solve    Gauss-Jordan inverse of Q_c in LDS and
+//                      T_c = -RgQ_c Q_c^-1 (Matrix.hpp:261-384,
+//                      TMatrixResource.hpp:712-730,1102-1119)
+// K5 k_converge        per-grain order test (TMatrixResource.hpp:743-775)
+// K5 k_full_sums       per-grain Q_sca/Q_ext over all m (TMatrix.hpp:2055-2110)
+#include "ctm_device.hpp"
+#include "ctm_kernels.hpp"
+
+namespace ctm {
+
+// Starting order of the downward ratio recurrence (SpecialFunctions.hpp:40).
+constexpr int kBesselStart = 800;
+
+// ---------------------------------------------------------------------------
+// K0: Gauss-Legendre nodes.  Thread i runs the Newton iteration of root i of
+// the 2G-point rule (Lether-Wenston guess, tolerance 1e-16 relaxed x10 per
+// iteration past 100) and stores node i = -root, the first half in ascending
+// order.
+__global__ void k_gauss_legendre(const GlKey *keys, double *gl) {
+  const GlKey key = keys[blockIdx.x];
+  const int G = key.G;
+  double *x = gl + key.off;
+  double *w = x + G;
+  const unsigned order = 2u * (unsigned)G;
+  const double inv = 1.0 / order;
+  const double inv2 = inv * inv;
+  for (int i = threadIdx.x; i < G; i += blockDim.x) {
+    const double t = M_PI * (i + 0.75) / (order + 0.5);
+    double xi = (1. - 0.125 * inv2 + (5. / 38.) * inv * inv2 -
+                 (2. / 25.) * inv2 * inv2 * (1. - (14. / 39.) / (t * t))) *
+                cos(t);
+    double tol = 1.e-16;
+    int iter = 0;
+    double pinv, step;
+    do {
+      ++iter;
+      if (iter > 100) tol *= 10.;
+      double p_prev = 1.;
+      double p_cur = xi;
+      for (unsigned j = 1; j < order; ++j) {
+        const double p_old = p_prev;
+        p_prev = p_cur;
+        p_cur = xi * p_prev + (xi * p_prev - p_old) * j / (j + 1.);
+      }
+      pinv = 1. / ((p_prev - xi * p_cur) * order);
+      step = pinv * p_cur * (1. - xi * xi);
+      xi = xi - step;
+    } while (fabs(step) > tol * fabs(xi));
+    x[i] = -xi;
+    w[i] = 2. * pinv * pinv * (1. - xi * xi);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: Wigner d^n_{0m}(theta) and its derivative at the cached nodes.
+__global__ void k_wigner(const WigKey *keys, const double *gl, double *wig) {
+  const WigKey key = keys[blockIdx.x];
+  const int N = key.N, m = key.m, G = key.G;
+  const int nmin = m > 0 ? m : 1;
+  const int nm = N + 1 - nmin;
+  double *wd = wig + key.off;
+  double *wdd = wd + (int64_t)nm * G;
+  const double *cth = gl + key.gl_off;
+  for (int ig = threadIdx.x; ig < G; ig += blockDim.x) {
+    const double cosx = cth[ig];
+    const double sinx = sqrt((1. - cosx) * (1. + cosx));
+    const double sinxinv = 1. / sinx;
+    if (m == 0) {
+      double dprev = 1., dcur = cosx;
+      for (int n = 0; n < N; ++n) {
+        const double np1 = n + 1., np2 = n + 2.;
+        const double tnp3 = 2. * np1 + 1.;
+        const double dnext = (tnp3 * cosx * dcur - np1 * dprev) / np2;
+        const double deriv = sinxinv * np1 * np2 * (dnext - dprev) / tnp3;
+        wd[(int64_t)n * G + ig] = dcur;
+        wdd[(int64_t)n * G + ig] = deriv;
+        dprev = dcur;
+        dcur = dnext;
+      }
+    } else {
+      const double m2 = (double)m * (double)m;
+      double dprev = 0., dcur = 1.;
+      for (int i = 0; i < m; ++i) {
+        const double i2 = 2. * (i + 1.);
+        dcur *= sqrt((i2 - 1.) / i2) * sinx;
+      }
+      for (int n = m - 1; n < N; ++n) {
+        const double np1 = n + 1., np2 = n + 2.;
+        const double tnp3 = 2. * np1 + 1.;
+        const double s1 = sqrt(np1 * np1 - m2);
+        const double s2 = sqrt(np2 * np2 - m2);
+        const double dnext = (tnp3 * cosx * dcur - s1 * dprev) / s2;
+        const double deriv =
+            sinxinv * (np1 * s2 * dnext - np2 * s1 * dprev) / tnp3;
+        const int nl = n + 1 - nmin;
+        wd[(int64_t)nl * G + ig] = dcur;
+        wdd[(int64_t)nl * G + ig] = deriv;
+        dprev = dcur;
+        dcur = dnext;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K1: per-(grain, order) node scalars and Bessel tables.  One thread per
+// node; the ratio recurrence of j_n runs down from n = 800 and parks the
+// ratios for n < N in the output rows before the forward pass overwrites
+// them with j_n.
+__global__ void k_tables(const TableSet *sets, const GrainParams gp,
+                         const double *gl, double *tables) {
+  const TableSet s = sets[blockIdx.x];
+  const int N = s.N, G = s.G;
+  const double rv = gp.radius[s.grain];
+  const double d = gp.axis_ratio[s.grain];
+  const double lam = gp.wavelength[s.grain];
+  const cdbl mr = cmk(gp.mr_re[s.grain], gp.mr_im[s.grain]);
+  const double *cth = gl + s.gl_off;
+  const double *wgt = cth + G;
+  double *S = tables + s.off;
+  double *B = S + (int64_t)kNodeScalars * G;
+  const int64_t NG = (int64_t)N * G;
+  double *bj = B, *bdj = B + NG, *by = B + 2 * NG, *bdy = B + 3 * NG;
+  double *bjmr = B + 4 * NG, *bjmi = B + 5 * NG;
+  double *bdjmr = B + 6 * NG, *bdjmi = B + 7 * NG;
+
+  const double k = 2. * M_PI / lam;
+  const cdbl kmr = cscale(mr, k);
+  const double a = cbrt(d);
+  const double a2 = a * a;
+  const double d2 = d * d;
+  const double d2m1 = d2 - 1.;
+  for (int ig = threadIdx.x; ig < G; ig += blockDim.x) {
+    const double c = cth[ig];
+    const double w = wgt[ig];
+    const double s2 = (1. - c) * (1. + c);
+    const double s2i = 1. / s2;
+    const double sn = sqrt(s2);
+    const double r2a2 = 1. / (s2 + d2 * (c * c));
+    const double r2 = a2 * r2a2;
+    const double dr = r2a2 * c * sn * d2m1;
+    const double r = sqrt(r2);
+    const double kr = k * r * rv;
+    const cdbl krmr = cmk(kmr.x * r * rv, kmr.y * r * rv);
+    const double krinv = 1. / kr;
+    const cdbl krmrinv = crcp(krmr);
+    S[0 * G + ig] = w * r2;
+    S[1 * G + ig] = dr;
+    S[2 * G + ig] = sqrt(s2i);
+    S[3 * G + ig] = s2i;
+    S[4 * G + ig] = krinv;
+    S[5 * G + ig] = krmrinv.x;
+    S[6 * G + ig] = krmrinv.y;
+
+    // j_n(kr): downward ratio recurrence, then forward products.
+    {
+      const double zinv = krinv;
+      double rho = kr / (2. * kBesselStart + 1.);
+      if (kBesselStart - 1 < N) bj[(int64_t)(kBesselStart - 1) * G + ig] = rho;
+      for (int idx = kBesselStart - 1; idx >= 1; --idx) {
+        rho = 1. / ((2. * idx + 1.) * zinv - rho);
+        if (idx - 1 < N) bj[(int64_t)(idx - 1) * G + ig] = rho;
+      }
+      const double j0 = sin(kr) * zinv;
+      double jprev = bj[ig] * j0;
+      bj[ig] = jprev;
+      bdj[ig] = j0 - jprev * zinv;
+      for (int i = 1; i < N; ++i) {
+        const double ji = bj[(int64_t)i * G + ig] * jprev;
+        bj[(int64_t)i * G + ig] = ji;
+        bdj[(int64_t)i * G + ig] = jprev - (i + 1.) * zinv * ji;
+        jprev = ji;
+      }
+    }
+    // y_n(kr): forward recurrence from the closed forms of y_1, y_2.
+    {
+      const double cz = cos(kr), sz = sin(kr);
+      const double zi = krinv, zi2 = zi * zi, zi3 = zi2 * zi;
+      double ym2 = -cz * zi2 - sz * zi;
+      by[ig] = ym2;
+      bdy[ig] = -zi * (cz + ym2);
+      if (N > 1) {
+        double ym1 = (-3. * zi3 + zi) * cz - 3. * zi2 * sz;
+        by[G + ig] = ym1;
+        bdy[G + ig] = ym2 - 2. * zi * ym1;
+        for (int i = 2; i < N; ++i) {
+          const double yi = (2. * i + 1.) * zi * ym1 - ym2;
+          by[(int64_t)i * G + ig] = yi;
+          bdy[(int64_t)i * G + ig] = ym1 - (i + 1.) * zi * yi;
+          ym2 = ym1;
+          ym1 = yi;
+        }
+      }
+    }
+    // j_n(m kr): complex downward ratio recurrence.
+    {
+      const cdbl zinv = krmrinv;
+      cdbl rho = cscale(krmr, 1. / (2. * kBesselStart + 1.));
+      for (int idx = kBesselStart - 1; idx >= 1; --idx) {
+        const double f = 2. * idx + 1.;
+        rho = crcp(csub(cscale(zinv, f), rho));
+        if (idx - 1 < N) {
+          bjmr[(int64_t)(idx - 1) * G + ig] = rho.x;
+          bjmi[(int64_t)(idx - 1) * G + ig] = rho.y;
+        }
+      }
+      // sin(z) = sin x cosh y + i cos x sinh y
+      const cdbl sz = cmk(sin(krmr.x) * cosh(krmr.y), cos(krmr.x) * sinh(krmr.y));
+      const cdbl j0 = cmul(sz, zinv);
+      cdbl jprev = cmul(cmk(bjmr[ig], bjmi[ig]), j0);
+      bjmr[ig] = jprev.x;
+      bjmi[ig] = jprev.y;
+      const cdbl dj0 = csub(j0, cmul(jprev, zinv));
+      bdjmr[ig] = dj0.x;
+      bdjmi[ig] = dj0.y;
+      for (int i = 1; i < N; ++i) {
+        const int64_t o = (int64_t)i * G + ig;
+        const cdbl ji = cmul(cmk(bjmr[o], bjmi[o]), jprev);
+        bjmr[o] = ji.x;
+        bjmi[o] = ji.y;
+        const cdbl dji = csub(jprev, cmul(cscale(zinv, i + 1.), ji));
+        bdjmr[o] = dji.x;
+        bdjmi[o] = dji.y;
+        jprev = ji;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3: Q/RgQ assembly.  Every (n1, n2) pair of an m block is a sum over the
+// quadrature nodes of products of a row factor (depends on n1 and the node)
+// and a column factor (n2 and the node).  The five column factors
+//   R1 = a2 d2 [z j]'_2   R2 = a2 d2' [z j]'_2   R3 = a2 n2(n2+1) d2 j_2
+//   R4 = a2 d2 j_2        R5 = a2 d2' j_2        (j_2 = j_n2(m kr))
+// are shared by all pairs; the row factors differ for pairs with n1 + n2
+// even (J21 = A1R1 + A2R2 + A3R3, J12 = A4R4 + A5R5) and odd
+// (J11 = i(B5R5 + B4R4), J22 = i(B2R2 + B1R1 + B3R3)), and the RgJ sums use
+// the same factors with h_n1 -> j_n1.  Then
+//   Qx = k^2 m J_a + k^2 J_b  -> row (half 0, n1), class n1 mod 2
+//   Qy = k^2 m J_b + k^2 J_a  -> row (half 1, n1), class (n1 + 1) mod 2
+// with (J_a, J_b) = (J21, J12) for even and (J11, J22) for odd pairs
+// (TMatrixResource.hpp:623-705, 993-1097).
+//
+// Workgroup: 256 threads = 4 waves on a 16x16 tile of pairs.  Wave w takes
+// rows of parity w & 1 and columns of parity w >> 1, so every wave sees a
+// single pair parity; lane (lr, lc) takes pair (2 lr + (w&1), 2 lc + (w>>1)).
+constexpr int kTile = 16;
+constexpr int kChunk = 8;
+constexpr int kLeftStride = 34;   // 32 factors + 2 pad doubles per row
+constexpr int kRightStride = 10;  // 5 complex factors per column
+
+__global__ __launch_bounds__(256) void k_assemble(
+    const TileTask *tasks, const Item *items, const TableSet *sets,
+    const WigKey *wkeys, const GrainParams gp, const double *tables,
+    const double *wig, cdbl *qbuf) {
+  __shared__ double Lsh[kChunk][kTile][kLeftStride];
+  __shared__ double Rsh[kChunk][kTile][kRightStride];
+
+  const TileTask task = tasks[blockIdx.x];
+  const Item it = items[task.item];
+  const TableSet ts = sets[it.set];
+  const WigKey wk = wkeys[it.wig];
+  const int N = it.N, m = it.m, G = it.G;
+  const int nmin = m > 0 ? m : 1;
+  const int nm = N + 1 - nmin;
+  const double *S = tables + ts.off;
+  const double *B = S + (int64_t)kNodeScalars * G;
+  const int64_t NG = (int64_t)N * G;
+  const double *Wd = wig + wk.off;
+  const double *Wdd = Wd + (int64_t)nm * G;
+
+  const int t = threadIdx.x;
+  const int wv = t >> 6, lane = t & 63;
+  const int rr = 2 * (lane >> 3) + (wv & 1);
+  const int cc = 2 * (lane & 7) + (wv >> 1);
+  const int p1 = task.tr * kTile + rr;
+  const int p2 = task.tc * kTile + cc;
+  const bool odd = ((wv & 1) ^ (wv >> 1)) != 0;
+  const bool active = !(m == 0 && odd);
+
+  cdbl Ja = cmk(0., 0.), Jb = cmk(0., 0.), RJa = cmk(0., 0.), RJb = cmk(0., 0.);
+  const double mm = (double)m;
+
+  for (int ig0 = 0; ig0 < G; ig0 += kChunk) {
+    __syncthreads();
+    if (t < kTile * kChunk) {
+      // row factors of row r at node ig0 + kk
+      const int r = t / kChunk, kk = t % kChunk;
+      const int ig = ig0 + kk;
+      const int p = task.tr * kTile + r;
+      double *L = Lsh[kk][r];
+      if (ig < G && p < nm) {
+        const int n1 = nmin + p;
+        const double d1 = Wd[(int64_t)p * G + ig];
+        const double dd1 = Wdd[(int64_t)p * G + ig];
+        const int64_t bo = (int64_t)(n1 - 1) * G + ig;
+        const double j1 = B[bo], dj1 = B[NG + bo];
+        const double y1 = B[2 * NG + bo], dy1 = B[3 * NG + bo];
+        const double wr2 = S[ig], dr = S[G + ig], s1 = S[2 * G + ig];
+        const double s2 = S[3 * G + ig], krinv = S[4 * G + ig];
+        const cdbl imkr = cmk(S[5 * G + ig], S[6 * G + ig]);
+        const double nn1 = n1 * (n1 + 1.);
+        const double a1 = sqrt((2. * n1 + 1.) / nn1);
+        const cdbl h1 = cmk(j1, y1), dh1 = cmk(dj1, dy1);
+        const double m2s2 = mm * mm * s2;
+        const double q = dr * krinv * nn1 * d1;
+        // even-pair row factors
+        const double fA1 = wr2 * m2s2 * d1 * a1;
+        const double fA2 = wr2 * dd1 * a1;
+        const double fA3 = wr2 * dr * dd1 * a1;
+        const double fA5 = -wr2 * a1;
+        const cdbl A1 = cscale(h1, fA1);
+        const cdbl A2 = cscale(h1, fA2);
+        const cdbl A3 = cmul(imkr, cscale(h1, fA3));
+        const cdbl A4 = cscale(dh1, -fA1);
+        const cdbl A5 = cscale(cadd(cscale(dh1, dd1), cscale(h1, q)), fA5);
+        L[0] = A1.x; L[1] = A1.y; L[2] = A2.x; L[3] = A2.y;
+        L[4] = A3.x; L[5] = A3.y; L[6] = A4.x; L[7] = A4.y;
+        L[8] = A5.x; L[9] = A5.y;
+        L[10] = fA1 * j1;
+        L[11] = fA2 * j1;
+        const cdbl A3r = cscale(imkr, fA3 * j1);
+        L[12] = A3r.x; L[13] = A3r.y;
+        L[14] = -fA1 * dj1;
+        L[15] = fA5 * (dd1 * dj1 + q * j1);
+        // odd-pair row factors (without the common factor i)
+        const double dsi = s1 * mm * wr2;
+        const double fB5 = dsi * d1 * a1;
+        const double fB4 = dsi * dd1 * a1;
+        const double fB1 = dsi * a1;
+        const double fB3 = dsi * dr * d1 * a1;
+        const cdbl B5 = cscale(h1, fB5);
+        const cdbl B4 = cscale(h1, fB4);
+        const cdbl B2 = cscale(dh1, fB5);
+        const cdbl B1 = cscale(cadd(cscale(dh1, dd1), cscale(h1, q)), fB1);
+        const cdbl B3 = cmul(imkr, cscale(dh1, fB3));
+        L[16] = B5.x; L[17] = B5.y; L[18] = B4.x; L[19] = B4.y;
+        L[20] = B2.x; L[21] = B2.y; L[22] = B1.x; L[23] = B1.y;
+        L[24] = B3.x; L[25] = B3.y;
+        L[26] = fB5 * j1;
+        L[27] = fB4 * j1;
+        L[28] = fB5 * dj1;
+        L[29] = fB1 * (dd1 * dj1 + q * j1);
+        const cdbl B3r = cscale(imkr, fB3 * dj1);
+        L[30] = B3r.x; L[31] = B3r.y;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) L[i] = 0.;
+      }
+    } else {
+      const int tt = t - kTile * kChunk;
+      const int c = tt / kChunk, kk = tt % kChunk;
+      const int ig = ig0 + kk;
+      const int p = task.tc * kTile + c;
+      double *R = Rsh[kk][c];
+      if (ig < G && p < nm) {
+        const int n2 = nmin + p;
+        const double d2 = Wd[(int64_t)p * G + ig];
+        const double dd2 = Wdd[(int64_t)p * G + ig];
+        const int64_t bo = (int64_t)(n2 - 1) * G + ig;
+        const cdbl jm = cmk(B[4 * NG + bo], B[5 * NG + bo]);
+        const cdbl djm = cmk(B[6 * NG + bo], B[7 * NG + bo]);
+        const double nn2 = n2 * (n2 + 1.);
+        const double a2 = sqrt((2. * n2 + 1.) / nn2);
+        const cdbl R1 = cscale(djm, a2 * d2);
+        const cdbl R2 = cscale(djm, a2 * dd2);
+        const cdbl R3 = cscale(jm, a2 * nn2 * d2);
+        const cdbl R4 = cscale(jm, a2 * d2);
+        const cdbl R5 = cscale(jm, a2 * dd2);
+        R[0] = R1.x; R[1] = R1.y; R[2] = R2.x; R[3] = R2.y;
+        R[4] = R3.x; R[5] = R3.y; R[6] = R4.x; R[7] = R4.y;
+        R[8] = R5.x; R[9] = R5.y;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 10; ++i) R[i] = 0.;
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const int kmax = (G - ig0) < kChunk ? (G - ig0) : kChunk;
+      if (!odd) {
+        for (int kk = 0; kk < kmax; ++kk) {
+          const double *L = Lsh[kk][rr];
+          const double *R = Rsh[kk][cc];
+          const cdbl R1 = cmk(R[0], R[1]), R2 = cmk(R[2], R[3]);
+          const cdbl R3 = cmk(R[4], R[5]), R4 = cmk(R[6], R[7]);
+          const cdbl R5 = cmk(R[8], R[9]);
+          cfma(Ja, cmk(L[0], L[1]), R1);
+          cfma(Ja, cmk(L[2], L[3]), R2);
+          cfma(Ja, cmk(L[4], L[5]), R3);
+          cfma(Jb, cmk(L[6], L[7]), R4);
+          cfma(Jb, cmk(L[8], L[9]), R5);
+          rfma(RJa, L[10], R1);
+          rfma(RJa, L[11], R2);
+          cfma(RJa, cmk(L[12], L[13]), R3);
+          rfma(RJb, L[14], R4);
+          rfma(RJb, L[15], R5);
+        }
+      } else {
+        for (int kk = 0; kk < kmax; ++kk) {
+          const double *L = Lsh[kk][rr] + 16;
+          const double *R = Rsh[kk][cc];
+          const cdbl R1 = cmk(R[0], R[1]), R2 = cmk(R[2], R[3]);
+          const cdbl R3 = cmk(R[4], R[5]), R4 = cmk(R[6], R[7]);
+          const cdbl R5 = cmk(R[8], R[9]);
+          cfma(Ja, cmk(L[0], L[1]), R5);
+          cfma(Ja, cmk(L[2], L[3]), R4);
+          cfma(Jb, cmk(L[4], L[5]), R2);
+          cfma(Jb, cmk(L[6], L[7]), R1);
+          cfma(Jb, cmk(L[8], L[9]), R3);
+          rfma(RJa, L[10], R5);
+          rfma(RJa, L[11], R4);
+          rfma(RJb, L[12], R2);
+          rfma(RJb, L[13], R1);
+          cfma(RJb, cmk(L[14], L[15]), R3);
+        }
+      }
+    }
+  }
+
+  if (p1 >= nm || p2 >= nm) return;
+  if (odd) {
+    // multiply by i: (x, y) -> (-y, x)
+    Ja = cmk(-Ja.y, Ja.x);
+    Jb = cmk(-Jb.y, Jb.x);
+    RJa = cmk(-RJa.y, RJa.x);
+    RJb = cmk(-RJb.y, RJb.x);
+  }
+  const double lam = gp.wavelength[it.grain];
+  const double k = 2. * M_PI / lam;
+  const double k2 = k * k;
+  const cdbl k2mr = cscale(cmk(gp.mr_re[it.grain], gp.mr_im[it.grain]), k2);
+  const cdbl Qx = cadd(cmul(k2mr, Ja), cscale(Jb, k2));
+  const cdbl Qy = cadd(cmul(k2mr, Jb), cscale(Ja, k2));
+  const cdbl RQx = cadd(cmul(k2mr, RJa), cscale(RJb, k2));
+  const cdbl RQy = cadd(cmul(k2mr, RJb), cscale(RJa, k2));
+  const int n1 = nmin + p1;
+  const int cx = n1 & 1;
+  const int64_t nm2 = (int64_t)nm * nm;
+  const int64_t e = (int64_t)p1 * nm + p2;
+  cdbl *Q = qbuf + it.q_off;
+  Q[cx * nm2 + e] = Qx;
+  Q[(1 - cx) * nm2 + e] = Qy;
+  Q[(2 + cx) * nm2 + e] = RQx;
+  Q[(2 + 1 - cx) * nm2 + e] = RQy;
+}
+
+// ---------------------------------------------------------------------------
+// K4: T_c = -RgQ_c Q_c^-1 for one (item, class).  Q_c is inverted in place in
+// LDS by Gauss-Jordan elimination with partial pivoting on |Q_ij| (largest
+// modulus, first index on ties: the rule of Matrix.hpp:280-293); the column
+// exchanges of the final unpermutation (Matrix.hpp:367-383) are folded into
+// the product's column index.  The product streams RgQ_c from HBM.
+__global__ __launch_bounds__(256) void k_invert_solve(
+    const int32_t *tasks, const Item *items, cdbl *qbuf, cdbl *tbuf,
+    double *qparts) {
+  extern __shared__ double smem[];
+  const int task = tasks[blockIdx.x];
+  const int item_index = task >> 1;
+  const int cls = task & 1;
+  const Item it = items[item_index];
+  const int N = it.N, m = it.m;
+  const int nmin = m > 0 ? m : 1;
+  const int nm = N + 1 - nmin;
+  const int64_t nm2 = (int64_t)nm * nm;
+  cdbl *A = reinterpret_cast<cdbl *>(smem);
+  cdbl *colk = A + nm2;
+  cdbl *rowk = colk + nm;
+  int *piv = reinterpret_cast<int *>(rowk + nm);
+  int *src = piv + nm;
+  __shared__ double red[4][4];
+
+  const cdbl *Qg = qbuf + it.q_off + cls * nm2;
+  const cdbl *RQg = qbuf + it.q_off + (2 + cls) * nm2;
+  const int t = threadIdx.x;
+  const int nt = blockDim.x;
+  for (int64_t e = t; e < nm2; e += nt) A[e] = Qg[e];
+  __syncthreads();
+
+  for (int k = 0; k < nm; ++k) {
+    if (t < 64) {
+      double best = -1.;
+      int bi = nm;
+      for (int i = k + t; i < nm; i += 64) {
+        const cdbl v = A[(int64_t)i * nm + k];
+        const double mag = v.x * v.x + v.y * v.y;
+        if (mag > best) {
+          best = mag;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (t == 0) piv[k] = bi;
+    }
+    __syncthreads();
+    const int p = piv[k];
+    if (p != k) {
+      for (int j = t; j < nm; j += nt) {
+        const cdbl a = A[(int64_t)k * nm + j];
+        A[(int64_t)k * nm + j] = A[(int64_t)p * nm + j];
+        A[(int64_t)p * nm + j] = a;
+      }
+      __syncthreads();
+    }
+    const cdbl inv = crcp(A[(int64_t)k * nm + k]);
+    for (int i = t; i < nm; i += nt) colk[i] = A[(int64_t)i * nm + k];
+    for (int j = t; j < nm; j += nt) rowk[j] = cmul(A[(int64_t)k * nm + j], inv);
+    __syncthreads();
+    for (int i = t >> 5; i < nm; i += (nt >> 5)) {
+     for (int j = t & 31; j < nm; j += 32) {
+      const int64_t e = (int64_t)i * nm + j;
+      cdbl v;
+      if (i == k) {
+        v = (j == k) ? inv : rowk[j];
+      } else if (j == k) {
+        v = cmul(colk[i], inv);
+        v.x = -v.x;
+        v.y = -v.y;
+      } else {
+        v = A[e];
+        const cdbl f = colk[i], g = rowk[j];
+        v.x -= f.x * g.x - f.y * g.y;
+        v.y -= f.x * g.y + f.y * g.x;
+      }
+      A[e] = v;
+     }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int j = 0; j < nm; ++j) src[j] = j;
+    for (int k = nm - 1; k >= 0; --k) {
+      const int p = piv[k];
+      if (p != k) {
+        const int tmp = src[k];
+        src[k] = src[p];
+        src[p] = tmp;
+      }
+    }
+  }
+  __syncthreads();
+
+  // T_c(r, c) = -sum_k RgQ_c(r, k) Ainv(k, src[c])
+  cdbl *T = tbuf ? tbuf + it.t_off + cls * nm2 : nullptr;
+  double s_diag2 = 0., e_diag = 0., s_all = 0., e_tr = 0.;
+  for (int r = t >> 5; r < nm; r += (nt >> 5)) {
+   for (int c = t & 31; c < nm; c += 32) {
+    const int64_t e = (int64_t)r * nm + c;
+    const cdbl *rq = RQg + (int64_t)r * nm;
+    const int sc = src[c];
+    cdbl acc = cmk(0., 0.);
+    for (int k = 0; k < nm; ++k) cfma(acc, rq[k], A[(int64_t)k * nm + sc]);
+    acc.x = -acc.x;
+    acc.y = -acc.y;
+    if (T) T[e] = acc;
+    const double a2 = acc.x * acc.x + acc.y * acc.y;
+    s_all += a2;
+    if (r == c) {
+      const double w = 2. * (nmin + r) + 1.;
+      s_diag2 += w * a2;
+      e_diag += w * acc.x;
+      e_tr += acc.x;
+    }
+   }
+  }
+  // deterministic block reduction: waves by shuffles, then wave 0
+  double v[4] = {s_diag2, e_diag, s_all, e_tr};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off);
+  }
+  if ((t & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[t >> 6][q] = v[q];
+  }
+  __syncthreads();
+  if (t == 0) {
+    const int nw = (nt + 63) / 64;
+    for (int q = 0; q < 4; ++q) {
+      double s = 0.;
+      for (int wv = 0; wv < nw; ++wv) s += red[wv][q];
+      qparts[(int64_t)task * kQParts + q] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5: convergence test over one window of orders for each active grain.
+// Items of grain g for the window are win_items[win_off[a] .. win_off[a+1])
+// in ascending order.
+__global__ void k_converge(const int32_t *active, int n_active,
+                           const int32_t *win_off, const int32_t *win_items,
+                           const Item *items, const double *qparts,
+                           double tol, int last_order, ConvState *state) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_active) return;
+  const int g = active[a];
+  ConvState st = state[g];
+  if (st.done) return;
+  for (int q = win_off[a]; q < win_off[a + 1]; ++q) {
+    const int ii = win_items[q];
+    const Item it = items[ii];
+    const double *p0 = qparts + (int64_t)(2 * ii) * kQParts;
+    const double *p1 = p0 + kQParts;
+    const double qs = p0[0] + p1[0];
+    const double qe = p0[1] + p1[1];
+    if (st.first) {
+      st.first = 0;
+      st.dqs = -1.;
+      st.dqe = -1.;
+    } else {
+      st.dqs = fabs((st.old_s - qs) / qs);
+      st.dqe = fabs((st.old_e - qe) / qe);
+    }
+    st.old_s = qs;
+    st.old_e = qe;
+    st.nmax = it.N;
+    st.item = ii;
+    if (st.dqs > 0. && st.dqs <= tol && st.dqe <= tol) {
+      st.done = 1;
+      st.status = 0;
+      break;
+    }
+    if (it.N >= last_order) {
+      st.done = 1;
+      st.status = 1;
+      break;
+    }
+  }
+  state[g] = st;
+}
+
+// Per-grain full-T sums in m order (TMatrix.hpp:2055-2110):
+// Q_sca = sum_m f_m sum_c |T_c|^2, Q_ext = sum_m f_m sum_c tr Re T_c.
+__global__ void k_full_sums(int n_grains, const int32_t *grain_items_off,
+                            const int32_t *grain_items, const double *qparts,
+                            double *qsca, double *qext) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_grains) return;
+  double s = 0., e = 0.;
+  for (int q = grain_items_off[g]; q < grain_items_off[g + 1]; ++q) {
+    const int ii = grain_items[q];
+    const double f = (q == grain_items_off[g]) ? 1. : 2.;
+    const double *p0 = qparts + (int64_t)(2 * ii) * kQParts;
+    const double *p1 = p0 + kQParts;
+    s += f * (p0[2] + p1[2]);
+    e += f * (p0[3] + p1[3]);
+  }
+  if (qsca) qsca[g] = s;
+  if (qext) qext[g] = e;
+}
+
+__global__ void k_write_orders(int n_grains, const ConvState *state,
+                               uint32_t *nmax_out, int32_t *status_out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_grains) return;
+  if (nmax_out) nmax_out[g] = (uint32_t)state[g].nmax;
+  if (status_out) status_out[g] = state[g].status;
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side).
+
+hipError_t launch_gauss_legendre(const GlKey *keys, int n, double *gl,
+                                 hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gauss_legendre, dim3(n), dim3(128), 0, s, keys, gl);
+  return hipGetLastError();
+}
+
+hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
+                         double *wig, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_wigner, dim3(n), dim3(128), 0, s, keys, gl, wig);
+  return hipGetLastError();
+}
+
+hipError_t launch_tables(const TableSet *sets, int n, const GrainParams &gp,
+                         const double *gl, double *tables, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tables, dim3(n), dim3(64), 0, s, sets, gp, gl, tables);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble(const TileTask *tasks, int n, const Item *items,
+                           const TableSet *sets, const WigKey *wkeys,
+                           const GrainParams &gp, const double *tables,
+                           const double *wig, cdbl *qbuf, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_assemble, dim3(n), dim3(256), 0, s, tasks, items, sets,
+                     wkeys, gp, tables, wig, qbuf);
+  return hipGetLastError();
+}
+
+size_t invert_solve_lds_bytes(int nm) {
+  return (size_t)nm * nm * sizeof(cdbl) + 2 * (size_t)nm * sizeof(cdbl) +
+         2 * (size_t)nm * sizeof(int);
+}
+
+hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
+                               const Item *items, cdbl *qbuf, cdbl *tbuf,
+                               double *qparts, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = invert_solve_lds_bytes(nm_max);
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(k_invert_solve),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_invert_solve, dim3(n), dim3(256), lds, s, tasks, items,
+                     qbuf, tbuf, qparts);
+  return hipGetLastError();
+}
+
+hipError_t launch_converge(const int32_t *active, int n_active,
+                           const int32_t *win_off, const int32_t *win_items,
+                           const Item *items, const double *qparts, double tol,
+                           int last_order, ConvState *state, hipStream_t s) {
+  if (n_active <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_converge, dim3((n_active + 127) / 128), dim3(128), 0, s,
+                     active, n_active, win_off, win_items, items, qparts, tol,
+                     last_order, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_full_sums(int n_grains, const int32_t *off,
+                            const int32_t *gi, const double *qparts,
+                            double *qsca, double *qext, hipStream_t s) {
+  if (n_grains <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_full_sums, dim3((n_grains + 127) / 128), dim3(128), 0,
+                     s, n_grains, off, gi, qparts, qsca, qext);
+  return hipGetLastError();
+}
+
+hipError_t launch_write_orders(int n_grains, const ConvState *state,
+                               uint32_t *nmax_out, int32_t *status_out,
+                               hipStream_t s) {
+  if (n_grains <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_write_orders, dim3((n_grains + 127) / 128), dim3(128),
+                     0, s, n_grains, state, nmax_out, status_out);
+  return hipGetLastError();
+}
+
+}  // namespace ctm

@@ -1,0 +1,44 @@
+// ctm_kernels.hpp -- launch wrappers of the gfx950 T-matrix kernels.
+#pragma once
+
+#include "ctm_device.hpp"
+
+namespace ctm {
+
+// Per-grain inputs (device arrays).
+struct GrainParams {
+  const double *radius;
+  const double *axis_ratio;
+  const double *wavelength;
+  const double *mr_re;
+  const double *mr_im;
+};
+
+hipError_t launch_gauss_legendre(const GlKey *keys, int n, double *gl,
+                                 hipStream_t s);
+hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
+                         double *wig, hipStream_t s);
+hipError_t launch_tables(const TableSet *sets, int n, const GrainParams &gp,
+                         const double *gl, double *tables, hipStream_t s);
+hipError_t launch_assemble(const TileTask *tasks, int n, const Item *items,
+                           const TableSet *sets, const WigKey *wkeys,
+                           const GrainParams &gp, const double *tables,
+                           const double *wig, cdbl *qbuf, hipStream_t s);
+size_t invert_solve_lds_bytes(int nm);
+// Largest block order the LDS-resident inversion holds.
+constexpr int kMaxLdsOrder = 99;
+hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
+                               const Item *items, cdbl *qbuf, cdbl *tbuf,
+                               double *qparts, hipStream_t s);
+hipError_t launch_converge(const int32_t *active, int n_active,
+                           const int32_t *win_off, const int32_t *win_items,
+                           const Item *items, const double *qparts, double tol,
+                           int last_order, ConvState *state, hipStream_t s);
+hipError_t launch_full_sums(int n_grains, const int32_t *off,
+                            const int32_t *gi, const double *qparts,
+                            double *qsca, double *qext, hipStream_t s);
+hipError_t launch_write_orders(int n_grains, const ConvState *state,
+                               uint32_t *nmax_out, int32_t *status_out,
+                               hipStream_t s);
+
+}  // namespace ctm

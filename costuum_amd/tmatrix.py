@@ -1,0 +1,172 @@
+"""Batched T-matrices on the GPU through the C ABI (include/costuum_amd.h).
+
+Host-side mirror of the reference's T-matrix entry points:
+  * ``tmatrix_batch`` -- the task-path chain InteractionTask ->
+    TMatrixM0Task (orders until converged) -> TMatrixMAllTask for a whole
+    batch of grains (TaskManager.hpp:1146-1197,1314-1327);
+  * ``tmatrix_fixed`` -- all m blocks at a forced (nmax, ngauss), the
+    TMatrix(wavelength, m, R_V, d, nmax, ngauss) constructor followed by
+    compute_additional_elements (TMatrix.hpp:532-992).
+Device memory and streams come from PyTorch; the arithmetic runs in the
+HIP library.  Lengths (radius, wavelength) share one unit.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native
+
+PRECISIONS = {"fp64": _native.CTM_PRECISION_FP64,
+              "mixed": _native.CTM_PRECISION_MIXED}
+
+
+class CtmError(RuntimeError):
+    pass
+
+
+class Context:
+    """A library context on one HIP device (owns a stream and scratch)."""
+
+    def __init__(self, device=0):
+        self._lib = _native.lib()
+        if not torch.cuda.is_available():
+            raise CtmError("no GPU visible to torch")
+        self.device = torch.device("cuda", device)
+        h = ctypes.c_void_p()
+        rc = self._lib.ctm_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise CtmError("ctm_create failed (%d)" % rc)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ctm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.ctm_last_error(self._h)
+            raise CtmError("%s failed (%d): %s" % (
+                what, rc, msg.decode() if msg else ""))
+
+    def synchronize(self):
+        self.check(self._lib.ctm_synchronize(self._h), "ctm_synchronize")
+
+    def stream_handle(self):
+        return self._lib.ctm_stream(self._h)
+
+
+_default = {}
+
+
+def default_context(device=0):
+    ctx = _default.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _default[device] = ctx
+    return ctx
+
+
+def _dev(x, device):
+    t = torch.as_tensor(x, dtype=torch.float64)
+    return t.to(device).contiguous()
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class BatchResult:
+    """Per-grain results of one batch; T-matrices stay on the device until
+    the next batch on the same context."""
+
+    def __init__(self, ctx, n, nmax, status, qsca, qext, ngauss=None):
+        self.ctx = ctx
+        self.n = n
+        self.nmax = nmax
+        self.status = status
+        self.qsca = qsca
+        self.qext = qext
+        self.ngauss = ngauss
+
+    def orders(self):
+        L = self.ctx._lib
+        nmax = np.zeros(self.n, dtype=np.uint32)
+        ngauss = np.zeros(self.n, dtype=np.uint32)
+        status = np.zeros(self.n, dtype=np.int32)
+        self.ctx.check(L.ctm_batch_orders(
+            self.ctx.handle, nmax.ctypes.data, ngauss.ctypes.data,
+            status.ctypes.data), "ctm_batch_orders")
+        return nmax, ngauss, status
+
+    def tmatrix(self, g):
+        """Grain g's T-matrix in the reference block layout (flat complex
+        array; see the oracle's tmatrix_blocks to split it per m)."""
+        L = self.ctx._lib
+        size = L.ctm_tmatrix_size(self.ctx.handle, g)
+        if size == 0:
+            raise CtmError("no T-matrix for grain %d" % g)
+        out = np.zeros(size, dtype=np.complex128)
+        self.ctx.check(L.ctm_tmatrix_get(self.ctx.handle, g, out.ctypes.data,
+                                         size), "ctm_tmatrix_get")
+        return out
+
+
+def tmatrix_batch(radius, axis_ratio, wavelength, mr_re, mr_im, min_order=4,
+                  max_order=100, glfac=2, tolerance=1e-4, precision="fp64",
+                  ctx=None, sync=True):
+    """Converged T-matrices for a batch of spheroids (task-path contract)."""
+    ctx = ctx or default_context()
+    dev = ctx.device
+    arrs = [_dev(a, dev) for a in (radius, axis_ratio, wavelength, mr_re,
+                                    mr_im)]
+    n = arrs[0].numel()
+    if any(a.numel() != n for a in arrs):
+        raise ValueError("grain arrays differ in length")
+    nmax = torch.zeros(n, dtype=torch.int32, device=dev)
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    qsca = torch.zeros(n, dtype=torch.float64, device=dev)
+    qext = torch.zeros(n, dtype=torch.float64, device=dev)
+    conv = _native.CtmConvergence(min_order, max_order, glfac,
+                                  PRECISIONS[precision], tolerance)
+    torch.cuda.current_stream(dev).synchronize()
+    rc = ctx._lib.ctm_tmatrix_batch(
+        ctx.handle, n, *[_ptr(a) for a in arrs], ctypes.byref(conv),
+        _ptr(nmax), _ptr(status), _ptr(qsca), _ptr(qext))
+    ctx.check(rc, "ctm_tmatrix_batch")
+    if sync:
+        ctx.synchronize()
+    return BatchResult(ctx, n, nmax, status, qsca, qext)
+
+
+def tmatrix_fixed(radius, axis_ratio, wavelength, mr_re, mr_im, nmax, ngauss,
+                  precision="fp64", ctx=None):
+    """All m blocks at forced (nmax, ngauss) per grain."""
+    ctx = ctx or default_context()
+    dev = ctx.device
+    arrs = [_dev(a, dev) for a in (radius, axis_ratio, wavelength, mr_re,
+                                    mr_im)]
+    n = arrs[0].numel()
+    nm = np.ascontiguousarray(np.broadcast_to(nmax, (n,)), dtype=np.uint32)
+    ng = np.ascontiguousarray(np.broadcast_to(ngauss, (n,)), dtype=np.uint32)
+    qsca = torch.zeros(n, dtype=torch.float64, device=dev)
+    qext = torch.zeros(n, dtype=torch.float64, device=dev)
+    torch.cuda.current_stream(dev).synchronize()
+    rc = ctx._lib.ctm_tmatrix_fixed(
+        ctx.handle, n, *[_ptr(a) for a in arrs], nm.ctypes.data,
+        ng.ctypes.data, PRECISIONS[precision], _ptr(qsca), _ptr(qext))
+    ctx.check(rc, "ctm_tmatrix_fixed")
+    ctx.synchronize()
+    return BatchResult(ctx, n, torch.as_tensor(nm.astype(np.int32)), None,
+                       qsca, qext, ngauss=ng)

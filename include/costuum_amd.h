@@ -1,0 +1,115 @@
+/*
+ * costuum_amd.h -- C ABI of the MI355X (gfx950) T-matrix library.
+ *
+ * The library computes converged EBCM T-matrices of spheroids for whole
+ * batches of grains on one GPU and the observables derived from them.  All
+ * signatures use plain pointers and sizes; complex values are interleaved
+ * (re, im) doubles.  Every call returns CTM_OK (0) or a negative status; the
+ * message of the last failure is available from ctm_last_error().  Nothing
+ * aborts the process.
+ *
+ * Pointers documented as "device" must point to GPU memory of the context's
+ * device (e.g. a torch tensor's data_ptr()); "host" pointers to CPU memory.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * SKIRT/CosTuuM).
+ */
+#ifndef COSTUUM_AMD_H
+#define COSTUUM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTM_OK 0
+#define CTM_ERR_ARGUMENT -1
+#define CTM_ERR_HIP -2
+#define CTM_ERR_STATE -3
+#define CTM_ERR_MEMORY -4
+
+/* Convergence status per grain. */
+#define CTM_CONVERGED 0
+#define CTM_NOT_CONVERGED 1 /* last order kept, as the task path does */
+
+/* Arithmetic of the batched path. */
+#define CTM_PRECISION_FP64 0
+#define CTM_PRECISION_MIXED 1
+
+typedef struct ctm_context ctm_context;
+
+/* Creates a context bound to HIP device `device` with its own stream
+ * (replaces the per-thread TMatrixAuxiliarySpaceManager scratch and the
+ * QuickSched runner: TMatrixResource.hpp:281-434,
+ * QuickSchedWrapper.hpp:302-318). */
+int ctm_create(ctm_context **ctx, int device);
+int ctm_destroy(ctm_context *ctx);
+const char *ctm_last_error(const ctm_context *ctx);
+/* The context's HIP stream as an opaque pointer (hipStream_t). */
+void *ctm_stream(ctm_context *ctx);
+int ctm_synchronize(ctm_context *ctx);
+
+/* Convergence parameters of the task path (TaskManager.hpp:371-384,
+ * CTMModule.cpp:160-175): orders min_order..max_order-1, ngauss = glfac *
+ * order nodes per hemisphere, relative tolerance on Q_sca and Q_ext. */
+typedef struct ctm_convergence {
+  uint32_t min_order;
+  uint32_t max_order;
+  uint32_t glfac;
+  uint32_t precision; /* CTM_PRECISION_* */
+  double tolerance;
+} ctm_convergence;
+
+/*
+ * Batched converged T-matrices: the task-path chain InteractionTask ->
+ * TMatrixM0Task (per order, until converged) -> TMatrixMAllTask (all m)
+ * (TaskManager.hpp:1146-1197,1314-1327; TMatrixResource.hpp:579-1120), for
+ * n_grains independent grains.
+ *
+ * Inputs (device, n_grains doubles each): equal-volume radius, axis ratio
+ * d = a/b (d < 1 prolate), wavelength (same length unit as the radius),
+ * real and imaginary refractive index.
+ * Outputs (device): converged order (uint32), status (int32), and the
+ * unnormalised full-T sums Q_sca, Q_ext (TMatrix.hpp:2055-2110; may be NULL).
+ * The T-matrices stay in the context until the next batch; fetch them with
+ * ctm_tmatrix_get() or use them in the observable calls below.
+ */
+int ctm_tmatrix_batch(ctm_context *ctx, int n_grains, const double *radius,
+                      const double *axis_ratio, const double *wavelength,
+                      const double *mr_re, const double *mr_im,
+                      const ctm_convergence *conv, uint32_t *nmax_out,
+                      int32_t *status_out, double *qsca_out,
+                      double *qext_out);
+
+/*
+ * The same chain at a forced order and ngauss per grain (host arrays), no
+ * convergence loop: the parity entry point (the reference's
+ * TMatrix(wavelength, m, R_V, d, nmax, ngauss) constructor plus
+ * compute_additional_elements, TMatrix.hpp:532-992).
+ */
+int ctm_tmatrix_fixed(ctm_context *ctx, int n_grains, const double *radius,
+                      const double *axis_ratio, const double *wavelength,
+                      const double *mr_re, const double *mr_im,
+                      const uint32_t *nmax_host, const uint32_t *ngauss_host,
+                      uint32_t precision, double *qsca_out, double *qext_out);
+
+/* Number of complex entries of grain g's T-matrix in the reference block
+ * layout (sum over m of (2 nm)^2), or 0 when g is out of range. */
+size_t ctm_tmatrix_size(ctm_context *ctx, int grain);
+/* Copies grain g's T-matrix to host memory in the reference block layout
+ * (TMatrix.hpp:1028-1046): for m = 0..nmax a row-major (2 nm) x (2 nm)
+ * complex block, row/col index i*nm + n - max(m,1). */
+int ctm_tmatrix_get(ctm_context *ctx, int grain, double *T_host,
+                    size_t capacity);
+
+/* Orders computed by the last batch (host copies of the per-grain results). */
+int ctm_batch_orders(ctm_context *ctx, uint32_t *nmax_host,
+                     uint32_t *ngauss_host, int32_t *status_host);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COSTUUM_AMD_H */

@@ -1,0 +1,228 @@
+"""ctypes bindings of the CPU FP64 oracle (liboracle.so).
+
+Test infrastructure only: the parity checker for the HIP path and the CPU
+baseline of bench.py.  Nothing in the costuum_amd package imports this.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    """Compile liboracle.so with the Makefile beside this file."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+_D = ctypes.c_double
+_U = ctypes.c_uint
+_I = ctypes.c_int
+_PD = ctypes.POINTER(ctypes.c_double)
+_PU = ctypes.POINTER(ctypes.c_uint)
+_PI = ctypes.POINTER(ctypes.c_int)
+
+
+def _declare(L):
+    L.ctmo_gauss_legendre.argtypes = [_U, _PD, _PD]
+    L.ctmo_gauss_legendre_ab.argtypes = [_U, _D, _D, _PD, _PD]
+    L.ctmo_rv_over_rs.argtypes = [_D]
+    L.ctmo_rv_over_rs.restype = _D
+    L.ctmo_r_dr_spheroid.argtypes = [_PD, _U, _D, _PD, _PD]
+    L.ctmo_sph_j_real.argtypes = [_U, _D, _PD, _PD]
+    L.ctmo_sph_j_cplx.argtypes = [_U, _PD, _PD, _PD]
+    L.ctmo_sph_y_real.argtypes = [_U, _D, _PD, _PD]
+    L.ctmo_wigner_dn_0m.argtypes = [_D, _U, _U, _PD, _PD]
+    L.ctmo_wigner_dn_0m_sinx.argtypes = [_D, _D, _D, _U, _U, _PD, _PD]
+    L.ctmo_plu_inverse.argtypes = [_U, _PD]
+    L.ctmo_clebsch_gordan.argtypes = [_I] * 6
+    L.ctmo_clebsch_gordan.restype = _D
+    L.ctmo_tmatrix_size.argtypes = [_U]
+    L.ctmo_tmatrix_size.restype = ctypes.c_size_t
+    L.ctmo_tmatrix_offset.argtypes = [_U, _U]
+    L.ctmo_tmatrix_offset.restype = ctypes.c_size_t
+    L.ctmo_tmatrix_m.argtypes = [_D, _D, _D, _D, _D, _U, _U, _U, _PD]
+    L.ctmo_tmatrix_all.argtypes = [_D, _D, _D, _D, _D, _U, _U, _PD]
+    L.ctmo_m0_qsums.argtypes = [_U, _PD, _PD, _PD]
+    L.ctmo_qcoeffs.argtypes = [_U, _PD, _PD, _PD]
+    L.ctmo_converge_task.argtypes = [_D, _D, _D, _D, _D, _U, _U, _U, _D,
+                                     _PU, _PU, _PD, _PD]
+    L.ctmo_converge_task.restype = _I
+    L.ctmo_converge_calculator.argtypes = [_D, _D, _D, _D, _U, _D, _U, _D, _D,
+                                           _U, _PU, _PU, _PD]
+    L.ctmo_converge_calculator.restype = _I
+    L.ctmo_batch_task.argtypes = [_I, _PD, _PD, _PD, _PD, _PD, _U, _U, _U, _D,
+                                  _I, _PU, _PI, _PD, _PD]
+
+
+def _p(a):
+    return a.ctypes.data_as(_PD)
+
+
+def gauss_legendre(order, a=None, b=None):
+    x = np.zeros(order)
+    w = np.zeros(order)
+    if a is None:
+        lib().ctmo_gauss_legendre(order, _p(x), _p(w))
+    else:
+        lib().ctmo_gauss_legendre_ab(order, a, b, _p(x), _p(w))
+    return x, w
+
+
+def rv_over_rs(d):
+    return lib().ctmo_rv_over_rs(d)
+
+
+def r_dr_spheroid(costheta, d):
+    costheta = np.ascontiguousarray(costheta, dtype=np.float64)
+    r2 = np.zeros_like(costheta)
+    dr = np.zeros_like(costheta)
+    lib().ctmo_r_dr_spheroid(_p(costheta), len(costheta), d, _p(r2), _p(dr))
+    return r2, dr
+
+
+def sph_j(nmax, z):
+    if isinstance(z, complex):
+        zz = np.array([z.real, z.imag])
+        j = np.zeros(nmax, dtype=np.complex128)
+        dj = np.zeros(nmax, dtype=np.complex128)
+        lib().ctmo_sph_j_cplx(nmax, _p(zz), _p(j), _p(dj))
+        return j, dj
+    j = np.zeros(nmax)
+    dj = np.zeros(nmax)
+    lib().ctmo_sph_j_real(nmax, float(z), _p(j), _p(dj))
+    return j, dj
+
+
+def sph_y(nmax, z):
+    y = np.zeros(nmax)
+    dy = np.zeros(nmax)
+    lib().ctmo_sph_y_real(nmax, float(z), _p(y), _p(dy))
+    return y, dy
+
+
+def wigner_dn_0m(cosx, nmax, m):
+    y = np.zeros(nmax)
+    dy = np.zeros(nmax)
+    lib().ctmo_wigner_dn_0m(cosx, nmax, m, _p(y), _p(dy))
+    return y, dy
+
+
+def wigner_dn_0m_sinx(cosx, sinx, sinx_inv, nmax, m):
+    y = np.zeros(nmax)
+    dy = np.zeros(nmax)
+    lib().ctmo_wigner_dn_0m_sinx(cosx, sinx, sinx_inv, nmax, m, _p(y), _p(dy))
+    return y, dy
+
+
+def plu_inverse(A):
+    A = np.ascontiguousarray(A, dtype=np.complex128).copy()
+    lib().ctmo_plu_inverse(A.shape[0], _p(A))
+    return A
+
+
+def clebsch_gordan(n1, m1, n2, m2, N, M):
+    return lib().ctmo_clebsch_gordan(n1, m1, n2, m2, N, M)
+
+
+def tmatrix_size(nmax):
+    return lib().ctmo_tmatrix_size(nmax)
+
+
+def tmatrix_offset(nmax, m):
+    return lib().ctmo_tmatrix_offset(nmax, m)
+
+
+def tmatrix_m(rv, d, lam, mr, nmax, ngauss, m):
+    nm = nmax if m == 0 else nmax + 1 - m
+    T = np.zeros((2 * nm, 2 * nm), dtype=np.complex128)
+    lib().ctmo_tmatrix_m(rv, d, lam, mr.real, mr.imag, nmax, ngauss, m, _p(T))
+    return T
+
+
+def tmatrix_all(rv, d, lam, mr, nmax, ngauss):
+    T = np.zeros(tmatrix_size(nmax), dtype=np.complex128)
+    lib().ctmo_tmatrix_all(rv, d, lam, mr.real, mr.imag, nmax, ngauss, _p(T))
+    return T
+
+
+def tmatrix_blocks(T, nmax):
+    """Split a flat full T-matrix into its list of m blocks."""
+    out = []
+    for m in range(nmax + 1):
+        nm = nmax if m == 0 else nmax + 1 - m
+        o = tmatrix_offset(nmax, m)
+        out.append(T[o:o + 4 * nm * nm].reshape(2 * nm, 2 * nm))
+    return out
+
+
+def m0_qsums(nmax, T0):
+    T0 = np.ascontiguousarray(T0, dtype=np.complex128)
+    s = ctypes.c_double()
+    e = ctypes.c_double()
+    lib().ctmo_m0_qsums(nmax, _p(T0), ctypes.byref(s), ctypes.byref(e))
+    return s.value, e.value
+
+
+def qcoeffs(nmax, T):
+    T = np.ascontiguousarray(T, dtype=np.complex128)
+    s = ctypes.c_double()
+    e = ctypes.c_double()
+    lib().ctmo_qcoeffs(nmax, _p(T), ctypes.byref(s), ctypes.byref(e))
+    return s.value, e.value
+
+
+def converge_task(rv, d, lam, mr, min_order=4, max_order=100, glfac=2,
+                  tol=1e-4):
+    n = ctypes.c_uint()
+    g = ctypes.c_uint()
+    dqs = ctypes.c_double()
+    dqe = ctypes.c_double()
+    st = lib().ctmo_converge_task(rv, d, lam, mr.real, mr.imag, min_order,
+                                  max_order, glfac, tol, ctypes.byref(n),
+                                  ctypes.byref(g), ctypes.byref(dqs),
+                                  ctypes.byref(dqe))
+    return st, n.value, g.value, dqs.value, dqe.value
+
+
+def converge_calculator(ratio_of_radii, d, axi, lam, mr, maximum_order=200,
+                        tol=1e-4, ndgs=2, maximum_ngauss=500):
+    n = ctypes.c_uint()
+    g = ctypes.c_uint()
+    rv = ctypes.c_double()
+    st = lib().ctmo_converge_calculator(ratio_of_radii, d, axi, lam,
+                                        maximum_order, tol, ndgs, mr.real,
+                                        mr.imag, maximum_ngauss,
+                                        ctypes.byref(n), ctypes.byref(g),
+                                        ctypes.byref(rv))
+    return st, n.value, g.value, rv.value
+
+
+def batch_task(rv, d, lam, mr_re, mr_im, min_order=4, max_order=100,
+               glfac=2, tol=1e-4, nthreads=0):
+    rv, d, lam, mr_re, mr_im = [np.ascontiguousarray(a, dtype=np.float64)
+                                for a in (rv, d, lam, mr_re, mr_im)]
+    n = len(rv)
+    nmax = np.zeros(n, dtype=np.uint32)
+    status = np.zeros(n, dtype=np.int32)
+    qs = np.zeros(n)
+    qe = np.zeros(n)
+    lib().ctmo_batch_task(n, _p(rv), _p(d), _p(lam), _p(mr_re), _p(mr_im),
+                          min_order, max_order, glfac, tol, nthreads,
+                          nmax.ctypes.data_as(_PU),
+                          status.ctypes.data_as(_PI), _p(qs), _p(qe))
+    return nmax, status, qs, qe
