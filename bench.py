@@ -1,0 +1,153 @@
+#!/usr/bin/env python
+"""Converged T-matrices/s on one MI355X for the 4096-spheroid batch.
+
+Workload: 4096 spheroids drawn from seed 0 (radius log-uniform in
+[0.01, 1] um, axis ratio uniform in [0.5, 2], wavelength log-uniform in
+[0.1, 10] um, Re(m) uniform in [1.3, 2], Im(m) log-uniform in [1e-4, 0.5]);
+task-path convergence with minimum order 4, maximum order 100, 2 Gauss
+nodes per order and tolerance 1e-4.  One step = one pass of the hot path
+over the whole batch with the inputs resident in HBM: every order of the
+convergence loop, all m blocks at the converged order, and the per-grain
+Q_sca/Q_ext sums.
+
+Prints one JSON line.  ``value`` counts the grains whose convergence test
+passed, per second of the timed steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--grains", type=int, default=4096)
+    ap.add_argument("--cpu-baseline", type=int, default=1,
+                    help="time the CPU oracle on a sample (rank 0, N=1)")
+    ap.add_argument("--cpu-stride", type=int, default=16,
+                    help="CPU sample = every k-th grain in size order")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(g, stride, threads):
+    """The CPU oracle (FP64 restatement of the reference algorithm) on a
+    size-stratified sample of the batch."""
+    import numpy as np
+    import oracle
+    x = 2 * np.pi * g["radius"] / g["wavelength"]
+    order = np.argsort(x)
+    sel = order[stride // 2::stride]
+    if threads <= 0:
+        threads = min(os.cpu_count() or 1, 16)
+    args = [g[k][sel] for k in ("radius", "axis_ratio", "wavelength",
+                                 "mr_re", "mr_im")]
+    t0 = time.perf_counter()
+    nmax, status, _, _ = oracle.batch_task(*args, min_order=4, max_order=100,
+                                           glfac=2, tol=1e-4,
+                                           nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": float((status == 0).sum() / dt),
+            "unit": "converged T-matrices/s", "cores": int(threads),
+            "kind": "port",
+            "sample": "%d of 4096 grains (every %dth in size-parameter "
+                      "order), %.2f s wall on %d threads" % (len(sel), stride,
+                                                            dt, threads)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU "
+              "path (grains shard with no communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import make_inputs
+    from costuum_amd.tmatrix import Context, tmatrix_batch
+
+    g = make_inputs.load(0, 4096)
+    if a.grains != 4096:
+        g = {k: v[:a.grains] for k, v in g.items()}
+    ctx = Context(0)
+    dev = ctx.device
+    keys = ("radius", "axis_ratio", "wavelength", "mr_re", "mr_im")
+    dg = [torch.as_tensor(g[k], dtype=torch.float64, device=dev) for k in keys]
+    conv = dict(min_order=4, max_order=100, glfac=2, tolerance=1e-4,
+                precision=a.precision)
+
+    res = None
+    for _ in range(a.warmup):
+        res = tmatrix_batch(*dg, ctx=ctx, **conv)
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = tmatrix_batch(*dg, ctx=ctx, **conv)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = ctx.profile_stats()
+    ms_per_step = (t1 - t0) * 1e3 / max(a.steps, 1)
+    status = res.status.cpu().numpy()
+    nconv = int((status == 0).sum())
+    value = nconv / (ms_per_step / 1e3)
+
+    launches, ms, flops, nbytes = stats["assemble"]
+    avg_ms = ms / max(launches, 1)
+    achieved = flops / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+    roofline = {"kernel": "k_assemble", "bound": "mfma",
+                "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS,
+                                                  4),
+                "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+                "launches": int(launches),
+                "algorithmic_flops_per_launch": flops / max(launches, 1)}
+    kernels = {k: {"launches": int(v[0]), "ms": round(v[1], 3),
+                   "tflops": round(v[2] / (v[1] / 1e3) / 1e12, 3)
+                   if v[1] > 0 and v[2] > 0 else None}
+               for k, v in stats.items()}
+    cpu = None
+    if a.cpu_baseline:
+        cpu = cpu_baseline(g, a.cpu_stride, a.cpu_threads)
+    out = {
+        "metric": "converged T-matrices/s on 1 GPU, batched mixed vs plain "
+                  "FP64, at <=1e-6 rel Q_ext err",
+        "value": round(value, 2),
+        "unit": "converged T-matrices/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "batched plain %s: %d spheroids, seed 0, "
+                               "task-path convergence (orders 4..99, "
+                               "ngauss = 2 order, tol 1e-4)"
+                               % (a.precision.upper(), len(g["radius"])),
+                   "grains": len(g["radius"]),
+                   "converged": nconv,
+                   "nmax_max": int(res.nmax.max().item()),
+                   "precision": a.precision},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -20,7 +20,18 @@ EXPORTS = (
     "ctm_create", "ctm_destroy", "ctm_last_error", "ctm_stream",
     "ctm_synchronize", "ctm_tmatrix_batch", "ctm_tmatrix_fixed",
     "ctm_tmatrix_size", "ctm_tmatrix_get", "ctm_batch_orders",
+    "ctm_profile_enable", "ctm_profile_reset", "ctm_profile_read",
 )
+
+KERNELS = ("gauss_legendre", "wigner", "tables", "assemble", "solve",
+           "converge", "sums")
+
+
+class CtmKernelStats(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_uint64),
+                ("milliseconds", ctypes.c_double),
+                ("flops", ctypes.c_double),
+                ("bytes", ctypes.c_double)]
 
 
 class CtmConvergence(ctypes.Structure):
@@ -68,5 +79,12 @@ def lib():
     L.ctm_tmatrix_get.restype = ctypes.c_int
     L.ctm_batch_orders.argtypes = [vp, dp, dp, dp]
     L.ctm_batch_orders.restype = ctypes.c_int
+    L.ctm_profile_enable.argtypes = [vp, ctypes.c_int]
+    L.ctm_profile_enable.restype = ctypes.c_int
+    L.ctm_profile_reset.argtypes = [vp]
+    L.ctm_profile_reset.restype = ctypes.c_int
+    L.ctm_profile_read.argtypes = [vp, ctypes.c_int,
+                                   ctypes.POINTER(CtmKernelStats)]
+    L.ctm_profile_read.restype = ctypes.c_int
     _lib = L
     return L

@@ -154,6 +154,72 @@ struct WorkList {
   }
 };
 
+// Per-kernel timing with HIP events recorded on the context's stream around
+// every launch, and the algorithmic work each launch stands for.
+struct Profiler {
+  struct Rec {
+    int kernel;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<Rec> pending;
+  uint64_t launches[CTM_KERNEL_COUNT] = {};
+  double ms[CTM_KERNEL_COUNT] = {};
+  double flops[CTM_KERNEL_COUNT] = {};
+  double bytes[CTM_KERNEL_COUNT] = {};
+  hipEvent_t take() {
+    if (pool.empty()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  // Records the start event; returns the record index or -1.
+  int begin(int kernel, hipStream_t s, double f, double b) {
+    if (!on) return -1;
+    Rec r{kernel, take(), take(), f, b};
+    if (!r.a || !r.b) return -1;
+    if (hipEventRecord(r.a, s) != hipSuccess) return -1;
+    pending.push_back(r);
+    return (int)pending.size() - 1;
+  }
+  void end(int idx, hipStream_t s) {
+    if (idx >= 0) (void)hipEventRecord(pending[idx].b, s);
+  }
+  // Folds finished records into the totals (the stream must be idle).
+  void collect() {
+    for (const Rec &r : pending) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+        launches[r.kernel] += 1;
+        ms[r.kernel] += t;
+        flops[r.kernel] += r.flops;
+        bytes[r.kernel] += r.bytes;
+      }
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  void reset() {
+    collect();
+    for (int k = 0; k < CTM_KERNEL_COUNT; ++k) {
+      launches[k] = 0;
+      ms[k] = flops[k] = bytes[k] = 0.;
+    }
+  }
+  void release() {
+    collect();
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+  }
+};
+
 }  // namespace
 
 struct ctm_context {
@@ -172,6 +238,7 @@ struct ctm_context {
   std::vector<Item> final_items;          // phase-2 items
   std::vector<int32_t> grain_items_off;   // per grain into final_items
   std::vector<int32_t> grain_items;
+  Profiler prof;
 };
 
 #define CTM_HIP(expr)                                                   \
@@ -181,6 +248,14 @@ struct ctm_context {
       ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);     \
       return CTM_ERR_HIP;                                               \
     }                                                                   \
+  } while (0)
+
+// Launch `call` between two profiler events when profiling is on.
+#define CTM_LAUNCH(kernel, flops, bytes, call)                          \
+  do {                                                                  \
+    const int _pi = ctx->prof.begin(kernel, ctx->stream, flops, bytes); \
+    CTM_HIP(call);                                                      \
+    ctx->prof.end(_pi, ctx->stream);                                    \
   } while (0)
 
 namespace {
@@ -222,8 +297,10 @@ int flush_shared(ctm_context *ctx) {
   if (sh.gl_done < ngl) {
     std::vector<GlKey> fresh(sh.gl_keys.begin() + sh.gl_done, sh.gl_keys.end());
     CTM_HIP(upload(ctx->d_glkeys, fresh, ctx->stream));
-    CTM_HIP(launch_gauss_legendre(ctx->d_glkeys.as<GlKey>(), (int)fresh.size(),
-                                  ctx->d_gl.as<double>(), ctx->stream));
+    CTM_LAUNCH(CTM_KERNEL_GAUSS_LEGENDRE, 0., 0.,
+               launch_gauss_legendre(ctx->d_glkeys.as<GlKey>(),
+                                     (int)fresh.size(), ctx->d_gl.as<double>(),
+                                     ctx->stream));
     // the key buffer is rewritten below; wait for the launch to read it
     CTM_HIP(hipStreamSynchronize(ctx->stream));
     sh.gl_done = ngl;
@@ -232,9 +309,10 @@ int flush_shared(ctm_context *ctx) {
     std::vector<WigKey> fresh(sh.wig_keys.begin() + sh.wig_done,
                               sh.wig_keys.end());
     CTM_HIP(upload(ctx->d_glkeys, fresh, ctx->stream));
-    CTM_HIP(launch_wigner(ctx->d_glkeys.as<WigKey>(), (int)fresh.size(),
-                          ctx->d_gl.as<double>(), ctx->d_wig.as<double>(),
-                          ctx->stream));
+    CTM_LAUNCH(CTM_KERNEL_WIGNER, 0., 0.,
+               launch_wigner(ctx->d_glkeys.as<WigKey>(), (int)fresh.size(),
+                             ctx->d_gl.as<double>(), ctx->d_wig.as<double>(),
+                             ctx->stream));
     CTM_HIP(hipStreamSynchronize(ctx->stream));
     sh.wig_done = nwig;
   }
@@ -290,13 +368,37 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     CTM_HIP(ctx->d_t.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
   CTM_HIP(ctx->d_qparts.ensure(std::max<size_t>(w.items.size(), 1) * 2 *
                                kQParts * sizeof(double)));
-  CTM_HIP(launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(), gp,
-                        ctx->d_gl.as<double>(), ctx->d_tables.as<double>(), s));
-  CTM_HIP(launch_assemble(ctx->d_tiles.as<TileTask>(), (int)w.tiles.size(),
-                          ctx->d_items.as<Item>(), ctx->d_sets.as<TableSet>(),
-                          ctx->d_wigkeys.as<WigKey>(), gp,
-                          ctx->d_tables.as<double>(), ctx->d_wig.as<double>(),
-                          ctx->d_q.as<cdbl>(), s));
+  // Algorithmic work of the launches (the per-block counts of the flop
+  // model F_T: 96 G nm^2 for an m > 0 assembly, 32 G n^2 for an m = 0
+  // order; (64/3) nm^3 and (16/3) n^3 for the parity-split solves) and the
+  // compulsory HBM traffic (tables and Wigner values read once, Q/RgQ
+  // written once by the assembly, read once by the solve, T written once).
+  double f_asm = 0., b_asm = 0., b_tab = 0.;
+  for (const Item &it : w.items) {
+    const int nmin = it.m > 0 ? it.m : 1;
+    const double nm = it.N + 1 - nmin;
+    if (it.m > 0) {
+      f_asm += 96. * it.G * nm * nm;
+    } else {
+      f_asm += 32. * it.G * nm * nm;
+    }
+    b_asm += 8. * (2. * nm * it.G) + 16. * 4. * nm * nm;
+  }
+  for (const TableSet &ts : w.sets)
+    b_tab += 8. * ((double)kNodeScalars * ts.G +
+                   (double)kBesselArrays * ts.N * ts.G);
+  b_asm += b_tab;
+  CTM_LAUNCH(CTM_KERNEL_TABLES, 0., b_tab,
+             launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(), gp,
+                           ctx->d_gl.as<double>(), ctx->d_tables.as<double>(),
+                           s));
+  CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, f_asm, b_asm,
+             launch_assemble(ctx->d_tiles.as<TileTask>(), (int)w.tiles.size(),
+                             ctx->d_items.as<Item>(),
+                             ctx->d_sets.as<TableSet>(),
+                             ctx->d_wigkeys.as<WigKey>(), gp,
+                             ctx->d_tables.as<double>(),
+                             ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(), s));
   // solve tasks in LDS-size buckets, largest first
   const int bounds[] = {kMaxLdsOrder, 72, 48, 24};
   size_t pos = 0;
@@ -318,10 +420,19 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
       ++end;
     }
     if (end > pos) {
-      CTM_HIP(launch_invert_solve(dsolve + pos, (int)(end - pos), nm_max,
-                                  ctx->d_items.as<Item>(), ctx->d_q.as<cdbl>(),
-                                  want_t ? ctx->d_t.as<cdbl>() : nullptr,
-                                  ctx->d_qparts.as<double>(), s));
+      double fs = 0., bs = 0.;
+      for (size_t q = pos; q < end; ++q) {
+        const Item &it = w.items[w.solve[q] >> 1];
+        const double nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+        fs += (it.m > 0 ? 32. / 3. : 8. / 3.) * nm * nm * nm;
+        bs += 16. * 2. * nm * nm + (want_t ? 16. * nm * nm : 0.);
+      }
+      CTM_LAUNCH(CTM_KERNEL_SOLVE, fs, bs,
+                 launch_invert_solve(dsolve + pos, (int)(end - pos), nm_max,
+                                     ctx->d_items.as<Item>(),
+                                     ctx->d_q.as<cdbl>(),
+                                     want_t ? ctx->d_t.as<cdbl>() : nullptr,
+                                     ctx->d_qparts.as<double>(), s));
     }
     pos = end;
   }
@@ -369,6 +480,7 @@ int ctm_destroy(ctm_context *ctx) {
                     &ctx->d_state,  &ctx->d_active,  &ctx->d_winoff,
                     &ctx->d_winitems, &ctx->d_gioff, &ctx->d_gitems};
   for (DevBuf *b : bufs) b->release();
+  ctx->prof.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CTM_OK;
@@ -422,10 +534,11 @@ static int run_all_m(ctm_context *ctx, const GrainParams &gp, int n,
   for (size_t i = 0; i < w.items.size(); ++i) ctx->grain_items[i] = (int32_t)i;
   CTM_HIP(upload(ctx->d_gioff, ctx->grain_items_off, ctx->stream));
   CTM_HIP(upload(ctx->d_gitems, ctx->grain_items, ctx->stream));
-  CTM_HIP(launch_full_sums(n, ctx->d_gioff.as<int32_t>(),
-                           ctx->d_gitems.as<int32_t>(),
-                           ctx->d_qparts.as<double>(), qsca_out, qext_out,
-                           ctx->stream));
+  CTM_LAUNCH(CTM_KERNEL_SUMS, 0., 0.,
+             launch_full_sums(n, ctx->d_gioff.as<int32_t>(),
+                              ctx->d_gitems.as<int32_t>(),
+                              ctx->d_qparts.as<double>(), qsca_out, qext_out,
+                              ctx->stream));
   ctx->final_items = w.items;
   return CTM_OK;
 }
@@ -513,12 +626,13 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     CTM_HIP(upload(ctx->d_active, active, s));
     CTM_HIP(upload(ctx->d_winoff, win_off, s));
     CTM_HIP(upload(ctx->d_winitems, win_items, s));
-    CTM_HIP(launch_converge(ctx->d_active.as<int32_t>(), (int)active.size(),
-                            ctx->d_winoff.as<int32_t>(),
-                            ctx->d_winitems.as<int32_t>(),
-                            ctx->d_items.as<Item>(),
-                            ctx->d_qparts.as<double>(), conv->tolerance,
-                            last_order, ctx->d_state.as<ConvState>(), s));
+    CTM_LAUNCH(CTM_KERNEL_CONVERGE, 0., 0.,
+               launch_converge(ctx->d_active.as<int32_t>(), (int)active.size(),
+                               ctx->d_winoff.as<int32_t>(),
+                               ctx->d_winitems.as<int32_t>(),
+                               ctx->d_items.as<Item>(),
+                               ctx->d_qparts.as<double>(), conv->tolerance,
+                               last_order, ctx->d_state.as<ConvState>(), s));
     CTM_HIP(hipMemcpyAsync(st.data(), ctx->d_state.p, n * sizeof(ConvState),
                            hipMemcpyDeviceToHost, s));
     CTM_HIP(hipStreamSynchronize(s));
@@ -632,6 +746,31 @@ int ctm_tmatrix_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
     }
     out_off += (size_t)n2m * n2m;
   }
+  return CTM_OK;
+}
+
+int ctm_profile_enable(ctm_context *ctx, int on) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  ctx->prof.on = on != 0;
+  return CTM_OK;
+}
+
+int ctm_profile_reset(ctm_context *ctx) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->prof.reset();
+  return CTM_OK;
+}
+
+int ctm_profile_read(ctm_context *ctx, int kernel, ctm_kernel_stats *out) {
+  if (!ctx || !out || kernel < 0 || kernel >= CTM_KERNEL_COUNT)
+    return CTM_ERR_ARGUMENT;
+  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->prof.collect();
+  out->launches = ctx->prof.launches[kernel];
+  out->milliseconds = ctx->prof.ms[kernel];
+  out->flops = ctx->prof.flops[kernel];
+  out->bytes = ctx->prof.bytes[kernel];
   return CTM_OK;
 }
 

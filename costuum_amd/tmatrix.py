@@ -66,6 +66,22 @@ class Context:
     def stream_handle(self):
         return self._lib.ctm_stream(self._h)
 
+    def profile(self, on=True):
+        self.check(self._lib.ctm_profile_enable(self._h, int(on)),
+                   "ctm_profile_enable")
+        self.check(self._lib.ctm_profile_reset(self._h), "ctm_profile_reset")
+
+    def profile_stats(self):
+        """{kernel: (launches, ms, flops, bytes)} since the last reset."""
+        out = {}
+        for k, name in enumerate(_native.KERNELS):
+            st = _native.CtmKernelStats()
+            self.check(self._lib.ctm_profile_read(self._h, k,
+                                                  ctypes.byref(st)),
+                       "ctm_profile_read")
+            out[name] = (st.launches, st.milliseconds, st.flops, st.bytes)
+        return out
+
 
 _default = {}
 

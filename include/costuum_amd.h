@@ -108,6 +108,32 @@ int ctm_tmatrix_get(ctm_context *ctx, int grain, double *T_host,
 int ctm_batch_orders(ctm_context *ctx, uint32_t *nmax_host,
                      uint32_t *ngauss_host, int32_t *status_host);
 
+/* Kernel timing.  With profiling on, every launch of the library is
+ * bracketed by two HIP events on the context's stream; ctm_profile_read
+ * returns, per kernel, the number of launches, the summed event time and the
+ * algorithmic FLOPs and compulsory HBM bytes those launches stand for.  No
+ * reference counterpart: it replaces QuickSched's per-task tic/toc log
+ * (qsched.c:572,620-622; QuickSchedWrapper.hpp:494-508). */
+#define CTM_KERNEL_GAUSS_LEGENDRE 0
+#define CTM_KERNEL_WIGNER 1
+#define CTM_KERNEL_TABLES 2
+#define CTM_KERNEL_ASSEMBLE 3
+#define CTM_KERNEL_SOLVE 4
+#define CTM_KERNEL_CONVERGE 5
+#define CTM_KERNEL_SUMS 6
+#define CTM_KERNEL_COUNT 7
+
+typedef struct ctm_kernel_stats {
+  uint64_t launches;
+  double milliseconds;
+  double flops;
+  double bytes;
+} ctm_kernel_stats;
+
+int ctm_profile_enable(ctm_context *ctx, int on);
+int ctm_profile_reset(ctm_context *ctx);
+int ctm_profile_read(ctm_context *ctx, int kernel, ctm_kernel_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -803,25 +803,90 @@ void ctmo_batch_task(int n, const double *rv, const double *d,
                      unsigned max_order, unsigned glfac, double tol,
                      int nthreads, unsigned *nmax_out, int *status_out,
                      double *qsca_out, double *qext_out) {
+  /* Same task decomposition as the reference's graph
+   * (TaskManager.hpp:1146-1197,1314-1327): one order chain per grain, then
+   * independent (grain, m) blocks, each level spread over the threads. */
 #ifdef _OPENMP
   if (nthreads > 0) {
     omp_set_num_threads(nthreads);
   }
+#endif
+  unsigned *ngauss = (unsigned *)malloc(sizeof(unsigned) * (n ? n : 1));
+#ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
   for (int g = 0; g < n; ++g) {
-    unsigned nmax, ngauss;
     double dqs, dqe;
     status_out[g] = ctmo_converge_task(rv[g], d[g], lambda[g], mr_re[g],
                                        mr_im[g], min_order, max_order, glfac,
-                                       tol, &nmax, &ngauss, &dqs, &dqe);
-    nmax_out[g] = nmax;
-    double *T = (double *)malloc(sizeof(cplx) * ctmo_tmatrix_size(nmax));
-    ctmo_tmatrix_all(rv[g], d[g], lambda[g], mr_re[g], mr_im[g], nmax,
-                     ngauss, T);
-    ctmo_qcoeffs(nmax, T, &qsca_out[g], &qext_out[g]);
+                                       tol, &nmax_out[g], &ngauss[g], &dqs,
+                                       &dqe);
+  }
+  tables_t *tabs = (tables_t *)malloc(sizeof(tables_t) * (n ? n : 1));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int g = 0; g < n; ++g) {
+    tables_build(&tabs[g], rv[g], d[g], lambda[g], mr_re[g] + I * mr_im[g],
+                 nmax_out[g], ngauss[g]);
+  }
+  /* (grain, m) work list, largest blocks first */
+  size_t *first = (size_t *)malloc(sizeof(size_t) * (n + 1));
+  first[0] = 0;
+  for (int g = 0; g < n; ++g) first[g + 1] = first[g] + nmax_out[g] + 1;
+  const size_t nwork = first[n];
+  int *wg = (int *)malloc(sizeof(int) * (nwork ? nwork : 1));
+  unsigned *wm = (unsigned *)malloc(sizeof(unsigned) * (nwork ? nwork : 1));
+  double *bs = (double *)calloc(nwork ? nwork : 1, sizeof(double));
+  double *be = (double *)calloc(nwork ? nwork : 1, sizeof(double));
+  for (int g = 0; g < n; ++g) {
+    for (unsigned m = 0; m <= nmax_out[g]; ++m) {
+      wg[first[g] + m] = g;
+      wm[first[g] + m] = m;
+    }
+  }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (long q = 0; q < (long)nwork; ++q) {
+    const int g = wg[q];
+    const unsigned m = wm[q];
+    const unsigned nmax = nmax_out[g];
+    const unsigned nm = (m == 0) ? nmax : nmax + 1 - m;
+    const unsigned n2m = 2 * nm;
+    cplx *T = (cplx *)malloc(sizeof(cplx) * (size_t)n2m * n2m);
+    tmatrix_m_from_tables(&tabs[g], m, T);
+    double s = 0., e = 0.;
+    for (size_t k = 0; k < (size_t)n2m * n2m; ++k) {
+      s += creal(T[k]) * creal(T[k]) + cimag(T[k]) * cimag(T[k]);
+    }
+    for (unsigned i = 0; i < nm; ++i) {
+      e += creal(T[(size_t)i * n2m + i]) + creal(T[(size_t)i * n2m + nm + i]) +
+           creal(T[(size_t)(nm + i) * n2m + i]) +
+           creal(T[(size_t)(nm + i) * n2m + nm + i]);
+    }
+    const double f = (m == 0) ? 1. : 2.;
+    bs[q] = f * s;
+    be[q] = f * e;
     free(T);
   }
+  for (int g = 0; g < n; ++g) {
+    double s = 0., e = 0.;
+    for (size_t q = first[g]; q < first[g + 1]; ++q) {
+      s += bs[q];
+      e += be[q];
+    }
+    qsca_out[g] = s;
+    qext_out[g] = e;
+    tables_free(&tabs[g]);
+  }
+  free(tabs);
+  free(first);
+  free(wg);
+  free(wm);
+  free(bs);
+  free(be);
+  free(ngauss);
 }
 
 /* ------------------------------------------------------------------------ */
