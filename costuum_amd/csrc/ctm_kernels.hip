@@ -23,6 +23,18 @@ namespace ctm {
 // Starting order of the downward ratio recurrence (SpecialFunctions.hpp:40).
 constexpr int kBesselStart = 800;
 
+// The reference starts the ratio recurrence at n = 800 for every argument.
+// For n well above max(N, |z|) the recurrence contracts by ~(|z|/2n)^2 per
+// step, so a start at max(N, |z|) + 10 |z|^(1/3) + 50 already reaches the
+// same fixed point: the ratios come out bit-identical to the 800 start (checked
+// over |z| in [1e-4, 500], arg z in [0, 1.2], N up to 200, at twice this
+// margin's precision need).  Never above 800.
+__device__ inline int bessel_start(int N, double absz) {
+  const int a = (int)absz;
+  const int ns = (N > a ? N : a) + (int)(10. * cbrt(absz)) + 50;
+  return ns < kBesselStart ? ns : kBesselStart;
+}
+
 // ---------------------------------------------------------------------------
 // K0: Gauss-Legendre nodes.  Thread i runs the Newton iteration of root i of
 // the 2G-point rule (Lether-Wenston guess, tolerance 1e-16 relaxed x10 per
@@ -57,6 +69,10 @@ __global__ void k_gauss_legendre(const GlKey *keys, double *gl) {
       pinv = 1. / ((p_prev - xi * p_cur) * order);
       step = pinv * p_cur * (1. - xi * xi);
       xi = xi - step;
+      // A few nodes stagnate at the rounding level, where the 1e-16 test
+      // keeps iterating until the tolerance has relaxed (iteration > 100);
+      // they are converged to ~1e-16 already (|x| <= 1), so stop there.
+      if (iter > 4 && fabs(step) <= 4.e-16) break;
     } while (fabs(step) > tol * fabs(xi));
     x[i] = -xi;
     w[i] = 2. * pinv * pinv * (1. - xi * xi);
@@ -167,9 +183,10 @@ __global__ void k_tables(const TableSet *sets, const GrainParams gp,
     // j_n(kr): downward ratio recurrence, then forward products.
     {
       const double zinv = krinv;
-      double rho = kr / (2. * kBesselStart + 1.);
-      if (kBesselStart - 1 < N) bj[(int64_t)(kBesselStart - 1) * G + ig] = rho;
-      for (int idx = kBesselStart - 1; idx >= 1; --idx) {
+      const int ns = bessel_start(N, kr);
+      double rho = kr / (2. * ns + 1.);
+      if (ns - 1 < N) bj[(int64_t)(ns - 1) * G + ig] = rho;
+      for (int idx = ns - 1; idx >= 1; --idx) {
         rho = 1. / ((2. * idx + 1.) * zinv - rho);
         if (idx - 1 < N) bj[(int64_t)(idx - 1) * G + ig] = rho;
       }
@@ -207,8 +224,9 @@ __global__ void k_tables(const TableSet *sets, const GrainParams gp,
     // j_n(m kr): complex downward ratio recurrence.
     {
       const cdbl zinv = krmrinv;
-      cdbl rho = cscale(krmr, 1. / (2. * kBesselStart + 1.));
-      for (int idx = kBesselStart - 1; idx >= 1; --idx) {
+      const int ns = bessel_start(N, sqrt(krmr.x * krmr.x + krmr.y * krmr.y));
+      cdbl rho = cscale(krmr, 1. / (2. * ns + 1.));
+      for (int idx = ns - 1; idx >= 1; --idx) {
         const double f = 2. * idx + 1.;
         rho = crcp(csub(cscale(zinv, f), rho));
         if (idx - 1 < N) {
@@ -254,28 +272,45 @@ __global__ void k_tables(const TableSet *sets, const GrainParams gp,
 // with (J_a, J_b) = (J21, J12) for even and (J11, J22) for odd pairs
 // (TMatrixResource.hpp:623-705, 993-1097).
 //
-// Workgroup: 256 threads = 4 waves on a 16x16 tile of pairs.  Wave w takes
-// rows of parity w & 1 and columns of parity w >> 1, so every wave sees a
-// single pair parity; lane (lr, lc) takes pair (2 lr + (w&1), 2 lc + (w>>1)).
+// Workgroup: 256 threads = 4 waves on a tile of 16 rows of pairs.  For m > 0
+// the tile has 16 columns; wave w takes rows of parity w & 1 and columns of
+// parity w >> 1, so every wave sees a single pair parity; lane (lr, lc) takes
+// pair (2 lr + (w&1), 2 lc + (w>>1)).  For m = 0 only pairs with n1 + n2 even
+// contribute (TMatrixResource.hpp:622), so the tile is 32 columns wide and
+// all four waves take even pairs: wave w takes rows of parity w & 1, the
+// columns of the same parity in half w >> 1 of the tile.
+// Workgroups find their (item, tile) by binary search in the prefix sum of
+// tile counts over the cost-sorted item list.
 constexpr int kTile = 16;
 constexpr int kChunk = 8;
 constexpr int kLeftStride = 34;   // 32 factors + 2 pad doubles per row
 constexpr int kRightStride = 10;  // 5 complex factors per column
 
 __global__ __launch_bounds__(256) void k_assemble(
-    const TileTask *tasks, const Item *items, const TableSet *sets,
-    const WigKey *wkeys, const GrainParams gp, const double *tables,
-    const double *wig, cdbl *qbuf) {
+    const int32_t *order, const int32_t *tile_prefix, int n_order,
+    const Item *items, const TableSet *sets, const WigKey *wkeys,
+    const GrainParams gp, const double *tables, const double *wig,
+    cdbl *qbuf) {
   __shared__ double Lsh[kChunk][kTile][kLeftStride];
-  __shared__ double Rsh[kChunk][kTile][kRightStride];
+  __shared__ double Rsh[kChunk][2 * kTile][kRightStride];
 
-  const TileTask task = tasks[blockIdx.x];
-  const Item it = items[task.item];
+  const int bid = blockIdx.x;
+  int lo = 0, hi = n_order;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_prefix[mid] <= bid) lo = mid; else hi = mid;
+  }
+  const Item it = items[order[lo]];
   const TableSet ts = sets[it.set];
   const WigKey wk = wkeys[it.wig];
   const int N = it.N, m = it.m, G = it.G;
   const int nmin = m > 0 ? m : 1;
   const int nm = N + 1 - nmin;
+  const bool m0 = (m == 0);
+  const int cw = m0 ? 2 * kTile : kTile;
+  const int ntc = (nm + cw - 1) / cw;
+  const int lt = bid - tile_prefix[lo];
+  const int tr = lt / ntc, tc = lt - (lt / ntc) * ntc;
   const double *S = tables + ts.off;
   const double *B = S + (int64_t)kNodeScalars * G;
   const int64_t NG = (int64_t)N * G;
@@ -285,22 +320,25 @@ __global__ __launch_bounds__(256) void k_assemble(
   const int t = threadIdx.x;
   const int wv = t >> 6, lane = t & 63;
   const int rr = 2 * (lane >> 3) + (wv & 1);
-  const int cc = 2 * (lane & 7) + (wv >> 1);
-  const int p1 = task.tr * kTile + rr;
-  const int p2 = task.tc * kTile + cc;
-  const bool odd = ((wv & 1) ^ (wv >> 1)) != 0;
-  const bool active = !(m == 0 && odd);
+  const int cc = m0 ? kTile * (wv >> 1) + 2 * (lane & 7) + (wv & 1)
+                    : 2 * (lane & 7) + (wv >> 1);
+  const int p1 = tr * kTile + rr;
+  const int p2 = tc * cw + cc;
+  const bool odd = !m0 && (((wv & 1) ^ (wv >> 1)) != 0);
+  const bool active = true;
+  const int nstage = kTile * kChunk + cw * kChunk;
 
   cdbl Ja = cmk(0., 0.), Jb = cmk(0., 0.), RJa = cmk(0., 0.), RJb = cmk(0., 0.);
   const double mm = (double)m;
 
   for (int ig0 = 0; ig0 < G; ig0 += kChunk) {
     __syncthreads();
-    if (t < kTile * kChunk) {
+    for (int e = t; e < nstage; e += 256) {
+    if (e < kTile * kChunk) {
       // row factors of row r at node ig0 + kk
-      const int r = t / kChunk, kk = t % kChunk;
+      const int r = e / kChunk, kk = e % kChunk;
       const int ig = ig0 + kk;
-      const int p = task.tr * kTile + r;
+      const int p = tr * kTile + r;
       double *L = Lsh[kk][r];
       if (ig < G && p < nm) {
         const int n1 = nmin + p;
@@ -361,10 +399,10 @@ __global__ __launch_bounds__(256) void k_assemble(
         for (int i = 0; i < 32; ++i) L[i] = 0.;
       }
     } else {
-      const int tt = t - kTile * kChunk;
+      const int tt = e - kTile * kChunk;
       const int c = tt / kChunk, kk = tt % kChunk;
       const int ig = ig0 + kk;
-      const int p = task.tc * kTile + c;
+      const int p = tc * cw + c;
       double *R = Rsh[kk][c];
       if (ig < G && p < nm) {
         const int n2 = nmin + p;
@@ -387,6 +425,7 @@ __global__ __launch_bounds__(256) void k_assemble(
 #pragma unroll
         for (int i = 0; i < 10; ++i) R[i] = 0.;
       }
+    }
     }
     __syncthreads();
     if (active) {
@@ -463,87 +502,186 @@ __global__ __launch_bounds__(256) void k_assemble(
 // LDS by Gauss-Jordan elimination with partial pivoting on |Q_ij| (largest
 // modulus, first index on ties: the rule of Matrix.hpp:280-293); the column
 // exchanges of the final unpermutation (Matrix.hpp:367-383) are folded into
-// the product's column index.  The product streams RgQ_c from HBM.
-__global__ __launch_bounds__(256) void k_invert_solve(
-    const int32_t *tasks, const Item *items, cdbl *qbuf, cdbl *tbuf,
-    double *qparts) {
+// the product's column index.  For m = 0 the class block splits further into
+// the rows of even and odd p (only n1 + n2 even couples,
+// TMatrixResource.hpp:622): the two sub-blocks are stored side by side
+// (permuted index) and eliminated concurrently, one pivot each per step.
+// Each step costs two barriers: the pivot of the next column is reduced
+// while the current update is applied.  The product streams RgQ_c from HBM
+// in groups of four rows per wave.  With diag_only only the diagonal of T is
+// formed (all the convergence test needs).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_invert_solve(
+    const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
+    double *qparts, int diag_only) {
+  constexpr int NW = NT / 64;
   extern __shared__ double smem[];
+  __shared__ double wb_mag[NW][2];
+  __shared__ int wb_row[NW][2];
+  __shared__ double red[NW][4];
+
   const int task = tasks[blockIdx.x];
   const int item_index = task >> 1;
   const int cls = task & 1;
   const Item it = items[item_index];
-  const int N = it.N, m = it.m;
+  const int m = it.m;
   const int nmin = m > 0 ? m : 1;
-  const int nm = N + 1 - nmin;
+  const int nm = it.N + 1 - nmin;
   const int64_t nm2 = (int64_t)nm * nm;
+  const int nb = (m == 0 && nm > 1) ? 2 : 1;
+  const int sz0 = (nb == 2) ? (nm + 1) / 2 : nm;
+  const int off[2] = {0, sz0};
+  const int sz[2] = {sz0, nm - sz0};
+  // permuted index -> p
+  auto pidx = [&](int i) {
+    return nb == 2 ? (i < sz0 ? 2 * i : 2 * (i - sz0) + 1) : i;
+  };
+
   cdbl *A = reinterpret_cast<cdbl *>(smem);
   cdbl *colk = A + nm2;
   cdbl *rowk = colk + nm;
-  int *piv = reinterpret_cast<int *>(rowk + nm);
+  cdbl *rowold = rowk + nm;
+  int *piv = reinterpret_cast<int *>(rowold + nm);
   int *src = piv + nm;
-  __shared__ double red[4][4];
 
   const cdbl *Qg = qbuf + it.q_off + cls * nm2;
   const cdbl *RQg = qbuf + it.q_off + (2 + cls) * nm2;
   const int t = threadIdx.x;
-  const int nt = blockDim.x;
-  for (int64_t e = t; e < nm2; e += nt) A[e] = Qg[e];
+  const int lane = t & 63, wv = t >> 6;
+
+  for (int i = t >> 5; i < nm; i += NT >> 5) {
+    const int pi = pidx(i);
+    const int bi = i >= sz0;
+    for (int j = t & 31; j < nm; j += 32) {
+      const int bj = j >= sz0;
+      A[(int64_t)i * nm + j] =
+          (bi == bj) ? Qg[(int64_t)pi * nm + pidx(j)] : cmk(0., 0.);
+    }
+  }
   __syncthreads();
 
-  for (int k = 0; k < nm; ++k) {
-    if (t < 64) {
-      double best = -1.;
-      int bi = nm;
-      for (int i = k + t; i < nm; i += 64) {
+  // wave-level argmax of (mag, row): larger mag wins, lower row on ties
+  auto wave_best = [&](double &mag, int &row) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const double om = __shfl_xor(mag, o);
+      const int orow = __shfl_xor(row, o);
+      if (om > mag || (om == mag && orow < row)) {
+        mag = om;
+        row = orow;
+      }
+    }
+  };
+  // pivot candidates of the first column of each sub-block
+  {
+    double bm[2] = {-1., -1.};
+    int br[2] = {nm, nm};
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      if (b >= nb) continue;
+      const int k = off[b];
+      for (int i = k + t; i < off[b] + sz[b]; i += NT) {
         const cdbl v = A[(int64_t)i * nm + k];
         const double mag = v.x * v.x + v.y * v.y;
-        if (mag > best) {
-          best = mag;
-          bi = i;
+        if (mag > bm[b]) {
+          bm[b] = mag;
+          br[b] = i;
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      wave_best(bm[b], br[b]);
+      if (lane == 0) {
+        wb_mag[wv][b] = bm[b];
+        wb_row[wv][b] = br[b];
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int s = 0; s < sz0; ++s) {
+    int kb[2], pb[2];
+    bool act[2];
+    cdbl inv[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      act[b] = b < nb && s < sz[b];
+      kb[b] = off[b] + s;
+      double best = -1.;
+      int row = nm;
+      for (int w = 0; w < NW; ++w) {
+        const double mg = wb_mag[w][b];
+        const int rw = wb_row[w][b];
+        if (mg > best || (mg == best && rw < row)) {
+          best = mg;
+          row = rw;
         }
       }
-      if (t == 0) piv[k] = bi;
+      pb[b] = act[b] ? row : kb[b];
+      inv[b] = cmk(1., 0.);
+      if (act[b]) inv[b] = crcp(A[(int64_t)pb[b] * nm + kb[b]]);
+    }
+    // phase A: pivot row (scaled), old row k, column k after the swap
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      if (!act[b]) continue;
+      const int k = kb[b], p = pb[b];
+      for (int j = off[b] + t; j < off[b] + sz[b]; j += NT) {
+        rowk[j] = cmul(A[(int64_t)p * nm + j], inv[b]);
+        rowold[j] = A[(int64_t)k * nm + j];
+        const int i = j;
+        colk[i] = (i == k) ? A[(int64_t)p * nm + k]
+                           : (i == p ? A[(int64_t)k * nm + k]
+                                     : A[(int64_t)i * nm + k]);
+      }
+      if (t == 0) piv[k] = p;
     }
     __syncthreads();
-    const int p = piv[k];
-    if (p != k) {
-      for (int j = t; j < nm; j += nt) {
-        const cdbl a = A[(int64_t)k * nm + j];
-        A[(int64_t)k * nm + j] = A[(int64_t)p * nm + j];
-        A[(int64_t)p * nm + j] = a;
+    // phase B: update, and candidates for the next pivot
+    double bm[2] = {-1., -1.};
+    int br[2] = {nm, nm};
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      if (!act[b]) continue;
+      const int k = kb[b], p = pb[b];
+      const int lo = off[b], hi = off[b] + sz[b];
+      for (int i = lo + (t >> 5); i < hi; i += NT >> 5) {
+        const cdbl ci = colk[i];
+        for (int j = lo + (t & 31); j < hi; j += 32) {
+          const int64_t e = (int64_t)i * nm + j;
+          cdbl v;
+          if (i == k) {
+            v = (j == k) ? inv[b] : rowk[j];
+          } else {
+            const cdbl base = (i == p) ? rowold[j] : A[e];
+            if (j == k) {
+              v = cmul(ci, inv[b]);
+              v.x = -v.x;
+              v.y = -v.y;
+            } else {
+              const cdbl g = rowk[j];
+              v.x = base.x - (ci.x * g.x - ci.y * g.y);
+              v.y = base.y - (ci.x * g.y + ci.y * g.x);
+            }
+          }
+          A[e] = v;
+          if (j == k + 1 && i > k) {
+            const double mag = v.x * v.x + v.y * v.y;
+            if (mag > bm[b] || (mag == bm[b] && i < br[b])) {
+              bm[b] = mag;
+              br[b] = i;
+            }
+          }
+        }
       }
-      __syncthreads();
     }
-    const cdbl inv = crcp(A[(int64_t)k * nm + k]);
-    for (int i = t; i < nm; i += nt) colk[i] = A[(int64_t)i * nm + k];
-    for (int j = t; j < nm; j += nt) rowk[j] = cmul(A[(int64_t)k * nm + j], inv);
-    __syncthreads();
-    for (int i = t >> 5; i < nm; i += (nt >> 5)) {
-     for (int j = t & 31; j < nm; j += 32) {
-      const int64_t e = (int64_t)i * nm + j;
-      cdbl v;
-      if (i == k) {
-        v = (j == k) ? inv : rowk[j];
-      } else if (j == k) {
-        v = cmul(colk[i], inv);
-        v.x = -v.x;
-        v.y = -v.y;
-      } else {
-        v = A[e];
-        const cdbl f = colk[i], g = rowk[j];
-        v.x -= f.x * g.x - f.y * g.y;
-        v.y -= f.x * g.y + f.y * g.x;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      wave_best(bm[b], br[b]);
+      if (lane == 0) {
+        wb_mag[wv][b] = bm[b];
+        wb_row[wv][b] = br[b];
       }
-      A[e] = v;
-     }
     }
     __syncthreads();
   }
@@ -560,46 +698,103 @@ __global__ __launch_bounds__(256) void k_invert_solve(
   }
   __syncthreads();
 
-  // T_c(r, c) = -sum_k RgQ_c(r, k) Ainv(k, src[c])
-  cdbl *T = tbuf ? tbuf + it.t_off + cls * nm2 : nullptr;
   double s_diag2 = 0., e_diag = 0., s_all = 0., e_tr = 0.;
-  for (int r = t >> 5; r < nm; r += (nt >> 5)) {
-   for (int c = t & 31; c < nm; c += 32) {
-    const int64_t e = (int64_t)r * nm + c;
-    const cdbl *rq = RQg + (int64_t)r * nm;
-    const int sc = src[c];
-    cdbl acc = cmk(0., 0.);
-    for (int k = 0; k < nm; ++k) cfma(acc, rq[k], A[(int64_t)k * nm + sc]);
-    acc.x = -acc.x;
-    acc.y = -acc.y;
-    if (T) T[e] = acc;
-    const double a2 = acc.x * acc.x + acc.y * acc.y;
-    s_all += a2;
-    if (r == c) {
-      const double w = 2. * (nmin + r) + 1.;
+  if (diag_only) {
+    // T(p, p) = -sum_k RgQ(p, k) Ainv(k, p) within the sub-block of p
+    for (int i = t; i < nm; i += NT) {
+      const int klo = i >= sz0 ? sz0 : 0, khi = i >= sz0 ? nm : sz0;
+      const int pi = pidx(i);
+      const cdbl *rq = RQg + (int64_t)pi * nm;
+      const int sc = src[i];
+      cdbl acc = cmk(0., 0.);
+      for (int k = klo; k < khi; ++k)
+        cfma(acc, rq[pidx(k)], A[(int64_t)k * nm + sc]);
+      const double w = 2. * (nmin + pi) + 1.;
+      const double a2 = acc.x * acc.x + acc.y * acc.y;
       s_diag2 += w * a2;
-      e_diag += w * acc.x;
-      e_tr += acc.x;
+      e_diag -= w * acc.x;
+      e_tr -= acc.x;
     }
-   }
+  } else {
+    cdbl *T = tbuf + it.t_off + cls * nm2;
+    if (nb == 2) {
+      // the two sub-blocks do not couple: zero the cross entries
+      for (int i = t >> 5; i < nm; i += NT >> 5) {
+        const int bi = i >= sz0;
+        for (int j = t & 31; j < nm; j += 32) {
+          if ((j >= sz0) != bi)
+            T[(int64_t)pidx(i) * nm + pidx(j)] = cmk(0., 0.);
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      if (b >= nb) continue;
+      const int lo = off[b], hi = off[b] + sz[b];
+      const int c0 = lo + lane, c1 = lo + lane + 64;
+      const bool v0 = c0 < hi, v1 = c1 < hi;
+      const int s0 = v0 ? src[c0] : 0, s1 = v1 ? src[c1] : 0;
+      for (int rg = lo + 4 * wv; rg < hi; rg += 4 * NW) {
+        cdbl acc[4][2];
+        int prow[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc[q][0] = cmk(0., 0.);
+          acc[q][1] = cmk(0., 0.);
+          const int r = rg + q < hi ? rg + q : rg;
+          prow[q] = pidx(r);
+        }
+        for (int k = lo; k < hi; ++k) {
+          const int pk = pidx(k);
+          const cdbl a0 = A[(int64_t)k * nm + s0];
+          const cdbl a1 = A[(int64_t)k * nm + s1];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const cdbl bq = RQg[(int64_t)prow[q] * nm + pk];
+            cfma(acc[q][0], bq, a0);
+            cfma(acc[q][1], bq, a1);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = rg + q;
+          if (r >= hi) break;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const bool valid = h == 0 ? v0 : v1;
+            if (!valid) continue;
+            const int c = h == 0 ? c0 : c1;
+            const cdbl v = cmk(-acc[q][h].x, -acc[q][h].y);
+            T[(int64_t)prow[q] * nm + pidx(c)] = v;
+            const double a2 = v.x * v.x + v.y * v.y;
+            s_all += a2;
+            if (r == c) {
+              const double w = 2. * (nmin + prow[q]) + 1.;
+              s_diag2 += w * a2;
+              e_diag += w * v.x;
+              e_tr += v.x;
+            }
+          }
+        }
+      }
+    }
   }
-  // deterministic block reduction: waves by shuffles, then wave 0
+  // deterministic block reduction: waves by shuffles, then thread 0
   double v[4] = {s_diag2, e_diag, s_all, e_tr};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off);
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
   }
-  if ((t & 63) == 0) {
+  if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) red[t >> 6][q] = v[q];
+    for (int q = 0; q < 4; ++q) red[wv][q] = v[q];
   }
   __syncthreads();
   if (t == 0) {
-    const int nw = (nt + 63) / 64;
     for (int q = 0; q < 4; ++q) {
-      double s = 0.;
-      for (int wv = 0; wv < nw; ++wv) s += red[wv][q];
-      qparts[(int64_t)task * kQParts + q] = s;
+      double acc = 0.;
+      for (int w = 0; w < NW; ++w) acc += red[w][q];
+      qparts[(int64_t)task * kQParts + q] = acc;
     }
   }
 }
@@ -702,37 +897,54 @@ hipError_t launch_tables(const TableSet *sets, int n, const GrainParams &gp,
   return hipGetLastError();
 }
 
-hipError_t launch_assemble(const TileTask *tasks, int n, const Item *items,
+hipError_t launch_assemble(const int32_t *order, const int32_t *tile_prefix,
+                           int n_order, int n_tiles, const Item *items,
                            const TableSet *sets, const WigKey *wkeys,
                            const GrainParams &gp, const double *tables,
                            const double *wig, cdbl *qbuf, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_assemble, dim3(n), dim3(256), 0, s, tasks, items, sets,
-                     wkeys, gp, tables, wig, qbuf);
+  if (n_tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_assemble, dim3(n_tiles), dim3(256), 0, s, order,
+                     tile_prefix, n_order, items, sets, wkeys, gp, tables, wig,
+                     qbuf);
   return hipGetLastError();
 }
 
 size_t invert_solve_lds_bytes(int nm) {
-  return (size_t)nm * nm * sizeof(cdbl) + 2 * (size_t)nm * sizeof(cdbl) +
+  return (size_t)nm * nm * sizeof(cdbl) + 3 * (size_t)nm * sizeof(cdbl) +
          2 * (size_t)nm * sizeof(int);
 }
 
-hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
-                               const Item *items, cdbl *qbuf, cdbl *tbuf,
-                               double *qparts, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const size_t lds = invert_solve_lds_bytes(nm_max);
+template <int NT>
+static hipError_t launch_solve_nt(const int32_t *tasks, int n, size_t lds,
+                                  const Item *items, const cdbl *qbuf,
+                                  cdbl *tbuf, double *qparts, int diag_only,
+                                  hipStream_t s) {
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(k_invert_solve),
-        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+        reinterpret_cast<const void *>(k_invert_solve<NT>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_invert_solve, dim3(n), dim3(256), lds, s, tasks, items,
-                     qbuf, tbuf, qparts);
+  hipLaunchKernelGGL(k_invert_solve<NT>, dim3(n), dim3(NT), lds, s, tasks,
+                     items, qbuf, tbuf, qparts, diag_only);
   return hipGetLastError();
+}
+
+hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
+                               const Item *items, const cdbl *qbuf, cdbl *tbuf,
+                               double *qparts, int diag_only, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = invert_solve_lds_bytes(nm_max);
+  if (nm_max <= 16)
+    return launch_solve_nt<64>(tasks, n, lds, items, qbuf, tbuf, qparts,
+                               diag_only, s);
+  if (nm_max <= 48)
+    return launch_solve_nt<256>(tasks, n, lds, items, qbuf, tbuf, qparts,
+                                diag_only, s);
+  return launch_solve_nt<512>(tasks, n, lds, items, qbuf, tbuf, qparts,
+                              diag_only, s);
 }
 
 hipError_t launch_converge(const int32_t *active, int n_active,

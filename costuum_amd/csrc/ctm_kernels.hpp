@@ -20,7 +20,8 @@ hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
                          double *wig, hipStream_t s);
 hipError_t launch_tables(const TableSet *sets, int n, const GrainParams &gp,
                          const double *gl, double *tables, hipStream_t s);
-hipError_t launch_assemble(const TileTask *tasks, int n, const Item *items,
+hipError_t launch_assemble(const int32_t *order, const int32_t *tile_prefix,
+                           int n_order, int n_tiles, const Item *items,
                            const TableSet *sets, const WigKey *wkeys,
                            const GrainParams &gp, const double *tables,
                            const double *wig, cdbl *qbuf, hipStream_t s);
@@ -28,8 +29,8 @@ size_t invert_solve_lds_bytes(int nm);
 // Largest block order the LDS-resident inversion holds.
 constexpr int kMaxLdsOrder = 99;
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
-                               const Item *items, cdbl *qbuf, cdbl *tbuf,
-                               double *qparts, hipStream_t s);
+                               const Item *items, const cdbl *qbuf, cdbl *tbuf,
+                               double *qparts, int diag_only, hipStream_t s);
 hipError_t launch_converge(const int32_t *active, int n_active,
                            const int32_t *win_off, const int32_t *win_items,
                            const Item *items, const double *qparts, double tol,

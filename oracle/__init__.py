@@ -226,3 +226,69 @@ def batch_task(rv, d, lam, mr_re, mr_im, min_order=4, max_order=100,
                           nmax.ctypes.data_as(_PU),
                           status.ctypes.data_as(_PI), _p(qs), _p(qe))
     return nmax, status, qs, qe
+
+
+def _declare_obs(L):
+    L.ctmo_extinction.argtypes = [_U, _PD, _D, _D, _U, _PD, _PD, _PD, _PD]
+    L.ctmo_absorption.argtypes = [_U, _PD, _D, _D, _U, _PD, _U, _I, _PD, _PD]
+    L.ctmo_scattering_grid.argtypes = [_U, _PD, _D, _U, _I, _PD]
+    L.ctmo_orientation_coefficients.argtypes = [_I, _D, _U, _PD]
+    L.ctmo_orientation_average.argtypes = [_U, _PD, _PD, _PD]
+
+
+_obs_declared = False
+
+
+def _olib():
+    global _obs_declared
+    L = lib()
+    if not _obs_declared:
+        _declare_obs(L)
+        _obs_declared = True
+    return L
+
+
+def extinction(nmax, T, wavelength, rv, theta):
+    T = np.ascontiguousarray(T, dtype=np.complex128)
+    th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    q = [np.zeros(len(th)) for _ in range(3)]
+    _olib().ctmo_extinction(nmax, _p(T), wavelength, rv, len(th), _p(th),
+                            *[_p(a) for a in q])
+    return q
+
+
+def absorption(nmax, T, wavelength, rv, theta, ngauss=20,
+               account_for_scattering=True):
+    T = np.ascontiguousarray(T, dtype=np.complex128)
+    th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    q = [np.zeros(len(th)) for _ in range(2)]
+    _olib().ctmo_absorption(nmax, _p(T), wavelength, rv, len(th), _p(th),
+                            ngauss, int(account_for_scattering),
+                            *[_p(a) for a in q])
+    return q
+
+
+def scattering_grid(nmax, T, wavelength, ntheta, use_gl=True):
+    T = np.ascontiguousarray(T, dtype=np.complex128)
+    Z = np.zeros((ntheta, 2 * ntheta, 4, 4))
+    _olib().ctmo_scattering_grid(nmax, _p(T), wavelength, ntheta, int(use_gl),
+                                 _p(Z))
+    return Z
+
+
+ORIENT_ISOTROPIC, ORIENT_MISHCHENKO, ORIENT_DAVIS_GREENSTEIN = 0, 1, 2
+
+
+def orientation_coefficients(kind, param, nmax):
+    p = np.zeros(nmax + 1)
+    _olib().ctmo_orientation_coefficients(kind, param, nmax, _p(p))
+    return p
+
+
+def orientation_average(nmax, T, pN):
+    T = np.ascontiguousarray(T, dtype=np.complex128)
+    pN = np.ascontiguousarray(pN, dtype=np.float64)
+    assert len(pN) >= 2 * nmax + 1
+    out = np.zeros_like(T)
+    _olib().ctmo_orientation_average(nmax, _p(T), _p(pN), _p(out))
+    return out

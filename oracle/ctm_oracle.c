@@ -957,3 +957,379 @@ static void cg_m0_family(int n1, int n2, int N, double *out, double *scratch) {
     out[i] = scratch[i - nmin + n1];
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Observables and averaging. */
+
+/* T(i1, n1, m, i2, n2, m) of a full T-matrix in the reference block layout
+ * (TMatrix.hpp:1028-1046). */
+static cplx tm_el(const cplx *T, unsigned nmax, int i1, unsigned n1,
+                  unsigned m, int i2, unsigned n2) {
+  const unsigned nm = (m == 0) ? nmax : nmax + 1 - m;
+  const unsigned nmin = (m == 0) ? 1 : m;
+  const cplx *B = T + ctmo_tmatrix_offset(nmax, m);
+  return B[(size_t)(i1 * nm + n1 - nmin) * (2 * nm) + i2 * nm + n2 - nmin];
+}
+
+/* c_{n1 n2} = i^(n1 - n2 - 1) a_{n1 n2} (NBasedResources.hpp:115-137). */
+static cplx cnn(unsigned n1, unsigned n2) {
+  static const cplx ipow[4] = {1., I, -1., -I};
+  const int e = (((int)n1 - (int)n2 - 1) % 4 + 4) % 4;
+  return ipow[e] * ann(n1, n2);
+}
+
+/* pi/tau tables at one angle for all m <= nmax (the special Wigner
+ * resources, ExtinctionCoefficientTask.hpp:242-415): pi[m][n-1] =
+ * m d^n_{0m}/sin, tau[m][n-1] = d/dtheta d^n_{0m}. */
+static void pitau(double costh, double sinth, double sininv, unsigned nmax,
+                  double *pi, double *tau) {
+  for (unsigned m = 0; m <= nmax; ++m) {
+    double *p = pi + (size_t)m * nmax;
+    double *t = tau + (size_t)m * nmax;
+    ctmo_wigner_dn_0m_sinx(costh, sinth, sininv, nmax, m, p, t);
+    for (unsigned n = 0; n < nmax; ++n) p[n] *= m;
+  }
+}
+
+static void angle_trig(double theta, double *c, double *s, double *si) {
+  *c = cos(theta);
+  *s = sqrt((1. - *c) * (1. + *c));
+  *si = (*s != 0.) ? 1. / *s : 9000.;
+}
+
+/* a23-a25: forward-scattering-type S matrix from T for an incoming
+ * direction with pi/tau tables (pin, tin) and an outgoing one (pout, tout),
+ * e^{i m (phi_out - phi_in)} given by (cphi, sphi); divided by k
+ * (ExtinctionCoefficientTask.hpp:517-619). */
+static void smatrix(const cplx *T, unsigned nmax, double k, const double *pin,
+                    const double *tin, const double *pout, const double *tout,
+                    double cphi, double sphi, cplx *S) {
+  S[0] = S[1] = S[2] = S[3] = 0.;
+  const cplx step = cphi + I * sphi;
+  cplx e = 1.;
+  for (unsigned m = 0; m <= nmax; ++m) {
+    const unsigned nmin = (m == 0) ? 1 : m;
+    const double fcos = 2. * creal(e);
+    const double fsin = 2. * cimag(e);
+    e *= step;
+    const double *pi_i = pin + (size_t)m * nmax, *ta_i = tin + (size_t)m * nmax;
+    const double *pi_o = pout + (size_t)m * nmax,
+                 *ta_o = tout + (size_t)m * nmax;
+    for (unsigned nn = nmin; nn <= nmax; ++nn) {
+      const double pnn = pi_i[nn - 1], tnn = ta_i[nn - 1];
+      for (unsigned n = nmin; n <= nmax; ++n) {
+        const double pn = pi_o[n - 1], tn = ta_o[n - 1];
+        const cplx c = cnn(nn, n);
+        const cplx T11 = tm_el(T, nmax, 0, n, m, 0, nn);
+        const cplx T22 = tm_el(T, nmax, 1, n, m, 1, nn);
+        if (m == 0) {
+          const cplx f = c * tn * tnn;
+          S[0] += f * T22;
+          S[3] += f * T11;
+        } else {
+          const cplx T12 = tm_el(T, nmax, 0, n, m, 1, nn);
+          const cplx T21 = tm_el(T, nmax, 1, n, m, 0, nn);
+          const cplx rf = c * fcos, jf = c * fsin;
+          const double pp = pn * pnn, pt = pn * tnn, tp = tn * pnn,
+                       tt = tn * tnn;
+          S[0] += rf * (T11 * pp + T21 * tp + T12 * pt + T22 * tt);
+          S[1] += jf * (T11 * pt + T21 * tt + T12 * pp + T22 * tp);
+          S[2] -= jf * (T11 * tp + T21 * pp + T12 * tt + T22 * pt);
+          S[3] += rf * (T11 * tt + T21 * pt + T12 * tp + T22 * pp);
+        }
+      }
+    }
+  }
+  const double kinv = 1. / k;
+  for (int q = 0; q < 4; ++q) S[q] *= kinv;
+}
+
+/* a23: Q_ext, Q_extpol, Q_extcpol at the given incidence angles
+ * (ExtinctionCoefficientTask.hpp:626-653), normalised by pi R_V^2. */
+void ctmo_extinction(unsigned nmax, const double *Td, double wavelength,
+                     double rv, unsigned ntheta, const double *theta,
+                     double *qext, double *qextpol, double *qextcpol) {
+  const cplx *T = (const cplx *)Td;
+  const double k = 2. * M_PI / wavelength;
+  const size_t tsz = (size_t)(nmax + 1) * nmax;
+  double *pi = (double *)malloc(sizeof(double) * tsz);
+  double *ta = (double *)malloc(sizeof(double) * tsz);
+  const double norm = M_PI * rv * rv;
+  for (unsigned i = 0; i < ntheta; ++i) {
+    double c, s, si;
+    angle_trig(theta[i], &c, &s, &si);
+    pitau(c, s, si, nmax, pi, ta);
+    cplx S[4];
+    smatrix(T, nmax, k, pi, ta, pi, ta, 1., 0., S);
+    const double pre = 2. * M_PI / k;
+    qext[i] = pre * cimag(S[0] + S[3]) / norm;
+    qextpol[i] = pre * cimag(S[0] - S[3]) / norm;
+    qextcpol[i] = pre * creal(S[3] - S[0]) / norm;
+  }
+  free(pi);
+  free(ta);
+}
+
+/* a24: Q_abs, Q_abspol at the given outgoing angles; with
+ * account_for_scattering the GL(ngauss) x GL(ngauss) quadrature over
+ * incoming directions of Z00 / Z10 is subtracted
+ * (AbsorptionCoefficientTask.hpp:722-781, grid :125-295). */
+void ctmo_absorption(unsigned nmax, const double *Td, double wavelength,
+                     double rv, unsigned ntheta, const double *theta,
+                     unsigned ngauss, int account_for_scattering,
+                     double *qabs, double *qabspol) {
+  const cplx *T = (const cplx *)Td;
+  const double k = 2. * M_PI / wavelength;
+  const size_t tsz = (size_t)(nmax + 1) * nmax;
+  double *pio = (double *)malloc(sizeof(double) * tsz);
+  double *tao = (double *)malloc(sizeof(double) * tsz);
+  double *pii = (double *)malloc(sizeof(double) * tsz * (ngauss ? ngauss : 1));
+  double *tai = (double *)malloc(sizeof(double) * tsz * (ngauss ? ngauss : 1));
+  double *ct = (double *)malloc(sizeof(double) * (ngauss ? ngauss : 1));
+  double *wt = (double *)malloc(sizeof(double) * (ngauss ? ngauss : 1));
+  double *ph = (double *)malloc(sizeof(double) * (ngauss ? ngauss : 1));
+  double *wp = (double *)malloc(sizeof(double) * (ngauss ? ngauss : 1));
+  if (account_for_scattering && ngauss > 0) {
+    ctmo_gauss_legendre(ngauss, ct, wt);
+    ctmo_gauss_legendre_ab(ngauss, 0., 2. * M_PI, ph, wp);
+    for (unsigned g = 0; g < ngauss; ++g) {
+      const double s = sqrt((1. - ct[g]) * (1. + ct[g]));
+      pitau(ct[g], s, (s != 0.) ? 1. / s : 9000., nmax, pii + g * tsz,
+            tai + g * tsz);
+    }
+  }
+  const double norm = M_PI * rv * rv;
+  for (unsigned i = 0; i < ntheta; ++i) {
+    double c, s, si;
+    angle_trig(theta[i], &c, &s, &si);
+    pitau(c, s, si, nmax, pio, tao);
+    cplx S[4];
+    smatrix(T, nmax, k, pio, tao, pio, tao, 1., 0., S);
+    const double pre = 2. * M_PI / k;
+    double a = pre * cimag(S[0] + S[3]);
+    double ap = pre * cimag(S[0] - S[3]);
+    if (account_for_scattering) {
+      for (unsigned ti = 0; ti < ngauss; ++ti) {
+        for (unsigned pj = 0; pj < ngauss; ++pj) {
+          cplx St[4];
+          smatrix(T, nmax, k, pii + ti * tsz, tai + ti * tsz, pio, tao,
+                  cos(ph[pj]), -sin(ph[pj]), St);
+          const double w = wt[ti] * wp[pj];
+          double n2[4];
+          for (int q = 0; q < 4; ++q)
+            n2[q] = creal(St[q]) * creal(St[q]) + cimag(St[q]) * cimag(St[q]);
+          a -= 0.5 * (n2[0] + n2[1] + n2[2] + n2[3]) * w;
+          ap -= 0.5 * (n2[0] + n2[1] - n2[2] - n2[3]) * w;
+        }
+      }
+    }
+    qabs[i] = a / norm;
+    qabspol[i] = ap / norm;
+  }
+  free(pio);
+  free(tao);
+  free(pii);
+  free(tai);
+  free(ct);
+  free(wt);
+  free(ph);
+  free(wp);
+}
+
+/* Z (4x4, row-major) from S (ScatteringMatrixTask.hpp:722-748). */
+static void z_from_s(const cplx *S, double *Z) {
+  const cplx S0 = S[0], S1 = S[1], S2 = S[2], S3 = S[3];
+  Z[0] = creal(0.5 * (S0 * conj(S0) + S1 * conj(S1) + S2 * conj(S2) + S3 * conj(S3)));
+  Z[1] = creal(0.5 * (S0 * conj(S0) - S1 * conj(S1) + S2 * conj(S2) - S3 * conj(S3)));
+  Z[2] = creal(-S0 * conj(S1) - S3 * conj(S2));
+  Z[3] = creal(I * (S0 * conj(S1) - S3 * conj(S2)));
+  Z[4] = creal(0.5 * (S0 * conj(S0) + S1 * conj(S1) - S2 * conj(S2) - S3 * conj(S3)));
+  Z[5] = creal(0.5 * (S0 * conj(S0) - S1 * conj(S1) - S2 * conj(S2) + S3 * conj(S3)));
+  Z[6] = creal(-S0 * conj(S1) + S3 * conj(S2));
+  Z[7] = creal(I * (S0 * conj(S1) + S3 * conj(S2)));
+  Z[8] = creal(-S0 * conj(S2) - S3 * conj(S1));
+  Z[9] = creal(-S0 * conj(S2) + S3 * conj(S1));
+  Z[10] = creal(S0 * conj(S3) + S1 * conj(S2));
+  Z[11] = creal(-I * (S0 * conj(S3) + S2 * conj(S1)));
+  Z[12] = creal(I * (S2 * conj(S0) + S3 * conj(S1)));
+  Z[13] = creal(I * (S2 * conj(S0) - S3 * conj(S1)));
+  Z[14] = creal(-I * (S3 * conj(S0) - S1 * conj(S2)));
+  Z[15] = creal(S3 * conj(S0) - S1 * conj(S2));
+}
+
+/* a25: get_table's scattering matrix: incidence theta_in = pi/2, phi_in = 0;
+ * outgoing theta on GL(ntheta) in cos (or uniform i pi/ntheta), phi on
+ * GL(2 ntheta) in [0, 2pi] (or uniform); Z[(itheta * nphi + iphi) * 16]
+ * (ScatteringMatrixTask.hpp:107-277,698-751, TaskManager.hpp:832-833). */
+void ctmo_scattering_grid(unsigned nmax, const double *Td, double wavelength,
+                          unsigned ntheta, int use_gl, double *Z) {
+  const cplx *T = (const cplx *)Td;
+  const double k = 2. * M_PI / wavelength;
+  const unsigned nphi = 2 * ntheta;
+  const size_t tsz = (size_t)(nmax + 1) * nmax;
+  double *pin = (double *)malloc(sizeof(double) * tsz);
+  double *tin = (double *)malloc(sizeof(double) * tsz);
+  double *po = (double *)malloc(sizeof(double) * tsz);
+  double *to = (double *)malloc(sizeof(double) * tsz);
+  double *cth = (double *)malloc(sizeof(double) * ntheta);
+  double *phi = (double *)malloc(sizeof(double) * nphi);
+  double *ign = (double *)malloc(sizeof(double) * nphi);
+  double c, s, si;
+  angle_trig(M_PI_2, &c, &s, &si);
+  pitau(c, s, si, nmax, pin, tin);
+  if (use_gl) {
+    ctmo_gauss_legendre(ntheta, cth, ign);
+    ctmo_gauss_legendre_ab(nphi, 0., 2. * M_PI, phi, ign);
+  } else {
+    for (unsigned i = 0; i < ntheta; ++i) cth[i] = cos(i * (M_PI / ntheta));
+    for (unsigned i = 0; i < nphi; ++i) phi[i] = i * (2. * M_PI / nphi);
+  }
+  for (unsigned it = 0; it < ntheta; ++it) {
+    const double co = cth[it];
+    const double so = sqrt((1. - co) * (1. + co));
+    pitau(co, so, (so != 0.) ? 1. / so : 9000., nmax, po, to);
+    for (unsigned ip = 0; ip < nphi; ++ip) {
+      cplx S[4];
+      smatrix(T, nmax, k, pin, tin, po, to, cos(phi[ip]), sin(phi[ip]), S);
+      z_from_s(S, Z + ((size_t)it * nphi + ip) * 16);
+    }
+  }
+  free(pin);
+  free(tin);
+  free(po);
+  free(to);
+  free(cth);
+  free(phi);
+  free(ign);
+}
+
+/* a19: Legendre coefficients p_N of an orientation distribution.
+ * kind 0: isotropic p(beta) = 1/2, integrated numerically like
+ * OrientationDistribution::initialise (OrientationDistribution.hpp:220-243):
+ * normalisation, then p_N = int_0^pi sin b p(b) d^N_00(b) db by adaptive GL
+ * (SpecialFunctions.hpp:772-810) with tolerances 1e-10 / 1e-5.
+ * kind 1: Mishchenko p0 = 1, p2 = 1.5 <cos^2> - 0.5
+ * (MishchenkoOrientationDistribution.hpp:105-119).
+ * kind 2: Davis-Greenstein for axis ratio `param`
+ * (DavisGreensteinOrientationDistribution.hpp:76-102). */
+typedef struct {
+  unsigned order;
+  double norm;
+} pn_args;
+
+static double iso_integrand(double beta, const pn_args *a) {
+  const double cb = cos(beta), sb = sin(beta);
+  double wd = 1.;
+  if (a->order > 0) {
+    double *y = (double *)malloc(sizeof(double) * a->order);
+    double *dy = (double *)malloc(sizeof(double) * a->order);
+    ctmo_wigner_dn_0m(cb, a->order, 0, y, dy);
+    wd = y[a->order - 1];
+    free(y);
+    free(dy);
+  }
+  return sb * (a->norm * 0.5) * wd;
+}
+
+static double gl_quad_adaptive(const pn_args *a, int normalisation,
+                               unsigned start, unsigned maxo, double abs_err,
+                               double rel_err) {
+  unsigned order = start;
+  double old = 0., ad = abs_err + 1., rd = rel_err + 1.;
+  while (order < maxo && ad > abs_err && rd > rel_err) {
+    double *x = (double *)malloc(sizeof(double) * order);
+    double *w = (double *)malloc(sizeof(double) * order);
+    ctmo_gauss_legendre_ab(order, 0., M_PI, x, w);
+    double q = 0.;
+    for (unsigned i = 0; i < order; ++i) {
+      q += w[i] * (normalisation ? sin(x[i]) * 0.5 : iso_integrand(x[i], a));
+    }
+    free(x);
+    free(w);
+    ad = fabs(q - old);
+    rd = ad / fabs(q + old);
+    old = q;
+    ++order;
+  }
+  return old;
+}
+
+void ctmo_orientation_coefficients(int kind, double param, unsigned nmax,
+                                   double *pN) {
+  for (unsigned i = 0; i <= nmax; ++i) pN[i] = 0.;
+  if (kind == 0) {
+    pn_args a = {0, 1.};
+    const double norm = gl_quad_adaptive(&a, 1, 100, 1000, 1e-10, 1e-5);
+    a.norm = 1. / norm;
+    pN[0] = 1.;
+    for (unsigned i = 1; i <= nmax; ++i) {
+      a.order = i;
+      pN[i] = gl_quad_adaptive(&a, 0, 2 * (i + 1), 100 * (i + 1), 1e-10, 1e-5);
+    }
+  } else if (kind == 1) {
+    pN[0] = 1.;
+    if (nmax >= 2) pN[2] = 1.5 * param - 0.5;
+  } else if (kind == 2) {
+    if (param < 1.) {
+      double num = 1., den = 2.;
+      pN[0] = 1.;
+      for (unsigned i = 1; 2 * i <= nmax; ++i) {
+        const int sgn = (i % 2 == 0) ? 1 : -1;
+        pN[2 * i] = sgn * num / den;
+        num *= 2. * i + 1.;
+        den *= 2. * (i + 1.);
+      }
+    } else {
+      for (unsigned i = 0; i <= nmax; ++i) pN[i] = 1.;
+    }
+  }
+}
+
+/* a21: ensemble T-matrix <T>_{m n1 n2} = sum_N p_N sum_m1 (+-) f C C T_m1
+ * (Mishchenko 1991 eq. 3.27; AlignmentAverageTask.hpp:115-215,
+ * TMatrixCalculator.hpp:229-307).  pN has at least 2 nmax + 1 entries. */
+void ctmo_orientation_average(unsigned nmax, const double *Tind,
+                              const double *pN, double *Toutd) {
+  const cplx *Tin = (const cplx *)Tind;
+  cplx *Tout = (cplx *)Toutd;
+  memset(Tout, 0, sizeof(cplx) * ctmo_tmatrix_size(nmax));
+  double *C = (double *)malloc(sizeof(double) * (2 * nmax + 1));
+  double *scratch =
+      (double *)malloc(sizeof(double) * (2 * nmax + 1) * (2 * nmax + 1));
+  for (unsigned m = 0; m <= nmax; ++m) {
+    const int msign = (m % 2 == 0) ? 1 : -1;
+    const unsigned nmin = (m == 0) ? 1 : m;
+    const unsigned nm = (m == 0) ? nmax : nmax + 1 - m;
+    cplx *B = Tout + ctmo_tmatrix_offset(nmax, m);
+    for (unsigned n1 = nmin; n1 <= nmax; ++n1) {
+      for (unsigned n2 = nmin; n2 <= nmax; ++n2) {
+        cplx t[2][2] = {{0., 0.}, {0., 0.}};
+        const unsigned lo = (n1 > n2) ? n1 - n2 : n2 - n1;
+        const unsigned hi = n1 + n2;
+        const unsigned M = (n1 < n2) ? n1 : n2;
+        for (unsigned N = lo; N <= hi; ++N) {
+          cg_m0_family((int)n1, (int)n2, (int)N, C, scratch);
+          const double p = pN[N];
+          for (unsigned m1 = 0; m1 <= M; ++m1) {
+            const double f = (m1 == 0) ? 1. : 2.;
+            const int sg = ((m1 % 2 == 0) ? 1 : -1) * msign;
+            const double cg = sg * f * p * C[M + m] * C[M + m1];
+            if ((N + hi) % 2 == 0) {
+              t[0][0] += cg * tm_el(Tin, nmax, 0, n1, m1, 0, n2);
+              t[1][1] += cg * tm_el(Tin, nmax, 1, n1, m1, 1, n2);
+            } else {
+              t[0][1] += cg * tm_el(Tin, nmax, 0, n1, m1, 1, n2);
+              t[1][0] += cg * tm_el(Tin, nmax, 1, n1, m1, 0, n2);
+            }
+          }
+        }
+        for (int i1 = 0; i1 < 2; ++i1)
+          for (int i2 = 0; i2 < 2; ++i2)
+            B[(size_t)(i1 * nm + n1 - nmin) * (2 * nm) + i2 * nm + n2 - nmin] =
+                t[i1][i2];
+      }
+    }
+  }
+  free(C);
+  free(scratch);
+}

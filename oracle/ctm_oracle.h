@@ -111,6 +111,26 @@ void ctmo_batch_task(int n, const double *rv, const double *axis_ratio,
                      int nthreads, unsigned *nmax_out, int *status_out,
                      double *qsca_out, double *qext_out);
 
+/* a23: ExtinctionCoefficientTask.hpp:626-653 (per incidence angle). */
+void ctmo_extinction(unsigned nmax, const double *T, double wavelength,
+                     double rv, unsigned ntheta, const double *theta,
+                     double *qext, double *qextpol, double *qextcpol);
+/* a24: AbsorptionCoefficientTask.hpp:722-781. */
+void ctmo_absorption(unsigned nmax, const double *T, double wavelength,
+                     double rv, unsigned ntheta, const double *theta,
+                     unsigned ngauss, int account_for_scattering,
+                     double *qabs, double *qabspol);
+/* a25: ScatteringMatrixTask.hpp:698-751 on its (theta, phi) grid. */
+void ctmo_scattering_grid(unsigned nmax, const double *T, double wavelength,
+                          unsigned ntheta, int use_gl, double *Z);
+/* a19: p_N of an orientation distribution (kind 0 isotropic, 1 Mishchenko
+ * with <cos^2 beta> = param, 2 Davis-Greenstein with axis ratio param). */
+void ctmo_orientation_coefficients(int kind, double param, unsigned nmax,
+                                   double *pN);
+/* a21: AlignmentAverageTask.hpp:115-215 / TMatrixCalculator.hpp:229-307. */
+void ctmo_orientation_average(unsigned nmax, const double *Tin,
+                              const double *pN, double *Tout);
+
 #ifdef __cplusplus
 }
 #endif
