@@ -21,10 +21,22 @@ EXPORTS = (
     "ctm_synchronize", "ctm_tmatrix_batch", "ctm_tmatrix_fixed",
     "ctm_tmatrix_size", "ctm_tmatrix_get", "ctm_batch_orders",
     "ctm_profile_enable", "ctm_profile_reset", "ctm_profile_read",
+    "ctm_observables",
 )
 
 KERNELS = ("gauss_legendre", "wigner", "tables", "assemble", "solve",
-           "converge", "sums")
+           "converge", "sums", "obs_x", "obs_sum", "average")
+
+
+class CtmObservableSpec(ctypes.Structure):
+    _fields_ = [("n_theta", ctypes.c_int32),
+                ("theta", ctypes.c_void_p),
+                ("do_extinction", ctypes.c_int32),
+                ("do_absorption", ctypes.c_int32),
+                ("account_for_scattering", ctypes.c_int32),
+                ("ngauss", ctypes.c_int32),
+                ("n_scatter_theta", ctypes.c_int32),
+                ("scatter_gl", ctypes.c_int32)]
 
 
 class CtmKernelStats(ctypes.Structure):
@@ -86,5 +98,8 @@ def lib():
     L.ctm_profile_read.argtypes = [vp, ctypes.c_int,
                                    ctypes.POINTER(CtmKernelStats)]
     L.ctm_profile_read.restype = ctypes.c_int
+    L.ctm_observables.argtypes = [vp, ctypes.POINTER(CtmObservableSpec),
+                                  dp, dp, dp, dp, dp, dp]
+    L.ctm_observables.restype = ctypes.c_int
     _lib = L
     return L

@@ -6,8 +6,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcostuum_amd.so")
-SOURCES = ["csrc/ctm_kernels.hip", "csrc/ctm_api.hip"]
+SOURCES = ["csrc/ctm_kernels.hip", "csrc/ctm_observables.hip",
+           "csrc/ctm_api.hip"]
 HEADERS = ["csrc/ctm_device.hpp", "csrc/ctm_kernels.hpp",
+           "csrc/ctm_observables.hpp",
            os.path.join("..", "include", "costuum_amd.h")]
 
 HIPCC_FLAGS = [

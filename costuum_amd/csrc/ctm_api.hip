@@ -29,6 +29,7 @@
 #include "../../include/costuum_amd.h"
 #include "ctm_device.hpp"
 #include "ctm_kernels.hpp"
+#include "ctm_observables.hpp"
 
 using namespace ctm;
 
@@ -234,7 +235,8 @@ struct ctm_context {
   WorkList work;
   DevBuf d_gl, d_glkeys, d_wig, d_wigkeys, d_tables, d_sets, d_items,
       d_tiles, d_solve, d_q, d_t, d_qparts, d_state, d_active, d_winoff,
-      d_winitems, d_gioff, d_gitems, d_gioff_tmp;
+      d_winitems, d_gioff, d_gitems, d_gioff_tmp, d_params, d_ang, d_pt,
+      d_x, d_spec;
   // Results of the last batch (host).
   int n_grains = 0;
   std::vector<uint32_t> nmax, ngauss;
@@ -242,6 +244,7 @@ struct ctm_context {
   std::vector<Item> final_items;          // phase-2 items
   std::vector<int32_t> grain_items_off;   // per grain into final_items
   std::vector<int32_t> grain_items;
+  GrainParams gp_saved{};  // context copies of the last batch's inputs
   Profiler prof;
 };
 
@@ -498,7 +501,8 @@ int ctm_destroy(ctm_context *ctx) {
                     &ctx->d_q,      &ctx->d_t,       &ctx->d_qparts,
                     &ctx->d_state,  &ctx->d_active,  &ctx->d_winoff,
                     &ctx->d_winitems, &ctx->d_gioff, &ctx->d_gitems,
-                    &ctx->d_gioff_tmp};
+                    &ctx->d_gioff_tmp, &ctx->d_params, &ctx->d_ang,
+                    &ctx->d_pt, &ctx->d_x, &ctx->d_spec};
   for (DevBuf *b : bufs) b->release();
   ctx->prof.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -548,6 +552,18 @@ static int run_all_m(ctm_context *ctx, const GrainParams &gp, int n,
     }
   }
   ctx->grain_items_off[n] = (int)w.items.size();
+  // keep the inputs for the observable calls on this batch
+  CTM_HIP(ctx->d_params.ensure(5 * (size_t)n * sizeof(double)));
+  {
+    double *P = ctx->d_params.as<double>();
+    const double *src[5] = {gp.radius, gp.axis_ratio, gp.wavelength, gp.mr_re,
+                            gp.mr_im};
+    for (int a = 0; a < 5; ++a)
+      CTM_HIP(hipMemcpyAsync(P + (size_t)a * n, src[a], n * sizeof(double),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->gp_saved = GrainParams{P, P + n, P + 2 * (size_t)n, P + 3 * (size_t)n,
+                                P + 4 * (size_t)n};
+  }
   int rc = run_work(ctx, gp, true);
   if (rc != CTM_OK) return rc;
   ctx->grain_items.resize(w.items.size());
@@ -769,6 +785,215 @@ int ctm_tmatrix_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
     }
     out_off += (size_t)n2m * n2m;
   }
+  return CTM_OK;
+}
+
+// Gauss-Legendre rule on [-1, 1] (SpecialFunctions.hpp:647-715), host side,
+// for the observable direction grids.
+static void gauss_legendre_host(unsigned order, double *x, double *w) {
+  const unsigned odd = order % 2u, half = order / 2u + odd;
+  const double inv = 1.0 / order, inv2 = inv * inv;
+  for (unsigned i = 0; i < half; ++i) {
+    const double t = M_PI * (i + 0.75) / (order + 0.5);
+    double xi = (1. - 0.125 * inv2 + (5. / 38.) * inv * inv2 -
+                 (2. / 25.) * inv2 * inv2 * (1. - (14. / 39.) / (t * t))) *
+                std::cos(t);
+    double tol = 1.e-16, pinv = 0., step = 0.;
+    unsigned iter = 0;
+    do {
+      ++iter;
+      if (iter > 100) tol *= 10.;
+      double p_prev = 1., p_cur = xi;
+      for (unsigned j = 1; j < order; ++j) {
+        const double p_old = p_prev;
+        p_prev = p_cur;
+        p_cur = xi * p_prev + (xi * p_prev - p_old) * j / (j + 1.);
+      }
+      pinv = 1. / ((p_prev - xi * p_cur) * order);
+      step = pinv * p_cur * (1. - xi * xi);
+      xi = xi - step;
+    } while (std::fabs(step) > tol * std::fabs(xi));
+    const unsigned k = order - i - 1;
+    x[k] = xi;
+    w[k] = 2. * pinv * pinv * (1. - xi * xi);
+    if (i != half - 1 || odd != 1) {
+      x[i] = -x[k];
+      w[i] = w[k];
+    }
+  }
+}
+
+static void push_angle(std::vector<double> &ang, double c) {
+  const double sn = std::sqrt((1. - c) * (1. + c));
+  ang.push_back(c);
+  ang.push_back(sn);
+  ang.push_back(sn != 0. ? 1. / sn : 9000.);
+}
+
+int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
+                    double *qext, double *qextpol, double *qextcpol,
+                    double *qabs, double *qabspol, double *Z) {
+  if (!ctx || !spec) return CTM_ERR_ARGUMENT;
+  if (ctx->final_items.empty()) {
+    ctx->err = "no T-matrices in the context";
+    return CTM_ERR_STATE;
+  }
+  const int nth = spec->n_theta;
+  const int ng = spec->account_for_scattering ? spec->ngauss : 0;
+  const int nst = spec->n_scatter_theta;
+  if (nth < 0 || ng < 0 || nst < 0 || (nth > 0 && !spec->theta) ||
+      (spec->account_for_scattering && spec->ngauss < 1)) {
+    ctx->err = "invalid observable specification";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int n = ctx->n_grains;
+  // direction table: outgoing theta list, GL incoming, scattering grid,
+  // and the scattering incidence theta = pi/2
+  std::vector<double> ang;
+  for (int j = 0; j < nth; ++j) push_angle(ang, std::cos(spec->theta[j]));
+  std::vector<double> gx(std::max(ng, 1)), gw(std::max(ng, 1));
+  std::vector<double> px(std::max(ng, 1)), pw(std::max(ng, 1));
+  std::vector<double> pc(std::max(ng, 1)), ps(std::max(ng, 1));
+  if (ng > 0) {
+    gauss_legendre_host(ng, gx.data(), gw.data());
+    gauss_legendre_host(ng, px.data(), pw.data());
+    for (int i = 0; i < ng; ++i) {  // rule on [0, 2 pi]
+      px[i] = M_PI * px[i] + M_PI;
+      pw[i] *= M_PI;
+      pc[i] = std::cos(px[i]);
+      ps[i] = std::sin(px[i]);
+    }
+    for (int i = 0; i < ng; ++i) push_angle(ang, gx[i]);
+  }
+  const int nsp = 2 * nst;
+  std::vector<double> sc(std::max(nsp, 1)), ss(std::max(nsp, 1));
+  if (nst > 0) {
+    std::vector<double> tx(nst), tw(nst), fx(nsp), fw(nsp);
+    if (spec->scatter_gl) {
+      gauss_legendre_host(nst, tx.data(), tw.data());
+      gauss_legendre_host(nsp, fx.data(), fw.data());
+      for (int i = 0; i < nsp; ++i) fx[i] = M_PI * fx[i] + M_PI;
+    } else {
+      for (int i = 0; i < nst; ++i) tx[i] = std::cos(i * (M_PI / nst));
+      for (int i = 0; i < nsp; ++i) fx[i] = i * (2. * M_PI / nsp);
+    }
+    for (int i = 0; i < nst; ++i) push_angle(ang, tx[i]);
+    for (int i = 0; i < nsp; ++i) {
+      sc[i] = std::cos(fx[i]);
+      ss[i] = std::sin(fx[i]);
+    }
+    push_angle(ang, std::cos(M_PI_2));
+  }
+  const int n_ang = (int)ang.size() / 3;
+  if (n_ang == 0) return CTM_OK;
+  // incoming / outgoing lists and pairs
+  std::vector<int32_t> in_l, out_l, p_in, p_out;
+  for (int j = 0; j < nth; ++j) {
+    in_l.push_back(j);
+    out_l.push_back(j);
+  }
+  for (int i = 0; i < ng; ++i) in_l.push_back(nth + i);
+  if (nst > 0) {
+    in_l.push_back(nth + ng + nst);
+    for (int i = 0; i < nst; ++i) out_l.push_back(nth + ng + i);
+  }
+  ObsSpec sp{};
+  sp.fwd_pair0 = 0;
+  for (int j = 0; j < nth; ++j) {
+    p_out.push_back(j);
+    p_in.push_back(j);
+  }
+  sp.abs_pair0 = (int)p_in.size();
+  for (int j = 0; j < nth; ++j)
+    for (int i = 0; i < ng; ++i) {
+      p_out.push_back(j);
+      p_in.push_back(nth + i);
+    }
+  sp.sca_pair0 = (int)p_in.size();
+  for (int i = 0; i < nst; ++i) {
+    p_out.push_back(nth + i);
+    p_in.push_back((int)in_l.size() - 1);
+  }
+  sp.n_ang = n_ang;
+  sp.n_in = (int)in_l.size();
+  sp.n_out = (int)out_l.size();
+  sp.n_pairs = (int)p_in.size();
+  sp.n_theta = nth;
+  sp.ngauss = ng;
+  sp.n_stheta = nst;
+  sp.n_sphi = nsp;
+  // device copies of the spec arrays (one buffer)
+  std::vector<char> blob;
+  auto put = [&](const void *src, size_t bytes) {
+    size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + bytes);
+    if (bytes) std::memcpy(blob.data() + off, src, bytes);
+    return off;
+  };
+  const size_t o_ang = put(ang.data(), ang.size() * sizeof(double));
+  const size_t o_in = put(in_l.data(), in_l.size() * 4);
+  const size_t o_out = put(out_l.data(), out_l.size() * 4);
+  const size_t o_pin = put(p_in.data(), p_in.size() * 4);
+  const size_t o_pout = put(p_out.data(), p_out.size() * 4);
+  const size_t o_tw = put(gw.data(), gw.size() * 8);
+  const size_t o_pc = put(pc.data(), pc.size() * 8);
+  const size_t o_ps = put(ps.data(), ps.size() * 8);
+  const size_t o_pw = put(pw.data(), pw.size() * 8);
+  const size_t o_sc = put(sc.data(), sc.size() * 8);
+  const size_t o_ss = put(ss.data(), ss.size() * 8);
+  CTM_HIP(ctx->d_spec.ensure(blob.size()));
+  CTM_HIP(hipMemcpyAsync(ctx->d_spec.p, blob.data(), blob.size(),
+                         hipMemcpyHostToDevice, s));
+  char *base = ctx->d_spec.as<char>();
+  const double *d_ang = reinterpret_cast<const double *>(base + o_ang);
+  sp.in_angle = reinterpret_cast<const int32_t *>(base + o_in);
+  sp.out_angle = reinterpret_cast<const int32_t *>(base + o_out);
+  sp.pair_in = reinterpret_cast<const int32_t *>(base + o_pin);
+  sp.pair_out = reinterpret_cast<const int32_t *>(base + o_pout);
+  sp.theta_w = reinterpret_cast<const double *>(base + o_tw);
+  sp.phi_cos = reinterpret_cast<const double *>(base + o_pc);
+  sp.phi_sin = reinterpret_cast<const double *>(base + o_ps);
+  sp.phi_w = reinterpret_cast<const double *>(base + o_pw);
+  sp.sphi_cos = reinterpret_cast<const double *>(base + o_sc);
+  sp.sphi_sin = reinterpret_cast<const double *>(base + o_ss);
+
+  int nmax_all = 1, nm_max = 1;
+  for (int g = 0; g < n; ++g) nmax_all = std::max(nmax_all, (int)ctx->nmax[g]);
+  nm_max = nmax_all;
+  const size_t nitems = ctx->final_items.size();
+  CTM_HIP(ctx->d_pt.ensure((size_t)(nmax_all + 1) * n_ang * nmax_all * 2 *
+                           sizeof(double)));
+  CTM_HIP(ctx->d_x.ensure(nitems * (size_t)sp.n_pairs * 4 * sizeof(cdbl)));
+  CTM_LAUNCH(CTM_KERNEL_OBS_X, 0., 0.,
+             launch_pitau(d_ang, n_ang, nmax_all, ctx->d_pt.as<double>(), s));
+  double fx = 0.;
+  for (const Item &it : ctx->final_items) {
+    const double nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+    fx += 8. * (4. * nm * nm * sp.n_in + 8. * nm * sp.n_pairs);
+  }
+  CTM_LAUNCH(CTM_KERNEL_OBS_X, fx, 0.,
+             launch_obs_x(sp, (int)nitems, nm_max, ctx->d_items.as<Item>(),
+                          ctx->d_t.as<cdbl>(), ctx->d_pt.as<double>(),
+                          nmax_all, ctx->d_x.as<cdbl>(), s));
+  if (nth > 0 && (qext || qextpol || qextcpol || qabs || qabspol)) {
+    CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
+               launch_obs_q(sp, n, ctx->d_gioff.as<int32_t>(), ctx->gp_saved,
+                            ctx->d_x.as<cdbl>(),
+                            spec->do_extinction ? qext : nullptr,
+                            spec->do_extinction ? qextpol : nullptr,
+                            spec->do_extinction ? qextcpol : nullptr,
+                            spec->do_absorption ? qabs : nullptr,
+                            spec->do_absorption ? qabspol : nullptr, s));
+  }
+  if (nst > 0 && Z) {
+    CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
+               launch_obs_z(sp, n, ctx->d_gioff.as<int32_t>(), ctx->gp_saved,
+                            ctx->d_x.as<cdbl>(), Z, s));
+  }
+  // the spec blob and the host vectors must outlive the queued copies
+  CTM_HIP(hipStreamSynchronize(s));
   return CTM_OK;
 }
 

@@ -1,0 +1,388 @@
+// ctm_observables.hip -- gfx950 kernels for the observables of a batch of
+// T-matrices: forward-scattering extinction, absorption (optionally minus the
+// scattered light, by the reference's GL x GL quadrature over incoming
+// directions) and the scattering matrix on get_table's (theta, phi) grid.
+//
+// Every observable is built from the S matrix of an (outgoing, incoming)
+// direction pair (ExtinctionCoefficientTask.hpp:517-619 and its copies in
+// AbsorptionCoefficientTask.hpp:612-715 and ScatteringMatrixTask.hpp:588-691):
+//   S_k(o, i, dphi) = (1/k) sum_m g_m^k(dphi) X_m^k(o, i)
+// where, with pi/tau of the outgoing (a) and incoming (b) direction and
+// M = c_{nn' n} T_m,
+//   X^0 = a^T M b,  X^3 = a'^T M b',  X^1 = a^T M b',  X^2 = -a'^T M b,
+//   a = (pi, tau), a' = (tau, pi) over the two halves of the m block,
+// g^0 = g^3 = 1 (m = 0) or 2 cos(m dphi), g^1 = g^2 = 0 (m = 0) or
+// 2 sin(m dphi).  The reference evaluates S pair by pair in O(N^3); here the
+// m-block products are shared by all pairs of a grain: K6 forms X per
+// (grain, m) work item, K7 sums over m per (grain, direction).
+#include "ctm_device.hpp"
+#include "ctm_kernels.hpp"
+#include "ctm_observables.hpp"
+
+namespace ctm {
+
+// pi/tau of d^n_{0m} at one direction (SpecialFunctions.hpp:398-495):
+// pi = m d/sin, tau = d(d)/dtheta, for n = 1..nmax.
+__device__ void pitau_dev(double cosx, double sinx, double sinx_inv, int nmax,
+                          int m, double *pi, double *tau, int stride) {
+  if (fabs(1. - fabs(cosx)) <= 1.e-10) {
+    if (m == 1) {
+      int sign = 1;
+      for (int n = 1; n <= nmax; ++n) {
+        double dn = 0.5 * sqrt(n * (n + 1.));
+        if (cosx < 0.) dn *= sign;
+        pi[(n - 1) * stride] = m * dn;
+        if (cosx < 0.) dn = -dn;
+        tau[(n - 1) * stride] = dn;
+        sign = -sign;
+      }
+    } else {
+      for (int n = 0; n < nmax; ++n) {
+        pi[n * stride] = 0.;
+        tau[n * stride] = 0.;
+      }
+    }
+    return;
+  }
+  if (m == 0) {
+    double dprev = 1., dcur = cosx;
+    for (int n = 0; n < nmax; ++n) {
+      const double np1 = n + 1., np2 = n + 2.;
+      const double tnp3 = 2. * np1 + 1.;
+      const double dnext = (tnp3 * cosx * dcur - np1 * dprev) / np2;
+      const double deriv = sinx_inv * np1 * np2 * (dnext - dprev) / tnp3;
+      pi[n * stride] = 0. * (dcur * sinx_inv);
+      tau[n * stride] = deriv;
+      dprev = dcur;
+      dcur = dnext;
+    }
+  } else {
+    const double m2 = (double)m * (double)m;
+    double dprev = 0., dcur = 1.;
+    for (int i = 0; i < m; ++i) {
+      const double i2 = 2. * (i + 1.);
+      dcur *= sqrt((i2 - 1.) / i2) * sinx;
+    }
+    for (int n = 0; n + 1 < m && n < nmax; ++n) {
+      pi[n * stride] = 0.;
+      tau[n * stride] = 0.;
+    }
+    for (int n = m - 1; n < nmax; ++n) {
+      const double np1 = n + 1., np2 = n + 2.;
+      const double tnp3 = 2. * np1 + 1.;
+      const double s1 = sqrt(np1 * np1 - m2);
+      const double s2 = sqrt(np2 * np2 - m2);
+      const double dnext = (tnp3 * cosx * dcur - s1 * dprev) / s2;
+      const double deriv =
+          sinx_inv * (np1 * s2 * dnext - np2 * s1 * dprev) / tnp3;
+      pi[n * stride] = m * (dcur * sinx_inv);
+      tau[n * stride] = deriv;
+      dprev = dcur;
+      dcur = dnext;
+    }
+  }
+}
+
+// Grain-independent pi/tau table PT[m][angle][n][2] for m, n <= nmax_all.
+__global__ void k_pitau(const double *ang, int n_ang, int nmax_all,
+                        double *PT) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (nmax_all + 1) * n_ang) return;
+  const int m = idx / n_ang, a = idx - m * n_ang;
+  double *base = PT + ((int64_t)(m * n_ang + a) * nmax_all) * 2;
+  pitau_dev(ang[3 * a], ang[3 * a + 1], ang[3 * a + 2], nmax_all, m, base,
+            base + 1, 2);
+}
+
+// c_{nn n} = i^(nn - n - 1) a_nn a_n (NBasedResources.hpp:115-137).
+__device__ inline cdbl cfac(int nn, int n) {
+  const double a = sqrt((2. * nn + 1.) / (nn * (nn + 1.))) *
+                   sqrt((2. * n + 1.) / (n * (n + 1.)));
+  const int e = (((nn - n - 1) % 4) + 4) % 4;
+  return e == 0 ? cmk(a, 0.) : e == 1 ? cmk(0., a) : e == 2 ? cmk(-a, 0.)
+                                                           : cmk(0., -a);
+}
+
+// K6: X_m^k for every direction pair of one (grain, m) item.  The incoming
+// directions are processed in chunks of kInChunk: Y(h n; i) = sum_nn
+// c_{nn n} T(h n; h' nn) b(h' nn; i) for b and b' (h' fixed by the parity
+// class), then the pair contractions with the outgoing pi/tau.
+constexpr int kInChunk = 8;
+
+__global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
+                                               const Item *items,
+                                               const cdbl *tbuf,
+                                               const double *PT,
+                                               int nmax_all, cdbl *X) {
+  extern __shared__ double smem[];
+  const Item it = items[blockIdx.x];
+  const int m = it.m;
+  const int nmin = m > 0 ? m : 1;
+  const int nm = it.N + 1 - nmin;
+  const int nm2 = nm * nm;
+  const cdbl *T = tbuf + it.t_off;
+  cdbl *Ysh = reinterpret_cast<cdbl *>(smem);  // [kInChunk][2][2 nm]
+  const int t = threadIdx.x;
+  const int n_ang = spec.n_ang;
+  auto ptv = [&](int a, int n, int which) -> double {
+    return PT[(((int64_t)m * n_ang + a) * nmax_all + (n - 1)) * 2 + which];
+  };
+  cdbl *Xo = X + (int64_t)blockIdx.x * spec.n_pairs * 4;
+
+  for (int c0 = 0; c0 < spec.n_in; c0 += kInChunk) {
+    const int nc = min(kInChunk, spec.n_in - c0);
+    __syncthreads();
+    // Y for the chunk: outputs (ic, bb, row), row = h * nm + p
+    const int nout = nc * 2 * 2 * nm;
+    for (int o = t; o < nout; o += 256) {
+      const int row = o % (2 * nm);
+      const int bb = (o / (2 * nm)) & 1;
+      const int ic = o / (4 * nm);
+      const int a_in = spec.in_angle[c0 + ic];
+      const int h = row / nm, p = row - (row / nm) * nm;
+      const int n = nmin + p;
+      const int cls = (h + n) & 1;
+      const cdbl *Trow = T + (int64_t)cls * nm2 + (int64_t)p * nm;
+      cdbl acc = cmk(0., 0.);
+      for (int q = 0; q < nm; ++q) {
+        const int nn = nmin + q;
+        const int h2 = (cls + nn) & 1;
+        // b = (pi, tau) over h2; b' = (tau, pi)
+        const int which = (h2 ^ bb) & 1;  // 0 -> pi, 1 -> tau
+        const double bv = ptv(a_in, nn, which);
+        if (bv == 0.) continue;
+        const cdbl cm = cmul(cfac(nn, n), Trow[q]);
+        acc.x = fma(cm.x, bv, acc.x);
+        acc.y = fma(cm.y, bv, acc.y);
+      }
+      Ysh[(ic * 2 + bb) * 2 * nm + row] = acc;
+    }
+    __syncthreads();
+    // pair contractions for pairs whose incoming direction is in the chunk
+    for (int e = t; e < spec.n_pairs * 4; e += 256) {
+      const int pr = e >> 2, k = e & 3;
+      const int ii = spec.pair_in[pr] - c0;
+      if (ii < 0 || ii >= nc) continue;
+      const int a_out = spec.out_angle[spec.pair_out[pr]];
+      // X^0 = a . Y_b, X^3 = a' . Y_b', X^1 = a . Y_b', X^2 = -a' . Y_b
+      const int bb = (k == 1 || k == 3) ? 1 : 0;
+      const int swap = (k == 2 || k == 3) ? 1 : 0;
+      const cdbl *Y = Ysh + (ii * 2 + bb) * 2 * nm;
+      cdbl acc = cmk(0., 0.);
+      for (int row = 0; row < 2 * nm; ++row) {
+        const int h = row / nm;
+        const int n = nmin + row - h * nm;
+        const double av = ptv(a_out, n, (h ^ swap) & 1);
+        acc.x = fma(av, Y[row].x, acc.x);
+        acc.y = fma(av, Y[row].y, acc.y);
+      }
+      if (k == 2) {
+        acc.x = -acc.x;
+        acc.y = -acc.y;
+      }
+      Xo[(int64_t)pr * 4 + k] = acc;
+    }
+  }
+}
+
+// K7a: extinction and absorption per (grain, outgoing direction j).
+// Forward pair j: S = (1/k) sum_m f_m X_m(j).  Absorption pairs (j, i) with
+// every quadrature azimuth: S(phi) = (1/k) sum_m g_m(-phi) X_m(j, i).
+__global__ __launch_bounds__(256) void k_obs_q(
+    const ObsSpec spec, const int32_t *gi_off, const GrainParams gp,
+    const cdbl *X, double *qext, double *qextpol, double *qextcpol,
+    double *qabs, double *qabspol) {
+  const int g = blockIdx.x / spec.n_theta;
+  const int j = blockIdx.x - g * spec.n_theta;
+  const int i0 = gi_off[g], i1 = gi_off[g + 1];
+  const double lam = gp.wavelength[g];
+  const double k = 2. * M_PI / lam;
+  const double rv = gp.radius[g];
+  const double norm = M_PI * rv * rv;
+  const double pre = 2. * M_PI / k;
+  const int t = threadIdx.x;
+  __shared__ double red[4][2];
+  __shared__ double fwd[2][2];
+  if (t == 0) {
+    cdbl S0 = cmk(0., 0.), S3 = cmk(0., 0.);
+    const int pr = spec.fwd_pair0 + j;
+    for (int q = i0; q < i1; ++q) {
+      const double f = (q == i0) ? 1. : 2.;
+      const cdbl *Xp = X + ((int64_t)q * spec.n_pairs + pr) * 4;
+      S0.x += f * Xp[0].x;
+      S0.y += f * Xp[0].y;
+      S3.x += f * Xp[3].x;
+      S3.y += f * Xp[3].y;
+    }
+    const double kinv = 1. / k;
+    S0 = cscale(S0, kinv);
+    S3 = cscale(S3, kinv);
+    fwd[0][0] = S0.x;
+    fwd[0][1] = S0.y;
+    fwd[1][0] = S3.x;
+    fwd[1][1] = S3.y;
+  }
+  double zsum = 0., zpol = 0.;
+  if (spec.ngauss > 0) {
+    const int npair = spec.ngauss * spec.ngauss;
+    for (int e = t; e < npair; e += 256) {
+      const int ti = e / spec.ngauss, pj = e - ti * spec.ngauss;
+      const int pr = spec.abs_pair0 + j * spec.ngauss + ti;
+      const double cphi = spec.phi_cos[pj], sphi = -spec.phi_sin[pj];
+      cdbl S[4] = {cmk(0., 0.), cmk(0., 0.), cmk(0., 0.), cmk(0., 0.)};
+      // e^{i m dphi} by recurrence, as the reference
+      cdbl ex = cmk(1., 0.);
+      const cdbl st = cmk(cphi, sphi);
+      for (int q = i0; q < i1; ++q) {
+        const int m = q - i0;
+        const cdbl *Xp = X + ((int64_t)q * spec.n_pairs + pr) * 4;
+        const double fc = (m == 0) ? 1. : 2. * ex.x;
+        const double fs = (m == 0) ? 0. : 2. * ex.y;
+        ex = cmul(ex, st);
+        rfma(S[0], fc, Xp[0]);
+        rfma(S[3], fc, Xp[3]);
+        rfma(S[1], fs, Xp[1]);
+        rfma(S[2], fs, Xp[2]);
+      }
+      const double kinv = 1. / k;
+      double n2[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const cdbl s = cscale(S[q], kinv);
+        n2[q] = s.x * s.x + s.y * s.y;
+      }
+      const double w = spec.theta_w[ti] * spec.phi_w[pj];
+      zsum += 0.5 * (n2[0] + n2[1] + n2[2] + n2[3]) * w;
+      zpol += 0.5 * (n2[0] + n2[1] - n2[2] - n2[3]) * w;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    zsum += __shfl_xor(zsum, o);
+    zpol += __shfl_xor(zpol, o);
+  }
+  if ((t & 63) == 0) {
+    red[t >> 6][0] = zsum;
+    red[t >> 6][1] = zpol;
+  }
+  __syncthreads();
+  if (t == 0) {
+    const double zs = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    const double zp = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    const double e_ = pre * (fwd[0][1] + fwd[1][1]);
+    const double ep = pre * (fwd[0][1] - fwd[1][1]);
+    const double ec = pre * (fwd[1][0] - fwd[0][0]);
+    const int64_t o = (int64_t)g * spec.n_theta + j;
+    if (qext) qext[o] = e_ / norm;
+    if (qextpol) qextpol[o] = ep / norm;
+    if (qextcpol) qextcpol[o] = ec / norm;
+    if (qabs) qabs[o] = (e_ - zs) / norm;
+    if (qabspol) qabspol[o] = (ep - zp) / norm;
+  }
+}
+
+// K7b: scattering matrix Z on the (theta, phi) grid for incidence
+// theta = pi/2, phi = 0 (ScatteringMatrixTask.hpp:698-751); one thread per
+// (grain, theta, phi); Z row-major 4x4 per point.
+__global__ void k_obs_z(const ObsSpec spec, int n_grains,
+                        const int32_t *gi_off, const GrainParams gp,
+                        const cdbl *X, double *Z) {
+  const int nphi = spec.n_sphi;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)n_grains * spec.n_stheta * nphi;
+  if (idx >= total) return;
+  const int g = (int)(idx / ((int64_t)spec.n_stheta * nphi));
+  const int rem = (int)(idx - (int64_t)g * spec.n_stheta * nphi);
+  const int it = rem / nphi, ip = rem - (rem / nphi) * nphi;
+  const int i0 = gi_off[g], i1 = gi_off[g + 1];
+  const double k = 2. * M_PI / gp.wavelength[g];
+  const int pr = spec.sca_pair0 + it;
+  cdbl S[4] = {cmk(0., 0.), cmk(0., 0.), cmk(0., 0.), cmk(0., 0.)};
+  cdbl ex = cmk(1., 0.);
+  const cdbl st = cmk(spec.sphi_cos[ip], spec.sphi_sin[ip]);
+  for (int q = i0; q < i1; ++q) {
+    const int m = q - i0;
+    const cdbl *Xp = X + ((int64_t)q * spec.n_pairs + pr) * 4;
+    const double fc = (m == 0) ? 1. : 2. * ex.x;
+    const double fs = (m == 0) ? 0. : 2. * ex.y;
+    ex = cmul(ex, st);
+    rfma(S[0], fc, Xp[0]);
+    rfma(S[3], fc, Xp[3]);
+    rfma(S[1], fs, Xp[1]);
+    rfma(S[2], fs, Xp[2]);
+  }
+  const double kinv = 1. / k;
+  for (int q = 0; q < 4; ++q) S[q] = cscale(S[q], kinv);
+  // products S_a conj(S_b)
+  auto pc = [&](int a, int b) {
+    return cmk(S[a].x * S[b].x + S[a].y * S[b].y,
+               S[a].y * S[b].x - S[a].x * S[b].y);
+  };
+  const cdbl s00 = pc(0, 0), s11 = pc(1, 1), s22 = pc(2, 2), s33 = pc(3, 3);
+  const cdbl s01 = pc(0, 1), s32 = pc(3, 2), s02 = pc(0, 2), s31 = pc(3, 1);
+  const cdbl s03 = pc(0, 3), s12 = pc(1, 2), s21 = pc(2, 1), s20 = pc(2, 0);
+  const cdbl s30 = pc(3, 0);
+  double *z = Z + idx * 16;
+  z[0] = 0.5 * (s00.x + s11.x + s22.x + s33.x);
+  z[1] = 0.5 * (s00.x - s11.x + s22.x - s33.x);
+  z[2] = -s01.x - s32.x;
+  z[3] = -(s01.y - s32.y);           // Re(i (S0 S1* - S3 S2*))
+  z[4] = 0.5 * (s00.x + s11.x - s22.x - s33.x);
+  z[5] = 0.5 * (s00.x - s11.x - s22.x + s33.x);
+  z[6] = -s01.x + s32.x;
+  z[7] = -(s01.y + s32.y);
+  z[8] = -s02.x - s31.x;
+  z[9] = -s02.x + s31.x;
+  z[10] = s03.x + s12.x;
+  z[11] = (s03.y + s21.y);           // Re(-i (S0 S3* + S2 S1*))
+  z[12] = -(s20.y + s31.y);          // Re(i (S2 S0* + S3 S1*))
+  z[13] = -(s20.y - s31.y);
+  z[14] = (s30.y - s12.y);           // Re(-i (S3 S0* - S1 S2*))
+  z[15] = s30.x - s12.x;
+}
+
+hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
+                        hipStream_t s) {
+  const int n = (nmax_all + 1) * n_ang;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pitau, dim3((n + 127) / 128), dim3(128), 0, s, ang,
+                     n_ang, nmax_all, PT);
+  return hipGetLastError();
+}
+
+size_t obs_x_lds_bytes(int nm_max) {
+  return (size_t)kInChunk * 2 * 2 * nm_max * sizeof(cdbl);
+}
+
+hipError_t launch_obs_x(const ObsSpec &spec, int n_items, int nm_max,
+                        const Item *items, const cdbl *tbuf, const double *PT,
+                        int nmax_all, cdbl *X, hipStream_t s) {
+  if (n_items <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_obs_x, dim3(n_items), dim3(256),
+                     obs_x_lds_bytes(nm_max), s, spec, items, tbuf, PT,
+                     nmax_all, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,
+                        const int32_t *gi_off, const GrainParams &gp,
+                        const cdbl *X, double *qext, double *qextpol,
+                        double *qextcpol, double *qabs, double *qabspol,
+                        hipStream_t s) {
+  if (n_grains <= 0 || spec.n_theta <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_obs_q, dim3(n_grains * spec.n_theta), dim3(256), 0, s,
+                     spec, gi_off, gp, X, qext, qextpol, qextcpol, qabs,
+                     qabspol);
+  return hipGetLastError();
+}
+
+hipError_t launch_obs_z(const ObsSpec &spec, int n_grains,
+                        const int32_t *gi_off, const GrainParams &gp,
+                        const cdbl *X, double *Z, hipStream_t s) {
+  const int64_t total = (int64_t)n_grains * spec.n_stheta * spec.n_sphi;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_obs_z, dim3((unsigned)((total + 127) / 128)), dim3(128),
+                     0, s, spec, n_grains, gi_off, gp, X, Z);
+  return hipGetLastError();
+}
+
+}  // namespace ctm

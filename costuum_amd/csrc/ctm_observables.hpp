@@ -1,0 +1,43 @@
+// ctm_observables.hpp -- direction-pair layout of the observable kernels.
+#pragma once
+
+#include "ctm_device.hpp"
+#include "ctm_kernels.hpp"
+
+namespace ctm {
+
+// Directions are rows of an angle table (cos, sin, 1/sin with the 9000
+// sentinel at sin = 0, ExtinctionCoefficientTask.hpp:222-226).  Incoming
+// and outgoing lists index that table; each direction pair (out, in) gets
+// the four X_m^k values.  Pair blocks: forward pairs j (extinction, and the
+// first term of absorption) at fwd_pair0 + j; absorption pairs (j, i) at
+// abs_pair0 + j * ngauss + i; scattering-matrix pairs at sca_pair0 + s.
+struct ObsSpec {
+  int32_t n_ang, n_in, n_out, n_pairs;
+  int32_t n_theta, ngauss, n_stheta, n_sphi;
+  int32_t fwd_pair0, abs_pair0, sca_pair0, pad;
+  const int32_t *in_angle;   // [n_in]
+  const int32_t *out_angle;  // [n_out]
+  const int32_t *pair_in;    // [n_pairs] index into the incoming list
+  const int32_t *pair_out;   // [n_pairs] index into the outgoing list
+  const double *theta_w;     // [ngauss] GL weights of cos(theta_in)
+  const double *phi_cos, *phi_sin, *phi_w;  // [ngauss] azimuths on [0, 2pi]
+  const double *sphi_cos, *sphi_sin;        // [n_sphi] scattering azimuths
+};
+
+hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
+                        hipStream_t s);
+size_t obs_x_lds_bytes(int nm_max);
+hipError_t launch_obs_x(const ObsSpec &spec, int n_items, int nm_max,
+                        const Item *items, const cdbl *tbuf, const double *PT,
+                        int nmax_all, cdbl *X, hipStream_t s);
+hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,
+                        const int32_t *gi_off, const GrainParams &gp,
+                        const cdbl *X, double *qext, double *qextpol,
+                        double *qextcpol, double *qabs, double *qabspol,
+                        hipStream_t s);
+hipError_t launch_obs_z(const ObsSpec &spec, int n_grains,
+                        const int32_t *gi_off, const GrainParams &gp,
+                        const cdbl *X, double *Z, hipStream_t s);
+
+}  // namespace ctm

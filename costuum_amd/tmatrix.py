@@ -139,6 +139,39 @@ class BatchResult:
         return out
 
 
+def observables(theta, ctx=None, do_extinction=True, do_absorption=True,
+                account_for_scattering=True, ngauss=20, n_scatter_theta=0,
+                scatter_gl=True, n_grains=None):
+    """Extinction/absorption at the angles `theta` (radians) and optionally
+    the scattering-matrix grid, for the T-matrices of the context's last
+    batch.  Returns a dict of device tensors."""
+    ctx = ctx or default_context()
+    dev = ctx.device
+    th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    n = n_grains
+    if n is None:
+        raise ValueError("n_grains is required")
+    nt = len(th)
+    out = {}
+    for k in ("qext", "qextpol", "qextcpol", "qabs", "qabspol"):
+        out[k] = torch.zeros((n, nt), dtype=torch.float64, device=dev)
+    Z = None
+    if n_scatter_theta > 0:
+        Z = torch.zeros((n, n_scatter_theta, 2 * n_scatter_theta, 4, 4),
+                        dtype=torch.float64, device=dev)
+        out["Z"] = Z
+    spec = _native.CtmObservableSpec(nt, th.ctypes.data, int(do_extinction),
+                                     int(do_absorption),
+                                     int(account_for_scattering), ngauss,
+                                     n_scatter_theta, int(scatter_gl))
+    rc = ctx._lib.ctm_observables(
+        ctx.handle, ctypes.byref(spec), _ptr(out["qext"]),
+        _ptr(out["qextpol"]), _ptr(out["qextcpol"]), _ptr(out["qabs"]),
+        _ptr(out["qabspol"]), _ptr(Z))
+    ctx.check(rc, "ctm_observables")
+    return out
+
+
 def tmatrix_batch(radius, axis_ratio, wavelength, mr_re, mr_im, min_order=4,
                   max_order=100, glfac=2, tolerance=1e-4, precision="fp64",
                   ctx=None, sync=True):

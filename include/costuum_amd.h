@@ -108,6 +108,35 @@ int ctm_tmatrix_get(ctm_context *ctx, int grain, double *T_host,
 int ctm_batch_orders(ctm_context *ctx, uint32_t *nmax_host,
                      uint32_t *ngauss_host, int32_t *status_host);
 
+/* Observables of the context's T-matrices (last batch, or its orientation
+ * average), per grain:
+ *  - extinction at the n_theta incidence angles: Q_ext, Q_extpol, Q_extcpol
+ *    (ExtinctionCoefficientTask.hpp:626-653);
+ *  - absorption at the same angles: Q_abs, Q_abspol, minus the GL(ngauss) x
+ *    GL(ngauss) quadrature of the scattered light over incoming directions
+ *    when account_for_scattering is set (AbsorptionCoefficientTask.hpp:
+ *    722-781);
+ *  - the 4x4 scattering matrix for incidence theta = pi/2 on an
+ *    n_scatter_theta x 2 n_scatter_theta (theta, phi) grid, GL or uniform
+ *    (ScatteringMatrixTask.hpp:698-751, grid :107-277).
+ * Q outputs are device arrays [n_grains][n_theta] normalised by pi R_V^2
+ * (any may be NULL); Z is a device array
+ * [n_grains][n_scatter_theta][2 n_scatter_theta][4][4] (may be NULL). */
+typedef struct ctm_observable_spec {
+  int32_t n_theta;
+  const double *theta; /* host, radians */
+  int32_t do_extinction;
+  int32_t do_absorption;
+  int32_t account_for_scattering;
+  int32_t ngauss;
+  int32_t n_scatter_theta;
+  int32_t scatter_gl;
+} ctm_observable_spec;
+
+int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
+                    double *qext, double *qextpol, double *qextcpol,
+                    double *qabs, double *qabspol, double *Z);
+
 /* Kernel timing.  With profiling on, every launch of the library is
  * bracketed by two HIP events on the context's stream; ctm_profile_read
  * returns, per kernel, the number of launches, the summed event time and the
@@ -121,7 +150,10 @@ int ctm_batch_orders(ctm_context *ctx, uint32_t *nmax_host,
 #define CTM_KERNEL_SOLVE 4
 #define CTM_KERNEL_CONVERGE 5
 #define CTM_KERNEL_SUMS 6
-#define CTM_KERNEL_COUNT 7
+#define CTM_KERNEL_OBS_X 7
+#define CTM_KERNEL_OBS_SUM 8
+#define CTM_KERNEL_AVERAGE 9
+#define CTM_KERNEL_COUNT 10
 
 typedef struct ctm_kernel_stats {
   uint64_t launches;
