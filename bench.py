@@ -7,8 +7,9 @@ Workload: 4096 spheroids drawn from seed 0 (radius log-uniform in
 task-path convergence with minimum order 4, maximum order 100, 2 Gauss
 nodes per order and tolerance 1e-4.  One step = one pass of the hot path
 over the whole batch with the inputs resident in HBM: every order of the
-convergence loop, all m blocks at the converged order, and the per-grain
-Q_sca/Q_ext sums.
+convergence loop, all m blocks at the converged order, the per-grain
+Q_sca/Q_ext sums, and Q_ext and Q_abs (minus the scattered light, 20 x 20
+quadrature) at 32 Gauss-Legendre incidence angles, averaged over them.
 
 Prints one JSON line.  ``value`` counts the grains whose convergence test
 passed, per second of the timed steps.
@@ -36,15 +37,16 @@ def parse():
     ap.add_argument("--grains", type=int, default=4096)
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the CPU oracle on a sample (rank 0, N=1)")
-    ap.add_argument("--cpu-stride", type=int, default=16,
+    ap.add_argument("--cpu-stride", type=int, default=64,
                     help="CPU sample = every k-th grain in size order")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
 
-def cpu_baseline(g, stride, threads):
+def cpu_baseline(g, stride, threads, theta, wtheta):
     """The CPU oracle (FP64 restatement of the reference algorithm) on a
-    size-stratified sample of the batch."""
+    size-stratified sample of the batch, doing the same work per grain as
+    the GPU step (T-matrix, then Q_ext and Q_abs at the same angles)."""
     import numpy as np
     import oracle
     x = 2 * np.pi * g["radius"] / g["wavelength"]
@@ -55,16 +57,19 @@ def cpu_baseline(g, stride, threads):
     args = [g[k][sel] for k in ("radius", "axis_ratio", "wavelength",
                                  "mr_re", "mr_im")]
     t0 = time.perf_counter()
-    nmax, status, _, _ = oracle.batch_task(*args, min_order=4, max_order=100,
-                                           glfac=2, tol=1e-4,
-                                           nthreads=threads)
+    nmax, status, qe, qa = oracle.batch_task_obs(*args, theta, wtheta,
+                                                 ngauss=20, min_order=4,
+                                                 max_order=100, glfac=2,
+                                                 tol=1e-4, nthreads=threads)
     dt = time.perf_counter() - t0
-    return {"value": float((status == 0).sum() / dt),
-            "unit": "converged T-matrices/s", "cores": int(threads),
-            "kind": "port",
-            "sample": "%d of 4096 grains (every %dth in size-parameter "
-                      "order), %.2f s wall on %d threads" % (len(sel), stride,
-                                                            dt, threads)}
+    return ({"value": float((status == 0).sum() / dt),
+             "unit": "converged T-matrices/s", "cores": int(threads),
+             "kind": "port",
+             "sample": "%d of 4096 grains (every %dth in size-parameter "
+                       "order, largest N %d), T-matrix plus Q_ext/Q_abs at "
+                       "32 angles, %.2f s wall on %d threads"
+                       % (len(sel), stride, int(nmax.max()), dt, threads)},
+            sel, nmax, qe, qa)
 
 
 def main():
@@ -77,7 +82,7 @@ def main():
     import numpy as np
     import torch
     import make_inputs
-    from costuum_amd.tmatrix import Context, tmatrix_batch
+    from costuum_amd.tmatrix import Context, observables, tmatrix_batch
 
     g = make_inputs.load(0, 4096)
     if a.grains != 4096:
@@ -89,14 +94,27 @@ def main():
     conv = dict(min_order=4, max_order=100, glfac=2, tolerance=1e-4,
                 precision=a.precision)
 
+    # 32 Gauss-Legendre incidence angles (cos theta nodes), alignment off
+    xg, wg = np.polynomial.legendre.leggauss(32)
+    theta = np.arccos(xg)
+    wgt = torch.as_tensor(0.5 * wg, dtype=torch.float64, device=dev)
+    n = len(g["radius"])
+
+    def step():
+        r = tmatrix_batch(*dg, ctx=ctx, sync=False, **conv)
+        obs = observables(theta, ctx=ctx, ngauss=20, n_grains=n)
+        qext = obs["qext"] @ wgt
+        qabs = obs["qabs"] @ wgt
+        return r, qext, qabs
+
     res = None
     for _ in range(a.warmup):
-        res = tmatrix_batch(*dg, ctx=ctx, **conv)
+        res, qext_avg, qabs_avg = step()
     ctx.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        res = tmatrix_batch(*dg, ctx=ctx, **conv)
+        res, qext_avg, qabs_avg = step()
     ctx.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -106,6 +124,8 @@ def main():
     nconv = int((status == 0).sum())
     value = nconv / (ms_per_step / 1e3)
 
+    qext_np = qext_avg.cpu().numpy()
+    qabs_np = qabs_avg.cpu().numpy()
     launches, ms, flops, nbytes = stats["assemble"]
     avg_ms = ms / max(launches, 1)
     achieved = flops / (ms / 1e3) / 1e12 if ms > 0 else 0.0
@@ -121,8 +141,20 @@ def main():
                    if v[1] > 0 and v[2] > 0 else None}
                for k, v in stats.items()}
     cpu = None
+    parity = None
     if a.cpu_baseline:
-        cpu = cpu_baseline(g, a.cpu_stride, a.cpu_threads)
+        cpu, sel, n_ref, qe_ref, qa_ref = cpu_baseline(
+            g, a.cpu_stride, a.cpu_threads, theta, 0.5 * wg)
+        nmax_gpu = res.nmax.cpu().numpy()[sel]
+        same = nmax_gpu == n_ref
+        ee = np.abs(qext_np[sel] - qe_ref) / np.abs(qe_ref)
+        ea = np.abs(qabs_np[sel] - qa_ref) / np.abs(qa_ref)
+        parity = {"grains": int(len(sel)),
+                  "same_nmax": int(same.sum()),
+                  "max_rel_err_qext": float(ee[same].max()),
+                  "max_rel_err_qabs": float(ea[same].max()),
+                  "p99_rel_err_qext": float(np.percentile(ee[same], 99)),
+                  "p99_rel_err_qabs": float(np.percentile(ea[same], 99))}
     out = {
         "metric": "converged T-matrices/s on 1 GPU, batched mixed vs plain "
                   "FP64, at <=1e-6 rel Q_ext err",
@@ -141,9 +173,13 @@ def main():
                    "grains": len(g["radius"]),
                    "converged": nconv,
                    "nmax_max": int(res.nmax.max().item()),
+                   "orientations": 32, "absorption_quadrature": "20x20",
+                   "mean_qext": float(np.mean(qext_np)),
+                   "mean_qabs": float(np.mean(qabs_np)),
                    "precision": a.precision},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

@@ -160,6 +160,9 @@ def observables(theta, ctx=None, do_extinction=True, do_absorption=True,
         Z = torch.zeros((n, n_scatter_theta, 2 * n_scatter_theta, 4, 4),
                         dtype=torch.float64, device=dev)
         out["Z"] = Z
+    # the output buffers were zeroed on torch's stream; the library writes
+    # them on its own stream
+    torch.cuda.current_stream(dev).synchronize()
     spec = _native.CtmObservableSpec(nt, th.ctypes.data, int(do_extinction),
                                      int(do_absorption),
                                      int(account_for_scattering), ngauss,

@@ -292,3 +292,27 @@ def orientation_average(nmax, T, pN):
     out = np.zeros_like(T)
     _olib().ctmo_orientation_average(nmax, _p(T), _p(pN), _p(out))
     return out
+
+
+def batch_task_obs(rv, d, lam, mr_re, mr_im, theta, wtheta, ngauss=20,
+                   min_order=4, max_order=100, glfac=2, tol=1e-4, nthreads=0):
+    L = _olib()
+    if not getattr(L, "_bto", False):
+        L.ctmo_batch_task_obs.argtypes = [_I, _PD, _PD, _PD, _PD, _PD, _U, _U,
+                                          _U, _D, _I, _U, _PD, _PD, _U, _PU,
+                                          _PI, _PD, _PD]
+        L._bto = True
+    rv, d, lam, mr_re, mr_im, theta, wtheta = [
+        np.ascontiguousarray(a, dtype=np.float64)
+        for a in (rv, d, lam, mr_re, mr_im, theta, wtheta)]
+    n = len(rv)
+    nmax = np.zeros(n, dtype=np.uint32)
+    status = np.zeros(n, dtype=np.int32)
+    qe = np.zeros(n)
+    qa = np.zeros(n)
+    L.ctmo_batch_task_obs(n, _p(rv), _p(d), _p(lam), _p(mr_re), _p(mr_im),
+                          min_order, max_order, glfac, tol, nthreads,
+                          len(theta), _p(theta), _p(wtheta), ngauss,
+                          nmax.ctypes.data_as(_PU), status.ctypes.data_as(_PI),
+                          _p(qe), _p(qa))
+    return nmax, status, qe, qa

@@ -1333,3 +1333,89 @@ void ctmo_orientation_average(unsigned nmax, const double *Tind,
   free(C);
   free(scratch);
 }
+
+/* The CPU baseline of the batched workload: task-path T-matrices as
+ * ctmo_batch_task, then Q_ext and Q_abs (with the scattered-light
+ * quadrature of ngauss x ngauss incoming directions) at ntheta incidence
+ * angles, averaged with weights wtheta; parallel over (grain, m) blocks and
+ * over (grain, angle) pairs. */
+void ctmo_batch_task_obs(int n, const double *rv, const double *d,
+                         const double *lambda, const double *mr_re,
+                         const double *mr_im, unsigned min_order,
+                         unsigned max_order, unsigned glfac, double tol,
+                         int nthreads, unsigned ntheta, const double *theta,
+                         const double *wtheta, unsigned ngauss,
+                         unsigned *nmax_out, int *status_out,
+                         double *qext_avg, double *qabs_avg) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  unsigned *ng = (unsigned *)malloc(sizeof(unsigned) * (n ? n : 1));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int g = 0; g < n; ++g) {
+    double dqs, dqe;
+    status_out[g] = ctmo_converge_task(rv[g], d[g], lambda[g], mr_re[g],
+                                       mr_im[g], min_order, max_order, glfac,
+                                       tol, &nmax_out[g], &ng[g], &dqs, &dqe);
+  }
+  tables_t *tabs = (tables_t *)malloc(sizeof(tables_t) * (n ? n : 1));
+  cplx **Ts = (cplx **)malloc(sizeof(cplx *) * (n ? n : 1));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int g = 0; g < n; ++g) {
+    tables_build(&tabs[g], rv[g], d[g], lambda[g], mr_re[g] + I * mr_im[g],
+                 nmax_out[g], ng[g]);
+    Ts[g] = (cplx *)malloc(sizeof(cplx) * ctmo_tmatrix_size(nmax_out[g]));
+  }
+  size_t *first = (size_t *)malloc(sizeof(size_t) * (n + 1));
+  first[0] = 0;
+  for (int g = 0; g < n; ++g) first[g + 1] = first[g] + nmax_out[g] + 1;
+  const long nwork = (long)first[n];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (long q = 0; q < nwork; ++q) {
+    int g = 0;
+    while ((size_t)q >= first[g + 1]) ++g;
+    const unsigned m = (unsigned)(q - (long)first[g]);
+    tmatrix_m_from_tables(&tabs[g], m,
+                          Ts[g] + ctmo_tmatrix_offset(nmax_out[g], m));
+  }
+  double *qe = (double *)calloc((size_t)n * ntheta + 1, sizeof(double));
+  double *qa = (double *)calloc((size_t)n * ntheta + 1, sizeof(double));
+  const long npairs = (long)n * ntheta;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (long q = 0; q < npairs; ++q) {
+    const int g = (int)(q / ntheta);
+    const unsigned j = (unsigned)(q % ntheta);
+    double e, ep, ec, a, ap;
+    ctmo_extinction(nmax_out[g], (const double *)Ts[g], lambda[g], rv[g], 1,
+                    theta + j, &e, &ep, &ec);
+    ctmo_absorption(nmax_out[g], (const double *)Ts[g], lambda[g], rv[g], 1,
+                    theta + j, ngauss, 1, &a, &ap);
+    qe[q] = e;
+    qa[q] = a;
+  }
+  for (int g = 0; g < n; ++g) {
+    double se = 0., sa = 0.;
+    for (unsigned j = 0; j < ntheta; ++j) {
+      se += wtheta[j] * qe[(size_t)g * ntheta + j];
+      sa += wtheta[j] * qa[(size_t)g * ntheta + j];
+    }
+    qext_avg[g] = se;
+    qabs_avg[g] = sa;
+    tables_free(&tabs[g]);
+    free(Ts[g]);
+  }
+  free(qe);
+  free(qa);
+  free(first);
+  free(tabs);
+  free(Ts);
+  free(ng);
+}

@@ -111,6 +111,18 @@ void ctmo_batch_task(int n, const double *rv, const double *axis_ratio,
                      int nthreads, unsigned *nmax_out, int *status_out,
                      double *qsca_out, double *qext_out);
 
+/* CPU baseline of the batched workload: T-matrices plus Q_ext and Q_abs
+ * (scattered-light quadrature ngauss x ngauss) at ntheta incidence angles,
+ * averaged with weights wtheta. */
+void ctmo_batch_task_obs(int n, const double *rv, const double *axis_ratio,
+                         const double *wavelength, const double *mr_re,
+                         const double *mr_im, unsigned min_order,
+                         unsigned max_order, unsigned glfac, double tolerance,
+                         int nthreads, unsigned ntheta, const double *theta,
+                         const double *wtheta, unsigned ngauss,
+                         unsigned *nmax_out, int *status_out,
+                         double *qext_avg, double *qabs_avg);
+
 /* a23: ExtinctionCoefficientTask.hpp:626-653 (per incidence angle). */
 void ctmo_extinction(unsigned nmax, const double *T, double wavelength,
                      double rv, unsigned ntheta, const double *theta,
