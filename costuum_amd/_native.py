@@ -21,7 +21,7 @@ EXPORTS = (
     "ctm_synchronize", "ctm_tmatrix_batch", "ctm_tmatrix_fixed",
     "ctm_tmatrix_size", "ctm_tmatrix_get", "ctm_batch_orders",
     "ctm_profile_enable", "ctm_profile_reset", "ctm_profile_read",
-    "ctm_observables",
+    "ctm_observables", "ctm_orientation_average", "ctm_average_get",
 )
 
 KERNELS = ("gauss_legendre", "wigner", "tables", "assemble", "solve",
@@ -101,5 +101,10 @@ def lib():
     L.ctm_observables.argtypes = [vp, ctypes.POINTER(CtmObservableSpec),
                                   dp, dp, dp, dp, dp, dp]
     L.ctm_observables.restype = ctypes.c_int
+    L.ctm_orientation_average.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                          dp, dp]
+    L.ctm_orientation_average.restype = ctypes.c_int
+    L.ctm_average_get.argtypes = [vp, ctypes.c_int, dp, ctypes.c_size_t]
+    L.ctm_average_get.restype = ctypes.c_int
     _lib = L
     return L

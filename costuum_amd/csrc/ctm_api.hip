@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <string>
@@ -30,6 +31,7 @@
 #include "ctm_device.hpp"
 #include "ctm_kernels.hpp"
 #include "ctm_observables.hpp"
+#include "ctm_average.hpp"
 
 using namespace ctm;
 
@@ -236,7 +238,8 @@ struct ctm_context {
   DevBuf d_gl, d_glkeys, d_wig, d_wigkeys, d_tables, d_sets, d_items,
       d_tiles, d_solve, d_q, d_t, d_qparts, d_state, d_active, d_winoff,
       d_winitems, d_gioff, d_gitems, d_gioff_tmp, d_params, d_ang, d_pt,
-      d_x, d_spec;
+      d_x, d_spec, d_cgkeys, d_cg, d_cgidx, d_avgtasks, d_avg, d_aoff, d_pn,
+      d_dist, d_obsitems;
   // Results of the last batch (host).
   int n_grains = 0;
   std::vector<uint32_t> nmax, ngauss;
@@ -245,6 +248,9 @@ struct ctm_context {
   std::vector<int32_t> grain_items_off;   // per grain into final_items
   std::vector<int32_t> grain_items;
   GrainParams gp_saved{};  // context copies of the last batch's inputs
+  bool averaged = false;   // d_avg holds ensemble T-matrices of the batch
+  std::vector<int64_t> aoff;  // full-layout offsets per final item
+  std::vector<int32_t> obs_items;
   Profiler prof;
 };
 
@@ -502,7 +508,10 @@ int ctm_destroy(ctm_context *ctx) {
                     &ctx->d_state,  &ctx->d_active,  &ctx->d_winoff,
                     &ctx->d_winitems, &ctx->d_gioff, &ctx->d_gitems,
                     &ctx->d_gioff_tmp, &ctx->d_params, &ctx->d_ang,
-                    &ctx->d_pt, &ctx->d_x, &ctx->d_spec};
+                    &ctx->d_pt, &ctx->d_x, &ctx->d_spec, &ctx->d_cgkeys,
+                    &ctx->d_cg, &ctx->d_cgidx, &ctx->d_avgtasks, &ctx->d_avg,
+                    &ctx->d_aoff, &ctx->d_pn, &ctx->d_dist,
+                    &ctx->d_obsitems};
   for (DevBuf *b : bufs) b->release();
   ctx->prof.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -576,6 +585,7 @@ static int run_all_m(ctm_context *ctx, const GrainParams &gp, int n,
                               ctx->d_qparts.as<double>(), qsca_out, qext_out,
                               ctx->stream));
   ctx->final_items = w.items;
+  ctx->averaged = false;
   return CTM_OK;
 }
 
@@ -968,15 +978,48 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   CTM_HIP(ctx->d_x.ensure(nitems * (size_t)sp.n_pairs * 4 * sizeof(cdbl)));
   CTM_LAUNCH(CTM_KERNEL_OBS_X, 0., 0.,
              launch_pitau(d_ang, n_ang, nmax_all, ctx->d_pt.as<double>(), s));
-  double fx = 0.;
-  for (const Item &it : ctx->final_items) {
-    const double nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-    fx += 8. * (4. * nm * nm * sp.n_in + 8. * nm * sp.n_pairs);
+  if (obs_x_lds_bytes(nmax_all, sp.n_out) > 159 * 1024) {
+    ctx->err = "too many outgoing directions for the largest order";
+    return CTM_ERR_ARGUMENT;
   }
-  CTM_LAUNCH(CTM_KERNEL_OBS_X, fx, 0.,
-             launch_obs_x(sp, (int)nitems, nm_max, ctx->d_items.as<Item>(),
-                          ctx->d_t.as<cdbl>(), ctx->d_pt.as<double>(),
-                          nmax_all, ctx->d_x.as<cdbl>(), s));
+  // items by block order (largest first), launched in LDS-size buckets
+  {
+    std::vector<int32_t> &lst = ctx->obs_items;
+    lst.resize(nitems);
+    std::vector<int32_t> cnt(nmax_all + 2, 0);
+    auto nm_of = [&](const Item &it) { return it.N + 1 - (it.m > 0 ? it.m : 1); };
+    for (const Item &it : ctx->final_items) cnt[nmax_all - nm_of(it) + 1]++;
+    for (int q = 1; q <= nmax_all + 1; ++q) cnt[q] += cnt[q - 1];
+    for (size_t i = 0; i < nitems; ++i)
+      lst[cnt[nmax_all - nm_of(ctx->final_items[i])]++] = (int32_t)i;
+    CTM_HIP(upload(ctx->d_obsitems, lst, s));
+    const int caps[] = {nmax_all, 48, 24, 12};
+    size_t pos = 0;
+    for (int b = 0; b < 4 && pos < nitems; ++b) {
+      const int cap = caps[b];
+      const int lo = b < 3 ? caps[b + 1] : 0;
+      if (cap <= lo) continue;
+      size_t end = pos;
+      double fb = 0.;
+      while (end < nitems && nm_of(ctx->final_items[lst[end]]) > lo) {
+        const double nm = nm_of(ctx->final_items[lst[end]]);
+        fb += 8. * (4. * nm * nm * sp.n_in + 8. * nm * sp.n_pairs);
+        ++end;
+      }
+      if (end > pos) {
+        CTM_LAUNCH(CTM_KERNEL_OBS_X, fb, 0.,
+                   launch_obs_x(sp, ctx->d_obsitems.as<int32_t>() + pos,
+                                (int)(end - pos), cap,
+                                ctx->d_items.as<Item>(), ctx->d_t.as<cdbl>(),
+                                ctx->averaged ? ctx->d_avg.as<cdbl>() : nullptr,
+                                ctx->averaged ? ctx->d_aoff.as<int64_t>()
+                                              : nullptr,
+                                ctx->d_pt.as<double>(), nmax_all,
+                                ctx->d_x.as<cdbl>(), s));
+      }
+      pos = end;
+    }
+  }
   if (nth > 0 && (qext || qextpol || qextcpol || qabs || qabspol)) {
     CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
                launch_obs_q(sp, n, ctx->d_gioff.as<int32_t>(), ctx->gp_saved,
@@ -994,6 +1037,121 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   }
   // the spec blob and the host vectors must outlive the queued copies
   CTM_HIP(hipStreamSynchronize(s));
+  return CTM_OK;
+}
+
+int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
+                            const double *pN_host,
+                            const int32_t *dist_of_grain) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (ctx->final_items.empty()) {
+    ctx->err = "no T-matrices in the context";
+    return CTM_ERR_STATE;
+  }
+  const int n = ctx->n_grains;
+  int nmax = 1;
+  for (int g = 0; g < n; ++g) nmax = std::max(nmax, (int)ctx->nmax[g]);
+  if (n_dist < 1 || !pN_host || ncoef < 2 * nmax + 1) {
+    ctx->err = "need >= 2 nmax + 1 expansion coefficients per distribution "
+               "(TMatrixCalculator.hpp:241-242)";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (nmax > kMaxAvgOrder / 2) {
+    ctx->err = "order too large for the averaging kernels";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (dist_of_grain) {
+    for (int g = 0; g < n; ++g)
+      if (dist_of_grain[g] < 0 || dist_of_grain[g] >= n_dist) {
+        ctx->err = "distribution index out of range";
+        return CTM_ERR_ARGUMENT;
+      }
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  // orders N with a non-zero coefficient in any distribution
+  std::vector<char> needN(2 * nmax + 1, 0);
+  for (int dd = 0; dd < n_dist; ++dd)
+    for (int N = 0; N <= 2 * nmax; ++N)
+      if (pN_host[(size_t)dd * ncoef + N] != 0.) needN[N] = 1;
+  // CG keys for (n1, n2, N) and the dense index table
+  const size_t idx_size = (size_t)(nmax + 1) * (nmax + 1) * (2 * nmax + 1);
+  std::vector<int32_t> cgidx(idx_size, -1);
+  std::vector<CgKey> keys;
+  int64_t cgsize = 0;
+  for (int n1 = 1; n1 <= nmax; ++n1)
+    for (int n2 = 1; n2 <= nmax; ++n2) {
+      const int lo = std::abs(n1 - n2), hi = n1 + n2;
+      for (int N = lo; N <= hi; ++N) {
+        if (!needN[N]) continue;
+        CgKey k{n1, n2, N, 0, cgsize};
+        cgidx[((size_t)n1 * (nmax + 1) + n2) * (2 * nmax + 1) + N] =
+            (int32_t)cgsize;
+        cgsize += std::min(n1, n2) + 1;
+        keys.push_back(k);
+      }
+    }
+  if (cgsize > INT32_MAX) {
+    ctx->err = "Clebsch-Gordan table too large";
+    return CTM_ERR_ARGUMENT;
+  }
+  std::vector<AvgTask> tasks;
+  ctx->aoff.assign(ctx->final_items.size(), 0);
+  int64_t asize = 0;
+  for (int g = 0; g < n; ++g) {
+    const int Ng = (int)ctx->nmax[g];
+    for (int n1 = 1; n1 <= Ng; ++n1)
+      for (int n2 = 1; n2 <= Ng; ++n2) tasks.push_back({g, n1, n2, 0});
+    for (int m = 0; m <= Ng; ++m) {
+      const int64_t nm = Ng + 1 - (m > 0 ? m : 1);
+      ctx->aoff[ctx->grain_items_off[g] + m] = asize;
+      asize += 4 * nm * nm;
+    }
+  }
+  std::vector<double> pn(pN_host, pN_host + (size_t)n_dist * ncoef);
+  std::vector<int32_t> dist;
+  if (dist_of_grain) dist.assign(dist_of_grain, dist_of_grain + n);
+  CTM_HIP(upload(ctx->d_cgkeys, keys, s));
+  CTM_HIP(upload(ctx->d_cgidx, cgidx, s));
+  CTM_HIP(upload(ctx->d_avgtasks, tasks, s));
+  CTM_HIP(upload(ctx->d_aoff, ctx->aoff, s));
+  CTM_HIP(upload(ctx->d_pn, pn, s));
+  if (dist_of_grain) CTM_HIP(upload(ctx->d_dist, dist, s));
+  CTM_HIP(ctx->d_cg.ensure(std::max<int64_t>(cgsize, 1) * sizeof(double)));
+  CTM_HIP(ctx->d_avg.ensure(std::max<int64_t>(asize, 1) * sizeof(cdbl)));
+  CTM_LAUNCH(CTM_KERNEL_AVERAGE, 0., 0.,
+             launch_cg(ctx->d_cgkeys.as<CgKey>(), (int)keys.size(),
+                       ctx->d_cg.as<double>(), s));
+  CTM_LAUNCH(CTM_KERNEL_AVERAGE, 0., 0.,
+             launch_average(ctx->d_avgtasks.as<AvgTask>(), (int)tasks.size(),
+                            ctx->d_items.as<Item>(),
+                            ctx->d_gioff.as<int32_t>(), ctx->d_t.as<cdbl>(),
+                            ctx->d_pn.as<double>(), ncoef,
+                            dist_of_grain ? ctx->d_dist.as<int32_t>() : nullptr,
+                            ctx->d_cgidx.as<int32_t>(), ctx->d_cg.as<double>(),
+                            nmax, ctx->d_avg.as<cdbl>(),
+                            ctx->d_aoff.as<int64_t>(), s));
+  CTM_HIP(hipStreamSynchronize(s));
+  ctx->averaged = true;
+  return CTM_OK;
+}
+
+int ctm_average_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (!ctx->averaged) {
+    ctx->err = "no ensemble T-matrices in the context";
+    return CTM_ERR_STATE;
+  }
+  const size_t need = ctm_tmatrix_size(ctx, g);
+  if (need == 0 || !T_host || capacity < need) {
+    ctx->err = "invalid grain index or output capacity";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  const int64_t o = ctx->aoff[ctx->grain_items_off[g]];
+  CTM_HIP(hipMemcpyAsync(T_host, ctx->d_avg.as<cdbl>() + o, need * sizeof(cdbl),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  CTM_HIP(hipStreamSynchronize(ctx->stream));
   return CTM_OK;
 }
 

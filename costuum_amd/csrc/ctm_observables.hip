@@ -105,82 +105,139 @@ __device__ inline cdbl cfac(int nn, int n) {
 
 // K6: X_m^k for every direction pair of one (grain, m) item.  The incoming
 // directions are processed in chunks of kInChunk: Y(h n; i) = sum_nn
-// c_{nn n} T(h n; h' nn) b(h' nn; i) for b and b' (h' fixed by the parity
-// class), then the pair contractions with the outgoing pi/tau.
+// c_{nn n} T(h n; h' nn) b(h' nn; i) for b and b' (for the parity-class
+// layout h' is fixed by the class), one thread per row (h n) with the whole
+// chunk in registers, then one thread per direction pair contracts with the
+// outgoing pi/tau for all four k.  pi/tau of the item's directions and the
+// n-factors a_n are staged in LDS.
 constexpr int kInChunk = 8;
 
 __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
+                                               const int32_t *item_list,
                                                const Item *items,
                                                const cdbl *tbuf,
+                                               const cdbl *abuf,
+                                               const int64_t *aoff,
                                                const double *PT,
-                                               int nmax_all, cdbl *X) {
+                                               int nmax_all, int nm_cap,
+                                               cdbl *X) {
   extern __shared__ double smem[];
-  const Item it = items[blockIdx.x];
+  const int ii = item_list[blockIdx.x];
+  const Item it = items[ii];
   const int m = it.m;
   const int nmin = m > 0 ? m : 1;
   const int nm = it.N + 1 - nmin;
-  const int nm2 = nm * nm;
+  const int64_t nm2 = (int64_t)nm * nm;
+  const int n2m = 2 * nm;
   const cdbl *T = tbuf + it.t_off;
-  cdbl *Ysh = reinterpret_cast<cdbl *>(smem);  // [kInChunk][2][2 nm]
+  const cdbl *A = abuf ? abuf + aoff[ii] : nullptr;
+  cdbl *Ysh = reinterpret_cast<cdbl *>(smem);            // [chunk][2][2 nm]
+  double *PTo = smem + 2 * (kInChunk * 2 * 2 * nm_cap);  // [out][p][2]
+  double *Bv = PTo + (int64_t)spec.n_out * nm_cap * 2;   // [chunk][p][2]
+  double *an = Bv + kInChunk * nm_cap * 2;               // [p]
   const int t = threadIdx.x;
   const int n_ang = spec.n_ang;
-  auto ptv = [&](int a, int n, int which) -> double {
-    return PT[(((int64_t)m * n_ang + a) * nmax_all + (n - 1)) * 2 + which];
-  };
-  cdbl *Xo = X + (int64_t)blockIdx.x * spec.n_pairs * 4;
+  const double *PTm = PT + (int64_t)m * n_ang * nmax_all * 2;
+  for (int e = t; e < spec.n_out * nm; e += 256) {
+    const int o = e / nm, p = e - (e / nm) * nm;
+    const double *src =
+        PTm + ((int64_t)spec.out_angle[o] * nmax_all + (nmin + p - 1)) * 2;
+    PTo[(o * nm + p) * 2] = src[0];
+    PTo[(o * nm + p) * 2 + 1] = src[1];
+  }
+  for (int p = t; p < nm; p += 256) {
+    const int n = nmin + p;
+    an[p] = sqrt((2. * n + 1.) / (n * (n + 1.)));
+  }
+  cdbl *Xo = X + (int64_t)ii * spec.n_pairs * 4;
 
   for (int c0 = 0; c0 < spec.n_in; c0 += kInChunk) {
     const int nc = min(kInChunk, spec.n_in - c0);
     __syncthreads();
-    // Y for the chunk: outputs (ic, bb, row), row = h * nm + p
-    const int nout = nc * 2 * 2 * nm;
-    for (int o = t; o < nout; o += 256) {
-      const int row = o % (2 * nm);
-      const int bb = (o / (2 * nm)) & 1;
-      const int ic = o / (4 * nm);
-      const int a_in = spec.in_angle[c0 + ic];
-      const int h = row / nm, p = row - (row / nm) * nm;
-      const int n = nmin + p;
-      const int cls = (h + n) & 1;
-      const cdbl *Trow = T + (int64_t)cls * nm2 + (int64_t)p * nm;
-      cdbl acc = cmk(0., 0.);
-      for (int q = 0; q < nm; ++q) {
-        const int nn = nmin + q;
-        const int h2 = (cls + nn) & 1;
-        // b = (pi, tau) over h2; b' = (tau, pi)
-        const int which = (h2 ^ bb) & 1;  // 0 -> pi, 1 -> tau
-        const double bv = ptv(a_in, nn, which);
-        if (bv == 0.) continue;
-        const cdbl cm = cmul(cfac(nn, n), Trow[q]);
-        acc.x = fma(cm.x, bv, acc.x);
-        acc.y = fma(cm.y, bv, acc.y);
+    for (int e = t; e < kInChunk * nm; e += 256) {
+      const int ic = e / nm, p = e - (e / nm) * nm;
+      double bp = 0., bt = 0.;
+      if (ic < nc) {
+        const double *src =
+            PTm +
+            ((int64_t)spec.in_angle[c0 + ic] * nmax_all + (nmin + p - 1)) * 2;
+        bp = src[0];
+        bt = src[1];
       }
-      Ysh[(ic * 2 + bb) * 2 * nm + row] = acc;
+      Bv[(ic * nm + p) * 2] = bp;
+      Bv[(ic * nm + p) * 2 + 1] = bt;
     }
     __syncthreads();
-    // pair contractions for pairs whose incoming direction is in the chunk
-    for (int e = t; e < spec.n_pairs * 4; e += 256) {
-      const int pr = e >> 2, k = e & 3;
-      const int ii = spec.pair_in[pr] - c0;
-      if (ii < 0 || ii >= nc) continue;
-      const int a_out = spec.out_angle[spec.pair_out[pr]];
-      // X^0 = a . Y_b, X^3 = a' . Y_b', X^1 = a . Y_b', X^2 = -a' . Y_b
-      const int bb = (k == 1 || k == 3) ? 1 : 0;
-      const int swap = (k == 2 || k == 3) ? 1 : 0;
-      const cdbl *Y = Ysh + (ii * 2 + bb) * 2 * nm;
-      cdbl acc = cmk(0., 0.);
-      for (int row = 0; row < 2 * nm; ++row) {
-        const int h = row / nm;
-        const int n = nmin + row - h * nm;
-        const double av = ptv(a_out, n, (h ^ swap) & 1);
-        acc.x = fma(av, Y[row].x, acc.x);
-        acc.y = fma(av, Y[row].y, acc.y);
+    for (int row = t; row < n2m; row += 256) {
+      const int h = row >= nm ? 1 : 0;
+      const int p = row - h * nm;
+      const int n = nmin + p;
+      const double a_n = an[p];
+      cdbl acc[kInChunk][2];
+#pragma unroll
+      for (int ic = 0; ic < kInChunk; ++ic) {
+        acc[ic][0] = cmk(0., 0.);
+        acc[ic][1] = cmk(0., 0.);
       }
-      if (k == 2) {
-        acc.x = -acc.x;
-        acc.y = -acc.y;
+      const int cls = (h + n) & 1;
+      const int nh2 = A ? 2 : 1;
+      for (int hh = 0; hh < nh2; ++hh) {
+        for (int q = 0; q < nm; ++q) {
+          const int nn = nmin + q;
+          const int h2 = A ? hh : ((cls + nn) & 1);
+          cdbl tv = A ? A[(int64_t)row * n2m + h2 * nm + q]
+                      : T[cls * nm2 + (int64_t)p * nm + q];
+          // c_{nn n} T = i^(nn - n - 1) a_nn a_n T
+          const double f = a_n * an[q];
+          const int e = (nn - n - 1) & 3;
+          cdbl cm;
+          if (e == 0) cm = cmk(f * tv.x, f * tv.y);
+          else if (e == 1) cm = cmk(-f * tv.y, f * tv.x);
+          else if (e == 2) cm = cmk(-f * tv.x, -f * tv.y);
+          else cm = cmk(f * tv.y, -f * tv.x);
+#pragma unroll
+          for (int ic = 0; ic < kInChunk; ++ic) {
+            const double bp = Bv[(ic * nm + q) * 2];
+            const double bt = Bv[(ic * nm + q) * 2 + 1];
+            // b = (pi, tau) over h2, b' = (tau, pi)
+            const double b0 = h2 ? bt : bp;
+            const double b1 = h2 ? bp : bt;
+            rfma(acc[ic][0], b0, cm);
+            rfma(acc[ic][1], b1, cm);
+          }
+        }
       }
-      Xo[(int64_t)pr * 4 + k] = acc;
+#pragma unroll
+      for (int ic = 0; ic < kInChunk; ++ic) {
+        Ysh[(ic * 2 + 0) * n2m + row] = acc[ic][0];
+        Ysh[(ic * 2 + 1) * n2m + row] = acc[ic][1];
+      }
+    }
+    __syncthreads();
+    // pair contractions: X^0 = a.Y_b, X^1 = a.Y_b', X^2 = -a'.Y_b,
+    // X^3 = a'.Y_b' with a = (pi, tau), a' = (tau, pi)
+    for (int pr = t; pr < spec.n_pairs; pr += 256) {
+      const int ic = spec.pair_in[pr] - c0;
+      if (ic < 0 || ic >= nc) continue;
+      const double *ao = PTo + (int64_t)spec.pair_out[pr] * nm * 2;
+      const cdbl *Yb = Ysh + (ic * 2 + 0) * n2m;
+      const cdbl *Ybp = Ysh + (ic * 2 + 1) * n2m;
+      cdbl x0 = cmk(0., 0.), x1 = cmk(0., 0.), x2 = cmk(0., 0.),
+           x3 = cmk(0., 0.);
+      for (int row = 0; row < n2m; ++row) {
+        const int h = row >= nm ? 1 : 0;
+        const int p = row - h * nm;
+        const double pi = ao[2 * p], ta = ao[2 * p + 1];
+        const double a = h ? ta : pi, as = h ? pi : ta;
+        rfma(x0, a, Yb[row]);
+        rfma(x1, a, Ybp[row]);
+        rfma(x2, as, Yb[row]);
+        rfma(x3, as, Ybp[row]);
+      }
+      Xo[(int64_t)pr * 4 + 0] = x0;
+      Xo[(int64_t)pr * 4 + 1] = x1;
+      Xo[(int64_t)pr * 4 + 2] = cmk(-x2.x, -x2.y);
+      Xo[(int64_t)pr * 4 + 3] = x3;
     }
   }
 }
@@ -349,17 +406,30 @@ hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
   return hipGetLastError();
 }
 
-size_t obs_x_lds_bytes(int nm_max) {
-  return (size_t)kInChunk * 2 * 2 * nm_max * sizeof(cdbl);
+size_t obs_x_lds_bytes(int nm_cap, int n_out) {
+  return (size_t)kInChunk * 2 * 2 * nm_cap * sizeof(cdbl) +
+         (size_t)n_out * nm_cap * 2 * sizeof(double) +
+         (size_t)kInChunk * nm_cap * 2 * sizeof(double) +
+         (size_t)nm_cap * sizeof(double);
 }
 
-hipError_t launch_obs_x(const ObsSpec &spec, int n_items, int nm_max,
-                        const Item *items, const cdbl *tbuf, const double *PT,
-                        int nmax_all, cdbl *X, hipStream_t s) {
+hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
+                        int n_items, int nm_cap, const Item *items,
+                        const cdbl *tbuf, const cdbl *abuf,
+                        const int64_t *aoff, const double *PT, int nmax_all,
+                        cdbl *X, hipStream_t s) {
   if (n_items <= 0) return hipSuccess;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(k_obs_x),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
   hipLaunchKernelGGL(k_obs_x, dim3(n_items), dim3(256),
-                     obs_x_lds_bytes(nm_max), s, spec, items, tbuf, PT,
-                     nmax_all, X);
+                     obs_x_lds_bytes(nm_cap, spec.n_out), s, spec, item_list,
+                     items, tbuf, abuf, aoff, PT, nmax_all, nm_cap, X);
   return hipGetLastError();
 }
 

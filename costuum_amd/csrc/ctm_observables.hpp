@@ -27,10 +27,14 @@ struct ObsSpec {
 
 hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
                         hipStream_t s);
-size_t obs_x_lds_bytes(int nm_max);
-hipError_t launch_obs_x(const ObsSpec &spec, int n_items, int nm_max,
-                        const Item *items, const cdbl *tbuf, const double *PT,
-                        int nmax_all, cdbl *X, hipStream_t s);
+size_t obs_x_lds_bytes(int nm_cap, int n_out);
+// item_list: items to process (largest first); abuf/aoff (may be null):
+// full-layout ensemble T-matrices per item.
+hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
+                        int n_items, int nm_cap, const Item *items,
+                        const cdbl *tbuf, const cdbl *abuf,
+                        const int64_t *aoff, const double *PT, int nmax_all,
+                        cdbl *X, hipStream_t s);
 hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,
                         const int32_t *gi_off, const GrainParams &gp,
                         const cdbl *X, double *qext, double *qextpol,

@@ -126,6 +126,27 @@ class BatchResult:
             status.ctypes.data), "ctm_batch_orders")
         return nmax, ngauss, status
 
+    def average(self, pN, dist_of_grain=None):
+        """Ensemble-average the batch's T-matrices over orientation
+        distribution(s) with Legendre coefficients pN ([n_dist][ncoef]);
+        later observables use the averages."""
+        pN = np.ascontiguousarray(np.atleast_2d(pN), dtype=np.float64)
+        d = None
+        if dist_of_grain is not None:
+            d = np.ascontiguousarray(dist_of_grain, dtype=np.int32)
+        self.ctx.check(self.ctx._lib.ctm_orientation_average(
+            self.ctx.handle, pN.shape[0], pN.shape[1], pN.ctypes.data,
+            d.ctypes.data if d is not None else None),
+            "ctm_orientation_average")
+
+    def averaged_tmatrix(self, g):
+        L = self.ctx._lib
+        size = L.ctm_tmatrix_size(self.ctx.handle, g)
+        out = np.zeros(size, dtype=np.complex128)
+        self.ctx.check(L.ctm_average_get(self.ctx.handle, g, out.ctypes.data,
+                                         size), "ctm_average_get")
+        return out
+
     def tmatrix(self, g):
         """Grain g's T-matrix in the reference block layout (flat complex
         array; see the oracle's tmatrix_blocks to split it per m)."""

@@ -137,6 +137,21 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
                     double *qext, double *qextpol, double *qextcpol,
                     double *qabs, double *qabspol, double *Z);
 
+/* Replaces, for the observable calls, the context's T-matrices by their
+ * ensemble averages over an axially symmetric orientation distribution with
+ * Legendre coefficients p_N (Mishchenko 1991 eq. 3.27;
+ * AlignmentAverageTask.hpp:115-215, TMatrixCalculator.hpp:229-307).
+ * pN_host: n_dist rows of ncoef >= 2 nmax + 1 coefficients; dist_of_grain
+ * (host, may be NULL = all 0) selects a row per grain, as a size-based
+ * alignment distribution does (SizeBasedAlignmentDistribution.hpp:68-146).
+ * Coefficients that are exactly 0 are skipped. */
+int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
+                            const double *pN_host,
+                            const int32_t *dist_of_grain);
+/* Grain g's ensemble T-matrix in the reference block layout. */
+int ctm_average_get(ctm_context *ctx, int grain, double *T_host,
+                    size_t capacity);
+
 /* Kernel timing.  With profiling on, every launch of the library is
  * bracketed by two HIP events on the context's stream; ctm_profile_read
  * returns, per kernel, the number of launches, the summed event time and the
