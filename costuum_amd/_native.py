@@ -22,7 +22,18 @@ EXPORTS = (
     "ctm_tmatrix_size", "ctm_tmatrix_get", "ctm_batch_orders",
     "ctm_profile_enable", "ctm_profile_reset", "ctm_profile_read",
     "ctm_observables", "ctm_orientation_average", "ctm_average_get",
+    "ctm_query", "ctm_m0_sums",
 )
+
+
+class CtmQuery(ctypes.Structure):
+    _fields_ = [("grain", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("theta_in", ctypes.c_double), ("phi_in", ctypes.c_double),
+                ("theta_out", ctypes.c_double), ("phi_out", ctypes.c_double)]
+
+
+QUERY_OUT = 40
 
 KERNELS = ("gauss_legendre", "wigner", "tables", "assemble", "solve",
            "converge", "sums", "obs_x", "obs_sum", "average")
@@ -106,5 +117,10 @@ def lib():
     L.ctm_orientation_average.restype = ctypes.c_int
     L.ctm_average_get.argtypes = [vp, ctypes.c_int, dp, ctypes.c_size_t]
     L.ctm_average_get.restype = ctypes.c_int
+    L.ctm_query.argtypes = [vp, ctypes.c_int, ctypes.POINTER(CtmQuery), dp]
+    L.ctm_query.restype = ctypes.c_int
+    L.ctm_m0_sums.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp,
+                              dp, dp]
+    L.ctm_m0_sums.restype = ctypes.c_int
     _lib = L
     return L

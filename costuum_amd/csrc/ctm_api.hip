@@ -239,7 +239,7 @@ struct ctm_context {
       d_tiles, d_solve, d_q, d_t, d_qparts, d_state, d_active, d_winoff,
       d_winitems, d_gioff, d_gitems, d_gioff_tmp, d_params, d_ang, d_pt,
       d_x, d_spec, d_cgkeys, d_cg, d_cgidx, d_avgtasks, d_avg, d_aoff, d_pn,
-      d_dist, d_obsitems;
+      d_dist, d_obsitems, d_query, d_qout;
   // Results of the last batch (host).
   int n_grains = 0;
   std::vector<uint32_t> nmax, ngauss;
@@ -511,7 +511,7 @@ int ctm_destroy(ctm_context *ctx) {
                     &ctx->d_pt, &ctx->d_x, &ctx->d_spec, &ctx->d_cgkeys,
                     &ctx->d_cg, &ctx->d_cgidx, &ctx->d_avgtasks, &ctx->d_avg,
                     &ctx->d_aoff, &ctx->d_pn, &ctx->d_dist,
-                    &ctx->d_obsitems};
+                    &ctx->d_obsitems, &ctx->d_query, &ctx->d_qout};
   for (DevBuf *b : bufs) b->release();
   ctx->prof.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1152,6 +1152,98 @@ int ctm_average_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
   CTM_HIP(hipMemcpyAsync(T_host, ctx->d_avg.as<cdbl>() + o, need * sizeof(cdbl),
                          hipMemcpyDeviceToHost, ctx->stream));
   CTM_HIP(hipStreamSynchronize(ctx->stream));
+  return CTM_OK;
+}
+
+int ctm_query(ctm_context *ctx, int n, const ctm_query_spec *q,
+              double *out_host) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (ctx->final_items.empty()) {
+    ctx->err = "no T-matrices in the context";
+    return CTM_ERR_STATE;
+  }
+  if (n < 0 || (n > 0 && (!q || !out_host))) {
+    ctx->err = "invalid query arrays";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (n == 0) return CTM_OK;
+  std::vector<Query> qs(n);
+  for (int i = 0; i < n; ++i) {
+    if (q[i].grain < 0 || q[i].grain >= ctx->n_grains) {
+      ctx->err = "query grain out of range";
+      return CTM_ERR_ARGUMENT;
+    }
+    if ((int)ctx->nmax[q[i].grain] >= kMaxQueryOrder) {
+      ctx->err = "order too large for the query kernel";
+      return CTM_ERR_ARGUMENT;
+    }
+    qs[i] = Query{q[i].grain, 0, q[i].alpha, q[i].beta, q[i].theta_in,
+                  q[i].phi_in, q[i].theta_out, q[i].phi_out};
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  CTM_HIP(upload(ctx->d_query, qs, s));
+  CTM_HIP(ctx->d_qout.ensure((size_t)n * kQueryOut * sizeof(double)));
+  CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
+             launch_query(ctx->d_query.as<Query>(), n, ctx->d_items.as<Item>(),
+                          ctx->d_gioff.as<int32_t>(), ctx->gp_saved,
+                          ctx->d_t.as<cdbl>(),
+                          ctx->averaged ? ctx->d_avg.as<cdbl>() : nullptr,
+                          ctx->averaged ? ctx->d_aoff.as<int64_t>() : nullptr,
+                          ctx->d_qout.as<double>(), s));
+  CTM_HIP(hipMemcpyAsync(out_host, ctx->d_qout.p,
+                         (size_t)n * kQueryOut * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  CTM_HIP(hipStreamSynchronize(s));
+  return CTM_OK;
+}
+
+int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
+                const double *axis_ratio, const double *wavelength,
+                const double *mr_re, const double *mr_im,
+                const uint32_t *nmax_host, const uint32_t *ngauss_host,
+                double *qsca_host, double *qext_host) {
+  int rc = check_inputs(ctx, n, radius, axis_ratio, wavelength, mr_re, mr_im);
+  if (rc != CTM_OK) return rc;
+  if (n > 0 && (!nmax_host || !ngauss_host || !qsca_host || !qext_host)) {
+    ctx->err = "missing candidate arrays";
+    return CTM_ERR_ARGUMENT;
+  }
+  for (int g = 0; g < n; ++g)
+    if (nmax_host[g] < 1 || (int)nmax_host[g] > kMaxLdsOrder ||
+        ngauss_host[g] < 1) {
+      ctx->err = "candidate order out of range (1..99) or ngauss < 1";
+      return CTM_ERR_ARGUMENT;
+    }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  CTM_HIP(hipStreamSynchronize(s));
+  // the work lists below replace the batch's items
+  ctx->final_items.clear();
+  ctx->averaged = false;
+  ctx->n_grains = 0;
+  ctx->shared.clear();
+  if (n == 0) return CTM_OK;
+  GrainParams gp{radius, axis_ratio, wavelength, mr_re, mr_im};
+  WorkList &w = ctx->work;
+  w.clear();
+  for (int g = 0; g < n; ++g) {
+    const int N = (int)nmax_host[g], G = (int)ngauss_host[g];
+    const int set = w.add_set(g, N, G, ctx->shared.gl_keys[ctx->shared.gl(G)].off);
+    w.add_item(g, N, 0, G, set, ctx->shared.wig(N, 0, G), false);
+  }
+  rc = run_work(ctx, gp, false);
+  if (rc != CTM_OK) return rc;
+  std::vector<double> qp((size_t)n * 2 * kQParts);
+  CTM_HIP(hipMemcpyAsync(qp.data(), ctx->d_qparts.p, qp.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  CTM_HIP(hipStreamSynchronize(s));
+  for (int g = 0; g < n; ++g) {
+    const double *p0 = qp.data() + (size_t)(2 * g) * kQParts;
+    const double *p1 = p0 + kQParts;
+    qsca_host[g] = p0[0] + p1[0];
+    qext_host[g] = p0[1] + p1[1];
+  }
   return CTM_OK;
 }
 

@@ -456,3 +456,261 @@ hipError_t launch_obs_z(const ObsSpec &spec, int n_grains,
 }
 
 }  // namespace ctm
+
+namespace ctm {
+
+// K9: S, Z and K for arbitrary (alpha, beta) particle orientation and
+// laboratory directions (TMatrix.hpp:1109-1229,1306-1482,1891-2010): the
+// directions are rotated to the particle frame (B, A_L, A_P matrices), S is
+// summed over m in the particle frame, then rotated back.  One workgroup
+// per query: m runs sequentially, the (n, n') double sum of each m is spread
+// over the threads.
+__global__ __launch_bounds__(256) void k_query(const Query *queries,
+                                               const Item *items,
+                                               const int32_t *gi_off,
+                                               const GrainParams gp,
+                                               const cdbl *tbuf,
+                                               const cdbl *abuf,
+                                               const int64_t *aoff,
+                                               double *out) {
+  __shared__ double geo[32];
+  __shared__ double ptab[4][kMaxQueryOrder];
+  __shared__ double red[4][8][2];
+  const Query q = queries[blockIdx.x];
+  const int g = q.grain;
+  const int i0 = gi_off[g];
+  const int N = items[i0].N;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const double ca = cos(q.alpha), sa = sin(q.alpha);
+    const double cb = cos(q.beta), sb = sin(q.beta);
+    const double cti = cos(q.theta_in), sti = sin(q.theta_in);
+    const double cto = cos(q.theta_out), sto = sin(q.theta_out);
+    const double cpi = cos(q.phi_in), spi = sin(q.phi_in);
+    const double cpo = cos(q.phi_out), spo = sin(q.phi_out);
+    double pf[2][6];  // per direction: cos th, sin th, 1/sin, cos ph, sin ph
+    double AL[2][3][2], AP[2][2][3];
+    for (int d = 0; d < 2; ++d) {
+      const double ct = d ? cto : cti, st = d ? sto : sti;
+      const double cp = d ? cpo : cpi, sp = d ? spo : spi;
+      const double cpr = cp * ca + sp * sa;
+      const double spr = sp * ca - cp * sa;
+      const double ctp = ct * cb + st * sb * cpr;
+      const double stp = sin(acos(ctp));
+      double cpp, spp, stpi;
+      if (stp != 0.) {
+        stpi = 1. / stp;
+        cpp = (cb * st * cpr - sb * ct) * stpi;
+        spp = st * spr * stpi;
+      } else {
+        cpp = 1.;
+        spp = 0.;
+        stpi = 9000.;
+      }
+      pf[d][0] = ctp; pf[d][1] = stp; pf[d][2] = stpi;
+      pf[d][3] = cpp; pf[d][4] = spp;
+      AL[d][0][0] = ct * cp; AL[d][0][1] = -sp;
+      AL[d][1][0] = ct * sp; AL[d][1][1] = cp;
+      AL[d][2][0] = -st;     AL[d][2][1] = 0.;
+      AP[d][0][0] = ctp * cpp; AP[d][0][1] = ctp * spp; AP[d][0][2] = -stp;
+      AP[d][1][0] = -spp;      AP[d][1][1] = cpp;       AP[d][1][2] = 0.;
+    }
+    const double B[3][3] = {{ca * cb, sa * cb, -sb},
+                            {-sa, ca, 0.},
+                            {ca * sb, sa * sb, cb}};
+    double R[2][2][2];
+    for (int d = 0; d < 2; ++d) {
+      double C[3][2];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 2; ++j) {
+          double s = 0.;
+          for (int k = 0; k < 3; ++k) s += B[i][k] * AL[d][k][j];
+          C[i][j] = s;
+        }
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+          double s = 0.;
+          for (int k = 0; k < 3; ++k) s += AP[d][i][k] * C[k][j];
+          R[d][i][j] = s;
+        }
+    }
+    const double det = 1. / (R[1][0][0] * R[1][1][1] - R[1][0][1] * R[1][1][0]);
+    const double tmp = R[1][0][0];
+    R[1][0][0] = R[1][1][1] * det;
+    R[1][0][1] = -R[1][0][1] * det;
+    R[1][1][0] = -R[1][1][0] * det;
+    R[1][1][1] = tmp * det;
+    for (int d = 0; d < 2; ++d)
+      for (int i = 0; i < 5; ++i) geo[d * 5 + i] = pf[d][i];
+    for (int d = 0; d < 2; ++d)
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) geo[10 + d * 4 + i * 2 + j] = R[d][i][j];
+  }
+  __syncthreads();
+  // e^{i(phi_out - phi_in)} in the particle frame
+  const double cpi = geo[3], spi = geo[4], cpo = geo[8], spo = geo[9];
+  const cdbl step = cmk(cpo * cpi + spo * spi, spo * cpi - cpo * spi);
+  cdbl ex = cmk(1., 0.);
+  cdbl S[4] = {cmk(0., 0.), cmk(0., 0.), cmk(0., 0.), cmk(0., 0.)};
+  for (int m = 0; m <= N; ++m) {
+    const int nmin = m > 0 ? m : 1;
+    const int nm = N + 1 - nmin;
+    __syncthreads();
+    if (t < 2) {
+      const int d = t;  // 0: incoming, 1: outgoing
+      pitau_dev(geo[d * 5], geo[d * 5 + 1], geo[d * 5 + 2], N, m,
+                &ptab[2 * d][0], &ptab[2 * d + 1][0], 1);
+    }
+    __syncthreads();
+    const double fcos = (m == 0) ? 1. : 2. * ex.x;
+    const double fsin = (m == 0) ? 0. : 2. * ex.y;
+    ex = cmul(ex, step);
+    const Item &it = items[i0 + m];
+    const int64_t nm2 = (int64_t)nm * nm;
+    const cdbl *T = tbuf + it.t_off;
+    const cdbl *A = abuf ? abuf + aoff[i0 + m] : nullptr;
+    for (int e = t; e < nm * nm; e += 256) {
+      const int pn = e / nm, pnn = e - (e / nm) * nm;
+      const int n = nmin + pn, nn = nmin + pnn;
+      const double pin = ptab[0][nn - 1], tin = ptab[1][nn - 1];
+      const double pout = ptab[2][n - 1], tout = ptab[3][n - 1];
+      // T(h n; h' nn) for the four half blocks
+      cdbl Th[2][2];
+      for (int h = 0; h < 2; ++h)
+        for (int h2 = 0; h2 < 2; ++h2) {
+          if (A) {
+            Th[h][h2] = A[(int64_t)(h * nm + pn) * 2 * nm + h2 * nm + pnn];
+          } else {
+            const int c1 = (h + n) & 1, c2 = (h2 + nn) & 1;
+            Th[h][h2] = (c1 == c2) ? T[c1 * nm2 + (int64_t)pn * nm + pnn]
+                                   : cmk(0., 0.);
+          }
+        }
+      // c_{nn n} = i^(nn - n - 1) a_nn a_n
+      const double a = sqrt((2. * nn + 1.) / (nn * (nn + 1.))) *
+                       sqrt((2. * n + 1.) / (n * (n + 1.)));
+      const int ph = (nn - n - 1) & 3;
+      const cdbl c = ph == 0 ? cmk(a, 0.) : ph == 1 ? cmk(0., a)
+                   : ph == 2 ? cmk(-a, 0.) : cmk(0., -a);
+      if (m == 0) {
+        const cdbl f = cscale(c, tout * tin);
+        cfma(S[0], f, Th[1][1]);
+        cfma(S[3], f, Th[0][0]);
+      } else {
+        const double pp = pout * pin, pt = pout * tin, tp = tout * pin,
+                     tt = tout * tin;
+        // T11 = Th[0][0], T12 = Th[0][1], T21 = Th[1][0], T22 = Th[1][1]
+        cdbl s0 = cadd(cadd(cscale(Th[0][0], pp), cscale(Th[1][0], tp)),
+                       cadd(cscale(Th[0][1], pt), cscale(Th[1][1], tt)));
+        cdbl s1 = cadd(cadd(cscale(Th[0][0], pt), cscale(Th[1][0], tt)),
+                       cadd(cscale(Th[0][1], pp), cscale(Th[1][1], tp)));
+        cdbl s2 = cadd(cadd(cscale(Th[0][0], tp), cscale(Th[1][0], pp)),
+                       cadd(cscale(Th[0][1], tt), cscale(Th[1][1], pt)));
+        cdbl s3 = cadd(cadd(cscale(Th[0][0], tt), cscale(Th[1][0], pt)),
+                       cadd(cscale(Th[0][1], tp), cscale(Th[1][1], pp)));
+        cfma(S[0], cscale(c, fcos), s0);
+        cfma(S[1], cscale(c, fsin), s1);
+        cfma(S[2], cscale(c, -fsin), s2);
+        cfma(S[3], cscale(c, fcos), s3);
+      }
+    }
+  }
+  // block reduction of S
+  for (int k = 0; k < 4; ++k) {
+    double x = S[k].x, y = S[k].y;
+    for (int o = 32; o > 0; o >>= 1) {
+      x += __shfl_xor(x, o);
+      y += __shfl_xor(y, o);
+    }
+    if ((t & 63) == 0) {
+      red[k][t >> 6][0] = x;
+      red[k][t >> 6][1] = y;
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    const double kinv = gp.wavelength[g] / (2. * M_PI);
+    cdbl s[4];
+    for (int k = 0; k < 4; ++k) {
+      double x = 0., y = 0.;
+      for (int w = 0; w < 4; ++w) {
+        x += red[k][w][0];
+        y += red[k][w][1];
+      }
+      s[k] = cmk(x * kinv, y * kinv);
+    }
+    const double *Ri = geo + 10, *Ro = geo + 14;
+    // S' = R_out^-1 S R_in
+    const cdbl c11 = cadd(cscale(s[0], Ri[0]), cscale(s[1], Ri[2]));
+    const cdbl c12 = cadd(cscale(s[0], Ri[1]), cscale(s[1], Ri[3]));
+    const cdbl c21 = cadd(cscale(s[2], Ri[0]), cscale(s[3], Ri[2]));
+    const cdbl c22 = cadd(cscale(s[2], Ri[1]), cscale(s[3], Ri[3]));
+    cdbl F[4];
+    F[0] = cadd(cscale(c11, Ro[0]), cscale(c21, Ro[1]));
+    F[1] = cadd(cscale(c12, Ro[0]), cscale(c22, Ro[1]));
+    F[2] = cadd(cscale(c11, Ro[2]), cscale(c21, Ro[3]));
+    F[3] = cadd(cscale(c12, Ro[2]), cscale(c22, Ro[3]));
+    double *o = out + (int64_t)blockIdx.x * kQueryOut;
+    for (int k = 0; k < 4; ++k) {
+      o[2 * k] = F[k].x;
+      o[2 * k + 1] = F[k].y;
+    }
+    auto pc = [&](int a, int b) {
+      return cmk(F[a].x * F[b].x + F[a].y * F[b].y,
+                 F[a].y * F[b].x - F[a].x * F[b].y);
+    };
+    const cdbl s00 = pc(0, 0), s11 = pc(1, 1), s22 = pc(2, 2), s33 = pc(3, 3);
+    const cdbl s01 = pc(0, 1), s32 = pc(3, 2), s02 = pc(0, 2), s31 = pc(3, 1);
+    const cdbl s03 = pc(0, 3), s12 = pc(1, 2), s21 = pc(2, 1), s20 = pc(2, 0);
+    const cdbl s30 = pc(3, 0);
+    double *z = o + 8;
+    z[0] = 0.5 * (s00.x + s11.x + s22.x + s33.x);
+    z[1] = 0.5 * (s00.x - s11.x + s22.x - s33.x);
+    z[2] = -s01.x - s32.x;
+    z[3] = -(s01.y - s32.y);
+    z[4] = 0.5 * (s00.x + s11.x - s22.x - s33.x);
+    z[5] = 0.5 * (s00.x - s11.x - s22.x + s33.x);
+    z[6] = -s01.x + s32.x;
+    z[7] = -(s01.y + s32.y);
+    z[8] = -s02.x - s31.x;
+    z[9] = -s02.x + s31.x;
+    z[10] = s03.x + s12.x;
+    z[11] = (s03.y + s21.y);
+    z[12] = -(s20.y + s31.y);
+    z[13] = -(s20.y - s31.y);
+    z[14] = (s30.y - s12.y);
+    z[15] = s30.x - s12.x;
+    // extinction matrix (TMatrix.hpp:1984-2007) with S of the same
+    // direction pair
+    const double pre = 2. * M_PI / (2. * M_PI / gp.wavelength[g]);
+    double *K = o + 24;
+    K[0] = pre * (F[0].y + F[3].y);
+    K[5] = K[0];
+    K[10] = K[0];
+    K[15] = K[0];
+    K[1] = pre * (F[0].y - F[3].y);
+    K[4] = K[1];
+    K[2] = -pre * (F[1].y + F[2].y);
+    K[8] = K[2];
+    K[3] = pre * (F[2].x - F[1].x);
+    K[12] = K[3];
+    K[6] = pre * (F[2].y - F[1].y);
+    K[9] = -K[6];
+    K[7] = -pre * (F[1].x + F[2].x);
+    K[13] = -K[7];
+    K[11] = pre * (F[3].x - F[0].x);
+    K[14] = -K[11];
+  }
+}
+
+hipError_t launch_query(const Query *q, int n, const Item *items,
+                        const int32_t *gi_off, const GrainParams &gp,
+                        const cdbl *tbuf, const cdbl *abuf,
+                        const int64_t *aoff, double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_query, dim3(n), dim3(256), 0, s, q, items, gi_off, gp,
+                     tbuf, abuf, aoff, out);
+  return hipGetLastError();
+}
+
+}  // namespace ctm

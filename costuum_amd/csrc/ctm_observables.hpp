@@ -25,6 +25,20 @@ struct ObsSpec {
   const double *sphi_cos, *sphi_sin;        // [n_sphi] scattering azimuths
 };
 
+// An S/Z/K query for one grain in an arbitrary orientation (radians).
+struct Query {
+  int32_t grain, pad;
+  double alpha, beta, theta_in, phi_in, theta_out, phi_out;
+};
+// Per query: S (4 complex, row-major 2x2), Z (16), K (16) doubles.
+constexpr int kQueryOut = 8 + 16 + 16;
+constexpr int kMaxQueryOrder = 100;
+
+hipError_t launch_query(const Query *q, int n, const Item *items,
+                        const int32_t *gi_off, const GrainParams &gp,
+                        const cdbl *tbuf, const cdbl *abuf,
+                        const int64_t *aoff, double *out, hipStream_t s);
+
 hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
                         hipStream_t s);
 size_t obs_x_lds_bytes(int nm_cap, int n_out);

@@ -243,3 +243,51 @@ def tmatrix_fixed(radius, axis_ratio, wavelength, mr_re, mr_im, nmax, ngauss,
     ctx.synchronize()
     return BatchResult(ctx, n, torch.as_tensor(nm.astype(np.int32)), None,
                        qsca, qext, ngauss=ng)
+
+
+def query(ctx, grains, alpha, beta, theta_in, phi_in, theta_out, phi_out):
+    """Amplitude (S, 2x2 complex), phase (Z, 4x4) and extinction (K, 4x4)
+    matrices of the context's T-matrices (the averaged ones after
+    ``BatchResult.average``) for particles at orientation (alpha, beta) and
+    the given laboratory incident/scattered directions
+    (TMatrix.hpp:1246-1482,1891-2010).  All arguments broadcast; returns
+    (S [..., 2, 2], Z [..., 4, 4], K [..., 4, 4])."""
+    b = np.broadcast_arrays(np.asarray(grains, dtype=np.int64), alpha, beta,
+                            theta_in, phi_in, theta_out, phi_out)
+    shape = b[0].shape
+    n = b[0].size
+    qs = (_native.CtmQuery * max(n, 1))()
+    flat = [np.ravel(x) for x in b]
+    for i in range(n):
+        qs[i] = _native.CtmQuery(int(flat[0][i]), 0, *[float(x[i])
+                                                     for x in flat[1:]])
+    out = np.zeros((max(n, 1), _native.QUERY_OUT))
+    ctx.check(ctx._lib.ctm_query(ctx.handle, n, qs, out.ctypes.data),
+              "ctm_query")
+    out = out[:n]
+    S = (out[:, 0:8:2] + 1j * out[:, 1:8:2]).reshape(shape + (2, 2))
+    Z = out[:, 8:24].reshape(shape + (4, 4))
+    K = out[:, 24:40].reshape(shape + (4, 4))
+    return S, Z, K
+
+
+def m0_sums(radius, axis_ratio, wavelength, mr_re, mr_im, nmax, ngauss,
+            ctx=None):
+    """m = 0 convergence sums (sum (2n+1)|T_nn|^2, sum (2n+1) Re T_nn) of
+    candidate (nmax, ngauss) pairs, the inner quantity of
+    TMatrixCalculator::calculate_TMatrix (TMatrixCalculator.hpp:108-124).
+    Replaces the T-matrices held by the context."""
+    ctx = ctx or default_context()
+    dev = ctx.device
+    arrs = [_dev(a, dev) for a in (radius, axis_ratio, wavelength, mr_re,
+                                    mr_im)]
+    n = arrs[0].numel()
+    nm = np.ascontiguousarray(np.broadcast_to(nmax, (n,)), dtype=np.uint32)
+    ng = np.ascontiguousarray(np.broadcast_to(ngauss, (n,)), dtype=np.uint32)
+    qs = np.zeros(n)
+    qe = np.zeros(n)
+    torch.cuda.current_stream(dev).synchronize()
+    ctx.check(ctx._lib.ctm_m0_sums(
+        ctx.handle, n, *[_ptr(a) for a in arrs], nm.ctypes.data,
+        ng.ctypes.data, qs.ctypes.data, qe.ctypes.data), "ctm_m0_sums")
+    return qs, qe

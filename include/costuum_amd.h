@@ -152,6 +152,35 @@ int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
 int ctm_average_get(ctm_context *ctx, int grain, double *T_host,
                     size_t capacity);
 
+/* S (2x2 complex), Z (4x4) and K (4x4) of grain g for a particle orientation
+ * (alpha, beta) and laboratory incoming/outgoing directions, from the
+ * context's T-matrices (ensemble ones after ctm_orientation_average):
+ * TMatrix::get_forward_scattering_matrix / get_scattering_matrix /
+ * get_extinction_matrix (TMatrix.hpp:1306-1482,1891-2010).  out_host
+ * receives 40 doubles per query: S as 4 (re, im) pairs row-major, Z
+ * row-major, K row-major (K uses the S of the incoming direction, i.e. call
+ * with theta_out = theta_in, phi_out = phi_in for get_extinction_matrix). */
+typedef struct ctm_query_spec {
+  int32_t grain;
+  int32_t pad;
+  double alpha, beta, theta_in, phi_in, theta_out, phi_out;
+} ctm_query_spec;
+
+int ctm_query(ctm_context *ctx, int n, const ctm_query_spec *queries,
+              double *out_host);
+
+/* m = 0 convergence sums (sum (2n+1)|T_nn|^2, sum (2n+1) Re T_nn over the
+ * m = 0 block, TMatrixCalculator.hpp:108-124) for n candidate
+ * (grain, nmax, ngauss) triples; inputs are device arrays with one entry
+ * per candidate, nmax/ngauss and outputs host arrays.  Used for the
+ * two-phase convergence of TMatrixCalculator::calculate_TMatrix.
+ * Invalidates the T-matrices held by the context. */
+int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
+                const double *axis_ratio, const double *wavelength,
+                const double *mr_re, const double *mr_im,
+                const uint32_t *nmax_host, const uint32_t *ngauss_host,
+                double *qsca_host, double *qext_host);
+
 /* Kernel timing.  With profiling on, every launch of the library is
  * bracketed by two HIP events on the context's stream; ctm_profile_read
  * returns, per kernel, the number of launches, the summed event time and the

@@ -316,3 +316,19 @@ def batch_task_obs(rv, d, lam, mr_re, mr_im, theta, wtheta, ngauss=20,
                           nmax.ctypes.data_as(_PU), status.ctypes.data_as(_PI),
                           _p(qe), _p(qa))
     return nmax, status, qe, qa
+
+
+def scattering_query(nmax, T, wavelength, alpha, beta, th_in, ph_in, th_out,
+                     ph_out):
+    """Returns (S 2x2 complex, Z 4x4, K 4x4)."""
+    L = _olib()
+    if not getattr(L, "_sq", False):
+        L.ctmo_scattering_query.argtypes = [_U, _PD, _D, _D, _D, _D, _D, _D,
+                                            _D, _PD]
+        L._sq = True
+    T = np.ascontiguousarray(T, dtype=np.complex128)
+    out = np.zeros(40)
+    L.ctmo_scattering_query(nmax, _p(T), wavelength, alpha, beta, th_in,
+                            ph_in, th_out, ph_out, _p(out))
+    S = (out[0:8:2] + 1j * out[1:8:2]).reshape(2, 2)
+    return S, out[8:24].reshape(4, 4), out[24:40].reshape(4, 4)

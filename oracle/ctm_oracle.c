@@ -1419,3 +1419,105 @@ void ctmo_batch_task_obs(int n, const double *rv, const double *d,
   free(Ts);
   free(ng);
 }
+
+/* a26: S, Z and K for particle orientation (alpha, beta) and laboratory
+ * directions (TMatrix.hpp:1246-1482 rotations and particle-frame S,
+ * :1891-1952 Z, :1972-2010 K).  out: S (4 re/im pairs), Z[16], K[16]. */
+void ctmo_scattering_query(unsigned nmax, const double *Td, double wavelength,
+                           double alpha, double beta, double th_in,
+                           double ph_in, double th_out, double ph_out,
+                           double *out) {
+  const cplx *T = (const cplx *)Td;
+  const double k = 2. * M_PI / wavelength;
+  const double ca = cos(alpha), sa = sin(alpha), cb = cos(beta), sb = sin(beta);
+  double ctl[2] = {cos(th_in), cos(th_out)}, stl[2] = {sin(th_in), sin(th_out)};
+  double cpl[2] = {cos(ph_in), cos(ph_out)}, spl[2] = {sin(ph_in), sin(ph_out)};
+  double ctp[2], stp[2], stpi[2], cpp[2], spp[2];
+  double R[2][2][2];
+  const double B[3][3] = {{ca * cb, sa * cb, -sb}, {-sa, ca, 0.},
+                          {ca * sb, sa * sb, cb}};
+  for (int d = 0; d < 2; ++d) {
+    const double cpr = cpl[d] * ca + spl[d] * sa;
+    const double spr = spl[d] * ca - cpl[d] * sa;
+    ctp[d] = ctl[d] * cb + stl[d] * sb * cpr;
+    stp[d] = sin(acos(ctp[d]));
+    if (stp[d] != 0.) {
+      stpi[d] = 1. / stp[d];
+      cpp[d] = (cb * stl[d] * cpr - sb * ctl[d]) * stpi[d];
+      spp[d] = stl[d] * spr * stpi[d];
+    } else {
+      cpp[d] = 1.;
+      spp[d] = 0.;
+      stpi[d] = 9000.;
+    }
+    const double AL[3][2] = {{ctl[d] * cpl[d], -spl[d]},
+                             {ctl[d] * spl[d], cpl[d]},
+                             {-stl[d], 0.}};
+    const double AP[2][3] = {{ctp[d] * cpp[d], ctp[d] * spp[d], -stp[d]},
+                             {-spp[d], cpp[d], 0.}};
+    double C[3][2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 2; ++j) {
+        C[i][j] = 0.;
+        for (int q = 0; q < 3; ++q) C[i][j] += B[i][q] * AL[q][j];
+      }
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        R[d][i][j] = 0.;
+        for (int q = 0; q < 3; ++q) R[d][i][j] += AP[i][q] * C[q][j];
+      }
+  }
+  {
+    const double det = 1. / (R[1][0][0] * R[1][1][1] - R[1][0][1] * R[1][1][0]);
+    const double tmp = R[1][0][0];
+    R[1][0][0] = R[1][1][1] * det;
+    R[1][0][1] = -R[1][0][1] * det;
+    R[1][1][0] = -R[1][1][0] * det;
+    R[1][1][1] = tmp * det;
+  }
+  /* particle-frame S (TMatrix.hpp:1109-1229) */
+  const size_t tsz = (size_t)(nmax + 1) * nmax;
+  double *pin = (double *)malloc(sizeof(double) * tsz);
+  double *tin = (double *)malloc(sizeof(double) * tsz);
+  double *pout = (double *)malloc(sizeof(double) * tsz);
+  double *tout = (double *)malloc(sizeof(double) * tsz);
+  pitau(ctp[0], stp[0], stpi[0], nmax, pin, tin);
+  pitau(ctp[1], stp[1], stpi[1], nmax, pout, tout);
+  cplx S[4];
+  smatrix(T, nmax, k, pin, tin, pout, tout, cpp[1] * cpp[0] + spp[1] * spp[0],
+          spp[1] * cpp[0] - cpp[1] * spp[0], S);
+  free(pin);
+  free(tin);
+  free(pout);
+  free(tout);
+  const cplx c11 = S[0] * R[0][0][0] + S[1] * R[0][1][0];
+  const cplx c12 = S[0] * R[0][0][1] + S[1] * R[0][1][1];
+  const cplx c21 = S[2] * R[0][0][0] + S[3] * R[0][1][0];
+  const cplx c22 = S[2] * R[0][0][1] + S[3] * R[0][1][1];
+  cplx F[4];
+  F[0] = R[1][0][0] * c11 + R[1][0][1] * c21;
+  F[1] = R[1][0][0] * c12 + R[1][0][1] * c22;
+  F[2] = R[1][1][0] * c11 + R[1][1][1] * c21;
+  F[3] = R[1][1][0] * c12 + R[1][1][1] * c22;
+  for (int q = 0; q < 4; ++q) {
+    out[2 * q] = creal(F[q]);
+    out[2 * q + 1] = cimag(F[q]);
+  }
+  z_from_s(F, out + 8);
+  double *K = out + 24;
+  const double pre = 2. * M_PI / k;
+  K[0] = pre * cimag(F[0] + F[3]);
+  K[5] = K[10] = K[15] = K[0];
+  K[1] = pre * cimag(F[0] - F[3]);
+  K[4] = K[1];
+  K[2] = -pre * cimag(F[1] + F[2]);
+  K[8] = K[2];
+  K[3] = pre * creal(F[2] - F[1]);
+  K[12] = K[3];
+  K[6] = pre * cimag(F[2] - F[1]);
+  K[9] = -K[6];
+  K[7] = -pre * creal(F[1] + F[2]);
+  K[13] = -K[7];
+  K[11] = pre * creal(F[3] - F[0]);
+  K[14] = -K[11];
+}

@@ -135,6 +135,12 @@ void ctmo_absorption(unsigned nmax, const double *T, double wavelength,
 /* a25: ScatteringMatrixTask.hpp:698-751 on its (theta, phi) grid. */
 void ctmo_scattering_grid(unsigned nmax, const double *T, double wavelength,
                           unsigned ntheta, int use_gl, double *Z);
+/* a26: S, Z, K at particle orientation (alpha, beta) for laboratory
+ * directions (TMatrix.hpp:1246-1482,1891-2010); out gets 40 doubles. */
+void ctmo_scattering_query(unsigned nmax, const double *T, double wavelength,
+                           double alpha, double beta, double th_in,
+                           double ph_in, double th_out, double ph_out,
+                           double *out);
 /* a19: p_N of an orientation distribution (kind 0 isotropic, 1 Mishchenko
  * with <cos^2 beta> = param, 2 Davis-Greenstein with axis ratio param). */
 void ctmo_orientation_coefficients(int kind, double param, unsigned nmax,

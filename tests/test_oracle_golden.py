@@ -64,3 +64,20 @@ def test_planted_mistake_is_caught():
     blocks = O.tmatrix_blocks(T, n)
     qe_wrong = sum(np.trace(b).real for b in blocks)  # no f_m, no 12/21
     assert rel(qe_wrong, r[15]) > 1e-2
+
+
+@pytest.mark.parametrize("row", list(range(0, 101, 10)))
+def test_scattering_matrix_matches_fixture(row):
+    """Z at the fixture's orientation and angles, in um^2, to the
+    reference's 2e-2 relative bar (testTMatrixCalculator.cpp:115-133)."""
+    r = ROWS[row]
+    axi, rat, lam, mrr, mri, eps, ddelt, ndgs = r[:8]
+    alpha, beta, th0, th, ph0, ph = np.radians(r[8:14])
+    mr = complex(mrr, mri)
+    st, n, g, rv = O.converge_calculator(rat, eps, axi, lam, mr, 200,
+                                         ddelt * 0.1, int(ndgs), 500)
+    T = O.tmatrix_all(rv, eps, lam, mr, n, g)
+    S, Z, K = O.scattering_query(n, T, lam, alpha, beta, th0, ph0, th, ph)
+    Zref = r[17:33].reshape(4, 4)
+    err = np.abs(Z - Zref) / np.abs(Z + Zref)
+    assert err.max() < 2e-2, err.max()
