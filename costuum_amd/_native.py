@@ -22,8 +22,13 @@ EXPORTS = (
     "ctm_tmatrix_size", "ctm_tmatrix_get", "ctm_batch_orders",
     "ctm_profile_enable", "ctm_profile_reset", "ctm_profile_read",
     "ctm_observables", "ctm_orientation_average", "ctm_average_get",
-    "ctm_query", "ctm_m0_sums",
+    "ctm_query", "ctm_m0_sums", "ctm_gauss_legendre", "ctm_rv_over_rs",
+    "ctm_adaptive_quadrature", "ctm_orientation_coefficients",
+    "ctm_weighted_reduce",
 )
+
+# double f(double x, void *arg): the host callback type (ctm_real_fn)
+REAL_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
 
 
 class CtmQuery(ctypes.Structure):
@@ -122,5 +127,24 @@ def lib():
     L.ctm_m0_sums.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp,
                               dp, dp]
     L.ctm_m0_sums.restype = ctypes.c_int
+    u32 = ctypes.c_uint32
+    L.ctm_weighted_reduce.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int64, dp, dp, ctypes.c_int,
+                                      dp]
+    L.ctm_weighted_reduce.restype = ctypes.c_int
+    L.ctm_gauss_legendre.argtypes = [u32, ctypes.c_double, ctypes.c_double,
+                                     dp, dp]
+    L.ctm_gauss_legendre.restype = ctypes.c_int
+    L.ctm_rv_over_rs.argtypes = [ctypes.c_double]
+    L.ctm_rv_over_rs.restype = ctypes.c_double
+    L.ctm_adaptive_quadrature.argtypes = [REAL_FN, vp, ctypes.c_double,
+                                          ctypes.c_double, u32, u32,
+                                          ctypes.c_double, ctypes.c_double,
+                                          dp]
+    L.ctm_adaptive_quadrature.restype = ctypes.c_int
+    L.ctm_orientation_coefficients.argtypes = [REAL_FN, vp, u32,
+                                               ctypes.c_double,
+                                               ctypes.c_double, dp]
+    L.ctm_orientation_coefficients.restype = ctypes.c_int
     _lib = L
     return L

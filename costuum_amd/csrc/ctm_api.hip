@@ -32,6 +32,7 @@
 #include "ctm_kernels.hpp"
 #include "ctm_observables.hpp"
 #include "ctm_average.hpp"
+#include "ctm_host.hpp"
 
 using namespace ctm;
 
@@ -800,39 +801,6 @@ int ctm_tmatrix_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
 
 // Gauss-Legendre rule on [-1, 1] (SpecialFunctions.hpp:647-715), host side,
 // for the observable direction grids.
-static void gauss_legendre_host(unsigned order, double *x, double *w) {
-  const unsigned odd = order % 2u, half = order / 2u + odd;
-  const double inv = 1.0 / order, inv2 = inv * inv;
-  for (unsigned i = 0; i < half; ++i) {
-    const double t = M_PI * (i + 0.75) / (order + 0.5);
-    double xi = (1. - 0.125 * inv2 + (5. / 38.) * inv * inv2 -
-                 (2. / 25.) * inv2 * inv2 * (1. - (14. / 39.) / (t * t))) *
-                std::cos(t);
-    double tol = 1.e-16, pinv = 0., step = 0.;
-    unsigned iter = 0;
-    do {
-      ++iter;
-      if (iter > 100) tol *= 10.;
-      double p_prev = 1., p_cur = xi;
-      for (unsigned j = 1; j < order; ++j) {
-        const double p_old = p_prev;
-        p_prev = p_cur;
-        p_cur = xi * p_prev + (xi * p_prev - p_old) * j / (j + 1.);
-      }
-      pinv = 1. / ((p_prev - xi * p_cur) * order);
-      step = pinv * p_cur * (1. - xi * xi);
-      xi = xi - step;
-    } while (std::fabs(step) > tol * std::fabs(xi));
-    const unsigned k = order - i - 1;
-    x[k] = xi;
-    w[k] = 2. * pinv * pinv * (1. - xi * xi);
-    if (i != half - 1 || odd != 1) {
-      x[i] = -x[k];
-      w[i] = w[k];
-    }
-  }
-}
-
 static void push_angle(std::vector<double> &ang, double c) {
   const double sn = std::sqrt((1. - c) * (1. + c));
   ang.push_back(c);
@@ -1062,7 +1030,7 @@ int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
   }
   if (dist_of_grain) {
     for (int g = 0; g < n; ++g)
-      if (dist_of_grain[g] < 0 || dist_of_grain[g] >= n_dist) {
+      if (dist_of_grain[g] < -1 || dist_of_grain[g] >= n_dist) {
         ctx->err = "distribution index out of range";
         return CTM_ERR_ARGUMENT;
       }
@@ -1133,6 +1101,23 @@ int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
                             ctx->d_aoff.as<int64_t>(), s));
   CTM_HIP(hipStreamSynchronize(s));
   ctx->averaged = true;
+  return CTM_OK;
+}
+
+int ctm_weighted_reduce(ctm_context *ctx, int n_groups, int group,
+                        int64_t width, const double *in, const double *weights,
+                        int normalise, double *out) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (n_groups < 0 || group < 1 || width < 0 ||
+      (n_groups > 0 && width > 0 && (!in || !weights || !out))) {
+    ctx->err = "invalid reduction arguments";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  CTM_LAUNCH(CTM_KERNEL_AVERAGE, 0., 3. * 8. * n_groups * group * width,
+             launch_weighted_reduce(n_groups, group, width, in, weights,
+                                    normalise, out, s));
   return CTM_OK;
 }
 

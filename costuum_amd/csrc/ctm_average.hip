@@ -81,8 +81,28 @@ __global__ __launch_bounds__(64) void k_average(
   const int M = n1 < n2 ? n1 : n2;
   const int lo = n1 > n2 ? n1 - n2 : n2 - n1;
   const int hi = n1 + n2;
-  const double *p = pN + (int64_t)(dist ? dist[g] : 0) * ncoef;
   const int t = threadIdx.x;
+  if (dist && dist[g] < 0) {
+    // alignment disabled for this grain: the ensemble T-matrix is the
+    // grain's own (AlignmentAverageTask.hpp:195-214), written in the full
+    // (2 nm)^2 layout; pairs of different class are zero
+    for (int m = t; m <= M; m += 64) {
+      const int nmin = m > 0 ? m : 1;
+      const int nm = Ng + 1 - nmin;
+      const int64_t nm2 = (int64_t)nm * nm;
+      const cdbl *T = tbuf + items[i0 + m].t_off;
+      cdbl *A = abuf + aoff[i0 + m];
+      const int p1 = n1 - nmin, p2 = n2 - nmin;
+      for (int h1 = 0; h1 < 2; ++h1)
+        for (int h2 = 0; h2 < 2; ++h2) {
+          const int c1 = (h1 + n1) & 1, c2 = (h2 + n2) & 1;
+          A[(int64_t)(h1 * nm + p1) * (2 * nm) + h2 * nm + p2] =
+              c1 == c2 ? T[c1 * nm2 + (int64_t)p1 * nm + p2] : cmk(0., 0.);
+        }
+    }
+    return;
+  }
+  const double *p = pN + (int64_t)(dist ? dist[g] : 0) * ncoef;
   // C^N(n1 m, n2 -m) for m = 0..M lives at cg[cg_index[(n1, n2, N)] + m]
   auto cgp = [&](int N) -> const double * {
     const int idx = cg_index[((int64_t)n1 * (nmax_cg + 1) + n2) *
@@ -147,6 +167,37 @@ __global__ __launch_bounds__(64) void k_average(
       for (int h2 = 0; h2 < 2; ++h2)
         A[(int64_t)(h1 * nm + p1) * n2m + h2 * nm + p2] = acc[h1][h2];
   }
+}
+
+// One thread per output value; the weighted sum runs over the group in
+// order, as the reference's shape average does
+// (ShapeAveragingTask.hpp:127-166).  Consecutive threads read consecutive
+// values of one row, so every load is coalesced.
+__global__ __launch_bounds__(256) void k_weighted_reduce(
+    int64_t total, int group, int64_t width, const double *in,
+    const double *w, int normalise, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int64_t o = i / width, k = i - o * width;
+  const double *wg = w + o * group;
+  const double *x = in + o * group * width + k;
+  double acc = 0., norm = 0.;
+  for (int s = 0; s < group; ++s) {
+    acc += wg[s] * x[(int64_t)s * width];
+    norm += wg[s];
+  }
+  out[i] = normalise ? acc * (1. / norm) : acc;
+}
+
+hipError_t launch_weighted_reduce(int n_groups, int group, int64_t width,
+                                  const double *in, const double *w,
+                                  int normalise, double *out, hipStream_t s) {
+  const int64_t total = (int64_t)n_groups * width;
+  if (total <= 0) return hipSuccess;
+  const int64_t blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(k_weighted_reduce, dim3((unsigned)blocks), dim3(256), 0,
+                     s, total, group, width, in, w, normalise, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_cg(const CgKey *keys, int n, double *cg, hipStream_t s) {

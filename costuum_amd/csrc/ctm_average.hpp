@@ -25,4 +25,10 @@ hipError_t launch_average(const AvgTask *tasks, int n, const Item *items,
                           int nmax_cg, cdbl *abuf, const int64_t *aoff,
                           hipStream_t s);
 
+// out[o][k] = sum_s w[o][s] in[o][s][k] (divided by sum_s w[o][s] when
+// normalise != 0), s in order: shape averaging and size integration.
+hipError_t launch_weighted_reduce(int n_groups, int group, int64_t width,
+                                  const double *in, const double *w,
+                                  int normalise, double *out, hipStream_t s);
+
 }  // namespace ctm

@@ -143,11 +143,26 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
  * AlignmentAverageTask.hpp:115-215, TMatrixCalculator.hpp:229-307).
  * pN_host: n_dist rows of ncoef >= 2 nmax + 1 coefficients; dist_of_grain
  * (host, may be NULL = all 0) selects a row per grain, as a size-based
- * alignment distribution does (SizeBasedAlignmentDistribution.hpp:68-146).
- * Coefficients that are exactly 0 are skipped. */
+ * alignment distribution does (SizeBasedAlignmentDistribution.hpp:68-146);
+ * -1 marks a grain whose alignment is disabled, whose ensemble T-matrix is
+ * its own (DisabledAlignmentOrientationDistribution.hpp:36-81,
+ * AlignmentAverageTask.hpp:195-214).  Coefficients that are exactly 0 are
+ * skipped. */
 int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
                             const double *pN_host,
                             const int32_t *dist_of_grain);
+/* Weighted sums over groups of consecutive rows, on the context's stream:
+ * out[o][k] = sum_s w[o][s] in[o][s][k], s = 0..group-1 in order, divided by
+ * sum_s w[o][s] when normalise != 0.  in: n_groups x group x width, weights:
+ * n_groups x group, out: n_groups x width, all device arrays.  With
+ * normalise it is the shape average of ShapeAveragingTask::execute
+ * (ShapeAveragingTask.hpp:127-166, ShapeDistribution.hpp:95-106); without,
+ * the caller-side size integration sum_a w_a Q(a)
+ * (scripts/BBOPResult_sim.py:9-71). */
+int ctm_weighted_reduce(ctm_context *ctx, int n_groups, int group,
+                        int64_t width, const double *in, const double *weights,
+                        int normalise, double *out);
+
 /* Grain g's ensemble T-matrix in the reference block layout. */
 int ctm_average_get(ctm_context *ctx, int grain, double *T_host,
                     size_t capacity);
@@ -180,6 +195,39 @@ int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
                 const double *mr_re, const double *mr_im,
                 const uint32_t *nmax_host, const uint32_t *ngauss_host,
                 double *qsca_host, double *qext_host);
+
+/* Host-only helpers (no GPU work).  ctm_real_fn is a real function of one
+ * real variable with a caller argument, as the reference passes its Python
+ * callbacks (PyOrientationDistribution.hpp:501-533). */
+typedef double (*ctm_real_fn)(double x, void *arg);
+
+/* Gauss-Legendre nodes (ascending) and weights of the given order on
+ * [a, b]: SpecialFunctions::get_gauss_legendre_points_and_weights(_ab)
+ * (SpecialFunctions.hpp:647-751). */
+int ctm_gauss_legendre(uint32_t order, double a, double b, double *x,
+                       double *w);
+
+/* R_V / R_S of a spheroid with axis ratio d:
+ * get_equal_volume_to_equal_surface_area_sphere_ratio
+ * (SpecialFunctions.hpp:558-583). */
+double ctm_rv_over_rs(double axis_ratio);
+
+/* Adaptive Gauss-Legendre quadrature of f on [a, b], orders start..max-1
+ * until two successive results agree to abs_err or rel_err:
+ * SpecialFunctions::gauss_legendre_quadrature (SpecialFunctions.hpp:772-815).
+ * Returns CTM_NOT_CONVERGED (with the last estimate) where the reference
+ * warns. */
+int ctm_adaptive_quadrature(ctm_real_fn f, void *arg, double a, double b,
+                            uint32_t start_order, uint32_t max_order,
+                            double abs_err, double rel_err, double *result);
+
+/* Legendre coefficients p_0..p_order of an orientation distribution p(beta)
+ * (NULL: the isotropic p = 1/2), normalised numerically:
+ * OrientationDistribution::initialise (OrientationDistribution.hpp:220-243).
+ * coefficients has order + 1 entries. */
+int ctm_orientation_coefficients(ctm_real_fn p, void *arg, uint32_t order,
+                                 double abs_err, double rel_err,
+                                 double *coefficients);
 
 /* Kernel timing.  With profiling on, every launch of the library is
  * bracketed by two HIP events on the context's stream; ctm_profile_read
