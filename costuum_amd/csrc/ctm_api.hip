@@ -240,7 +240,7 @@ struct ctm_context {
       d_tiles, d_solve, d_q, d_t, d_qparts, d_state, d_active, d_winoff,
       d_winitems, d_gioff, d_gitems, d_gioff_tmp, d_params, d_ang, d_pt,
       d_x, d_spec, d_cgkeys, d_cg, d_cgidx, d_avgtasks, d_avg, d_aoff, d_pn,
-      d_dist, d_obsitems, d_query, d_qout;
+      d_dist, d_obsitems, d_query, d_qout, d_gsolve;
   // Results of the last batch (host).
   int n_grains = 0;
   std::vector<uint32_t> nmax, ngauss;
@@ -427,26 +427,37 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
                              ctx->d_wigkeys.as<WigKey>(), gp,
                              ctx->d_tables.as<double>(),
                              ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(), s));
-  // solve tasks in LDS-size buckets, largest first
-  const int bounds[] = {kMaxLdsOrder, 72, 48, 24};
+  // solve tasks in LDS-size buckets, largest first; blocks above the LDS
+  // capacity are inverted in global scratch, at most kGlobalSolveBytes of it
+  // per launch
+  constexpr size_t kGlobalSolveBytes = (size_t)1 << 30;
+  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 72, 48, 24};
+  constexpr int kBuckets = 5;
   size_t pos = 0;
   const int32_t *dsolve = ctx->d_solve.as<int32_t>();
-  for (int b = 0; b < 4 && pos < w.solve.size(); ++b) {
+  for (int b = 0; b < kBuckets && pos < w.solve.size(); ++b) {
     const int hi = bounds[b];
-    const int lo = (b < 3) ? bounds[b + 1] : 0;
+    const int lo = (b < kBuckets - 1) ? bounds[b + 1] : 0;
     size_t end = pos;
     int nm_max = 0;
-    while (end < w.solve.size()) {
+    size_t cap = w.solve.size();
+    while (end < w.solve.size() && end - pos < cap) {
       const Item &it = w.items[w.solve[end] >> 1];
       const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
       if (nm <= lo) break;
       if (nm > hi) {
-        ctx->err = "block order exceeds the LDS-resident solver";
+        ctx->err = "block order exceeds the largest supported order";
         return CTM_ERR_ARGUMENT;
       }
+      if (nm_max == 0 && nm > kMaxLdsOrder)
+        cap = std::max<size_t>(
+            1, kGlobalSolveBytes / ((size_t)nm * nm * sizeof(cdbl)));
       nm_max = std::max(nm_max, nm);
       ++end;
     }
+    if (nm_max > kMaxLdsOrder)
+      CTM_HIP(ctx->d_gsolve.ensure((end - pos) * (size_t)nm_max * nm_max *
+                                   sizeof(cdbl)));
     if (end > pos) {
       double fs = 0., bs = 0.;
       for (size_t q = pos; q < end; ++q) {
@@ -461,9 +472,15 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
                                      ctx->d_q.as<cdbl>(),
                                      want_t ? ctx->d_t.as<cdbl>() : nullptr,
                                      ctx->d_qparts.as<double>(),
-                                     want_t ? 0 : 1, s));
+                                     want_t ? 0 : 1,
+                                     ctx->d_gsolve.as<cdbl>(), s));
     }
     pos = end;
+    // the global bucket may need several launches
+    if (b == 0 && pos < w.solve.size()) {
+      const Item &it = w.items[w.solve[pos] >> 1];
+      if (it.N + 1 - (it.m > 0 ? it.m : 1) > kMaxLdsOrder) --b;
+    }
   }
   return CTM_OK;
 }
@@ -603,8 +620,8 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     ctx->err = "invalid convergence parameters";
     return CTM_ERR_ARGUMENT;
   }
-  if ((int)conv->max_order - 1 > kMaxLdsOrder) {
-    ctx->err = "maximum order above 100 is not supported by this build";
+  if ((int)conv->max_order - 1 > kMaxOrder) {
+    ctx->err = "maximum order above 201 is not supported by this build";
     return CTM_ERR_ARGUMENT;
   }
   CTM_HIP(hipSetDevice(ctx->device));
@@ -728,9 +745,9 @@ int ctm_tmatrix_fixed(ctm_context *ctx, int n, const double *radius,
     return CTM_ERR_ARGUMENT;
   }
   for (int g = 0; g < n; ++g) {
-    if (nmax_host[g] < 1 || (int)nmax_host[g] > kMaxLdsOrder ||
+    if (nmax_host[g] < 1 || (int)nmax_host[g] > kMaxOrder ||
         ngauss_host[g] < 1) {
-      ctx->err = "forced order out of range (1..99) or ngauss < 1";
+      ctx->err = "forced order out of range (1..200) or ngauss < 1";
       return CTM_ERR_ARGUMENT;
     }
   }
@@ -1195,9 +1212,9 @@ int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
     return CTM_ERR_ARGUMENT;
   }
   for (int g = 0; g < n; ++g)
-    if (nmax_host[g] < 1 || (int)nmax_host[g] > kMaxLdsOrder ||
+    if (nmax_host[g] < 1 || (int)nmax_host[g] > kMaxOrder ||
         ngauss_host[g] < 1) {
-      ctx->err = "candidate order out of range (1..99) or ngauss < 1";
+      ctx->err = "candidate order out of range (1..200) or ngauss < 1";
       return CTM_ERR_ARGUMENT;
     }
   CTM_HIP(hipSetDevice(ctx->device));

@@ -73,7 +73,8 @@ __global__ __launch_bounds__(64) void k_average(
     const cdbl *tbuf, const double *pN, int ncoef, const int32_t *dist,
     const int32_t *cg_index, const double *cg, int nmax_cg, cdbl *abuf,
     const int64_t *aoff) {
-  __shared__ cdbl Bsh[2 * kMaxAvgOrder + 1][2];
+  // B_N for N = lo..hi (at most 2 nmax_cg + 1 rows of two values)
+  extern __shared__ cdbl Bsh[];
   const AvgTask task = tasks[blockIdx.x];
   const int g = task.grain, n1 = task.n1, n2 = task.n2;
   const int i0 = gi_off[g];
@@ -137,8 +138,8 @@ __global__ __launch_bounds__(64) void k_average(
         }
       }
     }
-    Bsh[N - lo][0] = b0;
-    Bsh[N - lo][1] = b1;
+    Bsh[2 * (N - lo)] = b0;
+    Bsh[2 * (N - lo) + 1] = b1;
   }
   __syncthreads();
   for (int m = t; m <= M; m += 64) {
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(64) void k_average(
       if (!C) continue;
       const double w = ((m & 1) ? -1. : 1.) * p[N] * C[m];
       const bool even = ((N + hi) & 1) == 0;
-      const cdbl b0 = Bsh[N - lo][0], b1 = Bsh[N - lo][1];
+      const cdbl b0 = Bsh[2 * (N - lo)], b1 = Bsh[2 * (N - lo) + 1];
       if (even) {
         rfma(acc[0][0], w, b0);
         rfma(acc[1][1], w, b1);
@@ -213,8 +214,10 @@ hipError_t launch_average(const AvgTask *tasks, int n, const Item *items,
                           int nmax_cg, cdbl *abuf, const int64_t *aoff,
                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_average, dim3(n), dim3(64), 0, s, tasks, items, gi_off,
-                     tbuf, pN, ncoef, dist, cg_index, cg, nmax_cg, abuf, aoff);
+  const size_t lds = (size_t)(2 * nmax_cg + 1) * 2 * sizeof(cdbl);
+  hipLaunchKernelGGL(k_average, dim3(n), dim3(64), lds, s, tasks, items,
+                     gi_off, tbuf, pN, ncoef, dist, cg_index, cg, nmax_cg,
+                     abuf, aoff);
   return hipGetLastError();
 }
 

@@ -6,7 +6,7 @@
 namespace ctm {
 
 // Largest order the averaging kernels handle (LDS rows of 2 N + 1).
-constexpr int kMaxAvgOrder = 200;
+constexpr int kMaxAvgOrder = 400;
 
 struct CgKey {
   int32_t n1, n2, N, pad;

@@ -510,10 +510,15 @@ __global__ __launch_bounds__(256) void k_assemble(
 // while the current update is applied.  The product streams RgQ_c from HBM
 // in groups of four rows per wave.  With diag_only only the diagonal of T is
 // formed (all the convergence test needs).
-template <int NT>
+// GLOBAL: blocks of order above the LDS capacity keep Q_c in a per-block
+// slice (gstride complex values) of a global scratch buffer instead; the
+// pivot bookkeeping stays in LDS and the access pattern is unchanged
+// (512-byte row segments per half-wave), so the working set streams through
+// L2 and the Infinity Cache.
+template <int NT, bool GLOBAL>
 __global__ __launch_bounds__(NT) void k_invert_solve(
     const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
-    double *qparts, int diag_only) {
+    double *qparts, int diag_only, cdbl *gscratch, int64_t gstride) {
   constexpr int NW = NT / 64;
   extern __shared__ double smem[];
   __shared__ double wb_mag[NW][2];
@@ -537,8 +542,9 @@ __global__ __launch_bounds__(NT) void k_invert_solve(
     return nb == 2 ? (i < sz0 ? 2 * i : 2 * (i - sz0) + 1) : i;
   };
 
-  cdbl *A = reinterpret_cast<cdbl *>(smem);
-  cdbl *colk = A + nm2;
+  cdbl *A = GLOBAL ? gscratch + (int64_t)blockIdx.x * gstride
+                  : reinterpret_cast<cdbl *>(smem);
+  cdbl *colk = GLOBAL ? reinterpret_cast<cdbl *>(smem) : A + nm2;
   cdbl *rowk = colk + nm;
   cdbl *rowold = rowk + nm;
   int *piv = reinterpret_cast<int *>(rowold + nm);
@@ -914,37 +920,48 @@ size_t invert_solve_lds_bytes(int nm) {
          2 * (size_t)nm * sizeof(int);
 }
 
-template <int NT>
+template <int NT, bool GLOBAL>
 static hipError_t launch_solve_nt(const int32_t *tasks, int n, size_t lds,
                                   const Item *items, const cdbl *qbuf,
                                   cdbl *tbuf, double *qparts, int diag_only,
+                                  cdbl *gscratch, int64_t gstride,
                                   hipStream_t s) {
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(k_invert_solve<NT>),
+        reinterpret_cast<const void *>(k_invert_solve<NT, GLOBAL>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_invert_solve<NT>, dim3(n), dim3(NT), lds, s, tasks,
-                     items, qbuf, tbuf, qparts, diag_only);
+  hipLaunchKernelGGL((k_invert_solve<NT, GLOBAL>), dim3(n), dim3(NT), lds, s,
+                     tasks, items, qbuf, tbuf, qparts, diag_only, gscratch,
+                     gstride);
   return hipGetLastError();
 }
 
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
-                               double *qparts, int diag_only, hipStream_t s) {
+                               double *qparts, int diag_only, cdbl *gscratch,
+                               hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (nm_max > kMaxLdsOrder) {
+    // the matrix lives in gscratch (n * nm_max^2 complex values)
+    const size_t lds = invert_solve_lds_bytes(nm_max) -
+                       (size_t)nm_max * nm_max * sizeof(cdbl);
+    return launch_solve_nt<512, true>(tasks, n, lds, items, qbuf, tbuf,
+                                      qparts, diag_only, gscratch,
+                                      (int64_t)nm_max * nm_max, s);
+  }
   const size_t lds = invert_solve_lds_bytes(nm_max);
   if (nm_max <= 16)
-    return launch_solve_nt<64>(tasks, n, lds, items, qbuf, tbuf, qparts,
-                               diag_only, s);
+    return launch_solve_nt<64, false>(tasks, n, lds, items, qbuf, tbuf,
+                                      qparts, diag_only, nullptr, 0, s);
   if (nm_max <= 48)
-    return launch_solve_nt<256>(tasks, n, lds, items, qbuf, tbuf, qparts,
-                                diag_only, s);
-  return launch_solve_nt<512>(tasks, n, lds, items, qbuf, tbuf, qparts,
-                              diag_only, s);
+    return launch_solve_nt<256, false>(tasks, n, lds, items, qbuf, tbuf,
+                                       qparts, diag_only, nullptr, 0, s);
+  return launch_solve_nt<512, false>(tasks, n, lds, items, qbuf, tbuf, qparts,
+                                     diag_only, nullptr, 0, s);
 }
 
 hipError_t launch_converge(const int32_t *active, int n_active,

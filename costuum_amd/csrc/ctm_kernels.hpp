@@ -28,9 +28,14 @@ hipError_t launch_assemble(const int32_t *order, const int32_t *tile_prefix,
 size_t invert_solve_lds_bytes(int nm);
 // Largest block order the LDS-resident inversion holds.
 constexpr int kMaxLdsOrder = 99;
+// Largest order the library computes (larger blocks are inverted in global
+// scratch).
+constexpr int kMaxOrder = 200;
+// gscratch: n * nm_max^2 complex values, used when nm_max > kMaxLdsOrder.
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
-                               double *qparts, int diag_only, hipStream_t s);
+                               double *qparts, int diag_only, cdbl *gscratch,
+                               hipStream_t s);
 hipError_t launch_converge(const int32_t *active, int n_active,
                            const int32_t *win_off, const int32_t *win_items,
                            const Item *items, const double *qparts, double tol,

@@ -32,7 +32,7 @@ struct Query {
 };
 // Per query: S (4 complex, row-major 2x2), Z (16), K (16) doubles.
 constexpr int kQueryOut = 8 + 16 + 16;
-constexpr int kMaxQueryOrder = 100;
+constexpr int kMaxQueryOrder = 256;
 
 hipError_t launch_query(const Query *q, int n, const Item *items,
                         const int32_t *gi_off, const GrainParams &gp,

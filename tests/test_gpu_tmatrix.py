@@ -102,3 +102,46 @@ def test_batch_convergence_matches_oracle(ctx):
     ee = np.abs(qe - qe_ref) / np.abs(qe + qe_ref)
     print("max rel err Qsca %.3e Qext %.3e" % (es.max(), ee.max()))
     assert ee.max() < 1e-8 and es.max() < 1e-8
+
+
+def test_orders_above_lds_capacity_match_oracle(ctx):
+    """Blocks of order above 99 are inverted in global scratch; parity with
+    the oracle at order 104 (x = 50), bar 1e-8 or 50 times the oracle's own
+    sensitivity to one rounding of R_V."""
+    from costuum_amd.tmatrix import tmatrix_fixed
+    rv, d, lam, mr, N, G = 0.8, 1.1, 0.1, 1.5 + 0.01j, 104, 208
+    print("order %d, ngauss %d" % (N, G), flush=True)
+    r = tmatrix_fixed([rv], [d], [lam], [mr.real], [mr.imag], [N], [G],
+                      ctx=ctx)
+    T = r.tmatrix(0)
+    Tref = O.tmatrix_all(rv, d, lam, mr, N, G)
+    T2 = O.tmatrix_all(rv * (1 + 4e-16), d, lam, mr, N, G)
+    scale = np.abs(Tref).max()
+    bar = max(1e-8, 50 * np.abs(T2 - Tref).max() / scale)
+    err = np.abs(T - Tref).max() / scale
+    print("max rel err %.3e (bar %.3e)" % (err, bar))
+    assert err < bar
+    s_ref, e_ref = O.qcoeffs(N, Tref)
+    assert rel(r.qsca.cpu().numpy()[0], s_ref) < 1e-8
+    assert rel(r.qext.cpu().numpy()[0], e_ref) < 1e-8
+
+
+
+def test_large_grains_converge_like_oracle(ctx):
+    """Task-path convergence with maximum order 160 for x = 90 and 100
+    (orders 108 and 131, blocks inverted in global scratch): same converged
+    order as the oracle, Q_ext/Q_sca to 1e-8."""
+    from costuum_amd.tmatrix import tmatrix_batch
+    cases = [(1.0, 0.9, 0.07, 1.5 + 0.01j), (1.2, 1.3, 0.075, 1.35 + 0.05j)]
+    print("large grains", cases, flush=True)
+    r = tmatrix_batch([c[0] for c in cases], [c[1] for c in cases],
+                      [c[2] for c in cases], [c[3].real for c in cases],
+                      [c[3].imag for c in cases], min_order=4, max_order=160,
+                      glfac=2, tolerance=1e-4, ctx=ctx)
+    nmax, _, status = r.orders()
+    for g, (rv, d, lam, mr) in enumerate(cases):
+        st, n, ng, _, _ = O.converge_task(rv, d, lam, mr, 4, 160, 2, 1e-4)
+        assert (int(nmax[g]), int(status[g])) == (n, st)
+        s_ref, e_ref = O.qcoeffs(n, O.tmatrix_all(rv, d, lam, mr, n, ng))
+        assert rel(r.qsca.cpu().numpy()[g], s_ref) < 1e-8
+        assert rel(r.qext.cpu().numpy()[g], e_ref) < 1e-8
