@@ -40,7 +40,25 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=64,
                     help="CPU sample = every k-th grain in size order")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 512 MiB five times "
+                         "(reference kernel for the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain device-to-device copy of known size: 512 MiB read and 512 MiB
+    written per launch.  rocprofv3's FETCH_SIZE / WRITE_SIZE of this kernel
+    give the counter-to-byte scale used for the traffic figure."""
+    import torch
+    n = 512 * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": 8 * n, "launches": 5}),
+          flush=True)
 
 
 def cpu_baseline(g, stride, threads, theta, wtheta):
@@ -74,9 +92,12 @@ def cpu_baseline(g, stride, threads, theta, wtheta):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU "
-              "path (grains shard with no communication; see DESIGN.md)",
+        print("bench.py measures one GPU only: grains shard across GPUs "
+              "with no communication, and no multi-GPU path is built",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np

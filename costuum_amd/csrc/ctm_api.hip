@@ -120,7 +120,12 @@ struct WorkList {
   std::vector<TableSet> sets;
   std::vector<Item> items;
   std::vector<int32_t> order;        // items sorted by assembly cost
-  std::vector<int32_t> tile_prefix;  // tiles before order[i]
+  // Assembly order: items with blocks above kSmallBlock (2 x 2 micro-tiles)
+  // first, then the small ones, each part sorted by G (largest first);
+  // tile_prefix holds one prefix sum per part (n_large + 1 entries, then
+  // n - n_large + 1).
+  std::vector<int32_t> tile_prefix;
+  int n_large = 0;
   std::vector<int32_t> solve;        // item * 2 + class, by block order
   int64_t table_size = 0, q_size = 0, t_size = 0;
   void clear() {
@@ -345,18 +350,34 @@ void build_tasks(WorkList &w) {
     gmax = std::max(gmax, it.G);
     nmmax = std::max(nmmax, it.N + 1 - (it.m > 0 ? it.m : 1));
   }
-  std::vector<int32_t> cnt(gmax + 2, 0);
-  for (const Item &it : w.items) cnt[gmax - it.G + 1]++;
-  for (int g = 1; g <= gmax + 1; ++g) cnt[g] += cnt[g - 1];
+  // key: large blocks first, then by G descending
+  auto is_large = [](const Item &it) {
+    return it.N + 1 - (it.m > 0 ? it.m : 1) > kSmallBlock;
+  };
+  const int nkey = 2 * (gmax + 1);
+  std::vector<int32_t> cnt(nkey + 1, 0);
+  auto key = [&](const Item &it) {
+    return (is_large(it) ? 0 : gmax + 1) + (gmax - it.G);
+  };
+  for (const Item &it : w.items) cnt[key(it) + 1]++;
+  for (int g = 1; g <= nkey; ++g) cnt[g] += cnt[g - 1];
   w.order.assign(n, 0);
-  for (int i = 0; i < n; ++i) w.order[cnt[gmax - w.items[i].G]++] = i;
-  w.tile_prefix.assign(n + 1, 0);
+  w.n_large = 0;
+  for (int i = 0; i < n; ++i) {
+    w.order[cnt[key(w.items[i])]++] = i;
+    if (is_large(w.items[i])) ++w.n_large;
+  }
+  w.tile_prefix.assign((size_t)n + 2, 0);
+  int32_t *pre = w.tile_prefix.data();
   for (int q = 0; q < n; ++q) {
     const Item &it = w.items[w.order[q]];
     const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-    const int ntr = (nm + 15) / 16;
-    const int ntc = it.m == 0 ? (nm + 31) / 32 : ntr;
-    w.tile_prefix[q + 1] = w.tile_prefix[q] + ntr * ntc;
+    const int tile = q < w.n_large ? 32 : 16;
+    const int ntr = (nm + tile - 1) / tile;
+    const int ntc = it.m == 0 ? (nm + 2 * tile - 1) / (2 * tile) : ntr;
+    if (q == w.n_large) pre += w.n_large + 1;  // second part's prefix
+    const int qq = q < w.n_large ? q : q - w.n_large;
+    pre[qq + 1] = pre[qq] + ntr * ntc;
   }
   std::vector<int32_t> c2(nmmax + 2, 0);
   for (const Item &it : w.items)
@@ -418,20 +439,38 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
              launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(), gp,
                            ctx->d_gl.as<double>(), ctx->d_tables.as<double>(),
                            s));
-  CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, f_asm, b_asm,
-             launch_assemble(ctx->d_tiles.as<int32_t>(),
-                             ctx->d_gioff_tmp.as<int32_t>(),
-                             (int)w.order.size(), w.tile_prefix.back(),
-                             ctx->d_items.as<Item>(),
-                             ctx->d_sets.as<TableSet>(),
-                             ctx->d_wigkeys.as<WigKey>(), gp,
-                             ctx->d_tables.as<double>(),
-                             ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(), s));
+  {
+    // two launches: 2 x 2 micro-tiles for the large blocks, then 1 x 1;
+    // the algorithmic work is credited to the first
+    const int nl = w.n_large, ns = (int)w.order.size() - nl;
+    const int32_t *ord = ctx->d_tiles.as<int32_t>();
+    const int32_t *pre = ctx->d_gioff_tmp.as<int32_t>();
+    if (nl > 0)
+      CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, f_asm, b_asm,
+                 launch_assemble(2, ord, pre, nl, w.tile_prefix[nl],
+                                 ctx->d_items.as<Item>(),
+                                 ctx->d_sets.as<TableSet>(),
+                                 ctx->d_wigkeys.as<WigKey>(), gp,
+                                 ctx->d_tables.as<double>(),
+                                 ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(),
+                                 s));
+    if (ns > 0)
+      CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, nl > 0 ? 0. : f_asm,
+                 nl > 0 ? 0. : b_asm,
+                 launch_assemble(1, ord + nl, pre + nl + 1, ns,
+                                 w.tile_prefix[nl + 1 + ns],
+                                 ctx->d_items.as<Item>(),
+                                 ctx->d_sets.as<TableSet>(),
+                                 ctx->d_wigkeys.as<WigKey>(), gp,
+                                 ctx->d_tables.as<double>(),
+                                 ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(),
+                                 s));
+  }
   // solve tasks in LDS-size buckets, largest first; blocks above the LDS
   // capacity are inverted in global scratch, at most kGlobalSolveBytes of it
   // per launch
   constexpr size_t kGlobalSolveBytes = (size_t)1 << 30;
-  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 72, 48, 24};
+  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 48, 32, 16};
   constexpr int kBuckets = 5;
   size_t pos = 0;
   const int32_t *dsolve = ctx->d_solve.as<int32_t>();

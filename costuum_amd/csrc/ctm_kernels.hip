@@ -272,27 +272,31 @@ __global__ void k_tables(const TableSet *sets, const GrainParams gp,
 // with (J_a, J_b) = (J21, J12) for even and (J11, J22) for odd pairs
 // (TMatrixResource.hpp:623-705, 993-1097).
 //
-// Workgroup: 256 threads = 4 waves on a tile of 16 rows of pairs.  For m > 0
-// the tile has 16 columns; wave w takes rows of parity w & 1 and columns of
-// parity w >> 1, so every wave sees a single pair parity; lane (lr, lc) takes
-// pair (2 lr + (w&1), 2 lc + (w>>1)).  For m = 0 only pairs with n1 + n2 even
-// contribute (TMatrixResource.hpp:622), so the tile is 32 columns wide and
-// all four waves take even pairs: wave w takes rows of parity w & 1, the
-// columns of the same parity in half w >> 1 of the tile.
+// Workgroup: 256 threads = 4 waves on a tile of 16 MT rows of pairs, each
+// thread accumulating an MT x MT micro-tile (MT = 1 for blocks of order up
+// to 16, MT = 2 above: twice the multiply-adds per LDS operand).  For m > 0
+// the tile has 16 MT columns; wave w takes rows of parity w & 1 and columns
+// of parity w >> 1, so every wave sees a single pair parity; lane (lr, lc)
+// takes rows 2 (lr + 8 a) + (w & 1) and columns 2 (lc + 8 b) + (w >> 1),
+// a, b < MT.  For m = 0 only pairs with n1 + n2 even contribute
+// (TMatrixResource.hpp:622), so the tile is 32 MT columns wide and all four
+// waves take even pairs: wave w takes rows of parity w & 1 and the columns
+// of the same parity in half w >> 1 of the tile.
 // Workgroups find their (item, tile) by binary search in the prefix sum of
 // tile counts over the cost-sorted item list.
-constexpr int kTile = 16;
-constexpr int kChunk = 8;
 constexpr int kLeftStride = 34;   // 32 factors + 2 pad doubles per row
 constexpr int kRightStride = 10;  // 5 complex factors per column
 
+template <int MT>
 __global__ __launch_bounds__(256) void k_assemble(
     const int32_t *order, const int32_t *tile_prefix, int n_order,
     const Item *items, const TableSet *sets, const WigKey *wkeys,
     const GrainParams gp, const double *tables, const double *wig,
     cdbl *qbuf) {
-  __shared__ double Lsh[kChunk][kTile][kLeftStride];
-  __shared__ double Rsh[kChunk][2 * kTile][kRightStride];
+  constexpr int TR = 16 * MT;            // rows of pairs per tile
+  constexpr int CH = MT == 1 ? 8 : 4;    // quadrature nodes per LDS stage
+  __shared__ double Lsh[CH][TR][kLeftStride];
+  __shared__ double Rsh[CH][2 * TR][kRightStride];
 
   const int bid = blockIdx.x;
   int lo = 0, hi = n_order;
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(256) void k_assemble(
   const int nmin = m > 0 ? m : 1;
   const int nm = N + 1 - nmin;
   const bool m0 = (m == 0);
-  const int cw = m0 ? 2 * kTile : kTile;
+  const int cw = m0 ? 2 * TR : TR;
   const int ntc = (nm + cw - 1) / cw;
   const int lt = bid - tile_prefix[lo];
   const int tr = lt / ntc, tc = lt - (lt / ntc) * ntc;
@@ -319,26 +323,38 @@ __global__ __launch_bounds__(256) void k_assemble(
 
   const int t = threadIdx.x;
   const int wv = t >> 6, lane = t & 63;
-  const int rr = 2 * (lane >> 3) + (wv & 1);
-  const int cc = m0 ? kTile * (wv >> 1) + 2 * (lane & 7) + (wv & 1)
-                    : 2 * (lane & 7) + (wv >> 1);
-  const int p1 = tr * kTile + rr;
-  const int p2 = tc * cw + cc;
-  const bool odd = !m0 && (((wv & 1) ^ (wv >> 1)) != 0);
-  const bool active = true;
-  const int nstage = kTile * kChunk + cw * kChunk;
+  const int prow = wv & 1;
+  const int pcol = m0 ? (wv & 1) : (wv >> 1);
+  const int cbase = m0 ? TR * (wv >> 1) : 0;
+  int rr[MT], cc[MT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a) {
+    rr[a] = 2 * ((lane >> 3) + 8 * a) + prow;
+    cc[a] = cbase + 2 * ((lane & 7) + 8 * a) + pcol;
+  }
+  const bool odd = !m0 && ((prow ^ pcol) != 0);
+  const int nstage = TR * CH + cw * CH;
 
-  cdbl Ja = cmk(0., 0.), Jb = cmk(0., 0.), RJa = cmk(0., 0.), RJb = cmk(0., 0.);
+  cdbl Ja[MT][MT], Jb[MT][MT], RJa[MT][MT], RJb[MT][MT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a)
+#pragma unroll
+    for (int b = 0; b < MT; ++b) {
+      Ja[a][b] = cmk(0., 0.);
+      Jb[a][b] = cmk(0., 0.);
+      RJa[a][b] = cmk(0., 0.);
+      RJb[a][b] = cmk(0., 0.);
+    }
   const double mm = (double)m;
 
-  for (int ig0 = 0; ig0 < G; ig0 += kChunk) {
+  for (int ig0 = 0; ig0 < G; ig0 += CH) {
     __syncthreads();
     for (int e = t; e < nstage; e += 256) {
-    if (e < kTile * kChunk) {
+    if (e < TR * CH) {
       // row factors of row r at node ig0 + kk
-      const int r = e / kChunk, kk = e % kChunk;
+      const int r = e / CH, kk = e % CH;
       const int ig = ig0 + kk;
-      const int p = tr * kTile + r;
+      const int p = tr * TR + r;
       double *L = Lsh[kk][r];
       if (ig < G && p < nm) {
         const int n1 = nmin + p;
@@ -399,8 +415,8 @@ __global__ __launch_bounds__(256) void k_assemble(
         for (int i = 0; i < 32; ++i) L[i] = 0.;
       }
     } else {
-      const int tt = e - kTile * kChunk;
-      const int c = tt / kChunk, kk = tt % kChunk;
+      const int tt = e - TR * CH;
+      const int c = tt / CH, kk = tt % CH;
       const int ig = ig0 + kk;
       const int p = tc * cw + c;
       double *R = Rsh[kk][c];
@@ -428,73 +444,108 @@ __global__ __launch_bounds__(256) void k_assemble(
     }
     }
     __syncthreads();
-    if (active) {
-      const int kmax = (G - ig0) < kChunk ? (G - ig0) : kChunk;
-      if (!odd) {
-        for (int kk = 0; kk < kmax; ++kk) {
-          const double *L = Lsh[kk][rr];
-          const double *R = Rsh[kk][cc];
-          const cdbl R1 = cmk(R[0], R[1]), R2 = cmk(R[2], R[3]);
-          const cdbl R3 = cmk(R[4], R[5]), R4 = cmk(R[6], R[7]);
-          const cdbl R5 = cmk(R[8], R[9]);
-          cfma(Ja, cmk(L[0], L[1]), R1);
-          cfma(Ja, cmk(L[2], L[3]), R2);
-          cfma(Ja, cmk(L[4], L[5]), R3);
-          cfma(Jb, cmk(L[6], L[7]), R4);
-          cfma(Jb, cmk(L[8], L[9]), R5);
-          rfma(RJa, L[10], R1);
-          rfma(RJa, L[11], R2);
-          cfma(RJa, cmk(L[12], L[13]), R3);
-          rfma(RJb, L[14], R4);
-          rfma(RJb, L[15], R5);
+    const int kmax = (G - ig0) < CH ? (G - ig0) : CH;
+    if (!odd) {
+      for (int kk = 0; kk < kmax; ++kk) {
+        cdbl R1[MT], R2[MT], R3[MT], R4[MT], R5[MT];
+#pragma unroll
+        for (int b = 0; b < MT; ++b) {
+          const double *R = Rsh[kk][cc[b]];
+          R1[b] = cmk(R[0], R[1]); R2[b] = cmk(R[2], R[3]);
+          R3[b] = cmk(R[4], R[5]); R4[b] = cmk(R[6], R[7]);
+          R5[b] = cmk(R[8], R[9]);
         }
-      } else {
-        for (int kk = 0; kk < kmax; ++kk) {
-          const double *L = Lsh[kk][rr] + 16;
-          const double *R = Rsh[kk][cc];
-          const cdbl R1 = cmk(R[0], R[1]), R2 = cmk(R[2], R[3]);
-          const cdbl R3 = cmk(R[4], R[5]), R4 = cmk(R[6], R[7]);
-          const cdbl R5 = cmk(R[8], R[9]);
-          cfma(Ja, cmk(L[0], L[1]), R5);
-          cfma(Ja, cmk(L[2], L[3]), R4);
-          cfma(Jb, cmk(L[4], L[5]), R2);
-          cfma(Jb, cmk(L[6], L[7]), R1);
-          cfma(Jb, cmk(L[8], L[9]), R3);
-          rfma(RJa, L[10], R5);
-          rfma(RJa, L[11], R4);
-          rfma(RJb, L[12], R2);
-          rfma(RJb, L[13], R1);
-          cfma(RJb, cmk(L[14], L[15]), R3);
+#pragma unroll
+        for (int a = 0; a < MT; ++a) {
+          const double *L = Lsh[kk][rr[a]];
+          const cdbl L1 = cmk(L[0], L[1]), L2 = cmk(L[2], L[3]);
+          const cdbl L3 = cmk(L[4], L[5]), L4 = cmk(L[6], L[7]);
+          const cdbl L5 = cmk(L[8], L[9]), L8 = cmk(L[12], L[13]);
+          const double l6 = L[10], l7 = L[11], l9 = L[14], l10 = L[15];
+#pragma unroll
+          for (int b = 0; b < MT; ++b) {
+            cfma(Ja[a][b], L1, R1[b]);
+            cfma(Ja[a][b], L2, R2[b]);
+            cfma(Ja[a][b], L3, R3[b]);
+            cfma(Jb[a][b], L4, R4[b]);
+            cfma(Jb[a][b], L5, R5[b]);
+            rfma(RJa[a][b], l6, R1[b]);
+            rfma(RJa[a][b], l7, R2[b]);
+            cfma(RJa[a][b], L8, R3[b]);
+            rfma(RJb[a][b], l9, R4[b]);
+            rfma(RJb[a][b], l10, R5[b]);
+          }
+        }
+      }
+    } else {
+      for (int kk = 0; kk < kmax; ++kk) {
+        cdbl R1[MT], R2[MT], R3[MT], R4[MT], R5[MT];
+#pragma unroll
+        for (int b = 0; b < MT; ++b) {
+          const double *R = Rsh[kk][cc[b]];
+          R1[b] = cmk(R[0], R[1]); R2[b] = cmk(R[2], R[3]);
+          R3[b] = cmk(R[4], R[5]); R4[b] = cmk(R[6], R[7]);
+          R5[b] = cmk(R[8], R[9]);
+        }
+#pragma unroll
+        for (int a = 0; a < MT; ++a) {
+          const double *L = Lsh[kk][rr[a]] + 16;
+          const cdbl L1 = cmk(L[0], L[1]), L2 = cmk(L[2], L[3]);
+          const cdbl L3 = cmk(L[4], L[5]), L4 = cmk(L[6], L[7]);
+          const cdbl L5 = cmk(L[8], L[9]), L8 = cmk(L[14], L[15]);
+          const double l6 = L[10], l7 = L[11], l9 = L[12], l10 = L[13];
+#pragma unroll
+          for (int b = 0; b < MT; ++b) {
+            cfma(Ja[a][b], L1, R5[b]);
+            cfma(Ja[a][b], L2, R4[b]);
+            cfma(Jb[a][b], L3, R2[b]);
+            cfma(Jb[a][b], L4, R1[b]);
+            cfma(Jb[a][b], L5, R3[b]);
+            rfma(RJa[a][b], l6, R5[b]);
+            rfma(RJa[a][b], l7, R4[b]);
+            rfma(RJb[a][b], l9, R2[b]);
+            rfma(RJb[a][b], l10, R1[b]);
+            cfma(RJb[a][b], L8, R3[b]);
+          }
         }
       }
     }
   }
 
-  if (p1 >= nm || p2 >= nm) return;
-  if (odd) {
-    // multiply by i: (x, y) -> (-y, x)
-    Ja = cmk(-Ja.y, Ja.x);
-    Jb = cmk(-Jb.y, Jb.x);
-    RJa = cmk(-RJa.y, RJa.x);
-    RJb = cmk(-RJb.y, RJb.x);
-  }
   const double lam = gp.wavelength[it.grain];
   const double k = 2. * M_PI / lam;
   const double k2 = k * k;
   const cdbl k2mr = cscale(cmk(gp.mr_re[it.grain], gp.mr_im[it.grain]), k2);
-  const cdbl Qx = cadd(cmul(k2mr, Ja), cscale(Jb, k2));
-  const cdbl Qy = cadd(cmul(k2mr, Jb), cscale(Ja, k2));
-  const cdbl RQx = cadd(cmul(k2mr, RJa), cscale(RJb, k2));
-  const cdbl RQy = cadd(cmul(k2mr, RJb), cscale(RJa, k2));
-  const int n1 = nmin + p1;
-  const int cx = n1 & 1;
   const int64_t nm2 = (int64_t)nm * nm;
-  const int64_t e = (int64_t)p1 * nm + p2;
   cdbl *Q = qbuf + it.q_off;
-  Q[cx * nm2 + e] = Qx;
-  Q[(1 - cx) * nm2 + e] = Qy;
-  Q[(2 + cx) * nm2 + e] = RQx;
-  Q[(2 + 1 - cx) * nm2 + e] = RQy;
+#pragma unroll
+  for (int a = 0; a < MT; ++a) {
+#pragma unroll
+    for (int b = 0; b < MT; ++b) {
+      const int p1 = tr * TR + rr[a];
+      const int p2 = tc * cw + cc[b];
+      if (p1 >= nm || p2 >= nm) continue;
+      cdbl ja = Ja[a][b], jb = Jb[a][b], rja = RJa[a][b], rjb = RJb[a][b];
+      if (odd) {
+        // multiply by i: (x, y) -> (-y, x)
+        ja = cmk(-ja.y, ja.x);
+        jb = cmk(-jb.y, jb.x);
+        rja = cmk(-rja.y, rja.x);
+        rjb = cmk(-rjb.y, rjb.x);
+      }
+      const cdbl Qx = cadd(cmul(k2mr, ja), cscale(jb, k2));
+      const cdbl Qy = cadd(cmul(k2mr, jb), cscale(ja, k2));
+      const cdbl RQx = cadd(cmul(k2mr, rja), cscale(rjb, k2));
+      const cdbl RQy = cadd(cmul(k2mr, rjb), cscale(rja, k2));
+      const int n1 = nmin + p1;
+      const int cx = n1 & 1;
+      const int64_t e = (int64_t)p1 * nm + p2;
+      Q[cx * nm2 + e] = Qx;
+      Q[(1 - cx) * nm2 + e] = Qy;
+      Q[(2 + cx) * nm2 + e] = RQx;
+      Q[(2 + 1 - cx) * nm2 + e] = RQy;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -515,6 +566,141 @@ __global__ __launch_bounds__(256) void k_assemble(
 // pivot bookkeeping stays in LDS and the access pattern is unchanged
 // (512-byte row segments per half-wave), so the working set streams through
 // L2 and the Infinity Cache.
+// Shared end of both solve kernels: the column unpermutation of the
+// Gauss-Jordan inverse Ainv (held in A, LDS or global), the product
+// T_c = -RgQ_c Ainv (or only its diagonal) and the per-block sums.
+template <int NT>
+__device__ __forceinline__ void solve_tail(
+    const Item &it, int task, int cls, int nm, int nb, int sz0,
+    const int *off, const int *sz, const cdbl *A, const int *piv, int *src,
+    const cdbl *RQg, cdbl *tbuf, double *qparts, int diag_only,
+    double (*red)[4]) {
+  constexpr int NW = NT / 64;
+  const int m = it.m;
+  const int nmin = m > 0 ? m : 1;
+  const int64_t nm2 = (int64_t)nm * nm;
+  auto pidx = [&](int i) {
+    return nb == 2 ? (i < sz0 ? 2 * i : 2 * (i - sz0) + 1) : i;
+  };
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  if (t == 0) {
+    for (int j = 0; j < nm; ++j) src[j] = j;
+    for (int k = nm - 1; k >= 0; --k) {
+      const int p = piv[k];
+      if (p != k) {
+        const int tmp = src[k];
+        src[k] = src[p];
+        src[p] = tmp;
+      }
+    }
+  }
+  __syncthreads();
+
+  double s_diag2 = 0., e_diag = 0., s_all = 0., e_tr = 0.;
+  if (diag_only) {
+    // T(p, p) = -sum_k RgQ(p, k) Ainv(k, p) within the sub-block of p
+    for (int i = t; i < nm; i += NT) {
+      const int klo = i >= sz0 ? sz0 : 0, khi = i >= sz0 ? nm : sz0;
+      const int pi = pidx(i);
+      const cdbl *rq = RQg + (int64_t)pi * nm;
+      const int sc = src[i];
+      cdbl acc = cmk(0., 0.);
+      for (int k = klo; k < khi; ++k)
+        cfma(acc, rq[pidx(k)], A[(int64_t)k * nm + sc]);
+      const double w = 2. * (nmin + pi) + 1.;
+      const double a2 = acc.x * acc.x + acc.y * acc.y;
+      s_diag2 += w * a2;
+      e_diag -= w * acc.x;
+      e_tr -= acc.x;
+    }
+  } else {
+    cdbl *T = tbuf + it.t_off + cls * nm2;
+    if (nb == 2) {
+      // the two sub-blocks do not couple: zero the cross entries
+      for (int i = t >> 5; i < nm; i += NT >> 5) {
+        const int bi = i >= sz0;
+        for (int j = t & 31; j < nm; j += 32) {
+          if ((j >= sz0) != bi)
+            T[(int64_t)pidx(i) * nm + pidx(j)] = cmk(0., 0.);
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      if (b >= nb) continue;
+      const int lo = off[b], hi = off[b] + sz[b];
+      // columns in groups of 128 (two per lane), rows in groups of four
+      for (int cb = lo; cb < hi; cb += 128) {
+      const int c0 = cb + lane, c1 = cb + lane + 64;
+      const bool v0 = c0 < hi, v1 = c1 < hi;
+      const int s0 = v0 ? src[c0] : 0, s1 = v1 ? src[c1] : 0;
+      for (int rg = lo + 4 * wv; rg < hi; rg += 4 * NW) {
+        cdbl acc[4][2];
+        int prow[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc[q][0] = cmk(0., 0.);
+          acc[q][1] = cmk(0., 0.);
+          const int r = rg + q < hi ? rg + q : rg;
+          prow[q] = pidx(r);
+        }
+        for (int k = lo; k < hi; ++k) {
+          const int pk = pidx(k);
+          const cdbl a0 = A[(int64_t)k * nm + s0];
+          const cdbl a1 = A[(int64_t)k * nm + s1];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const cdbl bq = RQg[(int64_t)prow[q] * nm + pk];
+            cfma(acc[q][0], bq, a0);
+            cfma(acc[q][1], bq, a1);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = rg + q;
+          if (r >= hi) break;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const bool valid = h == 0 ? v0 : v1;
+            if (!valid) continue;
+            const int c = h == 0 ? c0 : c1;
+            const cdbl v = cmk(-acc[q][h].x, -acc[q][h].y);
+            T[(int64_t)prow[q] * nm + pidx(c)] = v;
+            const double a2 = v.x * v.x + v.y * v.y;
+            s_all += a2;
+            if (r == c) {
+              const double w = 2. * (nmin + prow[q]) + 1.;
+              s_diag2 += w * a2;
+              e_diag += w * v.x;
+              e_tr += v.x;
+            }
+          }
+        }
+      }
+      }
+    }
+  }
+  // deterministic block reduction: waves by shuffles, then thread 0
+  double v[4] = {s_diag2, e_diag, s_all, e_tr};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wv][q] = v[q];
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 0; q < 4; ++q) {
+      double acc = 0.;
+      for (int w = 0; w < NW; ++w) acc += red[w][q];
+      qparts[(int64_t)task * kQParts + q] = acc;
+    }
+  }
+}
+
 template <int NT, bool GLOBAL>
 __global__ __launch_bounds__(NT) void k_invert_solve(
     const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
@@ -691,118 +877,242 @@ __global__ __launch_bounds__(NT) void k_invert_solve(
     }
     __syncthreads();
   }
-  if (t == 0) {
-    for (int j = 0; j < nm; ++j) src[j] = j;
-    for (int k = nm - 1; k >= 0; --k) {
-      const int p = piv[k];
-      if (p != k) {
-        const int tmp = src[k];
-        src[k] = src[p];
-        src[p] = tmp;
-      }
+  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, piv, src, RQg, tbuf,
+                 qparts, diag_only, red);
+}
+
+// K4 for blocks up to order 48: the same elimination (pivot rule, swaps,
+// update expressions; bit-identical results) with the matrix held in
+// registers.  Thread (ty, tx) of a TRR x TCC grid owns rows ty + TRR a
+// (a < RPT) and columns tx + TCC c (c < CPT).  A step exchanges only the
+// pivot row, the old row k and column k through LDS (3 nm values) instead of
+// reading and writing the whole matrix there; the pivot value of the next
+// step travels with the (magnitude, row) reduction.  The inverse is stored
+// to LDS once at the end for the product.
+template <int NT, int TCC, int RPT, int CPT>
+__global__ __launch_bounds__(NT) void k_invert_solve_reg(
+    const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
+    double *qparts, int diag_only) {
+  constexpr int NW = NT / 64;
+  constexpr int TRR = NT / TCC;
+  extern __shared__ double smem[];
+  __shared__ double wb_mag[NW][2];
+  __shared__ int wb_row[NW][2];
+  __shared__ cdbl wb_val[NW][2];
+  __shared__ double red[NW][4];
+
+  const int task = tasks[blockIdx.x];
+  const int item_index = task >> 1;
+  const int cls = task & 1;
+  const Item it = items[item_index];
+  const int m = it.m;
+  const int nmin = m > 0 ? m : 1;
+  const int nm = it.N + 1 - nmin;
+  const int64_t nm2 = (int64_t)nm * nm;
+  const int nb = (m == 0 && nm > 1) ? 2 : 1;
+  const int sz0 = (nb == 2) ? (nm + 1) / 2 : nm;
+  const int off[2] = {0, sz0};
+  const int sz[2] = {sz0, nm - sz0};
+  auto pidx = [&](int i) {
+    return nb == 2 ? (i < sz0 ? 2 * i : 2 * (i - sz0) + 1) : i;
+  };
+  auto blk = [&](int i) { return (nb == 2 && i >= sz0) ? 1 : 0; };
+
+  cdbl *A = reinterpret_cast<cdbl *>(smem);
+  cdbl *colk = A + nm2;
+  cdbl *rowk = colk + nm;
+  cdbl *rowold = rowk + nm;
+  int *piv = reinterpret_cast<int *>(rowold + nm);
+  int *src = piv + nm;
+
+  const cdbl *Qg = qbuf + it.q_off + cls * nm2;
+  const cdbl *RQg = qbuf + it.q_off + (2 + cls) * nm2;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int tx = t % TCC, ty = t / TCC;
+
+  cdbl R[RPT][CPT];
+#pragma unroll
+  for (int a = 0; a < RPT; ++a) {
+    const int i = ty + TRR * a;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int j = tx + TCC * c;
+      R[a][c] = (i < nm && j < nm && blk(i) == blk(j))
+                    ? Qg[(int64_t)pidx(i) * nm + pidx(j)]
+                    : cmk(0., 0.);
     }
   }
-  __syncthreads();
 
-  double s_diag2 = 0., e_diag = 0., s_all = 0., e_tr = 0.;
-  if (diag_only) {
-    // T(p, p) = -sum_k RgQ(p, k) Ainv(k, p) within the sub-block of p
-    for (int i = t; i < nm; i += NT) {
-      const int klo = i >= sz0 ? sz0 : 0, khi = i >= sz0 ? nm : sz0;
-      const int pi = pidx(i);
-      const cdbl *rq = RQg + (int64_t)pi * nm;
-      const int sc = src[i];
-      cdbl acc = cmk(0., 0.);
-      for (int k = klo; k < khi; ++k)
-        cfma(acc, rq[pidx(k)], A[(int64_t)k * nm + sc]);
-      const double w = 2. * (nmin + pi) + 1.;
-      const double a2 = acc.x * acc.x + acc.y * acc.y;
-      s_diag2 += w * a2;
-      e_diag -= w * acc.x;
-      e_tr -= acc.x;
-    }
-  } else {
-    cdbl *T = tbuf + it.t_off + cls * nm2;
-    if (nb == 2) {
-      // the two sub-blocks do not couple: zero the cross entries
-      for (int i = t >> 5; i < nm; i += NT >> 5) {
-        const int bi = i >= sz0;
-        for (int j = t & 31; j < nm; j += 32) {
-          if ((j >= sz0) != bi)
-            T[(int64_t)pidx(i) * nm + pidx(j)] = cmk(0., 0.);
-        }
+  auto wave_best = [&](double &mag, int &row, cdbl &val) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const double om = __shfl_xor(mag, o);
+      const int orow = __shfl_xor(row, o);
+      const double ox = __shfl_xor(val.x, o);
+      const double oy = __shfl_xor(val.y, o);
+      if (om > mag || (om == mag && orow < row)) {
+        mag = om;
+        row = orow;
+        val = cmk(ox, oy);
       }
     }
+  };
+  auto publish = [&](double *bm, int *br, cdbl *bv) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      wave_best(bm[b], br[b], bv[b]);
+      if (lane == 0) {
+        wb_mag[wv][b] = bm[b];
+        wb_row[wv][b] = br[b];
+        wb_val[wv][b] = bv[b];
+      }
+    }
+  };
+  // pivot candidates of the first column of each sub-block
+  {
+    double bm[2] = {-1., -1.};
+    int br[2] = {nm, nm};
+    cdbl bv[2] = {cmk(0., 0.), cmk(0., 0.)};
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       if (b >= nb) continue;
-      const int lo = off[b], hi = off[b] + sz[b];
-      const int c0 = lo + lane, c1 = lo + lane + 64;
-      const bool v0 = c0 < hi, v1 = c1 < hi;
-      const int s0 = v0 ? src[c0] : 0, s1 = v1 ? src[c1] : 0;
-      for (int rg = lo + 4 * wv; rg < hi; rg += 4 * NW) {
-        cdbl acc[4][2];
-        int prow[4];
+      const int k = off[b];
+      if (tx != k % TCC) continue;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          acc[q][0] = cmk(0., 0.);
-          acc[q][1] = cmk(0., 0.);
-          const int r = rg + q < hi ? rg + q : rg;
-          prow[q] = pidx(r);
-        }
-        for (int k = lo; k < hi; ++k) {
-          const int pk = pidx(k);
-          const cdbl a0 = A[(int64_t)k * nm + s0];
-          const cdbl a1 = A[(int64_t)k * nm + s1];
+      for (int c = 0; c < CPT; ++c) {
+        if (c != k / TCC) continue;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const cdbl bq = RQg[(int64_t)prow[q] * nm + pk];
-            cfma(acc[q][0], bq, a0);
-            cfma(acc[q][1], bq, a1);
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r = rg + q;
-          if (r >= hi) break;
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const bool valid = h == 0 ? v0 : v1;
-            if (!valid) continue;
-            const int c = h == 0 ? c0 : c1;
-            const cdbl v = cmk(-acc[q][h].x, -acc[q][h].y);
-            T[(int64_t)prow[q] * nm + pidx(c)] = v;
-            const double a2 = v.x * v.x + v.y * v.y;
-            s_all += a2;
-            if (r == c) {
-              const double w = 2. * (nmin + prow[q]) + 1.;
-              s_diag2 += w * a2;
-              e_diag += w * v.x;
-              e_tr += v.x;
-            }
+        for (int a = 0; a < RPT; ++a) {
+          const int i = ty + TRR * a;
+          if (i < off[b] || i >= off[b] + sz[b]) continue;
+          const cdbl v = R[a][c];
+          const double mag = v.x * v.x + v.y * v.y;
+          if (mag > bm[b] || (mag == bm[b] && i < br[b])) {
+            bm[b] = mag;
+            br[b] = i;
+            bv[b] = v;
           }
         }
       }
     }
-  }
-  // deterministic block reduction: waves by shuffles, then thread 0
-  double v[4] = {s_diag2, e_diag, s_all, e_tr};
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) red[wv][q] = v[q];
+    publish(bm, br, bv);
   }
   __syncthreads();
-  if (t == 0) {
-    for (int q = 0; q < 4; ++q) {
-      double acc = 0.;
-      for (int w = 0; w < NW; ++w) acc += red[w][q];
-      qparts[(int64_t)task * kQParts + q] = acc;
+
+  for (int s = 0; s < sz0; ++s) {
+    int kb[2], pb[2];
+    bool act[2];
+    cdbl inv[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      act[b] = b < nb && s < sz[b];
+      kb[b] = off[b] + s;
+      double best = -1.;
+      int row = nm;
+      cdbl val = cmk(1., 0.);
+      for (int w = 0; w < NW; ++w) {
+        const double mg = wb_mag[w][b];
+        const int rw = wb_row[w][b];
+        if (mg > best || (mg == best && rw < row)) {
+          best = mg;
+          row = rw;
+          val = wb_val[w][b];
+        }
+      }
+      pb[b] = act[b] ? row : kb[b];
+      inv[b] = act[b] ? crcp(val) : cmk(1., 0.);
+    }
+    // phase A: pivot row (scaled), old row k, column k after the swap
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+      const int i = ty + TRR * a;
+      if (i >= nm) continue;
+      const int b = blk(i);
+      if (!act[b]) continue;
+      const int k = kb[b], p = pb[b];
+      if (i == p || i == k) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+          const int j = tx + TCC * c;
+          if (j >= nm || blk(j) != b) continue;
+          if (i == p) rowk[j] = cmul(R[a][c], inv[b]);
+          if (i == k) rowold[j] = R[a][c];
+        }
+      }
+      if (tx == k % TCC) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+          if (c != k / TCC) continue;
+          const int dst = (i == p) ? k : (i == k ? p : i);
+          colk[dst] = R[a][c];
+        }
+      }
+    }
+    if (t == 0) {
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+        if (act[b]) piv[kb[b]] = pb[b];
+    }
+    __syncthreads();
+    // phase B: update, and candidates for the next pivot
+    double bm[2] = {-1., -1.};
+    int br[2] = {nm, nm};
+    cdbl bv[2] = {cmk(0., 0.), cmk(0., 0.)};
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+      const int i = ty + TRR * a;
+      if (i >= nm) continue;
+      const int b = blk(i);
+      if (!act[b]) continue;
+      const int k = kb[b], p = pb[b];
+      const cdbl ci = colk[i];
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        const int j = tx + TCC * c;
+        if (j >= nm || blk(j) != b) continue;
+        cdbl v;
+        if (i == k) {
+          v = (j == k) ? inv[b] : rowk[j];
+        } else {
+          const cdbl base = (i == p) ? rowold[j] : R[a][c];
+          if (j == k) {
+            v = cmul(ci, inv[b]);
+            v.x = -v.x;
+            v.y = -v.y;
+          } else {
+            const cdbl g = rowk[j];
+            v.x = base.x - (ci.x * g.x - ci.y * g.y);
+            v.y = base.y - (ci.x * g.y + ci.y * g.x);
+          }
+        }
+        R[a][c] = v;
+        if (j == k + 1 && i > k) {
+          const double mag = v.x * v.x + v.y * v.y;
+          if (mag > bm[b] || (mag == bm[b] && i < br[b])) {
+            bm[b] = mag;
+            br[b] = i;
+            bv[b] = v;
+          }
+        }
+      }
+    }
+    publish(bm, br, bv);
+    __syncthreads();
+  }
+  // the inverse to LDS for the product
+#pragma unroll
+  for (int a = 0; a < RPT; ++a) {
+    const int i = ty + TRR * a;
+    if (i >= nm) continue;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int j = tx + TCC * c;
+      if (j < nm) A[(int64_t)i * nm + j] = R[a][c];
     }
   }
+  __syncthreads();
+  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, piv, src, RQg, tbuf,
+                 qparts, diag_only, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -903,15 +1213,21 @@ hipError_t launch_tables(const TableSet *sets, int n, const GrainParams &gp,
   return hipGetLastError();
 }
 
-hipError_t launch_assemble(const int32_t *order, const int32_t *tile_prefix,
-                           int n_order, int n_tiles, const Item *items,
+hipError_t launch_assemble(int micro, const int32_t *order,
+                           const int32_t *tile_prefix, int n_order,
+                           int n_tiles, const Item *items,
                            const TableSet *sets, const WigKey *wkeys,
                            const GrainParams &gp, const double *tables,
                            const double *wig, cdbl *qbuf, hipStream_t s) {
   if (n_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_assemble, dim3(n_tiles), dim3(256), 0, s, order,
-                     tile_prefix, n_order, items, sets, wkeys, gp, tables, wig,
-                     qbuf);
+  if (micro == 2)
+    hipLaunchKernelGGL(k_assemble<2>, dim3(n_tiles), dim3(256), 0, s, order,
+                       tile_prefix, n_order, items, sets, wkeys, gp, tables,
+                       wig, qbuf);
+  else
+    hipLaunchKernelGGL(k_assemble<1>, dim3(n_tiles), dim3(256), 0, s, order,
+                       tile_prefix, n_order, items, sets, wkeys, gp, tables,
+                       wig, qbuf);
   return hipGetLastError();
 }
 
@@ -940,6 +1256,25 @@ static hipError_t launch_solve_nt(const int32_t *tasks, int n, size_t lds,
   return hipGetLastError();
 }
 
+template <int NT, int TCC, int RPT, int CPT>
+static hipError_t launch_solve_reg(const int32_t *tasks, int n, size_t lds,
+                                   const Item *items, const cdbl *qbuf,
+                                   cdbl *tbuf, double *qparts, int diag_only,
+                                   hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(k_invert_solve_reg<NT, TCC, RPT, CPT>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_invert_solve_reg<NT, TCC, RPT, CPT>), dim3(n),
+                     dim3(NT), lds, s, tasks, items, qbuf, tbuf, qparts,
+                     diag_only);
+  return hipGetLastError();
+}
+
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
@@ -953,13 +1288,19 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                       qparts, diag_only, gscratch,
                                       (int64_t)nm_max * nm_max, s);
   }
+  // register-resident elimination: grid and elements per thread by size
   const size_t lds = invert_solve_lds_bytes(nm_max);
   if (nm_max <= 16)
-    return launch_solve_nt<64, false>(tasks, n, lds, items, qbuf, tbuf,
-                                      qparts, diag_only, nullptr, 0, s);
+    return launch_solve_reg<64, 8, 2, 2>(tasks, n, lds, items, qbuf, tbuf,
+                                         qparts, diag_only, s);
+  if (nm_max <= 32)
+    return launch_solve_reg<256, 16, 2, 2>(tasks, n, lds, items, qbuf, tbuf,
+                                           qparts, diag_only, s);
   if (nm_max <= 48)
-    return launch_solve_nt<256, false>(tasks, n, lds, items, qbuf, tbuf,
-                                       qparts, diag_only, nullptr, 0, s);
+    return launch_solve_reg<256, 16, 3, 3>(tasks, n, lds, items, qbuf, tbuf,
+                                           qparts, diag_only, s);
+  // above order 48 one block fills a CU's LDS either way, and the
+  // LDS-resident elimination (fewer registers) measured faster
   return launch_solve_nt<512, false>(tasks, n, lds, items, qbuf, tbuf, qparts,
                                      diag_only, nullptr, 0, s);
 }

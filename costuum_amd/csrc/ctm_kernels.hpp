@@ -20,8 +20,12 @@ hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
                          double *wig, hipStream_t s);
 hipError_t launch_tables(const TableSet *sets, int n, const GrainParams &gp,
                          const double *gl, double *tables, hipStream_t s);
-hipError_t launch_assemble(const int32_t *order, const int32_t *tile_prefix,
-                           int n_order, int n_tiles, const Item *items,
+// micro = 1: 16 x 16 pair tiles (blocks up to order kSmallBlock);
+// micro = 2: 32 x 32 tiles of 2 x 2 pairs per thread.
+constexpr int kSmallBlock = 16;
+hipError_t launch_assemble(int micro, const int32_t *order,
+                           const int32_t *tile_prefix, int n_order,
+                           int n_tiles, const Item *items,
                            const TableSet *sets, const WigKey *wkeys,
                            const GrainParams &gp, const double *tables,
                            const double *wig, cdbl *qbuf, hipStream_t s);

@@ -142,6 +142,17 @@ def test_large_grains_converge_like_oracle(ctx):
     for g, (rv, d, lam, mr) in enumerate(cases):
         st, n, ng, _, _ = O.converge_task(rv, d, lam, mr, 4, 160, 2, 1e-4)
         assert (int(nmax[g]), int(status[g])) == (n, st)
-        s_ref, e_ref = O.qcoeffs(n, O.tmatrix_all(rv, d, lam, mr, n, ng))
+        Tref = O.tmatrix_all(rv, d, lam, mr, n, ng)
+        s_ref, e_ref = O.qcoeffs(n, Tref)
         assert rel(r.qsca.cpu().numpy()[g], s_ref) < 1e-8
         assert rel(r.qext.cpu().numpy()[g], e_ref) < 1e-8
+    # every entry of the order-131 T-matrix (m = 1 blocks are 131 wide:
+    # the product covers columns beyond 128), bar as above
+    rv, d, lam, mr = cases[1]
+    n = int(nmax[1])
+    ng = 2 * n
+    Tref = O.tmatrix_all(rv, d, lam, mr, n, ng)
+    T2 = O.tmatrix_all(rv * (1 + 4e-16), d, lam, mr, n, ng)
+    scale = np.abs(Tref).max()
+    bar = max(1e-8, 50 * np.abs(T2 - Tref).max() / scale)
+    assert np.abs(r.tmatrix(1) - Tref).max() / scale < bar
