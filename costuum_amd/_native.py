@@ -24,7 +24,8 @@ EXPORTS = (
     "ctm_observables", "ctm_orientation_average", "ctm_average_get",
     "ctm_query", "ctm_m0_sums", "ctm_gauss_legendre", "ctm_rv_over_rs",
     "ctm_adaptive_quadrature", "ctm_orientation_coefficients",
-    "ctm_weighted_reduce",
+    "ctm_weighted_reduce", "ctm_tmatrix_tangents", "ctm_tangent_get",
+    "ctm_observables_select", "ctm_vjp",
 )
 
 # double f(double x, void *arg): the host callback type (ctm_real_fn)
@@ -128,6 +129,15 @@ def lib():
                               dp, dp]
     L.ctm_m0_sums.restype = ctypes.c_int
     u32 = ctypes.c_uint32
+    L.ctm_tmatrix_tangents.argtypes = [vp, dp, dp]
+    L.ctm_tmatrix_tangents.restype = ctypes.c_int
+    L.ctm_tangent_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp,
+                                  ctypes.c_size_t]
+    L.ctm_tangent_get.restype = ctypes.c_int
+    L.ctm_observables_select.argtypes = [vp, ctypes.c_int]
+    L.ctm_observables_select.restype = ctypes.c_int
+    L.ctm_vjp.argtypes = [vp, dp, dp, dp]
+    L.ctm_vjp.restype = ctypes.c_int
     L.ctm_weighted_reduce.argtypes = [vp, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int64, dp, dp, ctypes.c_int,
                                       dp]

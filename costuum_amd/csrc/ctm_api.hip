@@ -33,6 +33,7 @@
 #include "ctm_observables.hpp"
 #include "ctm_average.hpp"
 #include "ctm_host.hpp"
+#include "ctm_tangent.hpp"
 
 using namespace ctm;
 
@@ -245,7 +246,8 @@ struct ctm_context {
       d_tiles, d_solve, d_q, d_t, d_qparts, d_state, d_active, d_winoff,
       d_winitems, d_gioff, d_gitems, d_gioff_tmp, d_params, d_ang, d_pt,
       d_x, d_spec, d_cgkeys, d_cg, d_cgidx, d_avgtasks, d_avg, d_aoff, d_pn,
-      d_dist, d_obsitems, d_query, d_qout, d_gsolve;
+      d_dist, d_obsitems, d_query, d_qout, d_gsolve, d_ttables, d_dq, d_ainv,
+      d_dt, d_tparts, d_dsum, d_ttiles, d_tprefix;
   // Results of the last batch (host).
   int n_grains = 0;
   std::vector<uint32_t> nmax, ngauss;
@@ -255,6 +257,11 @@ struct ctm_context {
   std::vector<int32_t> grain_items;
   GrainParams gp_saved{};  // context copies of the last batch's inputs
   bool averaged = false;   // d_avg holds ensemble T-matrices of the batch
+  // forward-mode tangents of the batch (d_dt) and the observable source:
+  // -1 the batch's T-matrices, 0..3 the tangent along one parameter
+  bool have_tangents = false;
+  int obs_source = -1;
+  std::vector<double> dqsca, dqext;  // [n][4] tangents of the full-T sums
   std::vector<int64_t> aoff;  // full-layout offsets per final item
   std::vector<int32_t> obs_items;
   Profiler prof;
@@ -395,6 +402,69 @@ void build_tasks(WorkList &w) {
   }
 }
 
+// Inversion and solve of the current work list's blocks (Q in ctx->d_q),
+// in LDS-size buckets; ainv_out (may be NULL) receives every Q_c^-1.
+static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
+  WorkList &w = ctx->work;
+  hipStream_t s = ctx->stream;
+  // solve tasks in LDS-size buckets, largest first; blocks above the LDS
+  // capacity are inverted in global scratch, at most kGlobalSolveBytes of it
+  // per launch
+  constexpr size_t kGlobalSolveBytes = (size_t)1 << 30;
+  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 48, 32, 16};
+  constexpr int kBuckets = 5;
+  size_t pos = 0;
+  const int32_t *dsolve = ctx->d_solve.as<int32_t>();
+  for (int b = 0; b < kBuckets && pos < w.solve.size(); ++b) {
+    const int hi = bounds[b];
+    const int lo = (b < kBuckets - 1) ? bounds[b + 1] : 0;
+    size_t end = pos;
+    int nm_max = 0;
+    size_t cap = w.solve.size();
+    while (end < w.solve.size() && end - pos < cap) {
+      const Item &it = w.items[w.solve[end] >> 1];
+      const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+      if (nm <= lo) break;
+      if (nm > hi) {
+        ctx->err = "block order exceeds the largest supported order";
+        return CTM_ERR_ARGUMENT;
+      }
+      if (nm_max == 0 && nm > kMaxLdsOrder)
+        cap = std::max<size_t>(
+            1, kGlobalSolveBytes / ((size_t)nm * nm * sizeof(cdbl)));
+      nm_max = std::max(nm_max, nm);
+      ++end;
+    }
+    if (nm_max > kMaxLdsOrder)
+      CTM_HIP(ctx->d_gsolve.ensure((end - pos) * (size_t)nm_max * nm_max *
+                                   sizeof(cdbl)));
+    if (end > pos) {
+      double fs = 0., bs = 0.;
+      for (size_t q = pos; q < end; ++q) {
+        const Item &it = w.items[w.solve[q] >> 1];
+        const double nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+        fs += (it.m > 0 ? 32. / 3. : 8. / 3.) * nm * nm * nm;
+        bs += 16. * 2. * nm * nm + (want_t ? 16. * nm * nm : 0.);
+      }
+      CTM_LAUNCH(CTM_KERNEL_SOLVE, fs, bs,
+                 launch_invert_solve(dsolve + pos, (int)(end - pos), nm_max,
+                                     ctx->d_items.as<Item>(),
+                                     ctx->d_q.as<cdbl>(),
+                                     want_t ? ctx->d_t.as<cdbl>() : nullptr,
+                                     ctx->d_qparts.as<double>(),
+                                     want_t ? 0 : 1,
+                                     ctx->d_gsolve.as<cdbl>(), ainv_out, s));
+    }
+    pos = end;
+    // the global bucket may need several launches
+    if (b == 0 && pos < w.solve.size()) {
+      const Item &it = w.items[w.solve[pos] >> 1];
+      if (it.N + 1 - (it.m > 0 ? it.m : 1) > kMaxLdsOrder) --b;
+    }
+  }
+  return CTM_OK;
+}
+
 // Runs tables + assembly + inversion/solve for the current work list.
 int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   WorkList &w = ctx->work;
@@ -466,62 +536,7 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
                                  ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(),
                                  s));
   }
-  // solve tasks in LDS-size buckets, largest first; blocks above the LDS
-  // capacity are inverted in global scratch, at most kGlobalSolveBytes of it
-  // per launch
-  constexpr size_t kGlobalSolveBytes = (size_t)1 << 30;
-  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 48, 32, 16};
-  constexpr int kBuckets = 5;
-  size_t pos = 0;
-  const int32_t *dsolve = ctx->d_solve.as<int32_t>();
-  for (int b = 0; b < kBuckets && pos < w.solve.size(); ++b) {
-    const int hi = bounds[b];
-    const int lo = (b < kBuckets - 1) ? bounds[b + 1] : 0;
-    size_t end = pos;
-    int nm_max = 0;
-    size_t cap = w.solve.size();
-    while (end < w.solve.size() && end - pos < cap) {
-      const Item &it = w.items[w.solve[end] >> 1];
-      const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-      if (nm <= lo) break;
-      if (nm > hi) {
-        ctx->err = "block order exceeds the largest supported order";
-        return CTM_ERR_ARGUMENT;
-      }
-      if (nm_max == 0 && nm > kMaxLdsOrder)
-        cap = std::max<size_t>(
-            1, kGlobalSolveBytes / ((size_t)nm * nm * sizeof(cdbl)));
-      nm_max = std::max(nm_max, nm);
-      ++end;
-    }
-    if (nm_max > kMaxLdsOrder)
-      CTM_HIP(ctx->d_gsolve.ensure((end - pos) * (size_t)nm_max * nm_max *
-                                   sizeof(cdbl)));
-    if (end > pos) {
-      double fs = 0., bs = 0.;
-      for (size_t q = pos; q < end; ++q) {
-        const Item &it = w.items[w.solve[q] >> 1];
-        const double nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-        fs += (it.m > 0 ? 32. / 3. : 8. / 3.) * nm * nm * nm;
-        bs += 16. * 2. * nm * nm + (want_t ? 16. * nm * nm : 0.);
-      }
-      CTM_LAUNCH(CTM_KERNEL_SOLVE, fs, bs,
-                 launch_invert_solve(dsolve + pos, (int)(end - pos), nm_max,
-                                     ctx->d_items.as<Item>(),
-                                     ctx->d_q.as<cdbl>(),
-                                     want_t ? ctx->d_t.as<cdbl>() : nullptr,
-                                     ctx->d_qparts.as<double>(),
-                                     want_t ? 0 : 1,
-                                     ctx->d_gsolve.as<cdbl>(), s));
-    }
-    pos = end;
-    // the global bucket may need several launches
-    if (b == 0 && pos < w.solve.size()) {
-      const Item &it = w.items[w.solve[pos] >> 1];
-      if (it.N + 1 - (it.m > 0 ? it.m : 1) > kMaxLdsOrder) --b;
-    }
-  }
-  return CTM_OK;
+  return run_solves(ctx, want_t, nullptr);
 }
 
 // Rough order estimate from the largest size parameter of the spheroid
@@ -618,6 +633,8 @@ static int run_all_m(ctm_context *ctx, const GrainParams &gp, int n,
     }
   }
   ctx->grain_items_off[n] = (int)w.items.size();
+  ctx->have_tangents = false;
+  ctx->obs_source = -1;
   // keep the inputs for the observable calls on this batch
   CTM_HIP(ctx->d_params.ensure(5 * (size_t)n * sizeof(double)));
   {
@@ -815,8 +832,10 @@ size_t ctm_tmatrix_size(ctm_context *ctx, int g) {
   return total;
 }
 
-int ctm_tmatrix_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
-  if (!ctx) return CTM_ERR_ARGUMENT;
+// Grain g's blocks of a class-layout buffer (T or a tangent) in the
+// reference block layout.
+static int get_blocks(ctm_context *ctx, int g, const cdbl *base,
+                      double *T_host, size_t capacity) {
   const size_t need = ctm_tmatrix_size(ctx, g);
   if (need == 0 || !T_host || capacity < need) {
     ctx->err = "invalid grain index or output capacity";
@@ -833,7 +852,7 @@ int ctm_tmatrix_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
     const int nm = N + 1 - nmin;
     const size_t nm2 = (size_t)nm * nm;
     blk.resize(2 * nm2);
-    CTM_HIP(hipMemcpyAsync(blk.data(), ctx->d_t.as<cdbl>() + it.t_off,
+    CTM_HIP(hipMemcpyAsync(blk.data(), base + it.t_off,
                            2 * nm2 * sizeof(cdbl), hipMemcpyDeviceToHost,
                            ctx->stream));
     CTM_HIP(hipStreamSynchronize(ctx->stream));
@@ -852,6 +871,132 @@ int ctm_tmatrix_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
     }
     out_off += (size_t)n2m * n2m;
   }
+  return CTM_OK;
+}
+
+int ctm_tmatrix_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  return get_blocks(ctx, g, ctx->d_t.as<cdbl>(), T_host, capacity);
+}
+
+int ctm_tmatrix_tangents(ctm_context *ctx, double *dqsca_host,
+                         double *dqext_host) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (ctx->final_items.empty()) {
+    ctx->err = "no T-matrices in the context";
+    return CTM_ERR_STATE;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  WorkList &w = ctx->work;  // the batch's all-m work list
+  const int n = ctx->n_grains;
+  const int ni = (int)w.items.size();
+  // tangent tables
+  CTM_HIP(ctx->d_ttables.ensure(std::max<int64_t>(w.table_size, 1) *
+                                kTangentDirs * sizeof(double)));
+  CTM_LAUNCH(CTM_KERNEL_TABLES, 0., 0.,
+             launch_tables_tangent(ctx->d_sets.as<TableSet>(),
+                                   (int)w.sets.size(), ctx->gp_saved,
+                                   ctx->d_gl.as<double>(),
+                                   ctx->d_tables.as<double>(), w.table_size,
+                                   ctx->d_ttables.as<double>(), s));
+  // tangent assembly over 16 x 16 pair tiles of every item
+  std::vector<int32_t> order(ni), prefix(ni + 1, 0);
+  for (int i = 0; i < ni; ++i) {
+    order[i] = i;
+    const Item &it = w.items[i];
+    const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+    const int ntr = (nm + 15) / 16;
+    const int ntc = it.m == 0 ? (nm + 31) / 32 : ntr;
+    prefix[i + 1] = prefix[i] + ntr * ntc;
+  }
+  CTM_HIP(upload(ctx->d_ttiles, order, s));
+  CTM_HIP(upload(ctx->d_tprefix, prefix, s));
+  CTM_HIP(ctx->d_dq.ensure(std::max<int64_t>(w.q_size, 1) * kTangentDirs *
+                           sizeof(cdbl)));
+  CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, 0., 0.,
+             launch_assemble_tangent(
+                 ctx->d_ttiles.as<int32_t>(), ctx->d_tprefix.as<int32_t>(), ni,
+                 prefix[ni], ctx->d_items.as<Item>(),
+                 ctx->d_sets.as<TableSet>(), ctx->d_wigkeys.as<WigKey>(),
+                 ctx->gp_saved, ctx->d_tables.as<double>(),
+                 ctx->d_ttables.as<double>(), w.table_size,
+                 ctx->d_wig.as<double>(), ctx->d_dq.as<cdbl>(), w.q_size, s));
+  // Q_c^-1 of every block (the solve is repeated; T comes out unchanged)
+  CTM_HIP(ctx->d_ainv.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
+  int rc = run_solves(ctx, true, ctx->d_ainv.as<cdbl>());
+  if (rc != CTM_OK) return rc;
+  CTM_HIP(ctx->d_dt.ensure(std::max<int64_t>(w.t_size, 1) * kTangentDirs *
+                           sizeof(cdbl)));
+  CTM_HIP(ctx->d_tparts.ensure((size_t)std::max(ni, 1) * 2 * kTangentDirs * 2 *
+                               sizeof(double)));
+  CTM_LAUNCH(CTM_KERNEL_SOLVE, 0., 0.,
+             launch_tangent_solve(2 * ni, ctx->d_items.as<Item>(),
+                                  ctx->d_t.as<cdbl>(), ctx->d_ainv.as<cdbl>(),
+                                  ctx->d_dq.as<cdbl>(), w.q_size,
+                                  ctx->d_dt.as<cdbl>(), w.t_size,
+                                  ctx->d_tparts.as<double>(), s));
+  CTM_HIP(ctx->d_dsum.ensure((size_t)std::max(n, 1) * 2 * kTangentDirs *
+                             sizeof(double)));
+  double *dsum = ctx->d_dsum.as<double>();
+  CTM_LAUNCH(CTM_KERNEL_SUMS, 0., 0.,
+             launch_tangent_sums(n, ctx->d_gioff.as<int32_t>(),
+                                 ctx->d_tparts.as<double>(), dsum,
+                                 dsum + (size_t)n * kTangentDirs, s));
+  ctx->dqsca.resize((size_t)n * kTangentDirs);
+  ctx->dqext.resize((size_t)n * kTangentDirs);
+  CTM_HIP(hipMemcpyAsync(ctx->dqsca.data(), dsum,
+                         ctx->dqsca.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  CTM_HIP(hipMemcpyAsync(ctx->dqext.data(), dsum + (size_t)n * kTangentDirs,
+                         ctx->dqext.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  CTM_HIP(hipStreamSynchronize(s));
+  if (dqsca_host)
+    std::copy(ctx->dqsca.begin(), ctx->dqsca.end(), dqsca_host);
+  if (dqext_host)
+    std::copy(ctx->dqext.begin(), ctx->dqext.end(), dqext_host);
+  ctx->have_tangents = true;
+  return CTM_OK;
+}
+
+int ctm_tangent_get(ctm_context *ctx, int g, int dir, double *dT_host,
+                    size_t capacity) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (!ctx->have_tangents || dir < 0 || dir >= kTangentDirs) {
+    ctx->err = "no tangents computed, or direction out of range";
+    return CTM_ERR_STATE;
+  }
+  return get_blocks(ctx, g,
+                    ctx->d_dt.as<cdbl>() + (int64_t)dir * ctx->work.t_size,
+                    dT_host, capacity);
+}
+
+int ctm_observables_select(ctm_context *ctx, int which) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (which < -1 || which >= kTangentDirs ||
+      (which >= 0 && !ctx->have_tangents)) {
+    ctx->err = "no such tangent (call ctm_tmatrix_tangents first)";
+    return CTM_ERR_ARGUMENT;
+  }
+  ctx->obs_source = which;
+  return CTM_OK;
+}
+
+int ctm_vjp(ctm_context *ctx, const double *cot_qsca, const double *cot_qext,
+            double *grad) {
+  if (!ctx || !grad) return CTM_ERR_ARGUMENT;
+  if (!ctx->have_tangents) {
+    ctx->err = "no tangents computed (call ctm_tmatrix_tangents first)";
+    return CTM_ERR_STATE;
+  }
+  const int n = ctx->n_grains;
+  for (int g = 0; g < n; ++g)
+    for (int p = 0; p < kTangentDirs; ++p) {
+      const size_t i = (size_t)g * kTangentDirs + p;
+      grad[i] = (cot_qsca ? cot_qsca[g] * ctx->dqsca[i] : 0.) +
+                (cot_qext ? cot_qext[g] * ctx->dqext[i] : 0.);
+    }
   return CTM_OK;
 }
 
@@ -878,6 +1023,12 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   if (nth < 0 || ng < 0 || nst < 0 || (nth > 0 && !spec->theta) ||
       (spec->account_for_scattering && spec->ngauss < 1)) {
     ctx->err = "invalid observable specification";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (ctx->obs_source >= 0 &&
+      (spec->account_for_scattering || nst > 0 || ctx->averaged)) {
+    ctx->err = "with a tangent selected only observables linear in T "
+               "(no scattered-light correction, no Z, no averaging)";
     return CTM_ERR_ARGUMENT;
   }
   CTM_HIP(hipSetDevice(ctx->device));
@@ -1034,7 +1185,12 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
         CTM_LAUNCH(CTM_KERNEL_OBS_X, fb, 0.,
                    launch_obs_x(sp, ctx->d_obsitems.as<int32_t>() + pos,
                                 (int)(end - pos), cap,
-                                ctx->d_items.as<Item>(), ctx->d_t.as<cdbl>(),
+                                ctx->d_items.as<Item>(),
+                                ctx->obs_source >= 0
+                                    ? ctx->d_dt.as<cdbl>() +
+                                          (int64_t)ctx->obs_source *
+                                              ctx->work.t_size
+                                    : ctx->d_t.as<cdbl>(),
                                 ctx->averaged ? ctx->d_avg.as<cdbl>() : nullptr,
                                 ctx->averaged ? ctx->d_aoff.as<int64_t>()
                                               : nullptr,
@@ -1262,6 +1418,8 @@ int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
   // the work lists below replace the batch's items
   ctx->final_items.clear();
   ctx->averaged = false;
+  ctx->have_tangents = false;
+  ctx->obs_source = -1;
   ctx->n_grains = 0;
   ctx->shared.clear();
   if (n == 0) return CTM_OK;

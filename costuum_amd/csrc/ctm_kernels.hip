@@ -574,7 +574,7 @@ __device__ __forceinline__ void solve_tail(
     const Item &it, int task, int cls, int nm, int nb, int sz0,
     const int *off, const int *sz, const cdbl *A, const int *piv, int *src,
     const cdbl *RQg, cdbl *tbuf, double *qparts, int diag_only,
-    double (*red)[4]) {
+    double (*red)[4], cdbl *ainv_out) {
   constexpr int NW = NT / 64;
   const int m = it.m;
   const int nmin = m > 0 ? m : 1;
@@ -596,6 +596,18 @@ __device__ __forceinline__ void solve_tail(
     }
   }
   __syncthreads();
+  if (ainv_out) {
+    // Q_c^-1 in the class layout: Q^-1(pidx(k), pidx(c)) = A(k, src(c))
+    cdbl *Ai = ainv_out + it.t_off + cls * nm2;
+    for (int k = t >> 5; k < nm; k += NT >> 5) {
+      const bool bk = k >= sz0;
+      for (int c = t & 31; c < nm; c += 32) {
+        const bool same = nb == 1 || (c >= sz0) == bk;
+        Ai[(int64_t)pidx(k) * nm + pidx(c)] =
+            same ? A[(int64_t)k * nm + src[c]] : cmk(0., 0.);
+      }
+    }
+  }
 
   double s_diag2 = 0., e_diag = 0., s_all = 0., e_tr = 0.;
   if (diag_only) {
@@ -704,7 +716,8 @@ __device__ __forceinline__ void solve_tail(
 template <int NT, bool GLOBAL>
 __global__ __launch_bounds__(NT) void k_invert_solve(
     const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
-    double *qparts, int diag_only, cdbl *gscratch, int64_t gstride) {
+    double *qparts, int diag_only, cdbl *gscratch, int64_t gstride,
+    cdbl *ainv_out) {
   constexpr int NW = NT / 64;
   extern __shared__ double smem[];
   __shared__ double wb_mag[NW][2];
@@ -878,7 +891,7 @@ __global__ __launch_bounds__(NT) void k_invert_solve(
     __syncthreads();
   }
   solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, piv, src, RQg, tbuf,
-                 qparts, diag_only, red);
+                 qparts, diag_only, red, ainv_out);
 }
 
 // K4 for blocks up to order 48: the same elimination (pivot rule, swaps,
@@ -892,7 +905,7 @@ __global__ __launch_bounds__(NT) void k_invert_solve(
 template <int NT, int TCC, int RPT, int CPT>
 __global__ __launch_bounds__(NT) void k_invert_solve_reg(
     const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
-    double *qparts, int diag_only) {
+    double *qparts, int diag_only, cdbl *ainv_out) {
   constexpr int NW = NT / 64;
   constexpr int TRR = NT / TCC;
   extern __shared__ double smem[];
@@ -1112,7 +1125,7 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
   }
   __syncthreads();
   solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, piv, src, RQg, tbuf,
-                 qparts, diag_only, red);
+                 qparts, diag_only, red, ainv_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1241,7 +1254,7 @@ static hipError_t launch_solve_nt(const int32_t *tasks, int n, size_t lds,
                                   const Item *items, const cdbl *qbuf,
                                   cdbl *tbuf, double *qparts, int diag_only,
                                   cdbl *gscratch, int64_t gstride,
-                                  hipStream_t s) {
+                                  cdbl *ainv_out, hipStream_t s) {
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
@@ -1252,7 +1265,7 @@ static hipError_t launch_solve_nt(const int32_t *tasks, int n, size_t lds,
   }
   hipLaunchKernelGGL((k_invert_solve<NT, GLOBAL>), dim3(n), dim3(NT), lds, s,
                      tasks, items, qbuf, tbuf, qparts, diag_only, gscratch,
-                     gstride);
+                     gstride, ainv_out);
   return hipGetLastError();
 }
 
@@ -1260,7 +1273,7 @@ template <int NT, int TCC, int RPT, int CPT>
 static hipError_t launch_solve_reg(const int32_t *tasks, int n, size_t lds,
                                    const Item *items, const cdbl *qbuf,
                                    cdbl *tbuf, double *qparts, int diag_only,
-                                   hipStream_t s) {
+                                   cdbl *ainv_out, hipStream_t s) {
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
@@ -1271,14 +1284,14 @@ static hipError_t launch_solve_reg(const int32_t *tasks, int n, size_t lds,
   }
   hipLaunchKernelGGL((k_invert_solve_reg<NT, TCC, RPT, CPT>), dim3(n),
                      dim3(NT), lds, s, tasks, items, qbuf, tbuf, qparts,
-                     diag_only);
+                     diag_only, ainv_out);
   return hipGetLastError();
 }
 
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
-                               hipStream_t s) {
+                               cdbl *ainv_out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (nm_max > kMaxLdsOrder) {
     // the matrix lives in gscratch (n * nm_max^2 complex values)
@@ -1286,23 +1299,23 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                        (size_t)nm_max * nm_max * sizeof(cdbl);
     return launch_solve_nt<512, true>(tasks, n, lds, items, qbuf, tbuf,
                                       qparts, diag_only, gscratch,
-                                      (int64_t)nm_max * nm_max, s);
+                                      (int64_t)nm_max * nm_max, ainv_out, s);
   }
   // register-resident elimination: grid and elements per thread by size
   const size_t lds = invert_solve_lds_bytes(nm_max);
   if (nm_max <= 16)
     return launch_solve_reg<64, 8, 2, 2>(tasks, n, lds, items, qbuf, tbuf,
-                                         qparts, diag_only, s);
+                                         qparts, diag_only, ainv_out, s);
   if (nm_max <= 32)
     return launch_solve_reg<256, 16, 2, 2>(tasks, n, lds, items, qbuf, tbuf,
-                                           qparts, diag_only, s);
+                                           qparts, diag_only, ainv_out, s);
   if (nm_max <= 48)
     return launch_solve_reg<256, 16, 3, 3>(tasks, n, lds, items, qbuf, tbuf,
-                                           qparts, diag_only, s);
+                                           qparts, diag_only, ainv_out, s);
   // above order 48 one block fills a CU's LDS either way, and the
   // LDS-resident elimination (fewer registers) measured faster
   return launch_solve_nt<512, false>(tasks, n, lds, items, qbuf, tbuf, qparts,
-                                     diag_only, nullptr, 0, s);
+                                     diag_only, nullptr, 0, ainv_out, s);
 }
 
 hipError_t launch_converge(const int32_t *active, int n_active,

@@ -35,11 +35,12 @@ constexpr int kMaxLdsOrder = 99;
 // Largest order the library computes (larger blocks are inverted in global
 // scratch).
 constexpr int kMaxOrder = 200;
-// gscratch: n * nm_max^2 complex values, used when nm_max > kMaxLdsOrder.
+// gscratch: n * nm_max^2 complex values, used when nm_max > kMaxLdsOrder;
+// ainv_out (may be NULL): Q_c^-1 per task in the T layout.
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
-                               hipStream_t s);
+                               cdbl *ainv_out, hipStream_t s);
 hipError_t launch_converge(const int32_t *active, int n_active,
                            const int32_t *win_off, const int32_t *win_items,
                            const Item *items, const double *qparts, double tol,

@@ -1,0 +1,39 @@
+// ctm_tangent.hpp -- forward-mode derivatives of the T-matrices with
+// respect to the grain parameters (R_V, d, Re m, Im m).
+#pragma once
+
+#include "ctm_device.hpp"
+#include "ctm_kernels.hpp"
+
+namespace ctm {
+
+// Parameter directions of the tangents.
+constexpr int kTangentDirs = 4;  // R_V, d, Re m, Im m
+
+// Tangent tables: ttables[dir * table_size + layout of tables].
+hipError_t launch_tables_tangent(const TableSet *sets, int n_sets,
+                                 const GrainParams &gp, const double *gl,
+                                 const double *tables, int64_t table_size,
+                                 double *ttables, hipStream_t s);
+// dQ, dRgQ per direction: dq[dir * q_size + layout of Q].  order /
+// tile_prefix: every item with 16 x 16 pair tiles.
+hipError_t launch_assemble_tangent(const int32_t *order,
+                                   const int32_t *tile_prefix, int n_order,
+                                   int n_tiles, const Item *items,
+                                   const TableSet *sets, const WigKey *wkeys,
+                                   const GrainParams &gp, const double *tables,
+                                   const double *ttables, int64_t table_size,
+                                   const double *wig, cdbl *dq, int64_t q_size,
+                                   hipStream_t s);
+// M = dRgQ + T dQ (in place of dRgQ), then dT = -M Q^-1 and the per-block
+// tangent sums tparts[(task * 4 + dir) * 2 + {0: d sum |T|^2, 1: d tr Re T}].
+hipError_t launch_tangent_solve(int n_tasks, const Item *items,
+                                const cdbl *tbuf, const cdbl *ainv, cdbl *dq,
+                                int64_t q_size, cdbl *dt, int64_t t_size,
+                                double *tparts, hipStream_t s);
+// Per-grain tangents of the full-T sums (f_m = 1 for m = 0, else 2).
+hipError_t launch_tangent_sums(int n_grains, const int32_t *gi_off,
+                               const double *tparts, double *dqsca,
+                               double *dqext, hipStream_t s);
+
+}  // namespace ctm

@@ -291,3 +291,107 @@ def m0_sums(radius, axis_ratio, wavelength, mr_re, mr_im, nmax, ngauss,
         ctx.handle, n, *[_ptr(a) for a in arrs], nm.ctypes.data,
         ng.ctypes.data, qs.ctypes.data, qe.ctypes.data), "ctm_m0_sums")
     return qs, qe
+
+
+# ---------------------------------------------------------------------------
+# Derivatives with respect to the grain parameters (forward mode in the
+# library; no reference counterpart)
+
+PARAMS = ("radius", "axis_ratio", "mr_re", "mr_im")
+
+
+def tangents(res):
+    """d(Q_sca, Q_ext full-T sums)/dp for p in PARAMS at the batch's frozen
+    orders: two arrays [n_grains, 4].  The tangent T-matrices stay in the
+    context for tangent_tmatrix / observables_tangent."""
+    ctx = res.ctx
+    dqs = np.zeros((res.n, 4))
+    dqe = np.zeros((res.n, 4))
+    ctx.check(ctx._lib.ctm_tmatrix_tangents(ctx.handle, dqs.ctypes.data,
+                                            dqe.ctypes.data),
+              "ctm_tmatrix_tangents")
+    return dqs, dqe
+
+
+def tangent_tmatrix(res, g, direction):
+    """dT/dp of grain g in the reference block layout."""
+    L = res.ctx._lib
+    size = L.ctm_tmatrix_size(res.ctx.handle, g)
+    out = np.zeros(size, dtype=np.complex128)
+    res.ctx.check(L.ctm_tangent_get(res.ctx.handle, g, direction,
+                                    out.ctypes.data, size), "ctm_tangent_get")
+    return out
+
+
+def observables_tangent(theta, direction, ctx, n_grains, radius=None,
+                        do_extinction=True, do_absorption=True):
+    """d/dp of the observables linear in T (extinction; absorption without
+    the scattered-light correction) at the angles theta, for p =
+    PARAMS[direction].  For p = R_V the normalisation 1 / (pi R_V^2) adds
+    -2 Q / R_V; radius (per grain) is required then."""
+    ctx.check(ctx._lib.ctm_observables_select(ctx.handle, int(direction)),
+              "ctm_observables_select")
+    try:
+        d = observables(theta, ctx=ctx, do_extinction=do_extinction,
+                        do_absorption=do_absorption,
+                        account_for_scattering=False, n_grains=n_grains)
+    finally:
+        ctx.check(ctx._lib.ctm_observables_select(ctx.handle, -1),
+                  "ctm_observables_select")
+    if direction == 0:
+        if radius is None:
+            raise ValueError("radius is required for the R_V direction")
+        q = observables(theta, ctx=ctx, do_extinction=do_extinction,
+                        do_absorption=do_absorption,
+                        account_for_scattering=False, n_grains=n_grains)
+        rv = torch.as_tensor(np.asarray(radius, dtype=np.float64),
+                             device=ctx.device).reshape(-1, 1)
+        for k in d:
+            d[k] = d[k] - 2.0 * q[k] / rv
+    return d
+
+
+def vjp(res, cot_qsca=None, cot_qext=None):
+    """Vector-Jacobian product of the full-T sums: [n_grains, 4]."""
+    ctx = res.ctx
+    cs = None if cot_qsca is None else np.ascontiguousarray(cot_qsca,
+                                                           dtype=np.float64)
+    ce = None if cot_qext is None else np.ascontiguousarray(cot_qext,
+                                                           dtype=np.float64)
+    out = np.zeros((res.n, 4))
+    ctx.check(ctx._lib.ctm_vjp(ctx.handle,
+                               cs.ctypes.data if cs is not None else None,
+                               ce.ctypes.data if ce is not None else None,
+                               out.ctypes.data), "ctm_vjp")
+    return out
+
+
+class _TMatrixSums(torch.autograd.Function):
+    """(radius, axis_ratio, mr_re, mr_im) -> (Q_sca, Q_ext) full-T sums at
+    frozen orders, differentiable through the library's tangents."""
+
+    @staticmethod
+    def forward(fctx, radius, axis_ratio, mr_re, mr_im, wavelength, nmax,
+                ngauss, ctx):
+        res = tmatrix_fixed(radius.detach(), axis_ratio.detach(),
+                            wavelength, mr_re.detach(), mr_im.detach(), nmax,
+                            ngauss, ctx=ctx)
+        dqs, dqe = tangents(res)
+        fctx.save_for_backward(torch.as_tensor(dqs, device=radius.device),
+                               torch.as_tensor(dqe, device=radius.device))
+        return res.qsca.clone(), res.qext.clone()
+
+    @staticmethod
+    def backward(fctx, gs, ge):
+        dqs, dqe = fctx.saved_tensors
+        g = gs.reshape(-1, 1) * dqs + ge.reshape(-1, 1) * dqe
+        return g[:, 0], g[:, 1], g[:, 2], g[:, 3], None, None, None, None
+
+
+def tmatrix_sums(radius, axis_ratio, wavelength, mr_re, mr_im, nmax, ngauss,
+                 ctx=None):
+    """Differentiable (Q_sca, Q_ext) full-T sums per grain (torch autograd
+    over radius, axis_ratio, mr_re, mr_im) at forced orders."""
+    ctx = ctx or default_context()
+    return _TMatrixSums.apply(radius, axis_ratio, mr_re, mr_im, wavelength,
+                              nmax, ngauss, ctx)

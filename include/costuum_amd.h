@@ -196,6 +196,31 @@ int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
                 const uint32_t *nmax_host, const uint32_t *ngauss_host,
                 double *qsca_host, double *qext_host);
 
+/* Forward-mode derivatives of the last batch's T-matrices with respect to
+ * the grain parameters p = (R_V, d, Re m, Im m), at the batch's orders
+ * (frozen nmax and ngauss).  No reference counterpart: the reference has no
+ * derivatives; this differentiates its assembly and T = -RgQ Q^-1
+ * (TMatrixResource.hpp:623-730,993-1119) exactly:
+ * dT = -(dRgQ + T dQ) Q^-1.  Outputs (host, may be NULL): [n_grains][4]
+ * derivatives of the full-T sums Q_sca and Q_ext (TMatrix.hpp:2055-2110). */
+int ctm_tmatrix_tangents(ctm_context *ctx, double *dqsca_host,
+                         double *dqext_host);
+/* dT/dp_dir of grain g in the reference block layout (dir 0..3). */
+int ctm_tangent_get(ctm_context *ctx, int grain, int dir, double *dT_host,
+                    size_t capacity);
+/* Which T-matrices ctm_observables reads: -1 the batch's own, 0..3 the
+ * tangent along p_dir.  With a tangent selected only the observables linear
+ * in T are allowed (extinction; absorption without the scattered-light
+ * correction; no Z, no orientation average): they return the derivative of
+ * the cross section divided by pi R_V^2 (the caller adds -2 Q / R_V for
+ * p = R_V). */
+int ctm_observables_select(ctm_context *ctx, int which);
+/* Vector-Jacobian product of the full-T sums for per-grain cotangents
+ * (host, may be NULL): grad[g][p] = cot_qsca[g] dQ_sca/dp +
+ * cot_qext[g] dQ_ext/dp, after ctm_tmatrix_tangents. */
+int ctm_vjp(ctm_context *ctx, const double *cot_qsca, const double *cot_qext,
+            double *grad);
+
 /* Host-only helpers (no GPU work).  ctm_real_fn is a real function of one
  * real variable with a caller argument, as the reference passes its Python
  * callbacks (PyOrientationDistribution.hpp:501-533). */

@@ -1,0 +1,156 @@
+"""Derivatives with respect to (R_V, d, Re m, Im m) at frozen orders:
+the library's forward-mode tangents against central differences of the GPU
+forward path and of the CPU oracle (relative step 1e-5), FP64."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# radius, d, wavelength, m, nmax, ngauss
+CASES = [
+    (0.2, 0.5, 0.55, 1.5 + 0.02j, 10, 20),
+    (0.3, 1.6, 0.7, 1.7 + 0.1j, 10, 20),
+    (0.05, 0.8, 2.0, 1.9 + 0.3j, 5, 10),
+    (0.5, 1.2, 0.9, 1.33 + 0.001j, 12, 24),
+]
+H = 1e-5
+
+
+def _params(c):
+    rv, d, lam, mr, n, g = c
+    return np.array([rv, d, mr.real, mr.imag])
+
+
+def _shift(cases, j, sign):
+    out = []
+    for c in cases:
+        p = _params(c)
+        p[j] += sign * H * max(abs(p[j]), 1e-3)
+        out.append((p[0], p[1], c[2], complex(p[2], p[3]), c[4], c[5]))
+    return out
+
+
+def _step(c, j):
+    return 2 * H * max(abs(_params(c)[j]), 1e-3)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from costuum_amd.tmatrix import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _fixed(ctx, cases):
+    from costuum_amd.tmatrix import tmatrix_fixed
+    return tmatrix_fixed([c[0] for c in cases], [c[1] for c in cases],
+                         [c[2] for c in cases], [c[3].real for c in cases],
+                         [c[3].imag for c in cases], [c[4] for c in cases],
+                         [c[5] for c in cases], ctx=ctx)
+
+
+def test_tangent_tmatrix_matches_central_differences(ctx):
+    from costuum_amd.tmatrix import tangent_tmatrix, tangents
+    print("tangent T-matrices, %d grains" % len(CASES), flush=True)
+    fd = {}
+    for j in range(4):
+        rp = _fixed(ctx, _shift(CASES, j, +1))
+        Tp = [rp.tmatrix(g) for g in range(len(CASES))]
+        rm = _fixed(ctx, _shift(CASES, j, -1))
+        Tm = [rm.tmatrix(g) for g in range(len(CASES))]
+        fd[j] = [(a - b) / _step(CASES[g], j)
+                 for g, (a, b) in enumerate(zip(Tp, Tm))]
+    res = _fixed(ctx, CASES)
+    tangents(res)
+    for j in range(4):
+        for g in range(len(CASES)):
+            dT = tangent_tmatrix(res, g, j)
+            err = np.abs(dT - fd[j][g]).max() / np.abs(fd[j][g]).max()
+            assert err < 1e-5, (j, g, err)
+
+
+def test_sum_tangents_match_oracle_differences(ctx):
+    """d(Q_sca, Q_ext)/dp against central differences of the oracle's
+    sums: pins the derivative to the reference algorithm."""
+    from costuum_amd.tmatrix import tangents, vjp
+    res = _fixed(ctx, CASES)
+    dqs, dqe = tangents(res)
+    for g, c in enumerate(CASES):
+        for j in range(4):
+            vals = []
+            for sign in (+1, -1):
+                rv, d, lam, mr, n, ng = _shift([c], j, sign)[0]
+                vals.append(O.qcoeffs(n, O.tmatrix_all(rv, d, lam, mr, n,
+                                                        ng)))
+            h = _step(c, j)
+            fs = (vals[0][0] - vals[1][0]) / h
+            fe = (vals[0][1] - vals[1][1]) / h
+            scale_s = max(abs(fs), 1e-6 * abs(vals[0][0]))
+            scale_e = max(abs(fe), 1e-6 * abs(vals[0][1]))
+            assert abs(dqs[g, j] - fs) < 1e-6 * scale_s + 1e-9, (g, j)
+            assert abs(dqe[g, j] - fe) < 1e-6 * scale_e + 1e-9, (g, j)
+    cot_s = np.array([1.0, -2.0, 0.5, 3.0])
+    cot_e = np.array([0.3, 0.0, -1.0, 2.0])
+    grad = vjp(res, cot_s, cot_e)
+    assert np.allclose(grad, cot_s[:, None] * dqs + cot_e[:, None] * dqe,
+                       rtol=0, atol=0)
+
+
+def test_extinction_tangents_match_differences(ctx):
+    from costuum_amd.tmatrix import observables, observables_tangent, \
+        tangents
+    theta = np.array([0.2, 0.9, 1.6, 2.5])
+    n = len(CASES)
+    fd = {}
+    for j in range(4):
+        out = []
+        for sign in (+1, -1):
+            _fixed(ctx, _shift(CASES, j, sign))
+            q = observables(theta, ctx=ctx, account_for_scattering=False,
+                            n_grains=n)
+            out.append({k: v.cpu().numpy() for k, v in q.items()})
+        steps = np.array([_step(c, j) for c in CASES])[:, None]
+        fd[j] = {k: (out[0][k] - out[1][k]) / steps for k in out[0]}
+    res = _fixed(ctx, CASES)
+    tangents(res)
+    for j in range(4):
+        d = observables_tangent(theta, j, ctx, n,
+                                radius=[c[0] for c in CASES])
+        for k in ("qext", "qextpol", "qextcpol", "qabs", "qabspol"):
+            got = d[k].cpu().numpy()
+            ref = fd[j][k]
+            scale = np.abs(fd[j]["qext"]).max()
+            assert np.abs(got - ref).max() < 1e-5 * scale, (j, k)
+
+
+def test_autograd_of_sums(ctx):
+    import torch
+    from costuum_amd.tmatrix import tmatrix_sums
+    dev = ctx.device
+    c = CASES[:2]
+    r = torch.tensor([x[0] for x in c], dtype=torch.float64, device=dev,
+                     requires_grad=True)
+    d = torch.tensor([x[1] for x in c], dtype=torch.float64, device=dev,
+                     requires_grad=True)
+    mre = torch.tensor([x[3].real for x in c], dtype=torch.float64,
+                       device=dev, requires_grad=True)
+    mim = torch.tensor([x[3].imag for x in c], dtype=torch.float64,
+                       device=dev, requires_grad=True)
+    qs, qe = tmatrix_sums(r, d, [x[2] for x in c], mre, mim,
+                          [x[4] for x in c], [x[5] for x in c], ctx=ctx)
+    loss = (qe - qs).sum()
+    loss.backward()
+    for g, case in enumerate(c):
+        rv, dd, lam, mr, n, ng = case
+        for j, tensor in enumerate((r, d, mre, mim)):
+            vals = []
+            for sign in (+1, -1):
+                rv2, d2, _, mr2, _, _ = _shift([case], j, sign)[0]
+                s, e = O.qcoeffs(n, O.tmatrix_all(rv2, d2, lam, mr2, n, ng))
+                vals.append(e - s)
+            fd = (vals[0] - vals[1]) / _step(case, j)
+            got = tensor.grad[g].item()
+            assert abs(got - fd) < 1e-6 * max(abs(fd), 1e-3), (g, j, got, fd)
