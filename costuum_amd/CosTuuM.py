@@ -505,15 +505,117 @@ class CustomDustProperties:
         return self._index(wavelength, size, type)
 
 
-class DraineDustProperties:
-    """Draine's tabulated refractive indices (DraineDustProperties.hpp:
-    70-239) need the reference's data tables, which this package does not
-    ship."""
+def _eps_to_mr(eps):
+    """Refractive index from the dielectric function
+    (DraineDustProperties.hpp:135-142)."""
+    a = abs(eps)
+    return complex(math.sqrt(0.5 * (a + eps.real)),
+                   math.sqrt(0.5 * (a - eps.real)))
 
-    def __init__(self, dust_temperature=20.0):
-        raise NotImplementedError(
-            "DraineDustProperties needs the callindex.out tables; use "
-            "CustomDustProperties with an interpolating function instead")
+
+def _carbon_free_electrons(T, wavelength, grain_size, parallel):
+    """Free-electron (Drude) contribution to the dielectric function of
+    graphite with the size-limited collision time
+    (DraineDustProperties.hpp:84-127); SI units."""
+    if parallel:
+        omega_p, tau_bulk, TF, vF = 1.53e14, 1.4e-14, 255.0, 3.7e4
+    else:
+        T2 = T * T
+        omega_p = 4.33e14 * math.sqrt(1.0 - 6.24e-3 * T + 3.66e-5 * T2)
+        tau_bulk = 4.2e-11 / (1.0 + 0.322 * T + 0.0013 * T2)
+        TF, vF = 255.0, 4.5e5
+    veff = vF * math.sqrt(1.0 + T / TF)
+    tau = 1.0 / (1.0 / tau_bulk + veff / grain_size)
+    omega = 2.0 * math.pi * 299792458.0 / wavelength
+    wt = omega * tau
+    wpt = omega_p * tau
+    return complex(-wpt * wpt, 0.0) / complex(wt * wt, wt)
+
+
+class _LinearTable:
+    """Columns of a whitespace text table with linear interpolation in the
+    first column, ascending or descending (Table.hpp:179-260)."""
+
+    def __init__(self, path):
+        rows = []
+        with open(path) as f:
+            for line in f:
+                t = line.split()
+                if not t or t[0].startswith("#"):
+                    continue
+                rows.append([float(v) for v in t[:5]])
+        self.rows = np.array(rows)
+
+    def row(self, x):
+        col = self.rows[:, 0]
+        lo, hi = 0, len(col) - 1
+        xlo, xhi = col[lo], col[hi]
+        if xlo == xhi:
+            raise CtmError("Table endpoints are the same!")
+        asc = xlo < xhi
+        if not (min(xlo, xhi) <= x <= max(xlo, xhi)):
+            raise ValueError("wavelength %g outside the table" % x)
+        while lo + 1 != hi:
+            mid = (lo + hi) >> 1
+            if (col[mid] <= x) if asc else (col[mid] >= x):
+                lo, xlo = mid, col[mid]
+            else:
+                hi, xhi = mid, col[mid]
+        f = (x - xlo) / (xhi - xlo)
+        return (1.0 - f) * self.rows[lo] + f * self.rows[hi]
+
+
+class DraineDustProperties:
+    """Draine's (2003) astrosilicate and graphite dielectric tables with the
+    graphite free-electron correction for the grain temperature and size
+    (DraineDustProperties.hpp:70-239).  The callindex.out_* tables are not
+    shipped with this package: data_directory (or the environment variable
+    COSTUUM_DRAINE_DATA) names the folder that holds callindex.out_silD03,
+    callindex.out_CpaD03_0.10 and callindex.out_CpeD03_0.10.  Wavelengths
+    and sizes in metres."""
+
+    def __init__(self, dust_temperature=20.0, data_directory=None):
+        import os
+        d = data_directory or os.environ.get("COSTUUM_DRAINE_DATA")
+        if not d:
+            raise NotImplementedError(
+                "DraineDustProperties needs the callindex.out tables: pass "
+                "data_directory or set COSTUUM_DRAINE_DATA")
+        self.dust_temperature = float(dust_temperature)
+        self._sil = self._load(os.path.join(d, "callindex.out_silD03"))
+        self._cpa = self._load(os.path.join(d, "callindex.out_CpaD03_0.10"),
+                               True)
+        self._cpe = self._load(os.path.join(d, "callindex.out_CpeD03_0.10"),
+                               False)
+
+    @staticmethod
+    def _load(path, parallel=None):
+        t = _LinearTable(path)
+        t.rows[:, 0] *= 1e-6
+        t.rows[:, 1] += 1.0
+        t.rows[:, 3] += 1.0
+        if parallel is not None:
+            # remove the free-electron term the table holds (20 K, 0.1 um)
+            for i in range(len(t.rows)):
+                c = _carbon_free_electrons(20.0, t.rows[i, 0], 1e-7, parallel)
+                t.rows[i, 1] -= c.real
+                t.rows[i, 2] -= c.imag
+        return t
+
+    def _index(self, wavelength, grain_size, grain_type):
+        if grain_type in (CARBON_PARALLEL, CARBON_PERPENDICULAR):
+            par = grain_type == CARBON_PARALLEL
+            row = (self._cpa if par else self._cpe).row(wavelength)
+            eps = complex(row[1], row[2]) + _carbon_free_electrons(
+                self.dust_temperature, wavelength, grain_size, par)
+            return _eps_to_mr(eps)
+        if grain_type == SILICON:
+            row = self._sil.row(wavelength)
+            return complex(row[3], row[4])
+        raise ValueError("Unknown dust grain type: %d!" % grain_type)
+
+    def get_refractive_index(self, type, size, wavelength):
+        return self._index(wavelength, size, type)
 
 
 # ---------------------------------------------------------------------------

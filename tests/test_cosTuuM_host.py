@@ -3,6 +3,7 @@ R_V/R_S, orientation-distribution coefficients, the Draine-Hensley shape
 distribution and the alignment selection, each pinned by closed forms or the
 oracle."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -185,11 +186,70 @@ def test_custom_dust_properties():
                                                              2.0 + 0j]))
     with pytest.raises(ValueError):
         arr.get_refractive_index(0, 1.0, 1.0)
-    with pytest.raises(NotImplementedError):
-        C.DraineDustProperties()
+    import os
+    if not os.environ.get("COSTUUM_DRAINE_DATA"):
+        with pytest.raises(NotImplementedError):
+            C.DraineDustProperties()
 
 
 def test_module_constants():
     assert (C.DAVIS_GREENSTEIN_ALIGNMENT, C.MISHCHENKO_ALIGNMENT,
             C.DISABLE_ALIGNMENT, C.CUSTOM_ALIGNMENT) == (0, 1, 2, 3)
     assert (C.CARBON_PARALLEL, C.CARBON_PERPENDICULAR, C.SILICON) == (0, 1, 2)
+
+
+DRAINE_DATA = "/root/reference/data"
+
+
+@pytest.mark.skipif(not os.path.isdir(DRAINE_DATA),
+                    reason="Draine's callindex tables are not available")
+def test_draine_dust_properties_against_tables():
+    d = C.DraineDustProperties(20.0, DRAINE_DATA)
+    sil = np.array([[float(v) for v in l.split()[:5]]
+                    for l in open(os.path.join(DRAINE_DATA,
+                                               "callindex.out_silD03"))
+                    if l.split() and not l.startswith("#")])
+    # at a table node the silicate index is the tabulated one
+    i = 300
+    lam = sil[i, 0] * 1e-6
+    m = d.get_refractive_index(C.SILICON, 1e-7, lam)
+    assert m == complex(sil[i, 3] + 1.0, sil[i, 4])
+    # halfway between two nodes: the average (linear interpolation)
+    mid = 0.5 * (sil[i, 0] + sil[i + 1, 0]) * 1e-6
+    mm = d.get_refractive_index(C.SILICON, 1e-7, mid)
+    assert abs(mm - 0.5 * (complex(sil[i, 3] + 1, sil[i, 4]) +
+                           complex(sil[i + 1, 3] + 1, sil[i + 1, 4]))) < 1e-12
+    # graphite at the tables' own temperature and size: the correction
+    # cancels, and m = sqrt(eps) (principal branch)
+    cpa = np.array([[float(v) for v in l.split()[:5]]
+                    for l in open(os.path.join(DRAINE_DATA,
+                                               "callindex.out_CpaD03_0.10"))
+                    if l.split() and not l.startswith("#")])
+    j = 200
+    lam = cpa[j, 0] * 1e-6
+    m = d.get_refractive_index(C.CARBON_PARALLEL, 1e-7, lam)
+    eps = complex(cpa[j, 1] + 1.0, cpa[j, 2])
+    assert abs(m - np.sqrt(eps)) < 1e-9 * abs(m)
+    # the table's own n agrees with sqrt(eps) to its printed precision
+    assert abs(m - complex(cpa[j, 3] + 1, cpa[j, 4])) < 2e-3 * abs(m)
+    with pytest.raises(ValueError):
+        d.get_refractive_index(7, 1e-7, lam)
+    with pytest.raises(ValueError):
+        d.get_refractive_index(C.SILICON, 1e-7, 10.0)
+
+
+def test_drude_correction_closed_form():
+    # -(w_p tau)^2 / ((w tau)^2 + i w tau) = -w_p^2 / (w^2 + i w / tau)
+    for par in (True, False):
+        for T, lam, a in ((20.0, 1e-6, 1e-7), (100.0, 3e-5, 5e-9)):
+            c = C._carbon_free_electrons(T, lam, a, par)
+            if par:
+                wp, tb, TF, vF = 1.53e14, 1.4e-14, 255.0, 3.7e4
+            else:
+                wp = 4.33e14 * math.sqrt(1 - 6.24e-3 * T + 3.66e-5 * T * T)
+                tb = 4.2e-11 / (1 + 0.322 * T + 0.0013 * T * T)
+                TF, vF = 255.0, 4.5e5
+            tau = 1 / (1 / tb + vF * math.sqrt(1 + T / TF) / a)
+            w = 2 * math.pi * 299792458.0 / lam
+            ref = -wp * wp / complex(w * w, w / tau)
+            assert abs(c - ref) < 1e-12 * abs(ref)
