@@ -676,6 +676,11 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     ctx->err = "invalid convergence parameters";
     return CTM_ERR_ARGUMENT;
   }
+  if (conv->precision != CTM_PRECISION_FP64) {
+    ctx->err = "only CTM_PRECISION_FP64 is built: the Q assembly needs FP64 "
+               "for 1e-6 Q_ext, and FP64 runs at the matrix-core rate here";
+    return CTM_ERR_ARGUMENT;
+  }
   if ((int)conv->max_order - 1 > kMaxOrder) {
     ctx->err = "maximum order above 201 is not supported by this build";
     return CTM_ERR_ARGUMENT;

@@ -34,7 +34,12 @@ extern "C" {
 #define CTM_CONVERGED 0
 #define CTM_NOT_CONVERGED 1 /* last order kept, as the task path does */
 
-/* Arithmetic of the batched path. */
+/* Arithmetic of the batched path.  Only CTM_PRECISION_FP64 is built:
+ * CTM_PRECISION_MIXED (FP32 elimination with FP64 refinement) is reserved
+ * and rejected with CTM_ERR_ARGUMENT.  The Q/RgQ assembly must stay FP64 to
+ * reach 1e-6 in Q_ext (rounding Q to FP32 is amplified by cond(Q)), and on
+ * this GPU FP64 runs at the same rate on the vector and matrix units, so a
+ * mixed solve could only touch the elimination, which is latency-bound. */
 #define CTM_PRECISION_FP64 0
 #define CTM_PRECISION_MIXED 1
 

@@ -211,3 +211,37 @@ def test_get_table_disabled_alignment_and_squeeze():
     qa, qap = O.absorption(n, T, 0.55, 0.2, thetas, ngauss=20,
                            account_for_scattering=False)
     assert relmax(q[:, 0], qa) < 1e-9 and relmax(q[:, 1], qap) < 1e-9
+
+
+@pytest.mark.parametrize("d", [0.5, 2.0])
+def test_single_grain_workload(d):
+    """The single-grain case: R_V = 0.2 um, lambda = 0.55 um, m = 1.5 +
+    0.02i, isotropic ensemble; Q_ext, Q_abs at theta = 30 deg and Z at 91
+    scattering angles 0, 2, ..., 180 deg (incidence along z, phi = 0)."""
+    from costuum_amd import CosTuuM as C
+    rv, lam, mr = 0.2, 0.55, 1.5 + 0.02j
+    tm = C.TMatrix(rv, d, lam, mr, cos2beta=1.0 / 3.0)
+    n, g = tm.get_nmax(), tm.get_ngauss()
+    T = O.tmatrix_all(rv, d, lam, mr, n, g)
+    pN = O.orientation_coefficients(O.ORIENT_ISOTROPIC, 0.0, 2 * n)
+    Tavg = O.orientation_average(n, T, pN)
+    th = np.radians(30.0)
+    area = math.pi * rv * rv
+    qa, _ = O.absorption(n, Tavg, lam, rv, [th], ngauss=100)
+    assert abs(tm.get_absorption_cross_section(th)[0] - qa[0] * area) < \
+        1e-9 * abs(qa[0] * area)
+    K = tm.get_extinction_matrix(th, 0.0)
+    qe, _, _ = O.extinction(n, Tavg, lam, rv, [th])
+    assert abs(K[0, 0] - qe[0] * area) < 1e-9 * abs(qe[0] * area)
+    Theta = np.radians(np.arange(0, 181, 2.0))
+    Z = tm.get_scattering_matrix(0.0, 0.0, Theta, 0.0)
+    assert Z.shape == (91, 4, 4)
+    for i in (0, 1, 45, 89, 90):
+        _, Zo, _ = O.scattering_query(n, Tavg, lam, 0.0, 0.0, 0.0, 0.0,
+                                      Theta[i], 0.0)
+        assert relmax(Z[i], Zo) < 1e-9, i
+    # backscattering invariant Z11 - Z22 + Z33 - Z44 = 0
+    # (testTMatrixCalculator.cpp:198-212)
+    zb = Z[90]
+    assert abs(zb[0, 0] - zb[1, 1] + zb[2, 2] - zb[3, 3]) < 1e-9 * zb[0, 0]
+    tm.close()

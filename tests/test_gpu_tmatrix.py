@@ -156,3 +156,10 @@ def test_large_grains_converge_like_oracle(ctx):
     scale = np.abs(Tref).max()
     bar = max(1e-8, 50 * np.abs(T2 - Tref).max() / scale)
     assert np.abs(r.tmatrix(1) - Tref).max() / scale < bar
+
+
+def test_mixed_precision_is_rejected(ctx):
+    from costuum_amd.tmatrix import CtmError, tmatrix_batch
+    with pytest.raises(CtmError):
+        tmatrix_batch([0.1], [1.5], [0.5], [1.5], [0.01], precision="mixed",
+                      ctx=ctx)
