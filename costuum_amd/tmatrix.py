@@ -395,3 +395,31 @@ def tmatrix_sums(radius, axis_ratio, wavelength, mr_re, mr_im, nmax, ngauss,
     ctx = ctx or default_context()
     return _TMatrixSums.apply(radius, axis_ratio, mr_re, mr_im, wavelength,
                               nmax, ngauss, ctx)
+
+
+def weighted_sums(values, weights, group, normalise=False, ctx=None):
+    """Weighted sums over consecutive groups of rows of a device tensor
+    values [n, ...] with weights [n], on the GPU: out[o] = sum_s w x (divided
+    by sum_s w when normalise).  normalise=True is the shape average of the
+    reference (ShapeAveragingTask.hpp:127-166); normalise=False the size
+    integration sum_a w_a Q(a) its scripts do (scripts/BBOPResult_sim.py:
+    9-71)."""
+    ctx = ctx or default_context()
+    x = torch.as_tensor(values, dtype=torch.float64, device=ctx.device)
+    n = x.shape[0]
+    if group < 1 or n % group:
+        raise ValueError("rows must split into groups of %d" % group)
+    width = int(np.prod(x.shape[1:])) if x.dim() > 1 else 1
+    xin = x.reshape(n, width).contiguous()
+    w = torch.as_tensor(weights, dtype=torch.float64,
+                        device=ctx.device).contiguous()
+    if w.numel() != n:
+        raise ValueError("one weight per row")
+    out = torch.zeros((n // group, width), dtype=torch.float64,
+                      device=ctx.device)
+    torch.cuda.current_stream(ctx.device).synchronize()
+    ctx.check(ctx._lib.ctm_weighted_reduce(
+        ctx.handle, n // group, group, width, _ptr(xin), _ptr(w),
+        int(bool(normalise)), _ptr(out)), "ctm_weighted_reduce")
+    ctx.synchronize()
+    return out.reshape((n // group,) + tuple(x.shape[1:]))

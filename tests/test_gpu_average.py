@@ -63,3 +63,25 @@ def test_observables_of_ensemble(batch):
         assert np.abs(qe[g] - e).max() < 1e-9 * s
         assert np.abs(qep[g] - ep).max() < 1e-9 * s
         assert np.abs(qa[g] - a).max() < 1e-9 * s
+
+
+def test_weighted_sums_shape_and_size():
+    """Shape averages (normalised) and size integrals (plain weighted sums)
+    on the GPU against numpy, summed in the same order."""
+    import torch
+    from costuum_amd.tmatrix import Context, weighted_sums
+    ctx = Context(0)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(6 * 7, 3, 5))
+    w = rng.uniform(0.1, 2.0, size=6 * 7)
+    got = weighted_sums(torch.as_tensor(x, device=ctx.device), w, 7,
+                        normalise=True, ctx=ctx).cpu().numpy()
+    xs = x.reshape(6, 7, 3, 5)
+    ws = w.reshape(6, 7)
+    ref = np.einsum("gs,gsij->gij", ws, xs) / ws.sum(axis=1)[:, None, None]
+    assert np.abs(got - ref).max() < 1e-14 * np.abs(ref).max()
+    got2 = weighted_sums(torch.as_tensor(x, device=ctx.device), w, 42,
+                         normalise=False, ctx=ctx).cpu().numpy()
+    ref2 = np.einsum("s,sij->ij", w, x)[None]
+    assert np.abs(got2 - ref2).max() < 1e-13 * np.abs(ref2).max()
+    ctx.close()
