@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=64,
                     help="CPU sample = every k-th grain in size order")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="batch", choices=["batch", "grid"],
+                    help="batch: the 4096-grain seed-0 batch (default); "
+                         "grid: the 61 440 T-matrix size x wavelength x "
+                         "shape grid at orders up to 160")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 512 MiB five times "
                          "(reference kernel for the HBM byte counters)")
@@ -90,10 +94,151 @@ def cpu_baseline(g, stride, threads, theta, wtheta):
             sel, nmax, qe, qa)
 
 
+def run_grid(a):
+    """The (size, wavelength, shape) grid: 64 wavelengths x 48 radii x 20
+    Draine-Hensley-weighted axis ratios in [0.5, 2], m(lambda) from a
+    seeded Lorentz dielectric function, task-path convergence up to order
+    159, Mishchenko alignment (<cos^2 beta> = 0.6 oblate, 0.2 prolate),
+    Q_ext and Q_abs (with the 20 x 20 scattered-light quadrature) at 8
+    Gauss-Legendre incidence angles, Z at 91 scattering angles 0..180 deg
+    (incidence along the alignment axis), all shape-averaged.  One step is
+    the whole grid in chunks of 204 (size, wavelength) cells."""
+    import numpy as np
+    import torch
+    import make_inputs
+    from costuum_amd.tmatrix import (Context, observables, query,
+                                     tmatrix_batch, weighted_sums)
+    g, wshape = make_inputs.grid()
+    nshape = len(wshape)
+    ncell = len(g["radius"]) // nshape
+    ctx = Context(0)
+    dev = ctx.device
+    keys = ("radius", "axis_ratio", "wavelength", "mr_re", "mr_im")
+    dg = [torch.as_tensor(g[k], dtype=torch.float64, device=dev) for k in keys]
+    xg, _ = np.polynomial.legendre.leggauss(8)
+    theta = np.arccos(xg)
+    Theta = np.radians(np.arange(0.0, 181.0, 2.0))
+    cells = 204
+    pN = np.zeros((2, 321))
+    pN[:, 0] = 1.0
+    pN[0, 2] = 1.5 * 0.6 - 0.5   # oblate (d > 1)
+    pN[1, 2] = 1.5 * 0.2 - 0.5   # prolate
+    wt = torch.as_tensor(np.tile(wshape, cells), device=dev)
+
+    # parity sample: grains of moderate order spread over the grid
+    x = 2 * np.pi * g["radius"] / g["wavelength"]
+    cand = np.flatnonzero((x > 0.5) & (x < 12.0))
+    sample = cand[np.linspace(0, len(cand) - 1, 6).astype(int)]
+    per_grain = {}
+
+    def step():
+        nconv = 0
+        q_all, z_all, st_all = [], [], []
+        for c0 in range(0, ncell, cells):
+            c1 = min(ncell, c0 + cells)
+            s0, s1 = c0 * nshape, c1 * nshape
+            n = s1 - s0
+            r = tmatrix_batch(*[t[s0:s1] for t in dg], ctx=ctx, min_order=4,
+                              max_order=160, glfac=2, tolerance=1e-4,
+                              sync=False)
+            st_all.append(r.status)
+            dist = np.where(g["axis_ratio"][s0:s1] > 1.0, 0, 1).astype(
+                np.int32)
+            r.average(pN, dist)
+            obs = observables(theta, ctx=ctx, ngauss=20, n_grains=n)
+            q = torch.stack([obs["qext"], obs["qabs"]], dim=2)
+            for i in sample[(sample >= s0) & (sample < s1)]:
+                per_grain[int(i)] = (q[i - s0], r.nmax[i - s0])
+            q_all.append(weighted_sums(q, wt[:n], nshape, normalise=True,
+                                       ctx=ctx))
+            _, Z, _ = query(ctx, np.arange(n)[:, None], 0.0, 0.0, 0.0, 0.0,
+                            Theta[None, :], 0.0)
+            z_all.append(weighted_sums(torch.as_tensor(Z, device=dev),
+                                       wt[:n], nshape, normalise=True,
+                                       ctx=ctx))
+        st = torch.cat(st_all)
+        return int((st == 0).sum().item()), torch.cat(q_all), \
+            torch.cat(z_all), st
+
+    for _ in range(a.warmup):
+        step()
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        nconv, q, z, st = step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = ctx.profile_stats()
+    ms = (t1 - t0) * 1e3 / max(a.steps, 1)
+    launches, kms, flops, _ = stats["assemble"]
+    achieved = flops / (kms / 1e3) / 1e12 if kms > 0 else 0.0
+    qn = q.cpu().numpy()
+    # cells whose shapes all converged (the others keep the last order, as
+    # in the reference, and carry meaningless values)
+    good = (st.cpu().numpy().reshape(-1, nshape) == 0).all(axis=1)
+    parity = None
+    if a.cpu_baseline:
+        import oracle
+        errs = []
+        for i, (qg, nm) in sorted(per_grain.items()):
+            rv, d, lam = g["radius"][i], g["axis_ratio"][i], \
+                g["wavelength"][i]
+            mr = complex(g["mr_re"][i], g["mr_im"][i])
+            stc, n, ng, _, _ = oracle.converge_task(rv, d, lam, mr, 4, 160, 2,
+                                                    1e-4)
+            T = oracle.tmatrix_all(rv, d, lam, mr, n, ng)
+            Ta = oracle.orientation_average(n, T, pN[0 if d > 1.0 else 1])
+            qe, _, _ = oracle.extinction(n, Ta, lam, rv, theta)
+            qa, _ = oracle.absorption(n, Ta, lam, rv, theta, ngauss=20)
+            qq = qg.cpu().numpy()
+            errs.append(max(np.abs(qq[:, 0] - qe).max() / np.abs(qe).max(),
+                            np.abs(qq[:, 1] - qa).max() / np.abs(qa).max()))
+            assert int(nm) == n, (i, int(nm), n)
+        parity = {"grains": len(errs), "same_nmax": True,
+                  "max_rel_err_q": float(max(errs))}
+    out = {
+        "metric": "converged T-matrices/s on 1 GPU, size x wavelength x "
+                  "shape grid",
+        "value": round(nconv / (ms / 1e3), 2),
+        "unit": "converged T-matrices/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "grid: 64 lambda x 48 radii x 20 DH shapes, "
+                               "Lorentz m(lambda), orders 4..159, "
+                               "Mishchenko alignment, Q at 8 angles, Z at "
+                               "91 angles",
+                   "t_matrices": int(len(g["radius"])), "converged": nconv,
+                   "cells_all_converged": int(good.sum()),
+                   "mean_qext_converged_cells": float(qn[good][..., 0].mean()),
+                   "mean_qabs_converged_cells": float(
+                       qn[good][..., 1].mean())},
+        "roofline": {"kernel": "k_assemble", "bound": "mfma",
+                     "achieved": round(achieved, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None},
+        "cpu_baseline": None,
+        "parity_vs_oracle": parity,
+        "kernels": {k: {"launches": int(v[0]), "ms": round(v[1], 3)}
+                    for k, v in stats.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.workload == "grid":
+        if a.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            sys.exit(2)
+        run_grid(a)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU only: grains shard across GPUs "

@@ -245,6 +245,13 @@ def tmatrix_fixed(radius, axis_ratio, wavelength, mr_re, mr_im, nmax, ngauss,
                        qsca, qext, ngauss=ng)
 
 
+_QUERY_DTYPE = np.dtype([("grain", "<i4"), ("pad", "<i4"), ("alpha", "<f8"),
+                         ("beta", "<f8"), ("theta_in", "<f8"),
+                         ("phi_in", "<f8"), ("theta_out", "<f8"),
+                         ("phi_out", "<f8")])
+assert _QUERY_DTYPE.itemsize == ctypes.sizeof(_native.CtmQuery)
+
+
 def query(ctx, grains, alpha, beta, theta_in, phi_in, theta_out, phi_out):
     """Amplitude (S, 2x2 complex), phase (Z, 4x4) and extinction (K, 4x4)
     matrices of the context's T-matrices (the averaged ones after
@@ -256,14 +263,15 @@ def query(ctx, grains, alpha, beta, theta_in, phi_in, theta_out, phi_out):
                             theta_in, phi_in, theta_out, phi_out)
     shape = b[0].shape
     n = b[0].size
-    qs = (_native.CtmQuery * max(n, 1))()
-    flat = [np.ravel(x) for x in b]
-    for i in range(n):
-        qs[i] = _native.CtmQuery(int(flat[0][i]), 0, *[float(x[i])
-                                                     for x in flat[1:]])
+    # packed like ctm_query_spec: int32 grain, int32 pad, six doubles
+    qs = np.zeros(max(n, 1), dtype=_QUERY_DTYPE)
+    qs["grain"][:n] = np.ravel(b[0])
+    for name, x in zip(_QUERY_DTYPE.names[2:], b[1:]):
+        qs[name][:n] = np.ravel(x)
     out = np.zeros((max(n, 1), _native.QUERY_OUT))
-    ctx.check(ctx._lib.ctm_query(ctx.handle, n, qs, out.ctypes.data),
-              "ctm_query")
+    ctx.check(ctx._lib.ctm_query(
+        ctx.handle, n, qs.ctypes.data_as(ctypes.POINTER(_native.CtmQuery)),
+        out.ctypes.data), "ctm_query")
     out = out[:n]
     S = (out[:, 0:8:2] + 1j * out[:, 1:8:2]).reshape(shape + (2, 2))
     Z = out[:, 8:24].reshape(shape + (4, 4))
