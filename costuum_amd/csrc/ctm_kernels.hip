@@ -287,69 +287,19 @@ __global__ void k_tables(const TableSet *sets, const GrainParams gp,
 constexpr int kLeftStride = 34;   // 32 factors + 2 pad doubles per row
 constexpr int kRightStride = 10;  // 5 complex factors per column
 
-template <int MT>
-__global__ __launch_bounds__(256) void k_assemble(
-    const int32_t *order, const int32_t *tile_prefix, int n_order,
-    const Item *items, const TableSet *sets, const WigKey *wkeys,
-    const GrainParams gp, const double *tables, const double *wig,
-    cdbl *qbuf) {
-  constexpr int TR = 16 * MT;            // rows of pairs per tile
-  constexpr int CH = MT == 1 ? 8 : 4;    // quadrature nodes per LDS stage
-  __shared__ double Lsh[CH][TR][kLeftStride];
-  __shared__ double Rsh[CH][2 * TR][kRightStride];
-
-  const int bid = blockIdx.x;
-  int lo = 0, hi = n_order;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (tile_prefix[mid] <= bid) lo = mid; else hi = mid;
-  }
-  const Item it = items[order[lo]];
-  const TableSet ts = sets[it.set];
-  const WigKey wk = wkeys[it.wig];
-  const int N = it.N, m = it.m, G = it.G;
-  const int nmin = m > 0 ? m : 1;
-  const int nm = N + 1 - nmin;
-  const bool m0 = (m == 0);
-  const int cw = m0 ? 2 * TR : TR;
-  const int ntc = (nm + cw - 1) / cw;
-  const int lt = bid - tile_prefix[lo];
-  const int tr = lt / ntc, tc = lt - (lt / ntc) * ntc;
-  const double *S = tables + ts.off;
-  const double *B = S + (int64_t)kNodeScalars * G;
-  const int64_t NG = (int64_t)N * G;
-  const double *Wd = wig + wk.off;
-  const double *Wdd = Wd + (int64_t)nm * G;
-
-  const int t = threadIdx.x;
-  const int wv = t >> 6, lane = t & 63;
-  const int prow = wv & 1;
-  const int pcol = m0 ? (wv & 1) : (wv >> 1);
-  const int cbase = m0 ? TR * (wv >> 1) : 0;
-  int rr[MT], cc[MT];
-#pragma unroll
-  for (int a = 0; a < MT; ++a) {
-    rr[a] = 2 * ((lane >> 3) + 8 * a) + prow;
-    cc[a] = cbase + 2 * ((lane & 7) + 8 * a) + pcol;
-  }
-  const bool odd = !m0 && ((prow ^ pcol) != 0);
+// Stages the row factors (32 per row and node: even-pair A1..A5, RgJ
+// factors, odd-pair B5..B3, RgJ factors) of the TR rows of tile row tr and
+// the column factors (R1..R5) of the cw columns of tile column tc for the
+// nodes ig0 .. ig0 + CH - 1 into LDS; entries beyond the block or the node
+// range are zero.
+template <int TR, int CH>
+__device__ __forceinline__ void stage_factors(
+    double (*Lsh)[TR][kLeftStride], double (*Rsh)[2 * TR][kRightStride],
+    int t, int nthreads, int ig0, int G, int nm, int nmin, int tr, int tc,
+    int cw, double mm, const double *S, const double *B, int64_t NG,
+    const double *Wd, const double *Wdd) {
   const int nstage = TR * CH + cw * CH;
-
-  cdbl Ja[MT][MT], Jb[MT][MT], RJa[MT][MT], RJb[MT][MT];
-#pragma unroll
-  for (int a = 0; a < MT; ++a)
-#pragma unroll
-    for (int b = 0; b < MT; ++b) {
-      Ja[a][b] = cmk(0., 0.);
-      Jb[a][b] = cmk(0., 0.);
-      RJa[a][b] = cmk(0., 0.);
-      RJb[a][b] = cmk(0., 0.);
-    }
-  const double mm = (double)m;
-
-  for (int ig0 = 0; ig0 < G; ig0 += CH) {
-    __syncthreads();
-    for (int e = t; e < nstage; e += 256) {
+    for (int e = t; e < nstage; e += nthreads) {
     if (e < TR * CH) {
       // row factors of row r at node ig0 + kk
       const int r = e / CH, kk = e % CH;
@@ -443,6 +393,71 @@ __global__ __launch_bounds__(256) void k_assemble(
       }
     }
     }
+}
+
+template <int MT>
+__global__ __launch_bounds__(256) void k_assemble(
+    const int32_t *order, const int32_t *tile_prefix, int n_order,
+    const Item *items, const TableSet *sets, const WigKey *wkeys,
+    const GrainParams gp, const double *tables, const double *wig,
+    cdbl *qbuf) {
+  constexpr int TR = 16 * MT;            // rows of pairs per tile
+  constexpr int CH = MT == 1 ? 8 : 4;    // quadrature nodes per LDS stage
+  __shared__ double Lsh[CH][TR][kLeftStride];
+  __shared__ double Rsh[CH][2 * TR][kRightStride];
+
+  const int bid = blockIdx.x;
+  int lo = 0, hi = n_order;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_prefix[mid] <= bid) lo = mid; else hi = mid;
+  }
+  const Item it = items[order[lo]];
+  const TableSet ts = sets[it.set];
+  const WigKey wk = wkeys[it.wig];
+  const int N = it.N, m = it.m, G = it.G;
+  const int nmin = m > 0 ? m : 1;
+  const int nm = N + 1 - nmin;
+  const bool m0 = (m == 0);
+  const int cw = m0 ? 2 * TR : TR;
+  const int ntc = (nm + cw - 1) / cw;
+  const int lt = bid - tile_prefix[lo];
+  const int tr = lt / ntc, tc = lt - (lt / ntc) * ntc;
+  const double *S = tables + ts.off;
+  const double *B = S + (int64_t)kNodeScalars * G;
+  const int64_t NG = (int64_t)N * G;
+  const double *Wd = wig + wk.off;
+  const double *Wdd = Wd + (int64_t)nm * G;
+
+  const int t = threadIdx.x;
+  const int wv = t >> 6, lane = t & 63;
+  const int prow = wv & 1;
+  const int pcol = m0 ? (wv & 1) : (wv >> 1);
+  const int cbase = m0 ? TR * (wv >> 1) : 0;
+  int rr[MT], cc[MT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a) {
+    rr[a] = 2 * ((lane >> 3) + 8 * a) + prow;
+    cc[a] = cbase + 2 * ((lane & 7) + 8 * a) + pcol;
+  }
+  const bool odd = !m0 && ((prow ^ pcol) != 0);
+
+  cdbl Ja[MT][MT], Jb[MT][MT], RJa[MT][MT], RJb[MT][MT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a)
+#pragma unroll
+    for (int b = 0; b < MT; ++b) {
+      Ja[a][b] = cmk(0., 0.);
+      Jb[a][b] = cmk(0., 0.);
+      RJa[a][b] = cmk(0., 0.);
+      RJb[a][b] = cmk(0., 0.);
+    }
+  const double mm = (double)m;
+
+  for (int ig0 = 0; ig0 < G; ig0 += CH) {
+    __syncthreads();
+    stage_factors<TR, CH>(Lsh, Rsh, t, 256, ig0, G, nm, nmin, tr, tc, cw, mm,
+                          S, B, NG, Wd, Wdd);
     __syncthreads();
     const int kmax = (G - ig0) < CH ? (G - ig0) : CH;
     if (!odd) {

@@ -384,10 +384,18 @@ __global__ __launch_bounds__(256) void k_tangent_m(const Item *items,
   cdbl *Q = dq + (int64_t)dir * q_size + it.q_off;
   const cdbl *dQ = Q + cls * nm2;
   cdbl *M = Q + (2 + cls) * nm2;  // dRgQ, replaced by dRgQ + T dQ
+  // m = 0: only indices of equal parity couple (TMatrixResource.hpp:622);
+  // the assembly leaves the other entries unwritten, so they are skipped
+  // here and zero in M
+  const int step = it.m == 0 ? 2 : 1;
   for (int64_t e = threadIdx.x; e < nm2; e += blockDim.x) {
     const int r = (int)(e / nm), c = (int)(e - (int64_t)(e / nm) * nm);
+    if (step == 2 && ((r ^ c) & 1)) {
+      M[e] = cmk(0., 0.);
+      continue;
+    }
     cdbl acc = M[e];
-    for (int k = 0; k < nm; ++k)
+    for (int k = step == 2 ? (r & 1) : 0; k < nm; k += step)
       cfma(acc, T[(int64_t)r * nm + k], dQ[(int64_t)k * nm + c]);
     M[e] = acc;
   }
@@ -411,11 +419,13 @@ __global__ __launch_bounds__(256) void k_tangent_t(const Item *items,
   const cdbl *M = dq + (int64_t)dir * q_size + it.q_off + (2 + cls) * nm2;
   cdbl *D = dt + (int64_t)dir * t_size + it.t_off + cls * nm2;
   double s_sca = 0., s_ext = 0.;
+  const int step = it.m == 0 ? 2 : 1;
   for (int64_t e = threadIdx.x; e < nm2; e += blockDim.x) {
     const int r = (int)(e / nm), c = (int)(e - (int64_t)(e / nm) * nm);
     cdbl acc = cmk(0., 0.);
-    for (int k = 0; k < nm; ++k)
-      cfma(acc, M[(int64_t)r * nm + k], Ai[(int64_t)k * nm + c]);
+    if (!(step == 2 && ((r ^ c) & 1)))
+      for (int k = step == 2 ? (r & 1) : 0; k < nm; k += step)
+        cfma(acc, M[(int64_t)r * nm + k], Ai[(int64_t)k * nm + c]);
     const cdbl v = cmk(-acc.x, -acc.y);
     D[e] = v;
     const cdbl tv = T[e];
