@@ -65,6 +65,20 @@ def copy_calibration():
           flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per launch of the large-block assembly kernel on this
+    workload, from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised
+    in profiles/ (scaled by the same counters on a copy of known size);
+    None when the summary is absent."""
+    import re
+    p = os.path.join(ROOT, "profiles", "k_assemble_batch4096_traffic.md")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)",
+                  open(p).read())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(g, stride, threads, theta, wtheta):
     """The CPU oracle (FP64 restatement of the reference algorithm) on a
     size-stratified sample of the batch, doing the same work per grain as
@@ -295,13 +309,15 @@ def main():
     launches, ms, flops, nbytes = stats["assemble"]
     avg_ms = ms / max(launches, 1)
     achieved = flops / (ms / 1e3) / 1e12 if ms > 0 else 0.0
-    roofline = {"kernel": "k_assemble", "bound": "mfma",
+    roofline = {"kernel": "k_assemble<2>", "bound": "mfma",
                 "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS,
                                                   4),
-                "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+                "traffic": measured_traffic() if a.grains == 4096 else None,
+                "avg_launch_ms": round(avg_ms, 4),
                 "launches": int(launches),
-                "algorithmic_flops_per_launch": flops / max(launches, 1)}
+                "algorithmic_flops_per_launch": flops / max(launches, 1),
+                "algorithmic_bytes_per_launch": nbytes / max(launches, 1)}
     kernels = {k: {"launches": int(v[0]), "ms": round(v[1], 3),
                    "tflops": round(v[2] / (v[1] / 1e3) / 1e12, 3)
                    if v[1] > 0 and v[2] > 0 else None}

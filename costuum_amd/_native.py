@@ -42,7 +42,8 @@ class CtmQuery(ctypes.Structure):
 QUERY_OUT = 40
 
 KERNELS = ("gauss_legendre", "wigner", "tables", "assemble", "solve",
-           "converge", "sums", "obs_x", "obs_sum", "average")
+           "converge", "sums", "obs_x", "obs_sum", "average",
+           "assemble_small")
 
 
 class CtmObservableSpec(ctypes.Structure):

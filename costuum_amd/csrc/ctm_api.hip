@@ -490,33 +490,30 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   // order; (64/3) nm^3 and (16/3) n^3 for the parity-split solves) and the
   // compulsory HBM traffic (tables and Wigner values read once, Q/RgQ
   // written once by the assembly, read once by the solve, T written once).
-  double f_asm = 0., b_asm = 0., b_tab = 0.;
+  // (index 0: blocks above kSmallBlock, 1: the others)
+  double f_asm[2] = {0., 0.}, b_asm[2] = {0., 0.}, b_tab = 0.;
   for (const Item &it : w.items) {
     const int nmin = it.m > 0 ? it.m : 1;
     const double nm = it.N + 1 - nmin;
-    if (it.m > 0) {
-      f_asm += 96. * it.G * nm * nm;
-    } else {
-      f_asm += 32. * it.G * nm * nm;
-    }
-    b_asm += 8. * (2. * nm * it.G) + 16. * 4. * nm * nm;
+    const int k = nm > kSmallBlock ? 0 : 1;
+    f_asm[k] += (it.m > 0 ? 96. : 32.) * it.G * nm * nm;
+    b_asm[k] += 8. * (2. * nm * it.G) + 16. * 4. * nm * nm;
   }
   for (const TableSet &ts : w.sets)
     b_tab += 8. * ((double)kNodeScalars * ts.G +
                    (double)kBesselArrays * ts.N * ts.G);
-  b_asm += b_tab;
   CTM_LAUNCH(CTM_KERNEL_TABLES, 0., b_tab,
              launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(), gp,
                            ctx->d_gl.as<double>(), ctx->d_tables.as<double>(),
                            s));
   {
-    // two launches: 2 x 2 micro-tiles for the large blocks, then 1 x 1;
-    // the algorithmic work is credited to the first
+    // two launches: 2 x 2 micro-tiles for the large blocks, then 1 x 1,
+    // each credited with the algorithmic work of its blocks
     const int nl = w.n_large, ns = (int)w.order.size() - nl;
     const int32_t *ord = ctx->d_tiles.as<int32_t>();
     const int32_t *pre = ctx->d_gioff_tmp.as<int32_t>();
     if (nl > 0)
-      CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, f_asm, b_asm,
+      CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, f_asm[0], b_asm[0],
                  launch_assemble(2, ord, pre, nl, w.tile_prefix[nl],
                                  ctx->d_items.as<Item>(),
                                  ctx->d_sets.as<TableSet>(),
@@ -525,8 +522,7 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
                                  ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(),
                                  s));
     if (ns > 0)
-      CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, nl > 0 ? 0. : f_asm,
-                 nl > 0 ? 0. : b_asm,
+      CTM_LAUNCH(CTM_KERNEL_ASSEMBLE_SMALL, f_asm[1], b_asm[1],
                  launch_assemble(1, ord + nl, pre + nl + 1, ns,
                                  w.tile_prefix[nl + 1 + ns],
                                  ctx->d_items.as<Item>(),

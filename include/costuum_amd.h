@@ -275,7 +275,8 @@ int ctm_orientation_coefficients(ctm_real_fn p, void *arg, uint32_t order,
 #define CTM_KERNEL_OBS_X 7
 #define CTM_KERNEL_OBS_SUM 8
 #define CTM_KERNEL_AVERAGE 9
-#define CTM_KERNEL_COUNT 10
+#define CTM_KERNEL_ASSEMBLE_SMALL 10 /* blocks of order <= 16 */
+#define CTM_KERNEL_COUNT 11
 
 typedef struct ctm_kernel_stats {
   uint64_t launches;
