@@ -40,10 +40,13 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=64,
                     help="CPU sample = every k-th grain in size order")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", default="batch", choices=["batch", "grid"],
+    ap.add_argument("--workload", default="batch",
+                    choices=["batch", "grid", "gradients"],
                     help="batch: the 4096-grain seed-0 batch (default); "
                          "grid: the 61 440 T-matrix size x wavelength x "
-                         "shape grid at orders up to 160")
+                         "shape grid at orders up to 160; gradients: the "
+                         "1024-grain seed-1 batch with d/d(R_V, d, Re m, "
+                         "Im m) of Q_ext, Q_abs, Q_sca")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 512 MiB five times "
                          "(reference kernel for the HBM byte counters)")
@@ -243,16 +246,75 @@ def run_grid(a):
     print(json.dumps(out), flush=True)
 
 
+def run_gradients(a):
+    """The 1024-grain seed-1 batch: converged T-matrices, Q_ext and Q_abs
+    (without the scattered-light term, the part linear in T) at 16
+    Gauss-Legendre incidence angles, the full-T Q_sca/Q_ext sums, and their
+    derivatives with respect to (R_V, d, Re m, Im m) at the converged orders
+    (forward mode).  One step does all of it."""
+    import numpy as np
+    import torch
+    import make_inputs
+    from costuum_amd.tmatrix import (Context, observables,
+                                     observables_tangent, tangents,
+                                     tmatrix_batch)
+    g = make_inputs.load(1, 1024)
+    ctx = Context(0)
+    dev = ctx.device
+    keys = ("radius", "axis_ratio", "wavelength", "mr_re", "mr_im")
+    dg = [torch.as_tensor(g[k], dtype=torch.float64, device=dev) for k in keys]
+    xg, _ = np.polynomial.legendre.leggauss(16)
+    theta = np.arccos(xg)
+    n = len(g["radius"])
+
+    def step():
+        r = tmatrix_batch(*dg, ctx=ctx, min_order=4, max_order=100, glfac=2,
+                          tolerance=1e-4, sync=False)
+        q = observables(theta, ctx=ctx, account_for_scattering=False,
+                        n_grains=n)
+        dqs, dqe = tangents(r)
+        dq = [observables_tangent(theta, j, ctx, n, radius=g["radius"])
+              for j in range(4)]
+        return r, q, dqs, dqe, dq
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        r, q, dqs, dqe, dq = step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / max(a.steps, 1)
+    st = r.status.cpu().numpy()
+    out = {
+        "metric": "converged T-matrices with gradients/s on 1 GPU",
+        "value": round(int((st == 0).sum()) / (ms / 1e3), 2),
+        "unit": "converged T-matrices with d/d(R_V, d, Re m, Im m) per s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "1024 spheroids, seed 1, orders 4..99; "
+                               "Q_ext, Q_abs at 16 angles and full-T sums "
+                               "with forward-mode derivatives",
+                   "grains": n, "converged": int((st == 0).sum()),
+                   "mean_dqext_dR": float(dq[0]["qext"].mean().item())},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
         copy_calibration()
         return
-    if a.workload == "grid":
+    if a.workload in ("grid", "gradients"):
         if a.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
             sys.exit(2)
-        run_grid(a)
+        (run_grid if a.workload == "grid" else run_gradients)(a)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU only: grains shard across GPUs "
