@@ -411,9 +411,9 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
   // capacity are inverted in global scratch, at most kGlobalSolveBytes of it
   // per launch
   constexpr size_t kGlobalSolveBytes = (size_t)1 << 30;
-  // (68 splits the LDS bucket where two blocks of up to order 68 fit a
-  // CU's 160 KiB of LDS together)
-  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 68, 48, 32, 16};
+  // (64 splits the LDS bucket where two blocks fit a CU's 160 KiB of LDS
+  // together and one row per lane suffices)
+  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 64, 48, 32, 16};
   constexpr int kBuckets = 6;
   size_t pos = 0;
   const int32_t *dsolve = ctx->d_solve.as<int32_t>();

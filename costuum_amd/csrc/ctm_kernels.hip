@@ -1143,6 +1143,257 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
                  qparts, diag_only, red, ainv_out);
 }
 
+// K4 for orders 49..99 with the matrix in LDS: blocked Gauss-Jordan.  The
+// columns of each sub-block are eliminated in panels of W.  One wave holds a
+// panel in registers (RS rows per lane) and runs the W elimination steps on
+// it alone -- pivot reduction, row exchange and update need no barrier, and
+// it records the pivot rows.  The panel's W steps compose to one
+// transformation that is the identity outside the panel's columns, whose
+// panel columns are the eliminated panel itself (F); every other column j
+// of the sub-block then becomes, after the panel's row exchanges (sigma),
+//   A(i, j) = [i not a panel row] A(sigma i, j) + sum_q F(i, q) A(sigma q, j),
+// one rank-W update per column, done by one wave per column (loads of the
+// column precede its stores in the wave's order, so no barrier).  Look-ahead:
+// the wave that eliminates the next panel first applies this update to that
+// panel's columns, while the other waves of the sub-block update the rest;
+// one barrier separates panels.  Same pivot rule as k_invert_solve (largest
+// |a_ik|^2 over rows i >= k, first row on ties); results equal it to
+// rounding, and the A / piv layout handed to solve_tail is the same.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) |
+                              (long long)(unsigned)lo);
+}
+
+template <int RS, int W>
+__device__ __forceinline__ void blk_panel(cdbl *A, int nm, int lo, int hi,
+                                          int c0, int w, int lane, int *piv,
+                                          int *steps) {
+  cdbl P[RS][W];
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int i = lo + lane + 64 * r;
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      P[r][j] = (i < hi && j < w) ? A[(int64_t)i * nm + c0 + j] : cmk(0., 0.);
+  }
+#pragma unroll
+  for (int s = 0; s < W; ++s) {
+    if (s >= w) break;
+    const int k = c0 + s;
+    // pivot: key = bits(|a|^2) + 1 orders non-negative magnitudes; 0 = none
+    unsigned long long key[RS], mx = 0ull;
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const int i = lo + lane + 64 * r;
+      const double mag = P[r][s].x * P[r][s].x + P[r][s].y * P[r][s].y;
+      key[r] = (i >= k && i < hi)
+                   ? (unsigned long long)__double_as_longlong(mag) + 1ull
+                   : 0ull;
+      mx = key[r] > mx ? key[r] : mx;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long om = __shfl_xor(mx, o);
+      mx = om > mx ? om : mx;
+    }
+    int p = k;
+#pragma unroll
+    for (int r = RS - 1; r >= 0; --r) {
+      const unsigned long long bal = __ballot(key[r] == mx);
+      if (bal) p = lo + 64 * r + __builtin_ctzll(bal);
+    }
+    p = __builtin_amdgcn_readfirstlane(p);
+    const int rk = k - lo, rp = p - lo;
+    cdbl vk[W], vp[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      cdbl a = P[0][j], b = P[0][j];
+#pragma unroll
+      for (int r = 1; r < RS; ++r) {
+        if ((rk >> 6) == r) a = P[r][j];
+        if ((rp >> 6) == r) b = P[r][j];
+      }
+      vk[j] = cmk(readlane_f64(a.x, rk & 63), readlane_f64(a.y, rk & 63));
+      vp[j] = cmk(readlane_f64(b.x, rp & 63), readlane_f64(b.y, rp & 63));
+    }
+    if (lane == 0) {
+      piv[k] = p;
+      steps[s] = p;
+    }
+    const cdbl inv = crcp(vp[s]);
+    cdbl g[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) g[j] = cmul(vp[j], inv);
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const int i = lo + lane + 64 * r;
+      if (i == k) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) P[r][j] = (j == s) ? inv : g[j];
+      } else {
+        // row p now holds the old row k
+        if (i == p) {
+#pragma unroll
+          for (int j = 0; j < W; ++j) P[r][j] = vk[j];
+        }
+        const cdbl ci = P[r][s];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          if (j == s) {
+            cdbl v = cmul(ci, inv);
+            P[r][j] = cmk(-v.x, -v.y);
+          } else {
+            P[r][j].x = P[r][j].x - (ci.x * g[j].x - ci.y * g[j].y);
+            P[r][j].y = P[r][j].y - (ci.x * g[j].y + ci.y * g[j].x);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int i = lo + lane + 64 * r;
+    if (i >= hi) continue;
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      if (j < w) A[(int64_t)i * nm + c0 + j] = P[r][j];
+  }
+}
+
+// The rank-W update of columns j = jbeg, jbeg + jstep, ... < jend of the
+// sub-block [lo, hi), skipping [skip0, skip1), by one wave, with the panel
+// at columns c0..c0+w-1 and its pivot rows steps[0..w).
+template <int RS, int W>
+__device__ __forceinline__ void blk_update(cdbl *A, int nm, int lo, int hi,
+                                           int c0, int w, const int *steps,
+                                           int jbeg, int jend, int jstep,
+                                           int skip0, int skip1, int lane) {
+  int sg[RS];
+  cdbl F[RS][W];
+  int sq[W];
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    int pos = c0 + q;
+    for (int s = w - 1; s >= 0; --s) {
+      const int k = c0 + s, p = steps[s];
+      pos = pos == k ? p : (pos == p ? k : pos);
+    }
+    sq[q] = q < w ? pos : c0;
+  }
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int i = lo + lane + 64 * r;
+    int pos = i;
+    for (int s = w - 1; s >= 0; --s) {
+      const int k = c0 + s, p = steps[s];
+      pos = pos == k ? p : (pos == p ? k : pos);
+    }
+    sg[r] = i < hi ? pos : lo;
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+      F[r][q] = (i < hi && q < w) ? A[(int64_t)i * nm + c0 + q] : cmk(0., 0.);
+  }
+  for (int j = jbeg; j < jend; j += jstep) {
+    if (j >= skip0 && j < skip1) continue;
+    cdbl po[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) po[q] = A[(int64_t)sq[q] * nm + j];
+    cdbl acc[RS];
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const int i = lo + lane + 64 * r;
+      const bool prow = i >= c0 && i < c0 + w;
+      acc[r] = prow ? cmk(0., 0.) : A[(int64_t)sg[r] * nm + j];
+#pragma unroll
+      for (int q = 0; q < W; ++q) cfma(acc[r], F[r][q], po[q]);
+    }
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const int i = lo + lane + 64 * r;
+      if (i < hi) A[(int64_t)i * nm + j] = acc[r];
+    }
+  }
+}
+
+template <int NT, int RS, int W>
+__global__ __launch_bounds__(NT) void k_invert_solve_blk(
+    const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
+    double *qparts, int diag_only, cdbl *ainv_out) {
+  constexpr int NW = NT / 64;
+  static_assert(NW >= 4, "two waves per sub-block at least");
+  extern __shared__ double smem[];
+  __shared__ int stp[2][2][W];
+  __shared__ double red[NW][4];
+
+  const int task = tasks[blockIdx.x];
+  const int item_index = task >> 1;
+  const int cls = task & 1;
+  const Item it = items[item_index];
+  const int m = it.m;
+  const int nmin = m > 0 ? m : 1;
+  const int nm = it.N + 1 - nmin;
+  const int64_t nm2 = (int64_t)nm * nm;
+  const int nb = (m == 0 && nm > 1) ? 2 : 1;
+  const int sz0 = (nb == 2) ? (nm + 1) / 2 : nm;
+  const int off[2] = {0, sz0};
+  const int sz[2] = {sz0, nm - sz0};
+  auto pidx = [&](int i) {
+    return nb == 2 ? (i < sz0 ? 2 * i : 2 * (i - sz0) + 1) : i;
+  };
+  cdbl *A = reinterpret_cast<cdbl *>(smem);
+  int *piv = reinterpret_cast<int *>(A + nm2);
+  int *src = piv + nm;
+  const cdbl *Qg = qbuf + it.q_off + cls * nm2;
+  const cdbl *RQg = qbuf + it.q_off + (2 + cls) * nm2;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+
+  for (int i = t >> 5; i < nm; i += NT >> 5) {
+    const int pi = pidx(i);
+    const int bi = i >= sz0;
+    for (int j = t & 31; j < nm; j += 32) {
+      const int bj = j >= sz0;
+      A[(int64_t)i * nm + j] =
+          (bi == bj) ? Qg[(int64_t)pi * nm + pidx(j)] : cmk(0., 0.);
+    }
+  }
+  __syncthreads();
+  // waves [b * GW, (b + 1) * GW) serve sub-block b; the first is its panel
+  // wave
+  const int GW = NW / nb;
+  const int b = wv / GW, gw = wv - b * GW;
+  const int lo = off[b], hi = off[b] + sz[b];
+  const int npan = (sz[b] + W - 1) / W;
+  const int npmax = (sz[0] + W - 1) / W;
+  if (gw == 0)
+    blk_panel<RS, W>(A, nm, lo, hi, lo, min(W, sz[b]), lane, piv, stp[0][b]);
+  __syncthreads();
+  for (int P = 0; P < npmax; ++P) {
+    if (P < npan) {
+      const int c0 = lo + P * W, w = min(W, hi - c0);
+      const int c1 = c0 + w, w1 = min(W, hi - c1);
+      const int *steps = stp[P & 1][b];
+      const bool ahead = P + 1 < npan;
+      if (ahead && gw == 0) {
+        blk_update<RS, W>(A, nm, lo, hi, c0, w, steps, c1, c1 + w1, 1, 0, 0,
+                          lane);
+        blk_panel<RS, W>(A, nm, lo, hi, c1, w1, lane, piv,
+                         stp[(P + 1) & 1][b]);
+      } else {
+        // columns outside the panel (and the next panel, if any)
+        const int nwk = ahead ? GW - 1 : GW, rank = ahead ? gw - 1 : gw;
+        blk_update<RS, W>(A, nm, lo, hi, c0, w, steps, lo + rank, hi, nwk,
+                          c0, ahead ? c1 + w1 : c1, lane);
+      }
+    }
+    __syncthreads();
+  }
+  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, piv, src, RQg, tbuf,
+                 qparts, diag_only, red, ainv_out);
+}
+
 // ---------------------------------------------------------------------------
 // K5: convergence test over one window of orders for each active grain.
 // Items of grain g for the window are win_items[win_off[a] .. win_off[a+1])
@@ -1303,6 +1554,24 @@ static hipError_t launch_solve_reg(const int32_t *tasks, int n, size_t lds,
   return hipGetLastError();
 }
 
+template <int NT, int RS, int W>
+static hipError_t launch_solve_blk(const int32_t *tasks, int n, size_t lds,
+                                   const Item *items, const cdbl *qbuf,
+                                   cdbl *tbuf, double *qparts, int diag_only,
+                                   cdbl *ainv_out, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(k_invert_solve_blk<NT, RS, W>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_invert_solve_blk<NT, RS, W>), dim3(n), dim3(NT), lds,
+                     s, tasks, items, qbuf, tbuf, qparts, diag_only, ainv_out);
+  return hipGetLastError();
+}
+
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
@@ -1327,10 +1596,15 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
   if (nm_max <= 48)
     return launch_solve_reg<256, 16, 3, 3>(tasks, n, lds, items, qbuf, tbuf,
                                            qparts, diag_only, ainv_out, s);
-  // above order 48 one block fills a CU's LDS either way, and the
-  // LDS-resident elimination (fewer registers) measured faster
-  return launch_solve_nt<512, false>(tasks, n, lds, items, qbuf, tbuf, qparts,
-                                     diag_only, nullptr, 0, ainv_out, s);
+  // above order 48: blocked elimination in LDS, two rows per lane above
+  // order 64
+  const size_t lds_blk =
+      (size_t)nm_max * nm_max * sizeof(cdbl) + 2 * (size_t)nm_max * sizeof(int);
+  if (nm_max <= 64)
+    return launch_solve_blk<512, 1, 8>(tasks, n, lds_blk, items, qbuf, tbuf,
+                                       qparts, diag_only, ainv_out, s);
+  return launch_solve_blk<512, 2, 8>(tasks, n, lds_blk, items, qbuf, tbuf,
+                                     qparts, diag_only, ainv_out, s);
 }
 
 hipError_t launch_converge(const int32_t *active, int n_active,
