@@ -1108,6 +1108,14 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
     p_out.push_back(nth + i);
     p_in.push_back((int)in_l.size() - 1);
   }
+  // pairs grouped by incoming direction (counting sort, stable)
+  std::vector<int32_t> in_off(in_l.size() + 1, 0), by_in(p_in.size());
+  for (int32_t v : p_in) in_off[v + 1]++;
+  for (size_t i = 0; i < in_l.size(); ++i) in_off[i + 1] += in_off[i];
+  {
+    std::vector<int32_t> fill(in_off.begin(), in_off.end() - 1);
+    for (size_t q = 0; q < p_in.size(); ++q) by_in[fill[p_in[q]]++] = (int32_t)q;
+  }
   sp.n_ang = n_ang;
   sp.n_in = (int)in_l.size();
   sp.n_out = (int)out_l.size();
@@ -1129,6 +1137,8 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   const size_t o_out = put(out_l.data(), out_l.size() * 4);
   const size_t o_pin = put(p_in.data(), p_in.size() * 4);
   const size_t o_pout = put(p_out.data(), p_out.size() * 4);
+  const size_t o_byin = put(by_in.data(), by_in.size() * 4);
+  const size_t o_inoff = put(in_off.data(), in_off.size() * 4);
   const size_t o_tw = put(gw.data(), gw.size() * 8);
   const size_t o_pc = put(pc.data(), pc.size() * 8);
   const size_t o_ps = put(ps.data(), ps.size() * 8);
@@ -1144,6 +1154,8 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   sp.out_angle = reinterpret_cast<const int32_t *>(base + o_out);
   sp.pair_in = reinterpret_cast<const int32_t *>(base + o_pin);
   sp.pair_out = reinterpret_cast<const int32_t *>(base + o_pout);
+  sp.pair_by_in = reinterpret_cast<const int32_t *>(base + o_byin);
+  sp.in_pair_off = reinterpret_cast<const int32_t *>(base + o_inoff);
   sp.theta_w = reinterpret_cast<const double *>(base + o_tw);
   sp.phi_cos = reinterpret_cast<const double *>(base + o_pc);
   sp.phi_sin = reinterpret_cast<const double *>(base + o_ps);
@@ -1160,7 +1172,7 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   CTM_HIP(ctx->d_x.ensure(nitems * (size_t)sp.n_pairs * 4 * sizeof(cdbl)));
   CTM_LAUNCH(CTM_KERNEL_OBS_X, 0., 0.,
              launch_pitau(d_ang, n_ang, nmax_all, ctx->d_pt.as<double>(), s));
-  if (obs_x_lds_bytes(nmax_all, sp.n_out) > 159 * 1024) {
+  if (obs_x_lds_bytes(nmax_all, sp.n_out, 8) > 159 * 1024) {
     ctx->err = "too many outgoing directions for the largest order";
     return CTM_ERR_ARGUMENT;
   }
@@ -1191,7 +1203,7 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
       if (end > pos) {
         CTM_LAUNCH(CTM_KERNEL_OBS_X, fb, 0.,
                    launch_obs_x(sp, ctx->d_obsitems.as<int32_t>() + pos,
-                                (int)(end - pos), cap,
+                                (int)(end - pos), cap, b == 0 ? 8 : 4,
                                 ctx->d_items.as<Item>(),
                                 ctx->obs_source >= 0
                                     ? ctx->d_dt.as<cdbl>() +

@@ -1585,25 +1585,23 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                       qparts, diag_only, gscratch,
                                       (int64_t)nm_max * nm_max, ainv_out, s);
   }
+  const size_t lds_blk =
+      (size_t)nm_max * nm_max * sizeof(cdbl) + 2 * (size_t)nm_max * sizeof(int);
   // register-resident elimination: grid and elements per thread by size
   const size_t lds = invert_solve_lds_bytes(nm_max);
   if (nm_max <= 16)
     return launch_solve_reg<64, 8, 2, 2>(tasks, n, lds, items, qbuf, tbuf,
                                          qparts, diag_only, ainv_out, s);
-  if (nm_max <= 32)
-    return launch_solve_reg<256, 16, 2, 2>(tasks, n, lds, items, qbuf, tbuf,
-                                           qparts, diag_only, ainv_out, s);
+  // above order 16: blocked elimination in LDS with panels of 4 columns
+  // (measured faster than 2, 6, 8 or 16), four waves per block up to order
+  // 48, eight above, two rows per lane above order 64
   if (nm_max <= 48)
-    return launch_solve_reg<256, 16, 3, 3>(tasks, n, lds, items, qbuf, tbuf,
-                                           qparts, diag_only, ainv_out, s);
-  // above order 48: blocked elimination in LDS, two rows per lane above
-  // order 64
-  const size_t lds_blk =
-      (size_t)nm_max * nm_max * sizeof(cdbl) + 2 * (size_t)nm_max * sizeof(int);
-  if (nm_max <= 64)
-    return launch_solve_blk<512, 1, 8>(tasks, n, lds_blk, items, qbuf, tbuf,
+    return launch_solve_blk<256, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
                                        qparts, diag_only, ainv_out, s);
-  return launch_solve_blk<512, 2, 8>(tasks, n, lds_blk, items, qbuf, tbuf,
+  if (nm_max <= 64)
+    return launch_solve_blk<512, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
+                                       qparts, diag_only, ainv_out, s);
+  return launch_solve_blk<512, 2, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
                                      qparts, diag_only, ainv_out, s);
 }
 

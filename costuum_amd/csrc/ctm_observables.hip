@@ -104,14 +104,18 @@ __device__ inline cdbl cfac(int nn, int n) {
 }
 
 // K6: X_m^k for every direction pair of one (grain, m) item.  The incoming
-// directions are processed in chunks of kInChunk: Y(h n; i) = sum_nn
-// c_{nn n} T(h n; h' nn) b(h' nn; i) for b and b' (for the parity-class
-// layout h' is fixed by the class), one thread per row (h n) with the whole
-// chunk in registers, then one thread per direction pair contracts with the
-// outgoing pi/tau for all four k.  pi/tau of the item's directions and the
-// n-factors a_n are staged in LDS.
-constexpr int kInChunk = 8;
-
+// directions are processed in chunks of CK: Y(h n; i) = sum_nn c_{nn n}
+// T(h n; h' nn) b(h' nn; i) for b and b' (for the parity-class layout h' is
+// fixed by the class), one thread per row (h n) with the chunk's
+// accumulators in registers.  When the 2 nm rows leave threads idle,
+// several chunks run side by side (G groups of 2 nm threads; G chunks per
+// round).  Then one thread per direction pair of the round's incoming
+// directions (pairs listed by incoming direction) contracts Y with the
+// outgoing pi/tau for all four k.  pi/tau of the outgoing directions
+// (stored [n][direction] so neighbouring pairs read neighbouring words),
+// of the round's incoming directions and the n-factors a_n are staged in
+// LDS.
+template <int CK>
 __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
                                                const int32_t *item_list,
                                                const Item *items,
@@ -131,36 +135,48 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
   const int n2m = 2 * nm;
   const cdbl *T = tbuf + it.t_off;
   const cdbl *A = abuf ? abuf + aoff[ii] : nullptr;
-  cdbl *Ysh = reinterpret_cast<cdbl *>(smem);            // [chunk][2][2 nm]
-  double *PTo = smem + 2 * (kInChunk * 2 * 2 * nm_cap);  // [out][p][2]
-  double *Bv = PTo + (int64_t)spec.n_out * nm_cap * 2;   // [chunk][p][2]
-  double *an = Bv + kInChunk * nm_cap * 2;               // [p]
+  // LDS: Ysh [G CK][2][2 nm] cdbl (G 2 nm <= max(256, 2 nm_cap)),
+  // PTo [nm][n_out][2], Bv [G CK][nm][2], an [nm]
+  const int rows_cap = max(256, 2 * nm_cap);
+  cdbl *Ysh = reinterpret_cast<cdbl *>(smem);
+  double *PTo = smem + 2 * (CK * 2 * rows_cap);
+  double *Bv = PTo + (int64_t)spec.n_out * nm_cap * 2;
+  double *an = Bv + (int64_t)CK * (rows_cap / 2) * 2;
   const int t = threadIdx.x;
   const int n_ang = spec.n_ang;
+  const int n_out = spec.n_out;
   const double *PTm = PT + (int64_t)m * n_ang * nmax_all * 2;
-  for (int e = t; e < spec.n_out * nm; e += 256) {
-    const int o = e / nm, p = e - (e / nm) * nm;
+  for (int e = t; e < n_out * nm; e += 256) {
+    const int p = e / n_out, o = e - (e / n_out) * n_out;
     const double *src =
         PTm + ((int64_t)spec.out_angle[o] * nmax_all + (nmin + p - 1)) * 2;
-    PTo[(o * nm + p) * 2] = src[0];
-    PTo[(o * nm + p) * 2 + 1] = src[1];
+    PTo[(p * n_out + o) * 2] = src[0];
+    PTo[(p * n_out + o) * 2 + 1] = src[1];
   }
   for (int p = t; p < nm; p += 256) {
     const int n = nmin + p;
     an[p] = sqrt((2. * n + 1.) / (n * (n + 1.)));
   }
   cdbl *Xo = X + (int64_t)ii * spec.n_pairs * 4;
+  // chunk groups per round
+  const int nchunk = (spec.n_in + CK - 1) / CK;
+  if (nchunk == 0) return;
+  const int gmax = n2m >= 256 ? 1 : min(nchunk, 256 / n2m);
+  const int rounds = (nchunk + gmax - 1) / gmax;
+  const int G = (nchunk + rounds - 1) / rounds;
+  const int gi = G == 1 ? 0 : t / n2m;
 
-  for (int c0 = 0; c0 < spec.n_in; c0 += kInChunk) {
-    const int nc = min(kInChunk, spec.n_in - c0);
+  for (int r = 0; r < rounds; ++r) {
+    const int in0 = r * G * CK;
+    const int in1 = min(spec.n_in, in0 + G * CK);
     __syncthreads();
-    for (int e = t; e < kInChunk * nm; e += 256) {
+    for (int e = t; e < G * CK * nm; e += 256) {
       const int ic = e / nm, p = e - (e / nm) * nm;
       double bp = 0., bt = 0.;
-      if (ic < nc) {
+      if (in0 + ic < in1) {
         const double *src =
             PTm +
-            ((int64_t)spec.in_angle[c0 + ic] * nmax_all + (nmin + p - 1)) * 2;
+            ((int64_t)spec.in_angle[in0 + ic] * nmax_all + (nmin + p - 1)) * 2;
         bp = src[0];
         bt = src[1];
       }
@@ -168,14 +184,17 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
       Bv[(ic * nm + p) * 2 + 1] = bt;
     }
     __syncthreads();
-    for (int row = t; row < n2m; row += 256) {
+    const int row0 = G == 1 ? t : t - gi * n2m;
+    const int rstep = G == 1 ? 256 : n2m;
+    for (int row = row0; gi < G && row < n2m; row += rstep) {
       const int h = row >= nm ? 1 : 0;
       const int p = row - h * nm;
       const int n = nmin + p;
       const double a_n = an[p];
-      cdbl acc[kInChunk][2];
+      const double *Bg = Bv + gi * CK * nm * 2;
+      cdbl acc[CK][2];
 #pragma unroll
-      for (int ic = 0; ic < kInChunk; ++ic) {
+      for (int ic = 0; ic < CK; ++ic) {
         acc[ic][0] = cmk(0., 0.);
         acc[ic][1] = cmk(0., 0.);
       }
@@ -196,9 +215,9 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
           else if (e == 2) cm = cmk(-f * tv.x, -f * tv.y);
           else cm = cmk(f * tv.y, -f * tv.x);
 #pragma unroll
-          for (int ic = 0; ic < kInChunk; ++ic) {
-            const double bp = Bv[(ic * nm + q) * 2];
-            const double bt = Bv[(ic * nm + q) * 2 + 1];
+          for (int ic = 0; ic < CK; ++ic) {
+            const double bp = Bg[(ic * nm + q) * 2];
+            const double bt = Bg[(ic * nm + q) * 2 + 1];
             // b = (pi, tau) over h2, b' = (tau, pi)
             const double b0 = h2 ? bt : bp;
             const double b1 = h2 ? bp : bt;
@@ -208,18 +227,20 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
         }
       }
 #pragma unroll
-      for (int ic = 0; ic < kInChunk; ++ic) {
-        Ysh[(ic * 2 + 0) * n2m + row] = acc[ic][0];
-        Ysh[(ic * 2 + 1) * n2m + row] = acc[ic][1];
+      for (int ic = 0; ic < CK; ++ic) {
+        const int icg = gi * CK + ic;
+        Ysh[(icg * 2 + 0) * n2m + row] = acc[ic][0];
+        Ysh[(icg * 2 + 1) * n2m + row] = acc[ic][1];
       }
     }
     __syncthreads();
     // pair contractions: X^0 = a.Y_b, X^1 = a.Y_b', X^2 = -a'.Y_b,
     // X^3 = a'.Y_b' with a = (pi, tau), a' = (tau, pi)
-    for (int pr = t; pr < spec.n_pairs; pr += 256) {
-      const int ic = spec.pair_in[pr] - c0;
-      if (ic < 0 || ic >= nc) continue;
-      const double *ao = PTo + (int64_t)spec.pair_out[pr] * nm * 2;
+    const int q0 = spec.in_pair_off[in0], q1 = spec.in_pair_off[in1];
+    for (int qi = q0 + t; qi < q1; qi += 256) {
+      const int pr = spec.pair_by_in[qi];
+      const int ic = spec.pair_in[pr] - in0;
+      const int o = spec.pair_out[pr];
       const cdbl *Yb = Ysh + (ic * 2 + 0) * n2m;
       const cdbl *Ybp = Ysh + (ic * 2 + 1) * n2m;
       cdbl x0 = cmk(0., 0.), x1 = cmk(0., 0.), x2 = cmk(0., 0.),
@@ -227,7 +248,8 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
       for (int row = 0; row < n2m; ++row) {
         const int h = row >= nm ? 1 : 0;
         const int p = row - h * nm;
-        const double pi = ao[2 * p], ta = ao[2 * p + 1];
+        const double pi = PTo[(p * n_out + o) * 2];
+        const double ta = PTo[(p * n_out + o) * 2 + 1];
         const double a = h ? ta : pi, as = h ? pi : ta;
         rfma(x0, a, Yb[row]);
         rfma(x1, a, Ybp[row]);
@@ -406,31 +428,47 @@ hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
   return hipGetLastError();
 }
 
-size_t obs_x_lds_bytes(int nm_cap, int n_out) {
-  return (size_t)kInChunk * 2 * 2 * nm_cap * sizeof(cdbl) +
+size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck) {
+  const size_t rows_cap = std::max(256, 2 * nm_cap);
+  return (size_t)ck * 2 * rows_cap * sizeof(cdbl) +
          (size_t)n_out * nm_cap * 2 * sizeof(double) +
-         (size_t)kInChunk * nm_cap * 2 * sizeof(double) +
+         (size_t)ck * (rows_cap / 2) * 2 * sizeof(double) +
          (size_t)nm_cap * sizeof(double);
 }
 
-hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
-                        int n_items, int nm_cap, const Item *items,
-                        const cdbl *tbuf, const cdbl *abuf,
-                        const int64_t *aoff, const double *PT, int nmax_all,
-                        cdbl *X, hipStream_t s) {
-  if (n_items <= 0) return hipSuccess;
+template <int CK>
+static hipError_t launch_obs_x_ck(const ObsSpec &spec,
+                                  const int32_t *item_list, int n_items,
+                                  int nm_cap, const Item *items,
+                                  const cdbl *tbuf, const cdbl *abuf,
+                                  const int64_t *aoff, const double *PT,
+                                  int nmax_all, cdbl *X, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(k_obs_x),
+        reinterpret_cast<const void *>(k_obs_x<CK>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL(k_obs_x, dim3(n_items), dim3(256),
-                     obs_x_lds_bytes(nm_cap, spec.n_out), s, spec, item_list,
-                     items, tbuf, abuf, aoff, PT, nmax_all, nm_cap, X);
+  hipLaunchKernelGGL(k_obs_x<CK>, dim3(n_items), dim3(256),
+                     obs_x_lds_bytes(nm_cap, spec.n_out, CK), s, spec,
+                     item_list, items, tbuf, abuf, aoff, PT, nmax_all, nm_cap,
+                     X);
   return hipGetLastError();
+}
+
+hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
+                        int n_items, int nm_cap, int ck, const Item *items,
+                        const cdbl *tbuf, const cdbl *abuf,
+                        const int64_t *aoff, const double *PT, int nmax_all,
+                        cdbl *X, hipStream_t s) {
+  if (n_items <= 0) return hipSuccess;
+  if (ck == 8)
+    return launch_obs_x_ck<8>(spec, item_list, n_items, nm_cap, items, tbuf,
+                              abuf, aoff, PT, nmax_all, X, s);
+  return launch_obs_x_ck<4>(spec, item_list, n_items, nm_cap, items, tbuf,
+                            abuf, aoff, PT, nmax_all, X, s);
 }
 
 hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,

@@ -20,6 +20,8 @@ struct ObsSpec {
   const int32_t *out_angle;  // [n_out]
   const int32_t *pair_in;    // [n_pairs] index into the incoming list
   const int32_t *pair_out;   // [n_pairs] index into the outgoing list
+  const int32_t *pair_by_in;  // [n_pairs] pairs ordered by incoming index
+  const int32_t *in_pair_off; // [n_in + 1] start of each incoming direction
   const double *theta_w;     // [ngauss] GL weights of cos(theta_in)
   const double *phi_cos, *phi_sin, *phi_w;  // [ngauss] azimuths on [0, 2pi]
   const double *sphi_cos, *sphi_sin;        // [n_sphi] scattering azimuths
@@ -41,11 +43,12 @@ hipError_t launch_query(const Query *q, int n, const Item *items,
 
 hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
                         hipStream_t s);
-size_t obs_x_lds_bytes(int nm_cap, int n_out);
+// ck: incoming directions per thread in the T b products (4 or 8)
+size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck);
 // item_list: items to process (largest first); abuf/aoff (may be null):
 // full-layout ensemble T-matrices per item.
 hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
-                        int n_items, int nm_cap, const Item *items,
+                        int n_items, int nm_cap, int ck, const Item *items,
                         const cdbl *tbuf, const cdbl *abuf,
                         const int64_t *aoff, const double *PT, int nmax_all,
                         cdbl *X, hipStream_t s);
