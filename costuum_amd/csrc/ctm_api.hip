@@ -68,38 +68,67 @@ struct DevBuf {
   }
 };
 
-// Grain-independent table caches (per batch).
+// Pinned host memory for uploads (see stage_copy).
+struct Staging {
+  char *p = nullptr;
+  size_t cap = 0, used = 0;
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = used = 0;
+  }
+};
+
+// Grain-independent table caches (per batch).  Gauss-Legendre keys are
+// found by G directly; Wigner keys through one slot array of N + 1 entries
+// (one per m) per (N, G), the last (N, G) looked up kept at hand (item
+// lists ask for all m of one (N, G) in a row).
 struct SharedTables {
-  std::map<int, int> gl_index;                 // G -> key index
-  std::unordered_map<uint64_t, int> wig_index;  // (N, m, G) -> key
+  std::vector<int32_t> gl_by_g;                 // G -> key index or -1
+  std::unordered_map<uint64_t, int32_t> wig_base;  // (N, G) -> first slot
+  std::vector<int32_t> wig_slot;                // key index or -1
   std::vector<GlKey> gl_keys;
   std::vector<WigKey> wig_keys;
   int64_t gl_size = 0, wig_size = 0;
   size_t gl_done = 0, wig_done = 0;  // keys already computed on device
+  int last_n = -1, last_g = -1, last_base = 0;
   void clear() {
-    gl_index.clear();
-    wig_index.clear();
+    gl_by_g.clear();
+    wig_base.clear();
+    wig_slot.clear();
     gl_keys.clear();
     wig_keys.clear();
     gl_size = wig_size = 0;
     gl_done = wig_done = 0;
+    last_n = last_g = -1;
   }
   int gl(int G) {
-    auto f = gl_index.find(G);
-    if (f != gl_index.end()) return f->second;
+    if (G >= (int)gl_by_g.size()) gl_by_g.resize(G + 1, -1);
+    if (gl_by_g[G] >= 0) return gl_by_g[G];
     GlKey k{};
     k.G = G;
     k.off = gl_size;
     gl_size += 2 * (int64_t)G;
     gl_keys.push_back(k);
-    const int idx = (int)gl_keys.size() - 1;
-    gl_index[G] = idx;
-    return idx;
+    gl_by_g[G] = (int)gl_keys.size() - 1;
+    return gl_by_g[G];
   }
   int wig(int N, int m, int G) {
-    const uint64_t key = (uint64_t)N | ((uint64_t)m << 16) | ((uint64_t)G << 32);
-    auto f = wig_index.find(key);
-    if (f != wig_index.end()) return f->second;
+    if (N != last_n || G != last_g) {
+      const uint64_t key = (uint64_t)(uint32_t)N | ((uint64_t)(uint32_t)G << 32);
+      auto f = wig_base.find(key);
+      if (f != wig_base.end()) {
+        last_base = f->second;
+      } else {
+        last_base = (int)wig_slot.size();
+        wig_slot.resize(wig_slot.size() + N + 1, -1);
+        wig_base.emplace(key, last_base);
+      }
+      last_n = N;
+      last_g = G;
+    }
+    int32_t &slot = wig_slot[last_base + m];
+    if (slot >= 0) return slot;
     const int g = gl(G);
     WigKey k{};
     k.N = N;
@@ -110,9 +139,8 @@ struct SharedTables {
     const int nmin = m > 0 ? m : 1;
     wig_size += 2 * (int64_t)(N + 1 - nmin) * G;
     wig_keys.push_back(k);
-    const int idx = (int)wig_keys.size() - 1;
-    wig_index[key] = idx;
-    return idx;
+    slot = (int)wig_keys.size() - 1;
+    return slot;
   }
 };
 
@@ -128,6 +156,7 @@ struct WorkList {
   std::vector<int32_t> tile_prefix;
   int n_large = 0;
   std::vector<int32_t> solve;        // item * 2 + class, by block order
+  std::vector<int32_t> scratch_a, scratch_b, scratch_c;  // build_tasks
   int64_t table_size = 0, q_size = 0, t_size = 0;
   void clear() {
     sets.clear();
@@ -247,7 +276,7 @@ struct ctm_context {
       d_winitems, d_gioff, d_gitems, d_gioff_tmp, d_params, d_ang, d_pt,
       d_x, d_spec, d_cgkeys, d_cg, d_cgidx, d_avgtasks, d_avg, d_aoff, d_pn,
       d_dist, d_obsitems, d_query, d_qout, d_gsolve, d_ttables, d_dq, d_ainv,
-      d_dt, d_tparts, d_dsum, d_ttiles, d_tprefix;
+      d_dt, d_tparts, d_dsum, d_ttiles, d_tprefix, d_wigfresh;
   // Results of the last batch (host).
   int n_grains = 0;
   std::vector<uint32_t> nmax, ngauss;
@@ -265,6 +294,7 @@ struct ctm_context {
   std::vector<int64_t> aoff;  // full-layout offsets per final item
   std::vector<int32_t> obs_items;
   Profiler prof;
+  Staging staging;
 };
 
 #define CTM_HIP(expr)                                                   \
@@ -286,13 +316,50 @@ struct ctm_context {
 
 namespace {
 
+// Host-to-device copy through the context's pinned staging arena: a copy
+// from pinned memory runs asynchronously, a pageable one stages through the
+// runtime and holds the host until the stream reaches it.  Arena space is
+// handed out in order and reused only after the stream has drained, so no
+// pending copy sees its source overwritten.
+hipError_t stage_copy(ctm_context *ctx, void *dst, const void *src,
+                      size_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  Staging &st = ctx->staging;
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (st.used + need > st.cap) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    st.used = 0;
+    if (need > st.cap) {
+      st.release();
+      const size_t want = std::max(2 * need, (size_t)16 << 20);
+      e = hipHostMalloc(reinterpret_cast<void **>(&st.p), want,
+                        hipHostMallocDefault);
+      if (e != hipSuccess) {
+        st.p = nullptr;
+        return e;
+      }
+      st.cap = want;
+    }
+  }
+  char *h = st.p + st.used;
+  std::memcpy(h, src, bytes);
+  st.used += need;
+  return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, ctx->stream);
+}
+
+// Waits for the context's stream; the staging arena is free afterwards.
+hipError_t drain(ctm_context *ctx) {
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) ctx->staging.used = 0;
+  return e;
+}
+
 template <typename T>
-hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+hipError_t upload(ctm_context *ctx, DevBuf &b, const std::vector<T> &v) {
   hipError_t e = b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
   if (e != hipSuccess) return e;
-  if (v.empty()) return hipSuccess;
-  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T),
-                        hipMemcpyHostToDevice, s);
+  return stage_copy(ctx, b.p, v.data(), v.size() * sizeof(T));
 }
 
 // Computes the grain-independent tables whose keys were added since the
@@ -322,28 +389,25 @@ int flush_shared(ctm_context *ctx) {
   CTM_HIP(grow(ctx->d_wig, std::max<int64_t>(sh.wig_size, 1) * sizeof(double)));
   if (sh.gl_done < ngl) {
     std::vector<GlKey> fresh(sh.gl_keys.begin() + sh.gl_done, sh.gl_keys.end());
-    CTM_HIP(upload(ctx->d_glkeys, fresh, ctx->stream));
+    CTM_HIP(upload(ctx, ctx->d_glkeys, fresh));
     CTM_LAUNCH(CTM_KERNEL_GAUSS_LEGENDRE, 0., 0.,
                launch_gauss_legendre(ctx->d_glkeys.as<GlKey>(),
                                      (int)fresh.size(), ctx->d_gl.as<double>(),
                                      ctx->stream));
-    // the key buffer is rewritten below; wait for the launch to read it
-    CTM_HIP(hipStreamSynchronize(ctx->stream));
     sh.gl_done = ngl;
   }
   if (sh.wig_done < nwig) {
     std::vector<WigKey> fresh(sh.wig_keys.begin() + sh.wig_done,
                               sh.wig_keys.end());
-    CTM_HIP(upload(ctx->d_glkeys, fresh, ctx->stream));
+    CTM_HIP(upload(ctx, ctx->d_wigfresh, fresh));
     CTM_LAUNCH(CTM_KERNEL_WIGNER, 0., 0.,
-               launch_wigner(ctx->d_glkeys.as<WigKey>(), (int)fresh.size(),
+               launch_wigner(ctx->d_wigfresh.as<WigKey>(), (int)fresh.size(),
                              ctx->d_gl.as<double>(), ctx->d_wig.as<double>(),
                              ctx->stream));
-    CTM_HIP(hipStreamSynchronize(ctx->stream));
     sh.wig_done = nwig;
   }
   // all Wigner keys on device for the assembly kernel
-  CTM_HIP(upload(ctx->d_wigkeys, sh.wig_keys, ctx->stream));
+  CTM_HIP(upload(ctx, ctx->d_wigkeys, sh.wig_keys));
   return CTM_OK;
 }
 
@@ -352,53 +416,59 @@ int flush_shared(ctm_context *ctx) {
 // counting sort.
 void build_tasks(WorkList &w) {
   const int n = (int)w.items.size();
-  int gmax = 0, nmmax = 0;
-  for (const Item &it : w.items) {
-    gmax = std::max(gmax, it.G);
-    nmmax = std::max(nmmax, it.N + 1 - (it.m > 0 ? it.m : 1));
-  }
-  // key: large blocks first, then by G descending
-  auto is_large = [](const Item &it) {
-    return it.N + 1 - (it.m > 0 ? it.m : 1) > kSmallBlock;
-  };
-  const int nkey = 2 * (gmax + 1);
-  std::vector<int32_t> cnt(nkey + 1, 0);
-  auto key = [&](const Item &it) {
-    return (is_large(it) ? 0 : gmax + 1) + (gmax - it.G);
-  };
-  for (const Item &it : w.items) cnt[key(it) + 1]++;
-  for (int g = 1; g <= nkey; ++g) cnt[g] += cnt[g - 1];
-  w.order.assign(n, 0);
-  w.n_large = 0;
-  for (int i = 0; i < n; ++i) {
-    w.order[cnt[key(w.items[i])]++] = i;
-    if (is_large(w.items[i])) ++w.n_large;
-  }
-  w.tile_prefix.assign((size_t)n + 2, 0);
-  int32_t *pre = w.tile_prefix.data();
-  for (int q = 0; q < n; ++q) {
-    const Item &it = w.items[w.order[q]];
-    const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-    const int tile = q < w.n_large ? 32 : 16;
-    const int ntr = (nm + tile - 1) / tile;
-    const int ntc = it.m == 0 ? (nm + 2 * tile - 1) / (2 * tile) : ntr;
-    if (q == w.n_large) pre += w.n_large + 1;  // second part's prefix
-    const int qq = q < w.n_large ? q : q - w.n_large;
-    pre[qq + 1] = pre[qq] + ntr * ntc;
-  }
-  std::vector<int32_t> c2(nmmax + 2, 0);
-  for (const Item &it : w.items)
-    c2[nmmax - (it.N + 1 - (it.m > 0 ? it.m : 1)) + 1]++;
-  for (int g = 1; g <= nmmax + 1; ++g) c2[g] += c2[g - 1];
-  std::vector<int32_t> byn(n);
+  // per item: block order and an m = 0 flag, then the assembly key
+  std::vector<int32_t> &info = w.scratch_a, &key = w.scratch_b;
+  info.resize(n);
+  key.resize(n);
+  int gmax = 0, nmmax = 0, n_large = 0;
   for (int i = 0; i < n; ++i) {
     const Item &it = w.items[i];
-    byn[c2[nmmax - (it.N + 1 - (it.m > 0 ? it.m : 1))]++] = i;
+    const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+    info[i] = nm | (it.m == 0 ? (1 << 30) : 0);
+    gmax = std::max(gmax, it.G);
+    nmmax = std::max(nmmax, nm);
+    n_large += nm > kSmallBlock;
   }
+  // assembly order: blocks above kSmallBlock first, each part by G
+  // descending (counting sort)
+  const int nkey = 2 * (gmax + 1);
+  std::vector<int32_t> &cnt = w.scratch_c;
+  cnt.assign(std::max(nkey, nmmax + 1) + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const int nm = info[i] & 0xffff;
+    key[i] = (nm > kSmallBlock ? 0 : gmax + 1) + (gmax - w.items[i].G);
+    cnt[key[i] + 1]++;
+  }
+  for (int g = 1; g <= nkey; ++g) cnt[g] += cnt[g - 1];
+  w.order.resize(n);
+  for (int i = 0; i < n; ++i) w.order[cnt[key[i]]++] = i;
+  w.n_large = n_large;
+  // tiles: 32-row (2 x 2 kernel) or 16-row tiles, twice as wide for m = 0
+  w.tile_prefix.resize((size_t)n + 2);
+  int32_t *pre = w.tile_prefix.data();
+  pre[0] = 0;
+  for (int q = 0; q < n_large; ++q) {
+    const int f = info[w.order[q]], nm = f & 0xffff;
+    const int ntr = (nm + 31) >> 5;
+    pre[q + 1] = pre[q] + ntr * ((f >> 30) ? (nm + 63) >> 6 : ntr);
+  }
+  pre += n_large + 1;
+  pre[0] = 0;
+  for (int q = n_large; q < n; ++q) {
+    const int f = info[w.order[q]], nm = f & 0xffff;
+    const int ntr = (nm + 15) >> 4;
+    pre[q - n_large + 1] =
+        pre[q - n_large] + ntr * ((f >> 30) ? (nm + 31) >> 5 : ntr);
+  }
+  // solve tasks (item, class) by block order descending
+  cnt.assign(nmmax + 2, 0);
+  for (int i = 0; i < n; ++i) cnt[nmmax - (info[i] & 0xffff) + 1]++;
+  for (int g = 1; g <= nmmax + 1; ++g) cnt[g] += cnt[g - 1];
   w.solve.resize(2 * (size_t)n);
-  for (int q = 0; q < n; ++q) {
-    w.solve[2 * q] = 2 * byn[q];
-    w.solve[2 * q + 1] = 2 * byn[q] + 1;
+  for (int i = 0; i < n; ++i) {
+    const int q = cnt[nmmax - (info[i] & 0xffff)]++;
+    w.solve[2 * q] = 2 * i;
+    w.solve[2 * q + 1] = 2 * i + 1;
   }
 }
 
@@ -475,11 +545,11 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   if (rc != CTM_OK) return rc;
   build_tasks(w);
   hipStream_t s = ctx->stream;
-  CTM_HIP(upload(ctx->d_sets, w.sets, s));
-  CTM_HIP(upload(ctx->d_items, w.items, s));
-  CTM_HIP(upload(ctx->d_tiles, w.order, s));
-  CTM_HIP(upload(ctx->d_gioff_tmp, w.tile_prefix, s));
-  CTM_HIP(upload(ctx->d_solve, w.solve, s));
+  CTM_HIP(upload(ctx, ctx->d_sets, w.sets));
+  CTM_HIP(upload(ctx, ctx->d_items, w.items));
+  CTM_HIP(upload(ctx, ctx->d_tiles, w.order));
+  CTM_HIP(upload(ctx, ctx->d_gioff_tmp, w.tile_prefix));
+  CTM_HIP(upload(ctx, ctx->d_solve, w.solve));
   CTM_HIP(ctx->d_tables.ensure(std::max<int64_t>(w.table_size, 1) *
                                sizeof(double)));
   CTM_HIP(ctx->d_q.ensure(std::max<int64_t>(w.q_size, 1) * sizeof(cdbl)));
@@ -581,9 +651,13 @@ int ctm_destroy(ctm_context *ctx) {
                     &ctx->d_pt, &ctx->d_x, &ctx->d_spec, &ctx->d_cgkeys,
                     &ctx->d_cg, &ctx->d_cgidx, &ctx->d_avgtasks, &ctx->d_avg,
                     &ctx->d_aoff, &ctx->d_pn, &ctx->d_dist,
-                    &ctx->d_obsitems, &ctx->d_query, &ctx->d_qout};
+                    &ctx->d_obsitems, &ctx->d_query, &ctx->d_qout,
+                    &ctx->d_gsolve, &ctx->d_ttables, &ctx->d_dq,
+                    &ctx->d_ainv, &ctx->d_dt, &ctx->d_tparts, &ctx->d_dsum,
+                    &ctx->d_ttiles, &ctx->d_tprefix, &ctx->d_wigfresh};
   for (DevBuf *b : bufs) b->release();
   ctx->prof.release();
+  ctx->staging.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CTM_OK;
@@ -597,7 +671,7 @@ void *ctm_stream(ctm_context *ctx) { return ctx ? (void *)ctx->stream : nullptr;
 
 int ctm_synchronize(ctm_context *ctx) {
   if (!ctx) return CTM_ERR_ARGUMENT;
-  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  CTM_HIP(drain(ctx));
   return CTM_OK;
 }
 
@@ -649,8 +723,8 @@ static int run_all_m(ctm_context *ctx, const GrainParams &gp, int n,
   if (rc != CTM_OK) return rc;
   ctx->grain_items.resize(w.items.size());
   for (size_t i = 0; i < w.items.size(); ++i) ctx->grain_items[i] = (int32_t)i;
-  CTM_HIP(upload(ctx->d_gioff, ctx->grain_items_off, ctx->stream));
-  CTM_HIP(upload(ctx->d_gitems, ctx->grain_items, ctx->stream));
+  CTM_HIP(upload(ctx, ctx->d_gioff, ctx->grain_items_off));
+  CTM_HIP(upload(ctx, ctx->d_gitems, ctx->grain_items));
   CTM_LAUNCH(CTM_KERNEL_SUMS, 0., 0.,
              launch_full_sums(n, ctx->d_gioff.as<int32_t>(),
                               ctx->d_gitems.as<int32_t>(),
@@ -686,7 +760,7 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
   CTM_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   // host work lists of the previous call may still feed pending copies
-  CTM_HIP(hipStreamSynchronize(s));
+  CTM_HIP(drain(ctx));
   ctx->err.clear();
   ctx->shared.clear();
   ctx->n_grains = n;
@@ -705,7 +779,7 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
                          hipMemcpyDeviceToHost, s));
   CTM_HIP(hipMemcpyAsync(h_lam.data(), wavelength, n * sizeof(double),
                          hipMemcpyDeviceToHost, s));
-  CTM_HIP(hipStreamSynchronize(s));
+  CTM_HIP(drain(ctx));
 
   std::vector<ConvState> st(n);
   for (int g = 0; g < n; ++g) {
@@ -719,7 +793,7 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     c.status = 1;
     st[g] = c;
   }
-  CTM_HIP(upload(ctx->d_state, st, s));
+  CTM_HIP(upload(ctx, ctx->d_state, st));
 
   std::vector<int> lo(n, min_order), hi(n);
   for (int g = 0; g < n; ++g) {
@@ -753,9 +827,9 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     }
     rc = run_work(ctx, gp, false);
     if (rc != CTM_OK) return rc;
-    CTM_HIP(upload(ctx->d_active, active, s));
-    CTM_HIP(upload(ctx->d_winoff, win_off, s));
-    CTM_HIP(upload(ctx->d_winitems, win_items, s));
+    CTM_HIP(upload(ctx, ctx->d_active, active));
+    CTM_HIP(upload(ctx, ctx->d_winoff, win_off));
+    CTM_HIP(upload(ctx, ctx->d_winitems, win_items));
     CTM_LAUNCH(CTM_KERNEL_CONVERGE, 0., 0.,
                launch_converge(ctx->d_active.as<int32_t>(), (int)active.size(),
                                ctx->d_winoff.as<int32_t>(),
@@ -765,7 +839,7 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
                                last_order, ctx->d_state.as<ConvState>(), s));
     CTM_HIP(hipMemcpyAsync(st.data(), ctx->d_state.p, n * sizeof(ConvState),
                            hipMemcpyDeviceToHost, s));
-    CTM_HIP(hipStreamSynchronize(s));
+    CTM_HIP(drain(ctx));
     std::vector<int32_t> next;
     for (int g : active) {
       if (st[g].done) continue;
@@ -815,7 +889,7 @@ int ctm_tmatrix_fixed(ctm_context *ctx, int n, const double *radius,
     }
   }
   CTM_HIP(hipSetDevice(ctx->device));
-  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  CTM_HIP(drain(ctx));
   ctx->err.clear();
   ctx->shared.clear();
   ctx->n_grains = n;
@@ -862,7 +936,7 @@ static int get_blocks(ctm_context *ctx, int g, const cdbl *base,
     CTM_HIP(hipMemcpyAsync(blk.data(), base + it.t_off,
                            2 * nm2 * sizeof(cdbl), hipMemcpyDeviceToHost,
                            ctx->stream));
-    CTM_HIP(hipStreamSynchronize(ctx->stream));
+    CTM_HIP(drain(ctx));
     const int n2m = 2 * nm;
     cdbl *out = T + out_off;
     for (size_t e = 0; e < (size_t)n2m * n2m; ++e) out[e] = cmk(0., 0.);
@@ -917,8 +991,8 @@ int ctm_tmatrix_tangents(ctm_context *ctx, double *dqsca_host,
     const int ntc = it.m == 0 ? (nm + 31) / 32 : ntr;
     prefix[i + 1] = prefix[i] + ntr * ntc;
   }
-  CTM_HIP(upload(ctx->d_ttiles, order, s));
-  CTM_HIP(upload(ctx->d_tprefix, prefix, s));
+  CTM_HIP(upload(ctx, ctx->d_ttiles, order));
+  CTM_HIP(upload(ctx, ctx->d_tprefix, prefix));
   CTM_HIP(ctx->d_dq.ensure(std::max<int64_t>(w.q_size, 1) * kTangentDirs *
                            sizeof(cdbl)));
   CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, 0., 0.,
@@ -958,7 +1032,7 @@ int ctm_tmatrix_tangents(ctm_context *ctx, double *dqsca_host,
   CTM_HIP(hipMemcpyAsync(ctx->dqext.data(), dsum + (size_t)n * kTangentDirs,
                          ctx->dqext.size() * sizeof(double),
                          hipMemcpyDeviceToHost, s));
-  CTM_HIP(hipStreamSynchronize(s));
+  CTM_HIP(drain(ctx));
   if (dqsca_host)
     std::copy(ctx->dqsca.begin(), ctx->dqsca.end(), dqsca_host);
   if (dqext_host)
@@ -1146,8 +1220,7 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   const size_t o_sc = put(sc.data(), sc.size() * 8);
   const size_t o_ss = put(ss.data(), ss.size() * 8);
   CTM_HIP(ctx->d_spec.ensure(blob.size()));
-  CTM_HIP(hipMemcpyAsync(ctx->d_spec.p, blob.data(), blob.size(),
-                         hipMemcpyHostToDevice, s));
+  CTM_HIP(stage_copy(ctx, ctx->d_spec.p, blob.data(), blob.size()));
   char *base = ctx->d_spec.as<char>();
   const double *d_ang = reinterpret_cast<const double *>(base + o_ang);
   sp.in_angle = reinterpret_cast<const int32_t *>(base + o_in);
@@ -1186,7 +1259,7 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
     for (int q = 1; q <= nmax_all + 1; ++q) cnt[q] += cnt[q - 1];
     for (size_t i = 0; i < nitems; ++i)
       lst[cnt[nmax_all - nm_of(ctx->final_items[i])]++] = (int32_t)i;
-    CTM_HIP(upload(ctx->d_obsitems, lst, s));
+    CTM_HIP(upload(ctx, ctx->d_obsitems, lst));
     const int caps[] = {nmax_all, 48, 24, 12};
     size_t pos = 0;
     for (int b = 0; b < 4 && pos < nitems; ++b) {
@@ -1235,7 +1308,7 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
                             ctx->d_x.as<cdbl>(), Z, s));
   }
   // the spec blob and the host vectors must outlive the queued copies
-  CTM_HIP(hipStreamSynchronize(s));
+  CTM_HIP(drain(ctx));
   return CTM_OK;
 }
 
@@ -1310,12 +1383,12 @@ int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
   std::vector<double> pn(pN_host, pN_host + (size_t)n_dist * ncoef);
   std::vector<int32_t> dist;
   if (dist_of_grain) dist.assign(dist_of_grain, dist_of_grain + n);
-  CTM_HIP(upload(ctx->d_cgkeys, keys, s));
-  CTM_HIP(upload(ctx->d_cgidx, cgidx, s));
-  CTM_HIP(upload(ctx->d_avgtasks, tasks, s));
-  CTM_HIP(upload(ctx->d_aoff, ctx->aoff, s));
-  CTM_HIP(upload(ctx->d_pn, pn, s));
-  if (dist_of_grain) CTM_HIP(upload(ctx->d_dist, dist, s));
+  CTM_HIP(upload(ctx, ctx->d_cgkeys, keys));
+  CTM_HIP(upload(ctx, ctx->d_cgidx, cgidx));
+  CTM_HIP(upload(ctx, ctx->d_avgtasks, tasks));
+  CTM_HIP(upload(ctx, ctx->d_aoff, ctx->aoff));
+  CTM_HIP(upload(ctx, ctx->d_pn, pn));
+  if (dist_of_grain) CTM_HIP(upload(ctx, ctx->d_dist, dist));
   CTM_HIP(ctx->d_cg.ensure(std::max<int64_t>(cgsize, 1) * sizeof(double)));
   CTM_HIP(ctx->d_avg.ensure(std::max<int64_t>(asize, 1) * sizeof(cdbl)));
   CTM_LAUNCH(CTM_KERNEL_AVERAGE, 0., 0.,
@@ -1330,7 +1403,7 @@ int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
                             ctx->d_cgidx.as<int32_t>(), ctx->d_cg.as<double>(),
                             nmax, ctx->d_avg.as<cdbl>(),
                             ctx->d_aoff.as<int64_t>(), s));
-  CTM_HIP(hipStreamSynchronize(s));
+  CTM_HIP(drain(ctx));
   ctx->averaged = true;
   return CTM_OK;
 }
@@ -1367,7 +1440,7 @@ int ctm_average_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
   const int64_t o = ctx->aoff[ctx->grain_items_off[g]];
   CTM_HIP(hipMemcpyAsync(T_host, ctx->d_avg.as<cdbl>() + o, need * sizeof(cdbl),
                          hipMemcpyDeviceToHost, ctx->stream));
-  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  CTM_HIP(drain(ctx));
   return CTM_OK;
 }
 
@@ -1398,7 +1471,7 @@ int ctm_query(ctm_context *ctx, int n, const ctm_query_spec *q,
   }
   CTM_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  CTM_HIP(upload(ctx->d_query, qs, s));
+  CTM_HIP(upload(ctx, ctx->d_query, qs));
   CTM_HIP(ctx->d_qout.ensure((size_t)n * kQueryOut * sizeof(double)));
   CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
              launch_query(ctx->d_query.as<Query>(), n, ctx->d_items.as<Item>(),
@@ -1410,7 +1483,7 @@ int ctm_query(ctm_context *ctx, int n, const ctm_query_spec *q,
   CTM_HIP(hipMemcpyAsync(out_host, ctx->d_qout.p,
                          (size_t)n * kQueryOut * sizeof(double),
                          hipMemcpyDeviceToHost, s));
-  CTM_HIP(hipStreamSynchronize(s));
+  CTM_HIP(drain(ctx));
   return CTM_OK;
 }
 
@@ -1433,7 +1506,7 @@ int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
     }
   CTM_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  CTM_HIP(hipStreamSynchronize(s));
+  CTM_HIP(drain(ctx));
   // the work lists below replace the batch's items
   ctx->final_items.clear();
   ctx->averaged = false;
@@ -1455,7 +1528,7 @@ int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
   std::vector<double> qp((size_t)n * 2 * kQParts);
   CTM_HIP(hipMemcpyAsync(qp.data(), ctx->d_qparts.p, qp.size() * sizeof(double),
                          hipMemcpyDeviceToHost, s));
-  CTM_HIP(hipStreamSynchronize(s));
+  CTM_HIP(drain(ctx));
   for (int g = 0; g < n; ++g) {
     const double *p0 = qp.data() + (size_t)(2 * g) * kQParts;
     const double *p1 = p0 + kQParts;
@@ -1473,7 +1546,7 @@ int ctm_profile_enable(ctm_context *ctx, int on) {
 
 int ctm_profile_reset(ctm_context *ctx) {
   if (!ctx) return CTM_ERR_ARGUMENT;
-  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  CTM_HIP(drain(ctx));
   ctx->prof.reset();
   return CTM_OK;
 }
@@ -1481,7 +1554,7 @@ int ctm_profile_reset(ctm_context *ctx) {
 int ctm_profile_read(ctm_context *ctx, int kernel, ctm_kernel_stats *out) {
   if (!ctx || !out || kernel < 0 || kernel >= CTM_KERNEL_COUNT)
     return CTM_ERR_ARGUMENT;
-  CTM_HIP(hipStreamSynchronize(ctx->stream));
+  CTM_HIP(drain(ctx));
   ctx->prof.collect();
   out->launches = ctx->prof.launches[kernel];
   out->milliseconds = ctx->prof.ms[kernel];
