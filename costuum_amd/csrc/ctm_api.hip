@@ -149,12 +149,11 @@ struct WorkList {
   std::vector<TableSet> sets;
   std::vector<Item> items;
   std::vector<int32_t> order;        // items sorted by assembly cost
-  // Assembly order: items with blocks above kSmallBlock (2 x 2 micro-tiles)
-  // first, then the small ones, each part sorted by G (largest first);
-  // tile_prefix holds one prefix sum per part (n_large + 1 entries, then
-  // n - n_large + 1).
+  // Assembly order: three parts by micro-tile (3, 2, then 1: see
+  // assemble_micro), each sorted by G (largest first); tile_prefix holds
+  // one prefix sum per part (n_part[k] + 1 entries each, in that order).
   std::vector<int32_t> tile_prefix;
-  int n_large = 0;
+  int n_part[3] = {0, 0, 0};
   std::vector<int32_t> solve;        // item * 2 + class, by block order
   std::vector<int32_t> scratch_a, scratch_b, scratch_c;  // build_tasks
   int64_t table_size = 0, q_size = 0, t_size = 0;
@@ -416,49 +415,47 @@ int flush_shared(ctm_context *ctx) {
 // counting sort.
 void build_tasks(WorkList &w) {
   const int n = (int)w.items.size();
-  // per item: block order and an m = 0 flag, then the assembly key
+  // per item: block order, an m = 0 flag and the part (micro-tile 3, 2, 1
+  // -> 0, 1, 2); then the assembly key
   std::vector<int32_t> &info = w.scratch_a, &key = w.scratch_b;
   info.resize(n);
   key.resize(n);
-  int gmax = 0, nmmax = 0, n_large = 0;
+  int gmax = 0, nmmax = 0;
+  w.n_part[0] = w.n_part[1] = w.n_part[2] = 0;
   for (int i = 0; i < n; ++i) {
     const Item &it = w.items[i];
     const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-    info[i] = nm | (it.m == 0 ? (1 << 30) : 0);
+    const int part = 3 - assemble_micro(nm, it.m == 0);
+    info[i] = nm | (part << 16) | (it.m == 0 ? (1 << 30) : 0);
+    w.n_part[part]++;
     gmax = std::max(gmax, it.G);
     nmmax = std::max(nmmax, nm);
-    n_large += nm > kSmallBlock;
   }
-  // assembly order: blocks above kSmallBlock first, each part by G
-  // descending (counting sort)
-  const int nkey = 2 * (gmax + 1);
+  // counting sort by (part, G descending)
+  const int nkey = 3 * (gmax + 1);
   std::vector<int32_t> &cnt = w.scratch_c;
   cnt.assign(std::max(nkey, nmmax + 1) + 1, 0);
   for (int i = 0; i < n; ++i) {
-    const int nm = info[i] & 0xffff;
-    key[i] = (nm > kSmallBlock ? 0 : gmax + 1) + (gmax - w.items[i].G);
+    key[i] = ((info[i] >> 16) & 3) * (gmax + 1) + (gmax - w.items[i].G);
     cnt[key[i] + 1]++;
   }
   for (int g = 1; g <= nkey; ++g) cnt[g] += cnt[g - 1];
   w.order.resize(n);
   for (int i = 0; i < n; ++i) w.order[cnt[key[i]]++] = i;
-  w.n_large = n_large;
-  // tiles: 32-row (2 x 2 kernel) or 16-row tiles, twice as wide for m = 0
-  w.tile_prefix.resize((size_t)n + 2);
+  // tiles of 16 MT rows, twice as wide for m = 0
+  w.tile_prefix.resize((size_t)n + 3);
   int32_t *pre = w.tile_prefix.data();
-  pre[0] = 0;
-  for (int q = 0; q < n_large; ++q) {
-    const int f = info[w.order[q]], nm = f & 0xffff;
-    const int ntr = (nm + 31) >> 5;
-    pre[q + 1] = pre[q] + ntr * ((f >> 30) ? (nm + 63) >> 6 : ntr);
-  }
-  pre += n_large + 1;
-  pre[0] = 0;
-  for (int q = n_large; q < n; ++q) {
-    const int f = info[w.order[q]], nm = f & 0xffff;
-    const int ntr = (nm + 15) >> 4;
-    pre[q - n_large + 1] =
-        pre[q - n_large] + ntr * ((f >> 30) ? (nm + 31) >> 5 : ntr);
+  int q = 0;
+  for (int part = 0; part < 3; ++part) {
+    const int tr = 16 * (3 - part);
+    pre[0] = 0;
+    for (int k = 0; k < w.n_part[part]; ++k, ++q) {
+      const int f = info[w.order[q]], nm = f & 0xffff;
+      const int ntr = (nm + tr - 1) / tr;
+      pre[k + 1] = pre[k] + ntr * ((f >> 30) ? (nm + 2 * tr - 1) / (2 * tr)
+                                             : ntr);
+    }
+    pre += w.n_part[part] + 1;
   }
   // solve tasks (item, class) by block order descending
   cnt.assign(nmmax + 2, 0);
@@ -562,12 +559,13 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   // order; (64/3) nm^3 and (16/3) n^3 for the parity-split solves) and the
   // compulsory HBM traffic (tables and Wigner values read once, Q/RgQ
   // written once by the assembly, read once by the solve, T written once).
-  // (index 0: blocks above kSmallBlock, 1: the others)
-  double f_asm[2] = {0., 0.}, b_asm[2] = {0., 0.}, b_tab = 0.;
+  // (index: part, micro-tile 3, 2, 1)
+  double f_asm[3] = {0., 0., 0.}, b_asm[3] = {0., 0., 0.}, b_tab = 0.;
   for (const Item &it : w.items) {
     const int nmin = it.m > 0 ? it.m : 1;
-    const double nm = it.N + 1 - nmin;
-    const int k = nm > kSmallBlock ? 0 : 1;
+    const int nmi = it.N + 1 - nmin;
+    const double nm = nmi;
+    const int k = 3 - assemble_micro(nmi, it.m == 0);
     f_asm[k] += (it.m > 0 ? 96. : 32.) * it.G * nm * nm;
     b_asm[k] += 8. * (2. * nm * it.G) + 16. * 4. * nm * nm;
   }
@@ -579,30 +577,28 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
                            ctx->d_gl.as<double>(), ctx->d_tables.as<double>(),
                            s));
   {
-    // two launches: 2 x 2 micro-tiles for the large blocks, then 1 x 1,
-    // each credited with the algorithmic work of its blocks
-    const int nl = w.n_large, ns = (int)w.order.size() - nl;
+    // one launch per part (3 x 3, 2 x 2, 1 x 1 micro-tiles), each credited
+    // with the algorithmic work of its blocks
     const int32_t *ord = ctx->d_tiles.as<int32_t>();
     const int32_t *pre = ctx->d_gioff_tmp.as<int32_t>();
-    if (nl > 0)
-      CTM_LAUNCH(CTM_KERNEL_ASSEMBLE, f_asm[0], b_asm[0],
-                 launch_assemble(2, ord, pre, nl, w.tile_prefix[nl],
-                                 ctx->d_items.as<Item>(),
-                                 ctx->d_sets.as<TableSet>(),
-                                 ctx->d_wigkeys.as<WigKey>(), gp,
-                                 ctx->d_tables.as<double>(),
-                                 ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(),
-                                 s));
-    if (ns > 0)
-      CTM_LAUNCH(CTM_KERNEL_ASSEMBLE_SMALL, f_asm[1], b_asm[1],
-                 launch_assemble(1, ord + nl, pre + nl + 1, ns,
-                                 w.tile_prefix[nl + 1 + ns],
-                                 ctx->d_items.as<Item>(),
-                                 ctx->d_sets.as<TableSet>(),
-                                 ctx->d_wigkeys.as<WigKey>(), gp,
-                                 ctx->d_tables.as<double>(),
-                                 ctx->d_wig.as<double>(), ctx->d_q.as<cdbl>(),
-                                 s));
+    const int kid[3] = {CTM_KERNEL_ASSEMBLE_WIDE, CTM_KERNEL_ASSEMBLE,
+                        CTM_KERNEL_ASSEMBLE_SMALL};
+    size_t po = 0;
+    for (int part = 0; part < 3; ++part) {
+      const int np = w.n_part[part];
+      if (np > 0)
+        CTM_LAUNCH(kid[part], f_asm[part], b_asm[part],
+                   launch_assemble(3 - part, ord, pre + po, np,
+                                   w.tile_prefix[po + np],
+                                   ctx->d_items.as<Item>(),
+                                   ctx->d_sets.as<TableSet>(),
+                                   ctx->d_wigkeys.as<WigKey>(), gp,
+                                   ctx->d_tables.as<double>(),
+                                   ctx->d_wig.as<double>(),
+                                   ctx->d_q.as<cdbl>(), s));
+      ord += np;
+      po += np + 1;
+    }
   }
   return run_solves(ctx, want_t, nullptr);
 }

@@ -402,7 +402,8 @@ __global__ __launch_bounds__(256) void k_assemble(
     const GrainParams gp, const double *tables, const double *wig,
     cdbl *qbuf) {
   constexpr int TR = 16 * MT;            // rows of pairs per tile
-  constexpr int CH = MT == 1 ? 8 : 4;    // quadrature nodes per LDS stage
+  // quadrature nodes per LDS stage (two blocks per CU for every MT)
+  constexpr int CH = MT == 1 ? 8 : MT == 2 ? 4 : 2;
   __shared__ double Lsh[CH][TR][kLeftStride];
   __shared__ double Rsh[CH][2 * TR][kRightStride];
 
@@ -1499,7 +1500,11 @@ hipError_t launch_assemble(int micro, const int32_t *order,
                            const GrainParams &gp, const double *tables,
                            const double *wig, cdbl *qbuf, hipStream_t s) {
   if (n_tiles <= 0) return hipSuccess;
-  if (micro == 2)
+  if (micro == 3)
+    hipLaunchKernelGGL(k_assemble<3>, dim3(n_tiles), dim3(256), 0, s, order,
+                       tile_prefix, n_order, items, sets, wkeys, gp, tables,
+                       wig, qbuf);
+  else if (micro == 2)
     hipLaunchKernelGGL(k_assemble<2>, dim3(n_tiles), dim3(256), 0, s, order,
                        tile_prefix, n_order, items, sets, wkeys, gp, tables,
                        wig, qbuf);

@@ -276,7 +276,8 @@ int ctm_orientation_coefficients(ctm_real_fn p, void *arg, uint32_t order,
 #define CTM_KERNEL_OBS_SUM 8
 #define CTM_KERNEL_AVERAGE 9
 #define CTM_KERNEL_ASSEMBLE_SMALL 10 /* blocks of order <= 16 */
-#define CTM_KERNEL_COUNT 11
+#define CTM_KERNEL_ASSEMBLE_WIDE 11  /* blocks on 48-row tiles */
+#define CTM_KERNEL_COUNT 12
 
 typedef struct ctm_kernel_stats {
   uint64_t launches;
