@@ -25,13 +25,14 @@ hipError_t launch_tables(const TableSet *sets, int n, const GrainParams &gp,
 // tiles of 3 x 3 pairs per thread.
 constexpr int kSmallBlock = 16;
 // Micro-tile of a block of order nm: 1 up to kSmallBlock, else 3 (48-row
-// tiles) where their padded pair count, at 0.9 of the cost per pair of
-// 32-row tiles, is smaller, else 2.  (m = 0 tiles are twice as wide.)
+// tiles) where their padded pair count, at half the cost per pair of
+// 32-row tiles (measured: the 3 x 3 micro-tile reads fewer LDS operands per
+// pair), is smaller, else 2.  (m = 0 tiles are twice as wide.)
 inline int assemble_micro(int nm, bool m0) {
   if (nm <= kSmallBlock) return 1;
   const long r32 = (nm + 31) / 32, c32 = m0 ? (nm + 63) / 64 : r32;
   const long r48 = (nm + 47) / 48, c48 = m0 ? (nm + 95) / 96 : r48;
-  return 0.9 * (double)(r48 * c48 * 2304) < (double)(r32 * c32 * 1024) ? 3
+  return 0.5 * (double)(r48 * c48 * 2304) < (double)(r32 * c32 * 1024) ? 3
                                                                        : 2;
 }
 hipError_t launch_assemble(int micro, const int32_t *order,
