@@ -434,12 +434,33 @@ __global__ __launch_bounds__(256) void k_assemble(
   const int wv = t >> 6, lane = t & 63;
   const int prow = wv & 1;
   const int pcol = m0 ? (wv & 1) : (wv >> 1);
-  const int cbase = m0 ? TR * (wv >> 1) : 0;
+  int cbase = m0 ? TR * (wv >> 1) : 0;
+  int lr = lane >> 3, lc = lane & 7;
+  // MT = 1 (orders <= 16): threads beyond the block sum a share of the
+  // nodes instead.  For m = 0 the second half of the tile's columns is
+  // always beyond the block, so waves 2 and 3 take the odd nodes of each
+  // stage; for orders <= 8 a lane's row and column bits 2 pick one of four
+  // node groups as well.  Node group sg of ns sums the stage nodes
+  // kk = sg, sg + ns, ...; the partial sums are added at the end.
+  int sg = 0, ns = 1;
+  if (MT == 1) {
+    if (m0) {
+      cbase = 0;
+      sg = wv >> 1;
+      ns = 2;
+    }
+    if (nm <= 8) {
+      sg = sg * 4 + ((lr >> 2) << 1) + (lc >> 2);
+      ns *= 4;
+      lr &= 3;
+      lc &= 3;
+    }
+  }
   int rr[MT], cc[MT];
 #pragma unroll
   for (int a = 0; a < MT; ++a) {
-    rr[a] = 2 * ((lane >> 3) + 8 * a) + prow;
-    cc[a] = cbase + 2 * ((lane & 7) + 8 * a) + pcol;
+    rr[a] = 2 * (lr + 8 * a) + prow;
+    cc[a] = cbase + 2 * (lc + 8 * a) + pcol;
   }
   const bool odd = !m0 && ((prow ^ pcol) != 0);
 
@@ -462,7 +483,7 @@ __global__ __launch_bounds__(256) void k_assemble(
     __syncthreads();
     const int kmax = (G - ig0) < CH ? (G - ig0) : CH;
     if (!odd) {
-      for (int kk = 0; kk < kmax; ++kk) {
+      for (int kk = sg; kk < kmax; kk += ns) {
         cdbl R1[MT], R2[MT], R3[MT], R4[MT], R5[MT];
 #pragma unroll
         for (int b = 0; b < MT; ++b) {
@@ -494,7 +515,7 @@ __global__ __launch_bounds__(256) void k_assemble(
         }
       }
     } else {
-      for (int kk = 0; kk < kmax; ++kk) {
+      for (int kk = sg; kk < kmax; kk += ns) {
         cdbl R1[MT], R2[MT], R3[MT], R4[MT], R5[MT];
 #pragma unroll
         for (int b = 0; b < MT; ++b) {
@@ -528,6 +549,37 @@ __global__ __launch_bounds__(256) void k_assemble(
     }
   }
 
+  if (MT == 1 && ns > 1) {
+    // add the node groups: lane groups by shuffles, then waves 2 and 3
+    // into waves 0 and 1 through LDS (the stage buffers are free now)
+    double v[8] = {Ja[0][0].x,  Ja[0][0].y,  Jb[0][0].x,  Jb[0][0].y,
+                   RJa[0][0].x, RJa[0][0].y, RJb[0][0].x, RJb[0][0].y};
+    if (nm <= 8) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        v[q] += __shfl_xor(v[q], 4);
+        v[q] += __shfl_xor(v[q], 32);
+      }
+    }
+    if (m0) {
+      double *red = &Lsh[0][0][0];
+      __syncthreads();
+      if (wv >= 2) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) red[q * 128 + (t - 128)] = v[q];
+      }
+      __syncthreads();
+      if (wv < 2) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] += red[q * 128 + t];
+      }
+    }
+    Ja[0][0] = cmk(v[0], v[1]);
+    Jb[0][0] = cmk(v[2], v[3]);
+    RJa[0][0] = cmk(v[4], v[5]);
+    RJb[0][0] = cmk(v[6], v[7]);
+    if (sg != 0) return;
+  }
   const double lam = gp.wavelength[it.grain];
   const double k = 2. * M_PI / lam;
   const double k2 = k * k;
