@@ -157,7 +157,9 @@ struct WorkList {
   std::vector<int32_t> solve;        // item * 2 + class, by block order
   std::vector<int32_t> scratch_a, scratch_b, scratch_c;  // build_tasks
   int64_t table_size = 0, q_size = 0, t_size = 0;
+  int32_t n_nodes = 0;  // quadrature nodes over all sets
   void clear() {
+    n_nodes = 0;
     sets.clear();
     items.clear();
     order.clear();
@@ -172,6 +174,8 @@ struct WorkList {
     s.G = G;
     s.gl_off = gl_off;
     s.off = table_size;
+    s.node0 = n_nodes;
+    n_nodes += G;
     table_size += (int64_t)kNodeScalars * G + (int64_t)kBesselArrays * N * G;
     sets.push_back(s);
     return (int)sets.size() - 1;
@@ -573,7 +577,8 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     b_tab += 8. * ((double)kNodeScalars * ts.G +
                    (double)kBesselArrays * ts.N * ts.G);
   CTM_LAUNCH(CTM_KERNEL_TABLES, 0., b_tab,
-             launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(), gp,
+             launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(),
+                           w.n_nodes, gp,
                            ctx->d_gl.as<double>(), ctx->d_tables.as<double>(),
                            s));
   {

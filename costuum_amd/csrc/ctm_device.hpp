@@ -75,7 +75,8 @@ struct WigKey {
 };
 
 struct TableSet {
-  int32_t grain, N, G, pad;
+  int32_t grain, N, G;
+  int32_t node0;  // nodes of the sets before this one in its list
   int64_t gl_off;
   int64_t off;
 };

@@ -132,12 +132,23 @@ __global__ void k_wigner(const WigKey *keys, const double *gl, double *wig) {
 
 // ---------------------------------------------------------------------------
 // K1: per-(grain, order) node scalars and Bessel tables.  One thread per
-// node; the ratio recurrence of j_n runs down from n = 800 and parks the
-// ratios for n < N in the output rows before the forward pass overwrites
-// them with j_n.
-__global__ void k_tables(const TableSet *sets, const GrainParams gp,
-                         const double *gl, double *tables) {
-  const TableSet s = sets[blockIdx.x];
+// node over all sets of the list (a thread finds its set by binary search
+// in the sets' node offsets); the ratio recurrence of j_n runs down from
+// the start order and parks the ratios for n < N in the output rows before
+// the forward pass overwrites them with j_n.
+__global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
+                                                int n_sets, int n_nodes,
+                                                const GrainParams gp,
+                                                const double *gl,
+                                                double *tables) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n_nodes) return;
+  int lo = 0, hi = n_sets;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sets[mid].node0 <= tid) lo = mid; else hi = mid;
+  }
+  const TableSet s = sets[lo];
   const int N = s.N, G = s.G;
   const double rv = gp.radius[s.grain];
   const double d = gp.axis_ratio[s.grain];
@@ -158,7 +169,8 @@ __global__ void k_tables(const TableSet *sets, const GrainParams gp,
   const double a2 = a * a;
   const double d2 = d * d;
   const double d2m1 = d2 - 1.;
-  for (int ig = threadIdx.x; ig < G; ig += blockDim.x) {
+  {
+    const int ig = tid - s.node0;
     const double c = cth[ig];
     const double w = wgt[ig];
     const double s2 = (1. - c) * (1. + c);
@@ -1538,10 +1550,12 @@ hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
   return hipGetLastError();
 }
 
-hipError_t launch_tables(const TableSet *sets, int n, const GrainParams &gp,
-                         const double *gl, double *tables, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tables, dim3(n), dim3(64), 0, s, sets, gp, gl, tables);
+hipError_t launch_tables(const TableSet *sets, int n, int n_nodes,
+                         const GrainParams &gp, const double *gl,
+                         double *tables, hipStream_t s) {
+  if (n <= 0 || n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tables, dim3((n_nodes + 127) / 128), dim3(128), 0, s,
+                     sets, n, n_nodes, gp, gl, tables);
   return hipGetLastError();
 }
 
