@@ -282,24 +282,30 @@ __global__ __launch_bounds__(256) void k_obs_q(
   const int t = threadIdx.x;
   __shared__ double red[4][2];
   __shared__ double fwd[2][2];
-  if (t == 0) {
-    cdbl S0 = cmk(0., 0.), S3 = cmk(0., 0.);
+  if ((t >> 6) == 3) {
+    // forward pair: the lanes of wave 3 take every 64th m, then a shuffle
+    // reduction
+    const int lane = t & 63;
+    double v[4] = {0., 0., 0., 0.};
     const int pr = spec.fwd_pair0 + j;
-    for (int q = i0; q < i1; ++q) {
+    for (int q = i0 + lane; q < i1; q += 64) {
       const double f = (q == i0) ? 1. : 2.;
       const cdbl *Xp = X + ((int64_t)q * spec.n_pairs + pr) * 4;
-      S0.x += f * Xp[0].x;
-      S0.y += f * Xp[0].y;
-      S3.x += f * Xp[3].x;
-      S3.y += f * Xp[3].y;
+      v[0] += f * Xp[0].x;
+      v[1] += f * Xp[0].y;
+      v[2] += f * Xp[3].x;
+      v[3] += f * Xp[3].y;
     }
-    const double kinv = 1. / k;
-    S0 = cscale(S0, kinv);
-    S3 = cscale(S3, kinv);
-    fwd[0][0] = S0.x;
-    fwd[0][1] = S0.y;
-    fwd[1][0] = S3.x;
-    fwd[1][1] = S3.y;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+    if (lane == 0) {
+      const double kinv = 1. / k;
+      fwd[0][0] = v[0] * kinv;
+      fwd[0][1] = v[1] * kinv;
+      fwd[1][0] = v[2] * kinv;
+      fwd[1][1] = v[3] * kinv;
+    }
   }
   double zsum = 0., zpol = 0.;
   if (spec.ngauss > 0) {
@@ -309,19 +315,37 @@ __global__ __launch_bounds__(256) void k_obs_q(
       const int pr = spec.abs_pair0 + j * spec.ngauss + ti;
       const double cphi = spec.phi_cos[pj], sphi = -spec.phi_sin[pj];
       cdbl S[4] = {cmk(0., 0.), cmk(0., 0.), cmk(0., 0.), cmk(0., 0.)};
-      // e^{i m dphi} by recurrence, as the reference
+      // e^{i m dphi} by recurrence, as the reference; X of four m values
+      // is loaded ahead of their sums (four loads in flight)
       cdbl ex = cmk(1., 0.);
       const cdbl st = cmk(cphi, sphi);
-      for (int q = i0; q < i1; ++q) {
-        const int m = q - i0;
-        const cdbl *Xp = X + ((int64_t)q * spec.n_pairs + pr) * 4;
+      const cdbl *Xb = X + ((int64_t)i0 * spec.n_pairs + pr) * 4;
+      const int64_t xs = (int64_t)spec.n_pairs * 4;
+      const int nq = i1 - i0;
+      auto term = [&](int m, const cdbl *xv) {
         const double fc = (m == 0) ? 1. : 2. * ex.x;
         const double fs = (m == 0) ? 0. : 2. * ex.y;
         ex = cmul(ex, st);
-        rfma(S[0], fc, Xp[0]);
-        rfma(S[3], fc, Xp[3]);
-        rfma(S[1], fs, Xp[1]);
-        rfma(S[2], fs, Xp[2]);
+        rfma(S[0], fc, xv[0]);
+        rfma(S[3], fc, xv[3]);
+        rfma(S[1], fs, xv[1]);
+        rfma(S[2], fs, xv[2]);
+      };
+      int m = 0;
+      for (; m + 4 <= nq; m += 4) {
+        cdbl xv[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) xv[u][c] = Xb[(m + u) * xs + c];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) term(m + u, xv[u]);
+      }
+      for (; m < nq; ++m) {
+        cdbl xv[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) xv[c] = Xb[m * xs + c];
+        term(m, xv);
       }
       const double kinv = 1. / k;
       double n2[4];
