@@ -206,11 +206,22 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
       double jprev = bj[ig] * j0;
       bj[ig] = jprev;
       bdj[ig] = j0 - jprev * zinv;
-      for (int i = 1; i < N; ++i) {
-        const double ji = bj[(int64_t)i * G + ig] * jprev;
-        bj[(int64_t)i * G + ig] = ji;
-        bdj[(int64_t)i * G + ig] = jprev - (i + 1.) * zinv * ji;
-        jprev = ji;
+      // the parked ratios are read in chunks of 8 ahead of the chunk's
+      // stores, so the loads overlap instead of waiting one by one
+      for (int i0 = 1; i0 < N; i0 += 8) {
+        double rho8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          rho8[u] = i0 + u < N ? bj[(int64_t)(i0 + u) * G + ig] : 0.;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int i = i0 + u;
+          if (i >= N) break;
+          const double ji = rho8[u] * jprev;
+          bj[(int64_t)i * G + ig] = ji;
+          bdj[(int64_t)i * G + ig] = jprev - (i + 1.) * zinv * ji;
+          jprev = ji;
+        }
       }
     }
     // y_n(kr): forward recurrence from the closed forms of y_1, y_2.
@@ -255,15 +266,26 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
       const cdbl dj0 = csub(j0, cmul(jprev, zinv));
       bdjmr[ig] = dj0.x;
       bdjmi[ig] = dj0.y;
-      for (int i = 1; i < N; ++i) {
-        const int64_t o = (int64_t)i * G + ig;
-        const cdbl ji = cmul(cmk(bjmr[o], bjmi[o]), jprev);
-        bjmr[o] = ji.x;
-        bjmi[o] = ji.y;
-        const cdbl dji = csub(jprev, cmul(cscale(zinv, i + 1.), ji));
-        bdjmr[o] = dji.x;
-        bdjmi[o] = dji.y;
-        jprev = ji;
+      for (int i0 = 1; i0 < N; i0 += 8) {
+        cdbl rho8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t o = (int64_t)(i0 + u) * G + ig;
+          rho8[u] = i0 + u < N ? cmk(bjmr[o], bjmi[o]) : cmk(0., 0.);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int i = i0 + u;
+          if (i >= N) break;
+          const int64_t o = (int64_t)i * G + ig;
+          const cdbl ji = cmul(rho8[u], jprev);
+          bjmr[o] = ji.x;
+          bjmi[o] = ji.y;
+          const cdbl dji = csub(jprev, cmul(cscale(zinv, i + 1.), ji));
+          bdjmr[o] = dji.x;
+          bdjmi[o] = dji.y;
+          jprev = ji;
+        }
       }
     }
   }
