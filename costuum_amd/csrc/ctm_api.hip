@@ -801,10 +801,12 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     const int est = order_estimate(h_rv[g], h_d[g], h_lam[g]);
     // The converged order minus the estimate (4096-grain seeded batch):
     // median 0 / -2 / +2 and 95th percentile +2 / +5 / +11 for est below
-    // 20 / 20-40 / above 40.  A window to est + 3 + est/8 followed by
-    // windows of a quarter of the order covers most grains in one or two
-    // rounds while computing few orders beyond convergence (each order
-    // costs ~n^3, each round a host synchronisation).
+    // 20 / 20-40 / above 40.  A window to est + 3 + est/8 covers most
+    // grains in one round while computing few orders beyond convergence
+    // (each order costs ~n^3).  The few grains left get a window of half
+    // their order: such a round is latency-bound (few grains), so extra
+    // orders in it cost little, while a further round costs a host
+    // synchronisation and a set of small launches.
     const int margin = 3 + est / 8;
     hi[g] = std::min(last_order, std::max(min_order + 1, est + margin));
   }
@@ -844,7 +846,7 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     std::vector<int32_t> next;
     for (int g : active) {
       if (st[g].done) continue;
-      const int width = std::max(4, (hi[g] + 3) / 4);
+      const int width = std::max(8, (hi[g] + 1) / 2);
       lo[g] = hi[g] + 1;
       hi[g] = std::min(last_order, hi[g] + width);
       next.push_back(g);
