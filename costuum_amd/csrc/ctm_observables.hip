@@ -264,30 +264,37 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
   }
 }
 
-// K7a: extinction and absorption per (grain, outgoing direction j).
-// Forward pair j: S = (1/k) sum_m f_m X_m(j).  Absorption pairs (j, i) with
-// every quadrature azimuth: S(phi) = (1/k) sum_m g_m(-phi) X_m(j, i).
+// K7a: extinction and absorption per (grain, outgoing direction j), four
+// directions per workgroup (kObsQJ).  Forward pair j: S = (1/k) sum_m f_m
+// X_m(j).  Absorption pairs (j, i) with every quadrature azimuth:
+// S(phi) = (1/k) sum_m g_m(-phi) X_m(j, i).
+constexpr int kObsQJ = 4;
+constexpr int kObsQPhi = 2;  // azimuths per thread (measured best of 2, 4, 5, 10)
+
 __global__ __launch_bounds__(256) void k_obs_q(
     const ObsSpec spec, const int32_t *gi_off, const GrainParams gp,
     const cdbl *X, double *qext, double *qextpol, double *qextcpol,
     double *qabs, double *qabspol) {
-  const int g = blockIdx.x / spec.n_theta;
-  const int j = blockIdx.x - g * spec.n_theta;
+  const int njb = (spec.n_theta + kObsQJ - 1) / kObsQJ;
+  const int g = blockIdx.x / njb;
+  const int j0 = (blockIdx.x - g * njb) * kObsQJ;
+  const int nj = min(kObsQJ, spec.n_theta - j0);
   const int i0 = gi_off[g], i1 = gi_off[g + 1];
   const double lam = gp.wavelength[g];
   const double k = 2. * M_PI / lam;
+  const double kinv = 1. / k;
   const double rv = gp.radius[g];
   const double norm = M_PI * rv * rv;
   const double pre = 2. * M_PI / k;
   const int t = threadIdx.x;
-  __shared__ double red[4][2];
-  __shared__ double fwd[2][2];
-  if ((t >> 6) == 3) {
-    // forward pair: the lanes of wave 3 take every 64th m, then a shuffle
+  const int wv = t >> 6, lane = t & 63;
+  __shared__ double red[4][kObsQJ][2];
+  __shared__ double fwd[kObsQJ][2][2];
+  if (wv < nj) {
+    // forward pair j0 + wv: the lanes take every 64th m, then a shuffle
     // reduction
-    const int lane = t & 63;
     double v[4] = {0., 0., 0., 0.};
-    const int pr = spec.fwd_pair0 + j;
+    const int pr = spec.fwd_pair0 + j0 + wv;
     for (int q = i0 + lane; q < i1; q += 64) {
       const double f = (q == i0) ? 1. : 2.;
       const cdbl *Xp = X + ((int64_t)q * spec.n_pairs + pr) * 4;
@@ -300,86 +307,110 @@ __global__ __launch_bounds__(256) void k_obs_q(
     for (int c = 0; c < 4; ++c)
       for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
     if (lane == 0) {
-      const double kinv = 1. / k;
-      fwd[0][0] = v[0] * kinv;
-      fwd[0][1] = v[1] * kinv;
-      fwd[1][0] = v[2] * kinv;
-      fwd[1][1] = v[3] * kinv;
+      fwd[wv][0][0] = v[0] * kinv;
+      fwd[wv][0][1] = v[1] * kinv;
+      fwd[wv][1][0] = v[2] * kinv;
+      fwd[wv][1][1] = v[3] * kinv;
     }
   }
-  double zsum = 0., zpol = 0.;
+  double zs[kObsQJ], zp[kObsQJ];
+#pragma unroll
+  for (int q = 0; q < kObsQJ; ++q) zs[q] = zp[q] = 0.;
   if (spec.ngauss > 0) {
-    const int npair = spec.ngauss * spec.ngauss;
-    for (int e = t; e < npair; e += 256) {
-      const int ti = e / spec.ngauss, pj = e - ti * spec.ngauss;
-      const int pr = spec.abs_pair0 + j * spec.ngauss + ti;
-      const double cphi = spec.phi_cos[pj], sphi = -spec.phi_sin[pj];
-      cdbl S[4] = {cmk(0., 0.), cmk(0., 0.), cmk(0., 0.), cmk(0., 0.)};
-      // e^{i m dphi} by recurrence, as the reference; X of four m values
-      // is loaded ahead of their sums (four loads in flight)
-      cdbl ex = cmk(1., 0.);
-      const cdbl st = cmk(cphi, sphi);
-      const cdbl *Xb = X + ((int64_t)i0 * spec.n_pairs + pr) * 4;
-      const int64_t xs = (int64_t)spec.n_pairs * 4;
-      const int nq = i1 - i0;
-      auto term = [&](int m, const cdbl *xv) {
-        const double fc = (m == 0) ? 1. : 2. * ex.x;
-        const double fs = (m == 0) ? 0. : 2. * ex.y;
-        ex = cmul(ex, st);
-        rfma(S[0], fc, xv[0]);
-        rfma(S[3], fc, xv[3]);
-        rfma(S[1], fs, xv[1]);
-        rfma(S[2], fs, xv[2]);
-      };
-      int m = 0;
-      for (; m + 4 <= nq; m += 4) {
-        cdbl xv[4][4];
+    // one thread per (direction, incoming node, group of kObsQPhi
+    // azimuths): X of a pair is loaded once for the group's azimuths
+    const int ng = spec.ngauss;
+    const int npg = (ng + kObsQPhi - 1) / kObsQPhi;
+    const int per_j = ng * npg;
+    const int64_t xs = (int64_t)spec.n_pairs * 4;
+    const int nq = i1 - i0;
+    for (int e = t; e < nj * per_j; e += 256) {
+      const int jj = e / per_j, r = e - jj * per_j;
+      const int ti = r / npg, pg = r - ti * npg;
+      const int pj0 = pg * kObsQPhi;
+      const int pr = spec.abs_pair0 + (j0 + jj) * ng + ti;
+      cdbl S[kObsQPhi][4], ex[kObsQPhi], st[kObsQPhi];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < kObsQPhi; ++u) {
+        const int pj = min(pj0 + u, ng - 1);
+        st[u] = cmk(spec.phi_cos[pj], -spec.phi_sin[pj]);
+        ex[u] = cmk(1., 0.);
 #pragma unroll
-          for (int c = 0; c < 4; ++c) xv[u][c] = Xb[(m + u) * xs + c];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) term(m + u, xv[u]);
+        for (int c = 0; c < 4; ++c) S[u][c] = cmk(0., 0.);
       }
-      for (; m < nq; ++m) {
+      // e^{i m dphi} by recurrence, as the reference
+      const cdbl *Xb = X + ((int64_t)i0 * spec.n_pairs + pr) * 4;
+      // X of the next m is loaded while the current one is summed
+      cdbl xn[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xn[c] = nq > 0 ? Xb[c] : cmk(0., 0.);
+      for (int m = 0; m < nq; ++m) {
         cdbl xv[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) xv[c] = Xb[m * xs + c];
-        term(m, xv);
-      }
-      const double kinv = 1. / k;
-      double n2[4];
+        for (int c = 0; c < 4; ++c) {
+          xv[c] = xn[c];
+          if (m + 1 < nq) xn[c] = Xb[(m + 1) * xs + c];
+        }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const cdbl s = cscale(S[q], kinv);
-        n2[q] = s.x * s.x + s.y * s.y;
+        for (int u = 0; u < kObsQPhi; ++u) {
+          const double fc = (m == 0) ? 1. : 2. * ex[u].x;
+          const double fs = (m == 0) ? 0. : 2. * ex[u].y;
+          ex[u] = cmul(ex[u], st[u]);
+          rfma(S[u][0], fc, xv[0]);
+          rfma(S[u][3], fc, xv[3]);
+          rfma(S[u][1], fs, xv[1]);
+          rfma(S[u][2], fs, xv[2]);
+        }
       }
-      const double w = spec.theta_w[ti] * spec.phi_w[pj];
-      zsum += 0.5 * (n2[0] + n2[1] + n2[2] + n2[3]) * w;
-      zpol += 0.5 * (n2[0] + n2[1] - n2[2] - n2[3]) * w;
+      double a = 0., b = 0.;
+#pragma unroll
+      for (int u = 0; u < kObsQPhi; ++u) {
+        if (pj0 + u >= ng) break;
+        double n2[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const cdbl sq = cscale(S[u][q], kinv);
+          n2[q] = sq.x * sq.x + sq.y * sq.y;
+        }
+        const double w = spec.theta_w[ti] * spec.phi_w[pj0 + u];
+        a += 0.5 * (n2[0] + n2[1] + n2[2] + n2[3]) * w;
+        b += 0.5 * (n2[0] + n2[1] - n2[2] - n2[3]) * w;
+      }
+#pragma unroll
+      for (int q = 0; q < kObsQJ; ++q) {
+        if (q == jj) {
+          zs[q] += a;
+          zp[q] += b;
+        }
+      }
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    zsum += __shfl_xor(zsum, o);
-    zpol += __shfl_xor(zpol, o);
-  }
-  if ((t & 63) == 0) {
-    red[t >> 6][0] = zsum;
-    red[t >> 6][1] = zpol;
+#pragma unroll
+  for (int q = 0; q < kObsQJ; ++q) {
+    for (int o = 32; o > 0; o >>= 1) {
+      zs[q] += __shfl_xor(zs[q], o);
+      zp[q] += __shfl_xor(zp[q], o);
+    }
+    if (lane == 0) {
+      red[wv][q][0] = zs[q];
+      red[wv][q][1] = zp[q];
+    }
   }
   __syncthreads();
-  if (t == 0) {
-    const double zs = red[0][0] + red[1][0] + red[2][0] + red[3][0];
-    const double zp = red[0][1] + red[1][1] + red[2][1] + red[3][1];
-    const double e_ = pre * (fwd[0][1] + fwd[1][1]);
-    const double ep = pre * (fwd[0][1] - fwd[1][1]);
-    const double ec = pre * (fwd[1][0] - fwd[0][0]);
-    const int64_t o = (int64_t)g * spec.n_theta + j;
+  if (t < nj) {
+    const double zsum = red[0][t][0] + red[1][t][0] + red[2][t][0] +
+                        red[3][t][0];
+    const double zpol = red[0][t][1] + red[1][t][1] + red[2][t][1] +
+                        red[3][t][1];
+    const double e_ = pre * (fwd[t][0][1] + fwd[t][1][1]);
+    const double ep = pre * (fwd[t][0][1] - fwd[t][1][1]);
+    const double ec = pre * (fwd[t][1][0] - fwd[t][0][0]);
+    const int64_t o = (int64_t)g * spec.n_theta + j0 + t;
     if (qext) qext[o] = e_ / norm;
     if (qextpol) qextpol[o] = ep / norm;
     if (qextcpol) qextcpol[o] = ec / norm;
-    if (qabs) qabs[o] = (e_ - zs) / norm;
-    if (qabspol) qabspol[o] = (ep - zp) / norm;
+    if (qabs) qabs[o] = (e_ - zsum) / norm;
+    if (qabspol) qabspol[o] = (ep - zpol) / norm;
   }
 }
 
@@ -501,7 +532,8 @@ hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,
                         double *qextcpol, double *qabs, double *qabspol,
                         hipStream_t s) {
   if (n_grains <= 0 || spec.n_theta <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_obs_q, dim3(n_grains * spec.n_theta), dim3(256), 0, s,
+  const int njb = (spec.n_theta + kObsQJ - 1) / kObsQJ;
+  hipLaunchKernelGGL(k_obs_q, dim3(n_grains * njb), dim3(256), 0, s,
                      spec, gi_off, gp, X, qext, qextpol, qextcpol, qabs,
                      qabspol);
   return hipGetLastError();
