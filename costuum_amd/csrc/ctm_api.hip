@@ -1248,7 +1248,7 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   CTM_HIP(ctx->d_x.ensure(nitems * (size_t)sp.n_pairs * 4 * sizeof(cdbl)));
   CTM_LAUNCH(CTM_KERNEL_OBS_X, 0., 0.,
              launch_pitau(d_ang, n_ang, nmax_all, ctx->d_pt.as<double>(), s));
-  if (obs_x_lds_bytes(nmax_all, sp.n_out, 8) > 159 * 1024) {
+  if (obs_x_lds_bytes(nmax_all, sp.n_out, 8, 256, 1) > kObsXMaxLds) {
     ctx->err = "too many outgoing directions for the largest order";
     return CTM_ERR_ARGUMENT;
   }
@@ -1263,11 +1263,14 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
     for (size_t i = 0; i < nitems; ++i)
       lst[cnt[nmax_all - nm_of(ctx->final_items[i])]++] = (int32_t)i;
     CTM_HIP(upload(ctx, ctx->d_obsitems, lst));
-    const int caps[] = {nmax_all, 48, 24, 12};
+    // (80 and 64: the blocks above 64 take 8 incoming directions per
+    // thread, and up to order 80 two such chunks side by side fit the LDS)
+    const int caps[] = {nmax_all, 80, 64, 32, 16};
+    constexpr int kCaps = 5;
     size_t pos = 0;
-    for (int b = 0; b < 4 && pos < nitems; ++b) {
+    for (int b = 0; b < kCaps && pos < nitems; ++b) {
       const int cap = caps[b];
-      const int lo = b < 3 ? caps[b + 1] : 0;
+      const int lo = b < kCaps - 1 ? caps[b + 1] : 0;
       if (cap <= lo) continue;
       size_t end = pos;
       double fb = 0.;
@@ -1279,7 +1282,7 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
       if (end > pos) {
         CTM_LAUNCH(CTM_KERNEL_OBS_X, fb, 0.,
                    launch_obs_x(sp, ctx->d_obsitems.as<int32_t>() + pos,
-                                (int)(end - pos), cap, b == 0 ? 8 : 4,
+                                (int)(end - pos), cap, cap > 64 ? 8 : 4,
                                 ctx->d_items.as<Item>(),
                                 ctx->obs_source >= 0
                                     ? ctx->d_dt.as<cdbl>() +

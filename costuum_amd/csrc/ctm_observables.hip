@@ -115,8 +115,8 @@ __device__ inline cdbl cfac(int nn, int n) {
 // (stored [n][direction] so neighbouring pairs read neighbouring words),
 // of the round's incoming directions and the n-factors a_n are staged in
 // LDS.
-template <int CK>
-__global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
+template <int CK, int NT, int GCAP>
+__global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
                                                const int32_t *item_list,
                                                const Item *items,
                                                const cdbl *tbuf,
@@ -135,9 +135,9 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
   const int n2m = 2 * nm;
   const cdbl *T = tbuf + it.t_off;
   const cdbl *A = abuf ? abuf + aoff[ii] : nullptr;
-  // LDS: Ysh [G CK][2][2 nm] cdbl (G 2 nm <= max(256, 2 nm_cap)),
+  // LDS: Ysh [G CK][2][2 nm] cdbl (G 2 nm <= rows_cap),
   // PTo [nm][n_out][2], Bv [G CK][nm][2], an [nm]
-  const int rows_cap = max(256, 2 * nm_cap);
+  const int rows_cap = obs_x_rows_cap(nm_cap, NT, GCAP);
   cdbl *Ysh = reinterpret_cast<cdbl *>(smem);
   double *PTo = smem + 2 * (CK * 2 * rows_cap);
   double *Bv = PTo + (int64_t)spec.n_out * nm_cap * 2;
@@ -146,14 +146,14 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
   const int n_ang = spec.n_ang;
   const int n_out = spec.n_out;
   const double *PTm = PT + (int64_t)m * n_ang * nmax_all * 2;
-  for (int e = t; e < n_out * nm; e += 256) {
+  for (int e = t; e < n_out * nm; e += NT) {
     const int p = e / n_out, o = e - (e / n_out) * n_out;
     const double *src =
         PTm + ((int64_t)spec.out_angle[o] * nmax_all + (nmin + p - 1)) * 2;
     PTo[(p * n_out + o) * 2] = src[0];
     PTo[(p * n_out + o) * 2 + 1] = src[1];
   }
-  for (int p = t; p < nm; p += 256) {
+  for (int p = t; p < nm; p += NT) {
     const int n = nmin + p;
     an[p] = sqrt((2. * n + 1.) / (n * (n + 1.)));
   }
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
   // chunk groups per round
   const int nchunk = (spec.n_in + CK - 1) / CK;
   if (nchunk == 0) return;
-  const int gmax = n2m >= 256 ? 1 : min(nchunk, 256 / n2m);
+  const int gmax = n2m >= NT ? 1 : min(min(nchunk, NT / n2m), GCAP);
   const int rounds = (nchunk + gmax - 1) / gmax;
   const int G = (nchunk + rounds - 1) / rounds;
   const int gi = G == 1 ? 0 : t / n2m;
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
     const int in0 = r * G * CK;
     const int in1 = min(spec.n_in, in0 + G * CK);
     __syncthreads();
-    for (int e = t; e < G * CK * nm; e += 256) {
+    for (int e = t; e < G * CK * nm; e += NT) {
       const int ic = e / nm, p = e - (e / nm) * nm;
       double bp = 0., bt = 0.;
       if (in0 + ic < in1) {
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
     }
     __syncthreads();
     const int row0 = G == 1 ? t : t - gi * n2m;
-    const int rstep = G == 1 ? 256 : n2m;
+    const int rstep = G == 1 ? NT : n2m;
     for (int row = row0; gi < G && row < n2m; row += rstep) {
       const int h = row >= nm ? 1 : 0;
       const int p = row - h * nm;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_obs_x(const ObsSpec spec,
     // pair contractions: X^0 = a.Y_b, X^1 = a.Y_b', X^2 = -a'.Y_b,
     // X^3 = a'.Y_b' with a = (pi, tau), a' = (tau, pi)
     const int q0 = spec.in_pair_off[in0], q1 = spec.in_pair_off[in1];
-    for (int qi = q0 + t; qi < q1; qi += 256) {
+    for (int qi = q0 + t; qi < q1; qi += NT) {
       const int pr = spec.pair_by_in[qi];
       const int ic = spec.pair_in[pr] - in0;
       const int o = spec.pair_out[pr];
@@ -483,15 +483,15 @@ hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
   return hipGetLastError();
 }
 
-size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck) {
-  const size_t rows_cap = std::max(256, 2 * nm_cap);
+size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck, int nt, int gcap) {
+  const size_t rows_cap = obs_x_rows_cap(nm_cap, nt, gcap);
   return (size_t)ck * 2 * rows_cap * sizeof(cdbl) +
          (size_t)n_out * nm_cap * 2 * sizeof(double) +
          (size_t)ck * (rows_cap / 2) * 2 * sizeof(double) +
          (size_t)nm_cap * sizeof(double);
 }
 
-template <int CK>
+template <int CK, int NT, int GCAP>
 static hipError_t launch_obs_x_ck(const ObsSpec &spec,
                                   const int32_t *item_list, int n_items,
                                   int nm_cap, const Item *items,
@@ -501,13 +501,14 @@ static hipError_t launch_obs_x_ck(const ObsSpec &spec,
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(k_obs_x<CK>),
+        reinterpret_cast<const void *>(k_obs_x<CK, NT, GCAP>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL(k_obs_x<CK>, dim3(n_items), dim3(256),
-                     obs_x_lds_bytes(nm_cap, spec.n_out, CK), s, spec,
+  hipLaunchKernelGGL((k_obs_x<CK, NT, GCAP>), dim3(n_items), dim3(NT),
+                     obs_x_lds_bytes(nm_cap, spec.n_out, CK, NT, GCAP), s,
+                     spec,
                      item_list, items, tbuf, abuf, aoff, PT, nmax_all, nm_cap,
                      X);
   return hipGetLastError();
@@ -519,11 +520,19 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
                         const int64_t *aoff, const double *PT, int nmax_all,
                         cdbl *X, hipStream_t s) {
   if (n_items <= 0) return hipSuccess;
-  if (ck == 8)
-    return launch_obs_x_ck<8>(spec, item_list, n_items, nm_cap, items, tbuf,
-                              abuf, aoff, PT, nmax_all, X, s);
-  return launch_obs_x_ck<4>(spec, item_list, n_items, nm_cap, items, tbuf,
-                            abuf, aoff, PT, nmax_all, X, s);
+  // ck 8: the largest blocks, 512 threads (two 8-direction chunks side by
+  // side) if the LDS allows, else 256
+  if (ck == 8) {
+    if (obs_x_lds_bytes(nm_cap, spec.n_out, 8, 512, 2) <= kObsXMaxLds)
+      return launch_obs_x_ck<8, 512, 2>(spec, item_list, n_items, nm_cap,
+                                        items, tbuf, abuf, aoff, PT, nmax_all,
+                                        X, s);
+    return launch_obs_x_ck<8, 256, 1>(spec, item_list, n_items, nm_cap, items,
+                                      tbuf, abuf, aoff, PT, nmax_all, X, s);
+  }
+  return launch_obs_x_ck<4, 256, 256>(spec, item_list, n_items, nm_cap,
+                                      items, tbuf, abuf, aoff, PT, nmax_all, X,
+                                      s);
 }
 
 hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,

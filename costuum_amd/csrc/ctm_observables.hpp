@@ -43,8 +43,17 @@ hipError_t launch_query(const Query *q, int n, const Item *items,
 
 hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
                         hipStream_t s);
-// ck: incoming directions per thread in the T b products (4 or 8)
-size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck);
+// ck: incoming directions per thread in the T b products (4 or 8); nt:
+// threads per workgroup; gcap: most chunks side by side.  Rows of the Y
+// buffer: G 2 nm for at most gcap groups that fit nt threads, and at least
+// one group of 2 nm_cap rows.
+__host__ __device__ inline int obs_x_rows_cap(int nm_cap, int nt, int gcap) {
+  const int two = 2 * nm_cap;
+  const int fit = gcap * two < nt ? gcap * two : nt;
+  return fit > two ? fit : two;
+}
+size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck, int nt, int gcap);
+constexpr size_t kObsXMaxLds = 159 * 1024;
 // item_list: items to process (largest first); abuf/aoff (may be null):
 // full-layout ensemble T-matrices per item.
 hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
