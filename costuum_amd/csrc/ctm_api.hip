@@ -81,8 +81,9 @@ struct Staging {
 
 // Grain-independent table caches (per batch).  Gauss-Legendre keys are
 // found by G directly; Wigner keys through one slot array of N + 1 entries
-// (one per m) per (N, G), the last (N, G) looked up kept at hand (item
-// lists ask for all m of one (N, G) in a row).
+// (one per m) per (N, G), found through a by-N entry (the last G seen for
+// that N) or a hash map, the last (N, G) looked up kept at hand (item lists
+// ask for all m of one (N, G) in a row).
 struct SharedTables {
   std::vector<int32_t> gl_by_g;                 // G -> key index or -1
   std::unordered_map<uint64_t, int32_t> wig_base;  // (N, G) -> first slot
@@ -92,7 +93,9 @@ struct SharedTables {
   int64_t gl_size = 0, wig_size = 0;
   size_t gl_done = 0, wig_done = 0;  // keys already computed on device
   int last_n = -1, last_g = -1, last_base = 0;
+  std::vector<int64_t> wig_by_n;  // N -> (G << 32) | first slot, or -1
   void clear() {
+    wig_by_n.clear();
     gl_by_g.clear();
     wig_base.clear();
     wig_slot.clear();
@@ -115,14 +118,22 @@ struct SharedTables {
   }
   int wig(int N, int m, int G) {
     if (N != last_n || G != last_g) {
-      const uint64_t key = (uint64_t)(uint32_t)N | ((uint64_t)(uint32_t)G << 32);
-      auto f = wig_base.find(key);
-      if (f != wig_base.end()) {
-        last_base = f->second;
+      if (N < (int)wig_by_n.size() && wig_by_n[N] >= 0 &&
+          (int)(wig_by_n[N] >> 32) == G) {
+        last_base = (int)(wig_by_n[N] & 0xffffffff);
       } else {
-        last_base = (int)wig_slot.size();
-        wig_slot.resize(wig_slot.size() + N + 1, -1);
-        wig_base.emplace(key, last_base);
+        const uint64_t key =
+            (uint64_t)(uint32_t)N | ((uint64_t)(uint32_t)G << 32);
+        auto f = wig_base.find(key);
+        if (f != wig_base.end()) {
+          last_base = f->second;
+        } else {
+          last_base = (int)wig_slot.size();
+          wig_slot.resize(wig_slot.size() + N + 1, -1);
+          wig_base.emplace(key, last_base);
+        }
+        if (N >= (int)wig_by_n.size()) wig_by_n.resize(N + 1, -1);
+        wig_by_n[N] = ((int64_t)G << 32) | (int64_t)last_base;
       }
       last_n = N;
       last_g = G;
@@ -544,27 +555,37 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   if (w.items.empty()) return CTM_OK;
   int rc = flush_shared(ctx);
   if (rc != CTM_OK) return rc;
-  build_tasks(w);
   hipStream_t s = ctx->stream;
+  // the tables need only the sets: launch them first and build the
+  // assembly and solve lists while they run
   CTM_HIP(upload(ctx, ctx->d_sets, w.sets));
-  CTM_HIP(upload(ctx, ctx->d_items, w.items));
-  CTM_HIP(upload(ctx, ctx->d_tiles, w.order));
-  CTM_HIP(upload(ctx, ctx->d_gioff_tmp, w.tile_prefix));
-  CTM_HIP(upload(ctx, ctx->d_solve, w.solve));
   CTM_HIP(ctx->d_tables.ensure(std::max<int64_t>(w.table_size, 1) *
                                sizeof(double)));
-  CTM_HIP(ctx->d_q.ensure(std::max<int64_t>(w.q_size, 1) * sizeof(cdbl)));
-  if (want_t)
-    CTM_HIP(ctx->d_t.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
-  CTM_HIP(ctx->d_qparts.ensure(std::max<size_t>(w.items.size(), 1) * 2 *
-                               kQParts * sizeof(double)));
   // Algorithmic work of the launches (the per-block counts of the flop
   // model F_T: 96 G nm^2 for an m > 0 assembly, 32 G n^2 for an m = 0
   // order; (64/3) nm^3 and (16/3) n^3 for the parity-split solves) and the
   // compulsory HBM traffic (tables and Wigner values read once, Q/RgQ
   // written once by the assembly, read once by the solve, T written once).
+  double b_tab = 0.;
+  for (const TableSet &ts : w.sets)
+    b_tab += 8. * ((double)kNodeScalars * ts.G +
+                   (double)kBesselArrays * ts.N * ts.G);
+  CTM_LAUNCH(CTM_KERNEL_TABLES, 0., b_tab,
+             launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(),
+                           w.n_nodes, gp, ctx->d_gl.as<double>(),
+                           ctx->d_tables.as<double>(), s));
+  build_tasks(w);
+  CTM_HIP(upload(ctx, ctx->d_items, w.items));
+  CTM_HIP(upload(ctx, ctx->d_tiles, w.order));
+  CTM_HIP(upload(ctx, ctx->d_gioff_tmp, w.tile_prefix));
+  CTM_HIP(upload(ctx, ctx->d_solve, w.solve));
+  CTM_HIP(ctx->d_q.ensure(std::max<int64_t>(w.q_size, 1) * sizeof(cdbl)));
+  if (want_t)
+    CTM_HIP(ctx->d_t.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
+  CTM_HIP(ctx->d_qparts.ensure(std::max<size_t>(w.items.size(), 1) * 2 *
+                               kQParts * sizeof(double)));
   // (index: part, micro-tile 3, 2, 1)
-  double f_asm[3] = {0., 0., 0.}, b_asm[3] = {0., 0., 0.}, b_tab = 0.;
+  double f_asm[3] = {0., 0., 0.}, b_asm[3] = {0., 0., 0.};
   for (const Item &it : w.items) {
     const int nmin = it.m > 0 ? it.m : 1;
     const int nmi = it.N + 1 - nmin;
@@ -573,14 +594,6 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     f_asm[k] += (it.m > 0 ? 96. : 32.) * it.G * nm * nm;
     b_asm[k] += 8. * (2. * nm * it.G) + 16. * 4. * nm * nm;
   }
-  for (const TableSet &ts : w.sets)
-    b_tab += 8. * ((double)kNodeScalars * ts.G +
-                   (double)kBesselArrays * ts.N * ts.G);
-  CTM_LAUNCH(CTM_KERNEL_TABLES, 0., b_tab,
-             launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(),
-                           w.n_nodes, gp,
-                           ctx->d_gl.as<double>(), ctx->d_tables.as<double>(),
-                           s));
   {
     // one launch per part (3 x 3, 2 x 2, 1 x 1 micro-tiles), each credited
     // with the algorithmic work of its blocks
