@@ -1254,6 +1254,39 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
                               (long long)(unsigned)lo);
 }
 
+// Wave maximum of a 64-bit key: DPP steps within each row of 16 lanes
+// (quad swaps, half-row and row mirrors), then the four rows' maxima by
+// readlane.  The result is uniform.
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_max_u64(
+    unsigned long long v) {
+  const int lo = (int)(unsigned)v, hi = (int)(v >> 32);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  const unsigned long long o =
+      ((unsigned long long)(unsigned)hi2 << 32) | (unsigned)lo2;
+  return o > v ? o : v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(
+    unsigned long long v) {
+  v = dpp_max_u64<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v = dpp_max_u64<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v = dpp_max_u64<0x141>(v);  // row_half_mirror
+  v = dpp_max_u64<0x140>(v);  // row_mirror
+  const int lo = (int)(unsigned)v, hi = (int)(v >> 32);
+  unsigned long long m = 0ull;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const unsigned long long x =
+        ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * r)
+         << 32) |
+        (unsigned)__builtin_amdgcn_readlane(lo, 16 * r);
+    m = x > m ? x : m;
+  }
+  return m;
+}
+
 template <int RS, int W>
 __device__ __forceinline__ void blk_panel(cdbl *A, int nm, int lo, int hi,
                                           int c0, int w, int lane, int *piv,
@@ -1281,10 +1314,7 @@ __device__ __forceinline__ void blk_panel(cdbl *A, int nm, int lo, int hi,
                    : 0ull;
       mx = key[r] > mx ? key[r] : mx;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      const unsigned long long om = __shfl_xor(mx, o);
-      mx = om > mx ? om : mx;
-    }
+    mx = wave_max_u64(mx);
     int p = k;
 #pragma unroll
     for (int r = RS - 1; r >= 0; --r) {
