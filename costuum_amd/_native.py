@@ -43,7 +43,7 @@ QUERY_OUT = 40
 
 KERNELS = ("gauss_legendre", "wigner", "tables", "assemble", "solve",
            "converge", "sums", "obs_x", "obs_sum", "average",
-           "assemble_small", "assemble_wide")
+           "assemble_small", "assemble_wide", "assemble_pack")
 
 
 class CtmObservableSpec(ctypes.Structure):

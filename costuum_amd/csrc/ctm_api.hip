@@ -160,11 +160,12 @@ struct WorkList {
   std::vector<TableSet> sets;
   std::vector<Item> items;
   std::vector<int32_t> order;        // items sorted by assembly cost
-  // Assembly order: three parts by micro-tile (3, 2, then 1: see
-  // assemble_micro), each sorted by G (largest first); tile_prefix holds
-  // one prefix sum per part (n_part[k] + 1 entries each, in that order).
+  // Assembly order: five parts, by micro-tile 3, 2, 1 (see assemble_micro)
+  // and the packed blocks with m > 0, then m = 0, each sorted by G (largest
+  // first); tile_prefix holds one prefix sum per tiled part (n_part[k] + 1
+  // entries each, parts 0-2 in that order).
   std::vector<int32_t> tile_prefix;
-  int n_part[3] = {0, 0, 0};
+  int n_part[5] = {0, 0, 0, 0, 0};
   std::vector<int32_t> solve;        // item * 2 + class, by block order
   std::vector<int32_t> scratch_a, scratch_b, scratch_c;  // build_tasks
   int64_t table_size = 0, q_size = 0, t_size = 0;
@@ -431,27 +432,28 @@ int flush_shared(ctm_context *ctx) {
 void build_tasks(WorkList &w) {
   const int n = (int)w.items.size();
   // per item: block order, an m = 0 flag and the part (micro-tile 3, 2, 1
-  // -> 0, 1, 2); then the assembly key
+  // -> 0, 1, 2; packed -> 3 (m > 0) or 4 (m = 0)); then the assembly key
   std::vector<int32_t> &info = w.scratch_a, &key = w.scratch_b;
   info.resize(n);
   key.resize(n);
   int gmax = 0, nmmax = 0;
-  w.n_part[0] = w.n_part[1] = w.n_part[2] = 0;
+  for (int k = 0; k < 5; ++k) w.n_part[k] = 0;
   for (int i = 0; i < n; ++i) {
     const Item &it = w.items[i];
     const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-    const int part = 3 - assemble_micro(nm, it.m == 0);
+    const int micro = assemble_micro(nm, it.m == 0);
+    const int part = micro == 4 ? (it.m == 0 ? 4 : 3) : 3 - micro;
     info[i] = nm | (part << 16) | (it.m == 0 ? (1 << 30) : 0);
     w.n_part[part]++;
     gmax = std::max(gmax, it.G);
     nmmax = std::max(nmmax, nm);
   }
   // counting sort by (part, G descending)
-  const int nkey = 3 * (gmax + 1);
+  const int nkey = 5 * (gmax + 1);
   std::vector<int32_t> &cnt = w.scratch_c;
   cnt.assign(std::max(nkey, nmmax + 1) + 1, 0);
   for (int i = 0; i < n; ++i) {
-    key[i] = ((info[i] >> 16) & 3) * (gmax + 1) + (gmax - w.items[i].G);
+    key[i] = ((info[i] >> 16) & 7) * (gmax + 1) + (gmax - w.items[i].G);
     cnt[key[i] + 1]++;
   }
   for (int g = 1; g <= nkey; ++g) cnt[g] += cnt[g - 1];
@@ -461,7 +463,7 @@ void build_tasks(WorkList &w) {
   w.tile_prefix.resize((size_t)n + 3);
   int32_t *pre = w.tile_prefix.data();
   int q = 0;
-  for (int part = 0; part < 3; ++part) {
+  for (int part = 0; part < 3; ++part) {  // the tiled parts
     const int tr = 16 * (3 - part);
     pre[0] = 0;
     for (int k = 0; k < w.n_part[part]; ++k, ++q) {
@@ -584,13 +586,14 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     CTM_HIP(ctx->d_t.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
   CTM_HIP(ctx->d_qparts.ensure(std::max<size_t>(w.items.size(), 1) * 2 *
                                kQParts * sizeof(double)));
-  // (index: part, micro-tile 3, 2, 1)
-  double f_asm[3] = {0., 0., 0.}, b_asm[3] = {0., 0., 0.};
+  // (index: part, micro-tile 3, 2, 1, packed m > 0, packed m = 0)
+  double f_asm[5] = {0., 0., 0., 0., 0.}, b_asm[5] = {0., 0., 0., 0., 0.};
   for (const Item &it : w.items) {
     const int nmin = it.m > 0 ? it.m : 1;
     const int nmi = it.N + 1 - nmin;
     const double nm = nmi;
-    const int k = 3 - assemble_micro(nmi, it.m == 0);
+    const int micro = assemble_micro(nmi, it.m == 0);
+    const int k = micro == 4 ? (it.m == 0 ? 4 : 3) : 3 - micro;
     f_asm[k] += (it.m > 0 ? 96. : 32.) * it.G * nm * nm;
     b_asm[k] += 8. * (2. * nm * it.G) + 16. * 4. * nm * nm;
   }
@@ -616,6 +619,19 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
                                    ctx->d_q.as<cdbl>(), s));
       ord += np;
       po += np + 1;
+    }
+    for (int part = 3; part < 5; ++part) {
+      const int np = w.n_part[part];
+      if (np > 0)
+        CTM_LAUNCH(CTM_KERNEL_ASSEMBLE_PACK, f_asm[part], b_asm[part],
+                   launch_assemble_pack4(ord, np, part == 4 ? 1 : 0,
+                                         ctx->d_items.as<Item>(),
+                                         ctx->d_sets.as<TableSet>(),
+                                         ctx->d_wigkeys.as<WigKey>(), gp,
+                                         ctx->d_tables.as<double>(),
+                                         ctx->d_wig.as<double>(),
+                                         ctx->d_q.as<cdbl>(), s));
+      ord += np;
     }
   }
   return run_solves(ctx, want_t, nullptr);

@@ -321,6 +321,100 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
 constexpr int kLeftStride = 34;   // 32 factors + 2 pad doubles per row
 constexpr int kRightStride = 10;  // 5 complex factors per column
 
+// Row factors (32 doubles) of block row p at node ig, zero beyond the block
+// or the node range.
+__device__ __forceinline__ void row_factor_entry(
+    double *L, int p, int ig, int G, int nm, int nmin, double mm,
+    const double *S, const double *B, int64_t NG, const double *Wd,
+    const double *Wdd) {
+  if (ig < G && p < nm) {
+    const int n1 = nmin + p;
+    const double d1 = Wd[(int64_t)p * G + ig];
+    const double dd1 = Wdd[(int64_t)p * G + ig];
+    const int64_t bo = (int64_t)(n1 - 1) * G + ig;
+    const double j1 = B[bo], dj1 = B[NG + bo];
+    const double y1 = B[2 * NG + bo], dy1 = B[3 * NG + bo];
+    const double wr2 = S[ig], dr = S[G + ig], s1 = S[2 * G + ig];
+    const double s2 = S[3 * G + ig], krinv = S[4 * G + ig];
+    const cdbl imkr = cmk(S[5 * G + ig], S[6 * G + ig]);
+    const double nn1 = n1 * (n1 + 1.);
+    const double a1 = sqrt((2. * n1 + 1.) / nn1);
+    const cdbl h1 = cmk(j1, y1), dh1 = cmk(dj1, dy1);
+    const double m2s2 = mm * mm * s2;
+    const double q = dr * krinv * nn1 * d1;
+    // even-pair row factors
+    const double fA1 = wr2 * m2s2 * d1 * a1;
+    const double fA2 = wr2 * dd1 * a1;
+    const double fA3 = wr2 * dr * dd1 * a1;
+    const double fA5 = -wr2 * a1;
+    const cdbl A1 = cscale(h1, fA1);
+    const cdbl A2 = cscale(h1, fA2);
+    const cdbl A3 = cmul(imkr, cscale(h1, fA3));
+    const cdbl A4 = cscale(dh1, -fA1);
+    const cdbl A5 = cscale(cadd(cscale(dh1, dd1), cscale(h1, q)), fA5);
+    L[0] = A1.x; L[1] = A1.y; L[2] = A2.x; L[3] = A2.y;
+    L[4] = A3.x; L[5] = A3.y; L[6] = A4.x; L[7] = A4.y;
+    L[8] = A5.x; L[9] = A5.y;
+    L[10] = fA1 * j1;
+    L[11] = fA2 * j1;
+    const cdbl A3r = cscale(imkr, fA3 * j1);
+    L[12] = A3r.x; L[13] = A3r.y;
+    L[14] = -fA1 * dj1;
+    L[15] = fA5 * (dd1 * dj1 + q * j1);
+    // odd-pair row factors (without the common factor i)
+    const double dsi = s1 * mm * wr2;
+    const double fB5 = dsi * d1 * a1;
+    const double fB4 = dsi * dd1 * a1;
+    const double fB1 = dsi * a1;
+    const double fB3 = dsi * dr * d1 * a1;
+    const cdbl B5 = cscale(h1, fB5);
+    const cdbl B4 = cscale(h1, fB4);
+    const cdbl B2 = cscale(dh1, fB5);
+    const cdbl B1 = cscale(cadd(cscale(dh1, dd1), cscale(h1, q)), fB1);
+    const cdbl B3 = cmul(imkr, cscale(dh1, fB3));
+    L[16] = B5.x; L[17] = B5.y; L[18] = B4.x; L[19] = B4.y;
+    L[20] = B2.x; L[21] = B2.y; L[22] = B1.x; L[23] = B1.y;
+    L[24] = B3.x; L[25] = B3.y;
+    L[26] = fB5 * j1;
+    L[27] = fB4 * j1;
+    L[28] = fB5 * dj1;
+    L[29] = fB1 * (dd1 * dj1 + q * j1);
+    const cdbl B3r = cscale(imkr, fB3 * dj1);
+    L[30] = B3r.x; L[31] = B3r.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 32; ++i) L[i] = 0.;
+  }
+}
+
+// Column factors (R1..R5, 10 doubles) of block column p at node ig, zero
+// beyond the block or the node range.
+__device__ __forceinline__ void col_factor_entry(
+    double *R, int p, int ig, int G, int nm, int nmin, const double *B,
+    int64_t NG, const double *Wd, const double *Wdd) {
+  if (ig < G && p < nm) {
+    const int n2 = nmin + p;
+    const double d2 = Wd[(int64_t)p * G + ig];
+    const double dd2 = Wdd[(int64_t)p * G + ig];
+    const int64_t bo = (int64_t)(n2 - 1) * G + ig;
+    const cdbl jm = cmk(B[4 * NG + bo], B[5 * NG + bo]);
+    const cdbl djm = cmk(B[6 * NG + bo], B[7 * NG + bo]);
+    const double nn2 = n2 * (n2 + 1.);
+    const double a2 = sqrt((2. * n2 + 1.) / nn2);
+    const cdbl R1 = cscale(djm, a2 * d2);
+    const cdbl R2 = cscale(djm, a2 * dd2);
+    const cdbl R3 = cscale(jm, a2 * nn2 * d2);
+    const cdbl R4 = cscale(jm, a2 * d2);
+    const cdbl R5 = cscale(jm, a2 * dd2);
+    R[0] = R1.x; R[1] = R1.y; R[2] = R2.x; R[3] = R2.y;
+    R[4] = R3.x; R[5] = R3.y; R[6] = R4.x; R[7] = R4.y;
+    R[8] = R5.x; R[9] = R5.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) R[i] = 0.;
+  }
+}
+
 // Stages the row factors (32 per row and node: even-pair A1..A5, RgJ
 // factors, odd-pair B5..B3, RgJ factors) of the TR rows of tile row tr and
 // the column factors (R1..R5) of the cw columns of tile column tc for the
@@ -333,100 +427,140 @@ __device__ __forceinline__ void stage_factors(
     int cw, double mm, const double *S, const double *B, int64_t NG,
     const double *Wd, const double *Wdd) {
   const int nstage = TR * CH + cw * CH;
-    for (int e = t; e < nstage; e += nthreads) {
+  for (int e = t; e < nstage; e += nthreads) {
     if (e < TR * CH) {
-      // row factors of row r at node ig0 + kk
       const int r = e / CH, kk = e % CH;
-      const int ig = ig0 + kk;
-      const int p = tr * TR + r;
-      double *L = Lsh[kk][r];
-      if (ig < G && p < nm) {
-        const int n1 = nmin + p;
-        const double d1 = Wd[(int64_t)p * G + ig];
-        const double dd1 = Wdd[(int64_t)p * G + ig];
-        const int64_t bo = (int64_t)(n1 - 1) * G + ig;
-        const double j1 = B[bo], dj1 = B[NG + bo];
-        const double y1 = B[2 * NG + bo], dy1 = B[3 * NG + bo];
-        const double wr2 = S[ig], dr = S[G + ig], s1 = S[2 * G + ig];
-        const double s2 = S[3 * G + ig], krinv = S[4 * G + ig];
-        const cdbl imkr = cmk(S[5 * G + ig], S[6 * G + ig]);
-        const double nn1 = n1 * (n1 + 1.);
-        const double a1 = sqrt((2. * n1 + 1.) / nn1);
-        const cdbl h1 = cmk(j1, y1), dh1 = cmk(dj1, dy1);
-        const double m2s2 = mm * mm * s2;
-        const double q = dr * krinv * nn1 * d1;
-        // even-pair row factors
-        const double fA1 = wr2 * m2s2 * d1 * a1;
-        const double fA2 = wr2 * dd1 * a1;
-        const double fA3 = wr2 * dr * dd1 * a1;
-        const double fA5 = -wr2 * a1;
-        const cdbl A1 = cscale(h1, fA1);
-        const cdbl A2 = cscale(h1, fA2);
-        const cdbl A3 = cmul(imkr, cscale(h1, fA3));
-        const cdbl A4 = cscale(dh1, -fA1);
-        const cdbl A5 = cscale(cadd(cscale(dh1, dd1), cscale(h1, q)), fA5);
-        L[0] = A1.x; L[1] = A1.y; L[2] = A2.x; L[3] = A2.y;
-        L[4] = A3.x; L[5] = A3.y; L[6] = A4.x; L[7] = A4.y;
-        L[8] = A5.x; L[9] = A5.y;
-        L[10] = fA1 * j1;
-        L[11] = fA2 * j1;
-        const cdbl A3r = cscale(imkr, fA3 * j1);
-        L[12] = A3r.x; L[13] = A3r.y;
-        L[14] = -fA1 * dj1;
-        L[15] = fA5 * (dd1 * dj1 + q * j1);
-        // odd-pair row factors (without the common factor i)
-        const double dsi = s1 * mm * wr2;
-        const double fB5 = dsi * d1 * a1;
-        const double fB4 = dsi * dd1 * a1;
-        const double fB1 = dsi * a1;
-        const double fB3 = dsi * dr * d1 * a1;
-        const cdbl B5 = cscale(h1, fB5);
-        const cdbl B4 = cscale(h1, fB4);
-        const cdbl B2 = cscale(dh1, fB5);
-        const cdbl B1 = cscale(cadd(cscale(dh1, dd1), cscale(h1, q)), fB1);
-        const cdbl B3 = cmul(imkr, cscale(dh1, fB3));
-        L[16] = B5.x; L[17] = B5.y; L[18] = B4.x; L[19] = B4.y;
-        L[20] = B2.x; L[21] = B2.y; L[22] = B1.x; L[23] = B1.y;
-        L[24] = B3.x; L[25] = B3.y;
-        L[26] = fB5 * j1;
-        L[27] = fB4 * j1;
-        L[28] = fB5 * dj1;
-        L[29] = fB1 * (dd1 * dj1 + q * j1);
-        const cdbl B3r = cscale(imkr, fB3 * dj1);
-        L[30] = B3r.x; L[31] = B3r.y;
-      } else {
-#pragma unroll
-        for (int i = 0; i < 32; ++i) L[i] = 0.;
-      }
+      row_factor_entry(Lsh[kk][r], tr * TR + r, ig0 + kk, G, nm, nmin, mm, S,
+                       B, NG, Wd, Wdd);
     } else {
       const int tt = e - TR * CH;
       const int c = tt / CH, kk = tt % CH;
-      const int ig = ig0 + kk;
-      const int p = tc * cw + c;
-      double *R = Rsh[kk][c];
-      if (ig < G && p < nm) {
-        const int n2 = nmin + p;
-        const double d2 = Wd[(int64_t)p * G + ig];
-        const double dd2 = Wdd[(int64_t)p * G + ig];
-        const int64_t bo = (int64_t)(n2 - 1) * G + ig;
-        const cdbl jm = cmk(B[4 * NG + bo], B[5 * NG + bo]);
-        const cdbl djm = cmk(B[6 * NG + bo], B[7 * NG + bo]);
-        const double nn2 = n2 * (n2 + 1.);
-        const double a2 = sqrt((2. * n2 + 1.) / nn2);
-        const cdbl R1 = cscale(djm, a2 * d2);
-        const cdbl R2 = cscale(djm, a2 * dd2);
-        const cdbl R3 = cscale(jm, a2 * nn2 * d2);
-        const cdbl R4 = cscale(jm, a2 * d2);
-        const cdbl R5 = cscale(jm, a2 * dd2);
-        R[0] = R1.x; R[1] = R1.y; R[2] = R2.x; R[3] = R2.y;
-        R[4] = R3.x; R[5] = R3.y; R[6] = R4.x; R[7] = R4.y;
-        R[8] = R5.x; R[9] = R5.y;
-      } else {
+      col_factor_entry(Rsh[kk][c], tc * cw + c, ig0 + kk, G, nm, nmin, B, NG,
+                       Wd, Wdd);
+    }
+  }
+}
+
+// Adds the staged nodes kbeg, kbeg + kstep, ... < kmax of one LDS stage to
+// the MT x MT micro-tile of pairs (rows rr, columns cc: indices into the
+// staged row and column factors) of a thread; odd selects the odd-pair
+// factors.  Lb/Rb: staged row/column factors, lnode/rnode doubles per node.
+template <int MT>
+__device__ __forceinline__ void accumulate_nodes(
+    const double *Lb, int lnode, const double *Rb, int rnode, int kbeg,
+    int kmax, int kstep, bool odd, const int (&rr)[MT], const int (&cc)[MT],
+    cdbl (&Ja)[MT][MT], cdbl (&Jb)[MT][MT], cdbl (&RJa)[MT][MT],
+    cdbl (&RJb)[MT][MT]) {
+  if (!odd) {
+    for (int kk = kbeg; kk < kmax; kk += kstep) {
+      cdbl R1[MT], R2[MT], R3[MT], R4[MT], R5[MT];
 #pragma unroll
-        for (int i = 0; i < 10; ++i) R[i] = 0.;
+      for (int b = 0; b < MT; ++b) {
+        const double *R = Rb + kk * rnode + cc[b] * kRightStride;
+        R1[b] = cmk(R[0], R[1]); R2[b] = cmk(R[2], R[3]);
+        R3[b] = cmk(R[4], R[5]); R4[b] = cmk(R[6], R[7]);
+        R5[b] = cmk(R[8], R[9]);
+      }
+#pragma unroll
+      for (int a = 0; a < MT; ++a) {
+        const double *L = Lb + kk * lnode + rr[a] * kLeftStride;
+        const cdbl L1 = cmk(L[0], L[1]), L2 = cmk(L[2], L[3]);
+        const cdbl L3 = cmk(L[4], L[5]), L4 = cmk(L[6], L[7]);
+        const cdbl L5 = cmk(L[8], L[9]), L8 = cmk(L[12], L[13]);
+        const double l6 = L[10], l7 = L[11], l9 = L[14], l10 = L[15];
+#pragma unroll
+        for (int b = 0; b < MT; ++b) {
+          cfma(Ja[a][b], L1, R1[b]);
+          cfma(Ja[a][b], L2, R2[b]);
+          cfma(Ja[a][b], L3, R3[b]);
+          cfma(Jb[a][b], L4, R4[b]);
+          cfma(Jb[a][b], L5, R5[b]);
+          rfma(RJa[a][b], l6, R1[b]);
+          rfma(RJa[a][b], l7, R2[b]);
+          cfma(RJa[a][b], L8, R3[b]);
+          rfma(RJb[a][b], l9, R4[b]);
+          rfma(RJb[a][b], l10, R5[b]);
+        }
       }
     }
+  } else {
+    for (int kk = kbeg; kk < kmax; kk += kstep) {
+      cdbl R1[MT], R2[MT], R3[MT], R4[MT], R5[MT];
+#pragma unroll
+      for (int b = 0; b < MT; ++b) {
+        const double *R = Rb + kk * rnode + cc[b] * kRightStride;
+        R1[b] = cmk(R[0], R[1]); R2[b] = cmk(R[2], R[3]);
+        R3[b] = cmk(R[4], R[5]); R4[b] = cmk(R[6], R[7]);
+        R5[b] = cmk(R[8], R[9]);
+      }
+#pragma unroll
+      for (int a = 0; a < MT; ++a) {
+        const double *L = Lb + kk * lnode + rr[a] * kLeftStride + 16;
+        const cdbl L1 = cmk(L[0], L[1]), L2 = cmk(L[2], L[3]);
+        const cdbl L3 = cmk(L[4], L[5]), L4 = cmk(L[6], L[7]);
+        const cdbl L5 = cmk(L[8], L[9]), L8 = cmk(L[14], L[15]);
+        const double l6 = L[10], l7 = L[11], l9 = L[12], l10 = L[13];
+#pragma unroll
+        for (int b = 0; b < MT; ++b) {
+          cfma(Ja[a][b], L1, R5[b]);
+          cfma(Ja[a][b], L2, R4[b]);
+          cfma(Jb[a][b], L3, R2[b]);
+          cfma(Jb[a][b], L4, R1[b]);
+          cfma(Jb[a][b], L5, R3[b]);
+          rfma(RJa[a][b], l6, R5[b]);
+          rfma(RJa[a][b], l7, R4[b]);
+          rfma(RJb[a][b], l9, R2[b]);
+          rfma(RJb[a][b], l10, R1[b]);
+          cfma(RJb[a][b], L8, R3[b]);
+        }
+      }
     }
+  }
+}
+
+// Writes the micro-tile's pairs (block rows pr, columns pc; those beyond the
+// block skipped) of Q and RgQ into the block's class layout: Qx = k^2 m J_a +
+// k^2 J_b to row class n1 mod 2, Qy = k^2 m J_b + k^2 J_a to the other
+// (odd pairs carry a factor i).
+template <int MT>
+__device__ __forceinline__ void store_pairs(
+    const GrainParams &gp, int grain, int nm, int nmin, cdbl *Q,
+    const int (&pr)[MT], const int (&pc)[MT], bool odd,
+    const cdbl (&Ja)[MT][MT], const cdbl (&Jb)[MT][MT],
+    const cdbl (&RJa)[MT][MT], const cdbl (&RJb)[MT][MT]) {
+  const double lam = gp.wavelength[grain];
+  const double k = 2. * M_PI / lam;
+  const double k2 = k * k;
+  const cdbl k2mr = cscale(cmk(gp.mr_re[grain], gp.mr_im[grain]), k2);
+  const int64_t nm2 = (int64_t)nm * nm;
+#pragma unroll
+  for (int a = 0; a < MT; ++a) {
+#pragma unroll
+    for (int b = 0; b < MT; ++b) {
+      const int p1 = pr[a];
+      const int p2 = pc[b];
+      if (p1 >= nm || p2 >= nm) continue;
+      cdbl ja = Ja[a][b], jb = Jb[a][b], rja = RJa[a][b], rjb = RJb[a][b];
+      if (odd) {
+        // multiply by i: (x, y) -> (-y, x)
+        ja = cmk(-ja.y, ja.x);
+        jb = cmk(-jb.y, jb.x);
+        rja = cmk(-rja.y, rja.x);
+        rjb = cmk(-rjb.y, rjb.x);
+      }
+      const cdbl Qx = cadd(cmul(k2mr, ja), cscale(jb, k2));
+      const cdbl Qy = cadd(cmul(k2mr, jb), cscale(ja, k2));
+      const cdbl RQx = cadd(cmul(k2mr, rja), cscale(rjb, k2));
+      const cdbl RQy = cadd(cmul(k2mr, rjb), cscale(rja, k2));
+      const int n1 = nmin + p1;
+      const int cx = n1 & 1;
+      const int64_t e = (int64_t)p1 * nm + p2;
+      Q[cx * nm2 + e] = Qx;
+      Q[(1 - cx) * nm2 + e] = Qy;
+      Q[(2 + cx) * nm2 + e] = RQx;
+      Q[(2 + 1 - cx) * nm2 + e] = RQy;
+    }
+  }
 }
 
 template <int MT>
@@ -516,71 +650,9 @@ __global__ __launch_bounds__(256) void k_assemble(
                           S, B, NG, Wd, Wdd);
     __syncthreads();
     const int kmax = (G - ig0) < CH ? (G - ig0) : CH;
-    if (!odd) {
-      for (int kk = sg; kk < kmax; kk += ns) {
-        cdbl R1[MT], R2[MT], R3[MT], R4[MT], R5[MT];
-#pragma unroll
-        for (int b = 0; b < MT; ++b) {
-          const double *R = Rsh[kk][cc[b]];
-          R1[b] = cmk(R[0], R[1]); R2[b] = cmk(R[2], R[3]);
-          R3[b] = cmk(R[4], R[5]); R4[b] = cmk(R[6], R[7]);
-          R5[b] = cmk(R[8], R[9]);
-        }
-#pragma unroll
-        for (int a = 0; a < MT; ++a) {
-          const double *L = Lsh[kk][rr[a]];
-          const cdbl L1 = cmk(L[0], L[1]), L2 = cmk(L[2], L[3]);
-          const cdbl L3 = cmk(L[4], L[5]), L4 = cmk(L[6], L[7]);
-          const cdbl L5 = cmk(L[8], L[9]), L8 = cmk(L[12], L[13]);
-          const double l6 = L[10], l7 = L[11], l9 = L[14], l10 = L[15];
-#pragma unroll
-          for (int b = 0; b < MT; ++b) {
-            cfma(Ja[a][b], L1, R1[b]);
-            cfma(Ja[a][b], L2, R2[b]);
-            cfma(Ja[a][b], L3, R3[b]);
-            cfma(Jb[a][b], L4, R4[b]);
-            cfma(Jb[a][b], L5, R5[b]);
-            rfma(RJa[a][b], l6, R1[b]);
-            rfma(RJa[a][b], l7, R2[b]);
-            cfma(RJa[a][b], L8, R3[b]);
-            rfma(RJb[a][b], l9, R4[b]);
-            rfma(RJb[a][b], l10, R5[b]);
-          }
-        }
-      }
-    } else {
-      for (int kk = sg; kk < kmax; kk += ns) {
-        cdbl R1[MT], R2[MT], R3[MT], R4[MT], R5[MT];
-#pragma unroll
-        for (int b = 0; b < MT; ++b) {
-          const double *R = Rsh[kk][cc[b]];
-          R1[b] = cmk(R[0], R[1]); R2[b] = cmk(R[2], R[3]);
-          R3[b] = cmk(R[4], R[5]); R4[b] = cmk(R[6], R[7]);
-          R5[b] = cmk(R[8], R[9]);
-        }
-#pragma unroll
-        for (int a = 0; a < MT; ++a) {
-          const double *L = Lsh[kk][rr[a]] + 16;
-          const cdbl L1 = cmk(L[0], L[1]), L2 = cmk(L[2], L[3]);
-          const cdbl L3 = cmk(L[4], L[5]), L4 = cmk(L[6], L[7]);
-          const cdbl L5 = cmk(L[8], L[9]), L8 = cmk(L[14], L[15]);
-          const double l6 = L[10], l7 = L[11], l9 = L[12], l10 = L[13];
-#pragma unroll
-          for (int b = 0; b < MT; ++b) {
-            cfma(Ja[a][b], L1, R5[b]);
-            cfma(Ja[a][b], L2, R4[b]);
-            cfma(Jb[a][b], L3, R2[b]);
-            cfma(Jb[a][b], L4, R1[b]);
-            cfma(Jb[a][b], L5, R3[b]);
-            rfma(RJa[a][b], l6, R5[b]);
-            rfma(RJa[a][b], l7, R4[b]);
-            rfma(RJb[a][b], l9, R2[b]);
-            rfma(RJb[a][b], l10, R1[b]);
-            cfma(RJb[a][b], L8, R3[b]);
-          }
-        }
-      }
-    }
+    accumulate_nodes<MT>(&Lsh[0][0][0], TR * kLeftStride, &Rsh[0][0][0],
+                         2 * TR * kRightStride, sg, kmax, ns, odd, rr, cc, Ja,
+                         Jb, RJa, RJb);
   }
 
   if (MT == 1 && ns > 1) {
@@ -614,40 +686,161 @@ __global__ __launch_bounds__(256) void k_assemble(
     RJb[0][0] = cmk(v[6], v[7]);
     if (sg != 0) return;
   }
-  const double lam = gp.wavelength[it.grain];
-  const double k = 2. * M_PI / lam;
-  const double k2 = k * k;
-  const cdbl k2mr = cscale(cmk(gp.mr_re[it.grain], gp.mr_im[it.grain]), k2);
-  const int64_t nm2 = (int64_t)nm * nm;
-  cdbl *Q = qbuf + it.q_off;
+  int pr[MT], pc[MT];
 #pragma unroll
   for (int a = 0; a < MT; ++a) {
-#pragma unroll
-    for (int b = 0; b < MT; ++b) {
-      const int p1 = tr * TR + rr[a];
-      const int p2 = tc * cw + cc[b];
-      if (p1 >= nm || p2 >= nm) continue;
-      cdbl ja = Ja[a][b], jb = Jb[a][b], rja = RJa[a][b], rjb = RJb[a][b];
-      if (odd) {
-        // multiply by i: (x, y) -> (-y, x)
-        ja = cmk(-ja.y, ja.x);
-        jb = cmk(-jb.y, jb.x);
-        rja = cmk(-rja.y, rja.x);
-        rjb = cmk(-rjb.y, rjb.x);
-      }
-      const cdbl Qx = cadd(cmul(k2mr, ja), cscale(jb, k2));
-      const cdbl Qy = cadd(cmul(k2mr, jb), cscale(ja, k2));
-      const cdbl RQx = cadd(cmul(k2mr, rja), cscale(rjb, k2));
-      const cdbl RQy = cadd(cmul(k2mr, rjb), cscale(rja, k2));
-      const int n1 = nmin + p1;
-      const int cx = n1 & 1;
-      const int64_t e = (int64_t)p1 * nm + p2;
-      Q[cx * nm2 + e] = Qx;
-      Q[(1 - cx) * nm2 + e] = Qy;
-      Q[(2 + cx) * nm2 + e] = RQx;
-      Q[(2 + 1 - cx) * nm2 + e] = RQy;
-    }
+    pr[a] = tr * TR + rr[a];
+    pc[a] = tc * cw + cc[a];
   }
+  store_pairs<MT>(gp, it.grain, nm, nmin, qbuf + it.q_off, pr, pc, odd, Ja,
+                  Jb, RJa, RJb);
+}
+
+// K3 for blocks of order up to kPackRows: four blocks per workgroup, each on
+// a kPackRows x kPackRows tile of 3 x 3 pairs per thread (the cheaper
+// micro-tile per pair), so such a block fills a larger share of its tile than
+// of a 32 x 32 or 16 x 16 one.  Lane (lr, lc) of a wave serves block
+// 2 (lr / 4) + lc / 4 at rows 2 (lr mod 4 + 4 a) + prow and columns
+// 2 (lc mod 4 + 4 b) + pcol; wave w takes the parity class (w & 1, w >> 1)
+// for m > 0 as in k_assemble.  For m = 0 only the classes (0, 0) and (1, 1)
+// exist: waves 2 and 3 repeat waves 0 and 1 on the odd nodes of each stage
+// and their sums are added at the end.  The four blocks (consecutive in the
+// list, which is sorted by G) are staged together, two nodes per stage;
+// nodes beyond a block's G stage as zero.
+struct PackBlock {
+  const double *S, *B, *Wd, *Wdd;
+  int64_t NG;
+  int nm, nmin, G, m, item, pad;
+};
+
+__global__ __launch_bounds__(256, 2) void k_assemble_pack4(
+    const int32_t *order, int n_order, int m0, const Item *items,
+    const TableSet *sets, const WigKey *wkeys, const GrainParams gp,
+    const double *tables, const double *wig, cdbl *qbuf) {
+  constexpr int MT = 3, CH = 2, NB = 4, PR = kPackRows;
+  __shared__ double Lsh[CH][NB * PR][kLeftStride];
+  __shared__ double Rsh[CH][NB * PR][kRightStride];
+  __shared__ PackBlock blk[NB];
+  const int t = threadIdx.x;
+  if (t < NB) {
+    const int q = blockIdx.x * NB + t;
+    PackBlock pb{};
+    pb.item = -1;
+    if (q < n_order) {
+      const Item it = items[order[q]];
+      const TableSet ts = sets[it.set];
+      const WigKey wk = wkeys[it.wig];
+      pb.m = it.m;
+      pb.nmin = it.m > 0 ? it.m : 1;
+      pb.nm = it.N + 1 - pb.nmin;
+      pb.G = it.G;
+      pb.S = tables + ts.off;
+      pb.B = pb.S + (int64_t)kNodeScalars * it.G;
+      pb.NG = (int64_t)it.N * it.G;
+      pb.Wd = wig + wk.off;
+      pb.Wdd = pb.Wd + (int64_t)pb.nm * it.G;
+      pb.item = order[q];
+    }
+    blk[t] = pb;
+  }
+  __syncthreads();
+  const int gmax = max(max(blk[0].G, blk[1].G), max(blk[2].G, blk[3].G));
+  const int wv = t >> 6, lane = t & 63;
+  const int lr = lane >> 3, lc = lane & 7;
+  const int b = ((lr >> 2) << 1) | (lc >> 2);
+  const int prow = wv & 1;
+  const int pcol = m0 ? prow : (wv >> 1);
+  const int sg = m0 ? (wv >> 1) : 0, ns = m0 ? 2 : 1;
+  int rr[MT], cc[MT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a) {
+    rr[a] = b * PR + 2 * ((lr & 3) + 4 * a) + prow;
+    cc[a] = b * PR + 2 * ((lc & 3) + 4 * a) + pcol;
+  }
+  const bool odd = !m0 && ((prow ^ pcol) != 0);
+  cdbl Ja[MT][MT], Jb[MT][MT], RJa[MT][MT], RJb[MT][MT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a)
+#pragma unroll
+    for (int c = 0; c < MT; ++c) {
+      Ja[a][c] = cmk(0., 0.);
+      Jb[a][c] = cmk(0., 0.);
+      RJa[a][c] = cmk(0., 0.);
+      RJb[a][c] = cmk(0., 0.);
+    }
+  for (int ig0 = 0; ig0 < gmax; ig0 += CH) {
+    __syncthreads();
+    for (int e = t; e < 2 * NB * PR * CH; e += 256) {
+      if (e < NB * PR * CH) {
+        const int r = e / CH, kk = e - (e / CH) * CH;
+        const int bb = r / PR, p = r - bb * PR;
+        const PackBlock &pb = blk[bb];
+        row_factor_entry(Lsh[kk][r], p, ig0 + kk, pb.G, pb.nm, pb.nmin,
+                         (double)pb.m, pb.S, pb.B, pb.NG, pb.Wd, pb.Wdd);
+      } else {
+        const int tt = e - NB * PR * CH;
+        const int c = tt / CH, kk = tt - (tt / CH) * CH;
+        const int bb = c / PR, p = c - bb * PR;
+        const PackBlock &pb = blk[bb];
+        col_factor_entry(Rsh[kk][c], p, ig0 + kk, pb.G, pb.nm, pb.nmin, pb.B,
+                         pb.NG, pb.Wd, pb.Wdd);
+      }
+    }
+    __syncthreads();
+    const int kmax = (gmax - ig0) < CH ? (gmax - ig0) : CH;
+    accumulate_nodes<MT>(&Lsh[0][0][0], NB * PR * kLeftStride, &Rsh[0][0][0],
+                         NB * PR * kRightStride, sg, kmax, ns, odd, rr, cc, Ja,
+                         Jb, RJa, RJb);
+  }
+  if (m0) {
+    // add the odd-node sums of waves 2 and 3 to waves 0 and 1 (J, then RgJ)
+    double *red = &Lsh[0][0][0];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      __syncthreads();
+      if (wv >= 2) {
+#pragma unroll
+        for (int a = 0; a < MT; ++a)
+#pragma unroll
+          for (int c = 0; c < MT; ++c) {
+            const cdbl u = half ? RJa[a][c] : Ja[a][c];
+            const cdbl v = half ? RJb[a][c] : Jb[a][c];
+            const int q = (a * MT + c) * 4;
+            red[(q + 0) * 128 + t - 128] = u.x;
+            red[(q + 1) * 128 + t - 128] = u.y;
+            red[(q + 2) * 128 + t - 128] = v.x;
+            red[(q + 3) * 128 + t - 128] = v.y;
+          }
+      }
+      __syncthreads();
+      if (wv < 2) {
+#pragma unroll
+        for (int a = 0; a < MT; ++a)
+#pragma unroll
+          for (int c = 0; c < MT; ++c) {
+            const int q = (a * MT + c) * 4;
+            cdbl &u = half ? RJa[a][c] : Ja[a][c];
+            cdbl &v = half ? RJb[a][c] : Jb[a][c];
+            u.x += red[(q + 0) * 128 + t];
+            u.y += red[(q + 1) * 128 + t];
+            v.x += red[(q + 2) * 128 + t];
+            v.y += red[(q + 3) * 128 + t];
+          }
+      }
+    }
+    if (wv >= 2) return;
+  }
+  const PackBlock &pb = blk[b];
+  if (pb.item < 0) return;
+  const Item it = items[pb.item];
+  int pr[MT], pc[MT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a) {
+    pr[a] = rr[a] - b * PR;
+    pc[a] = cc[a] - b * PR;
+  }
+  store_pairs<MT>(gp, it.grain, pb.nm, pb.nmin, qbuf + it.q_off, pr, pc, odd,
+                  Ja, Jb, RJa, RJb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1630,6 +1823,18 @@ hipError_t launch_assemble(int micro, const int32_t *order,
     hipLaunchKernelGGL(k_assemble<1>, dim3(n_tiles), dim3(256), 0, s, order,
                        tile_prefix, n_order, items, sets, wkeys, gp, tables,
                        wig, qbuf);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble_pack4(const int32_t *order, int n_order, int m0,
+                                 const Item *items, const TableSet *sets,
+                                 const WigKey *wkeys, const GrainParams &gp,
+                                 const double *tables, const double *wig,
+                                 cdbl *qbuf, hipStream_t s) {
+  if (n_order <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_assemble_pack4, dim3((n_order + 3) / 4), dim3(256), 0,
+                     s, order, n_order, m0, items, sets, wkeys, gp, tables,
+                     wig, qbuf);
   return hipGetLastError();
 }
 

@@ -25,11 +25,22 @@ hipError_t launch_tables(const TableSet *sets, int n, int n_nodes,
 // micro = 2: 32 x 32 tiles of 2 x 2 pairs per thread; micro = 3: 48 x 48
 // tiles of 3 x 3 pairs per thread.
 constexpr int kSmallBlock = 16;
-// Micro-tile of a block of order nm: 1 up to kSmallBlock, else 3 (48-row
+// Blocks of order up to kPackRows, four per workgroup (order: their item
+// indices, all with m = 0 when m0 is set, else all with m > 0).
+constexpr int kPackRows = 24;
+constexpr int kPackMin = 9;  // smallest order assembled four per workgroup
+hipError_t launch_assemble_pack4(const int32_t *order, int n_order, int m0,
+                                 const Item *items, const TableSet *sets,
+                                 const WigKey *wkeys, const GrainParams &gp,
+                                 const double *tables, const double *wig,
+                                 cdbl *qbuf, hipStream_t s);
+// Micro-tile of a block of order nm: 4 (four blocks per workgroup on
+// kPackRows-row tiles) from kPackMin to kPackRows, 1 below, else 3 (48-row
 // tiles) where their padded pair count, at half the cost per pair of
 // 32-row tiles (measured: the 3 x 3 micro-tile reads fewer LDS operands per
 // pair), is smaller, else 2.  (m = 0 tiles are twice as wide.)
 inline int assemble_micro(int nm, bool m0) {
+  if (nm >= kPackMin && nm <= kPackRows) return 4;
   if (nm <= kSmallBlock) return 1;
   const long r32 = (nm + 31) / 32, c32 = m0 ? (nm + 63) / 64 : r32;
   const long r48 = (nm + 47) / 48, c48 = m0 ? (nm + 95) / 96 : r48;

@@ -277,7 +277,8 @@ int ctm_orientation_coefficients(ctm_real_fn p, void *arg, uint32_t order,
 #define CTM_KERNEL_AVERAGE 9
 #define CTM_KERNEL_ASSEMBLE_SMALL 10 /* blocks of order <= 16 */
 #define CTM_KERNEL_ASSEMBLE_WIDE 11  /* blocks on 48-row tiles */
-#define CTM_KERNEL_COUNT 12
+#define CTM_KERNEL_ASSEMBLE_PACK 12  /* blocks of order 9-24, four per group */
+#define CTM_KERNEL_COUNT 13
 
 typedef struct ctm_kernel_stats {
   uint64_t launches;
