@@ -526,14 +526,23 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
                                    sizeof(cdbl)));
     if (end > pos) {
       double fs = 0., bs = 0.;
+      int64_t store_max = 0;
+      int rows_max = 0;
       for (size_t q = pos; q < end; ++q) {
         const Item &it = w.items[w.solve[q] >> 1];
-        const double nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+        const int nmi = it.N + 1 - (it.m > 0 ? it.m : 1);
+        const double nm = nmi;
         fs += (it.m > 0 ? 32. / 3. : 8. / 3.) * nm * nm * nm;
         bs += 16. * 2. * nm * nm + (want_t ? 16. * nm * nm : 0.);
+        // m = 0: two parity sub-blocks of (nm + 1) / 2 and nm / 2 rows
+        const int64_t r0 = it.m == 0 && nmi > 1 ? (nmi + 1) / 2 : nmi;
+        const int64_t r1 = nmi - r0;
+        store_max = std::max(store_max, r0 * r0 + r1 * r1);
+        rows_max = std::max(rows_max, (int)r0);
       }
       CTM_LAUNCH(CTM_KERNEL_SOLVE, fs, bs,
                  launch_invert_solve(dsolve + pos, (int)(end - pos), nm_max,
+                                     store_max, rows_max,
                                      ctx->d_items.as<Item>(),
                                      ctx->d_q.as<cdbl>(),
                                      want_t ? ctx->d_t.as<cdbl>() : nullptr,

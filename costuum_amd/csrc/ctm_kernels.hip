@@ -867,7 +867,8 @@ __global__ __launch_bounds__(256, 2) void k_assemble_pack4(
 template <int NT>
 __device__ __forceinline__ void solve_tail(
     const Item &it, int task, int cls, int nm, int nb, int sz0,
-    const int *off, const int *sz, const cdbl *A, const int *piv, int *src,
+    const int *off, const int *sz, const cdbl *A0, int ld0, const cdbl *A1,
+    int ld1, const int *piv, int *src,
     const cdbl *RQg, cdbl *tbuf, double *qparts, int diag_only,
     double (*red)[4], cdbl *ainv_out) {
   constexpr int NW = NT / 64;
@@ -879,6 +880,13 @@ __device__ __forceinline__ void solve_tail(
   };
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
+  // element (k, c) of sub-block b: sub-block b starts at Ab[b] (its element
+  // (off[b], off[b])) with row stride ld[b]
+  const cdbl *Ab[2] = {A0, A1};
+  const int ld[2] = {ld0, ld1};
+  auto aget = [&](int b, int k, int c) -> cdbl {
+    return Ab[b][(int64_t)(k - off[b]) * ld[b] + (c - off[b])];
+  };
   if (t == 0) {
     for (int j = 0; j < nm; ++j) src[j] = j;
     for (int k = nm - 1; k >= 0; --k) {
@@ -899,7 +907,7 @@ __device__ __forceinline__ void solve_tail(
       for (int c = t & 31; c < nm; c += 32) {
         const bool same = nb == 1 || (c >= sz0) == bk;
         Ai[(int64_t)pidx(k) * nm + pidx(c)] =
-            same ? A[(int64_t)k * nm + src[c]] : cmk(0., 0.);
+            same ? aget(bk ? 1 : 0, k, src[c]) : cmk(0., 0.);
       }
     }
   }
@@ -912,9 +920,10 @@ __device__ __forceinline__ void solve_tail(
       const int pi = pidx(i);
       const cdbl *rq = RQg + (int64_t)pi * nm;
       const int sc = src[i];
+      const int bi = (nb == 2 && i >= sz0) ? 1 : 0;
       cdbl acc = cmk(0., 0.);
       for (int k = klo; k < khi; ++k)
-        cfma(acc, rq[pidx(k)], A[(int64_t)k * nm + sc]);
+        cfma(acc, rq[pidx(k)], aget(bi, k, sc));
       const double w = 2. * (nmin + pi) + 1.;
       const double a2 = acc.x * acc.x + acc.y * acc.y;
       s_diag2 += w * a2;
@@ -941,7 +950,7 @@ __device__ __forceinline__ void solve_tail(
       for (int cb = lo; cb < hi; cb += 128) {
       const int c0 = cb + lane, c1 = cb + lane + 64;
       const bool v0 = c0 < hi, v1 = c1 < hi;
-      const int s0 = v0 ? src[c0] : 0, s1 = v1 ? src[c1] : 0;
+      const int s0 = v0 ? src[c0] : lo, s1 = v1 ? src[c1] : lo;
       for (int rg = lo + 4 * wv; rg < hi; rg += 4 * NW) {
         cdbl acc[4][2];
         int prow[4];
@@ -954,8 +963,8 @@ __device__ __forceinline__ void solve_tail(
         }
         for (int k = lo; k < hi; ++k) {
           const int pk = pidx(k);
-          const cdbl a0 = A[(int64_t)k * nm + s0];
-          const cdbl a1 = A[(int64_t)k * nm + s1];
+          const cdbl a0 = aget(b, k, s0);
+          const cdbl a1 = aget(b, k, s1);
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const cdbl bq = RQg[(int64_t)prow[q] * nm + pk];
@@ -1185,8 +1194,9 @@ __global__ __launch_bounds__(NT) void k_invert_solve(
     }
     __syncthreads();
   }
-  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, piv, src, RQg, tbuf,
-                 qparts, diag_only, red, ainv_out);
+  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, nm,
+                 nb == 2 ? A + (int64_t)sz0 * nm + sz0 : A, nm, piv, src, RQg,
+                 tbuf, qparts, diag_only, red, ainv_out);
 }
 
 // K4 for blocks up to order 48: the same elimination (pivot rule, swaps,
@@ -1419,8 +1429,9 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
     }
   }
   __syncthreads();
-  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, piv, src, RQg, tbuf,
-                 qparts, diag_only, red, ainv_out);
+  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, nm,
+                 nb == 2 ? A + (int64_t)sz0 * nm + sz0 : A, nm, piv, src, RQg,
+                 tbuf, qparts, diag_only, red, ainv_out);
 }
 
 // K4 for orders 49..99 with the matrix in LDS: blocked Gauss-Jordan.  The
@@ -1438,7 +1449,8 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
 // panel's columns, while the other waves of the sub-block update the rest;
 // one barrier separates panels.  Same pivot rule as k_invert_solve (largest
 // |a_ik|^2 over rows i >= k, first row on ties); results equal it to
-// rounding, and the A / piv layout handed to solve_tail is the same.
+// rounding.  Each sub-block is stored compactly (sz x sz, one after the
+// other), so an m = 0 block holds half the LDS of its full square.
 __device__ __forceinline__ double readlane_f64(double v, int l) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, l);
@@ -1481,7 +1493,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(
 }
 
 template <int RS, int W>
-__device__ __forceinline__ void blk_panel(cdbl *A, int nm, int lo, int hi,
+__device__ __forceinline__ void blk_panel(cdbl *Ab, int ld, int lo, int hi,
                                           int c0, int w, int lane, int *piv,
                                           int *steps) {
   cdbl P[RS][W];
@@ -1490,7 +1502,7 @@ __device__ __forceinline__ void blk_panel(cdbl *A, int nm, int lo, int hi,
     const int i = lo + lane + 64 * r;
 #pragma unroll
     for (int j = 0; j < W; ++j)
-      P[r][j] = (i < hi && j < w) ? A[(int64_t)i * nm + c0 + j] : cmk(0., 0.);
+      P[r][j] = (i < hi && j < w) ? Ab[(int64_t)(i - lo) * ld + (c0 - lo) + j] : cmk(0., 0.);
   }
 #pragma unroll
   for (int s = 0; s < W; ++s) {
@@ -1568,7 +1580,7 @@ __device__ __forceinline__ void blk_panel(cdbl *A, int nm, int lo, int hi,
     if (i >= hi) continue;
 #pragma unroll
     for (int j = 0; j < W; ++j)
-      if (j < w) A[(int64_t)i * nm + c0 + j] = P[r][j];
+      if (j < w) Ab[(int64_t)(i - lo) * ld + (c0 - lo) + j] = P[r][j];
   }
 }
 
@@ -1576,7 +1588,7 @@ __device__ __forceinline__ void blk_panel(cdbl *A, int nm, int lo, int hi,
 // sub-block [lo, hi), skipping [skip0, skip1), by one wave, with the panel
 // at columns c0..c0+w-1 and its pivot rows steps[0..w).
 template <int RS, int W>
-__device__ __forceinline__ void blk_update(cdbl *A, int nm, int lo, int hi,
+__device__ __forceinline__ void blk_update(cdbl *Ab, int ld, int lo, int hi,
                                            int c0, int w, const int *steps,
                                            int jbeg, int jend, int jstep,
                                            int skip0, int skip1, int lane) {
@@ -1603,26 +1615,26 @@ __device__ __forceinline__ void blk_update(cdbl *A, int nm, int lo, int hi,
     sg[r] = i < hi ? pos : lo;
 #pragma unroll
     for (int q = 0; q < W; ++q)
-      F[r][q] = (i < hi && q < w) ? A[(int64_t)i * nm + c0 + q] : cmk(0., 0.);
+      F[r][q] = (i < hi && q < w) ? Ab[(int64_t)(i - lo) * ld + (c0 - lo) + q] : cmk(0., 0.);
   }
   for (int j = jbeg; j < jend; j += jstep) {
     if (j >= skip0 && j < skip1) continue;
     cdbl po[W];
 #pragma unroll
-    for (int q = 0; q < W; ++q) po[q] = A[(int64_t)sq[q] * nm + j];
+    for (int q = 0; q < W; ++q) po[q] = Ab[(int64_t)(sq[q] - lo) * ld + (j - lo)];
     cdbl acc[RS];
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
       const int i = lo + lane + 64 * r;
       const bool prow = i >= c0 && i < c0 + w;
-      acc[r] = prow ? cmk(0., 0.) : A[(int64_t)sg[r] * nm + j];
+      acc[r] = prow ? cmk(0., 0.) : Ab[(int64_t)(sg[r] - lo) * ld + (j - lo)];
 #pragma unroll
       for (int q = 0; q < W; ++q) cfma(acc[r], F[r][q], po[q]);
     }
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
       const int i = lo + lane + 64 * r;
-      if (i < hi) A[(int64_t)i * nm + j] = acc[r];
+      if (i < hi) Ab[(int64_t)(i - lo) * ld + (j - lo)] = acc[r];
     }
   }
 }
@@ -1652,21 +1664,27 @@ __global__ __launch_bounds__(NT) void k_invert_solve_blk(
   auto pidx = [&](int i) {
     return nb == 2 ? (i < sz0 ? 2 * i : 2 * (i - sz0) + 1) : i;
   };
+  // the sub-blocks stored one after the other, each sz[b] x sz[b] (for
+  // m = 0 half the LDS of the full block, whose cross blocks are zero)
   cdbl *A = reinterpret_cast<cdbl *>(smem);
-  int *piv = reinterpret_cast<int *>(A + nm2);
+  cdbl *Ab[2] = {A, A + (int64_t)sz0 * sz0};
+  const int ldb[2] = {sz[0], sz[1]};
+  const int64_t nstore = (int64_t)sz[0] * sz[0] + (int64_t)sz[1] * sz[1];
+  int *piv = reinterpret_cast<int *>(A + nstore);
   int *src = piv + nm;
   const cdbl *Qg = qbuf + it.q_off + cls * nm2;
   const cdbl *RQg = qbuf + it.q_off + (2 + cls) * nm2;
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
 
-  for (int i = t >> 5; i < nm; i += NT >> 5) {
-    const int pi = pidx(i);
-    const int bi = i >= sz0;
-    for (int j = t & 31; j < nm; j += 32) {
-      const int bj = j >= sz0;
-      A[(int64_t)i * nm + j] =
-          (bi == bj) ? Qg[(int64_t)pi * nm + pidx(j)] : cmk(0., 0.);
+#pragma unroll
+  for (int bb = 0; bb < 2; ++bb) {
+    if (bb >= nb) continue;
+    const int o = off[bb], n = sz[bb];
+    for (int i = t >> 5; i < n; i += NT >> 5) {
+      const int pi = pidx(o + i);
+      for (int j = t & 31; j < n; j += 32)
+        Ab[bb][(int64_t)i * n + j] = Qg[(int64_t)pi * nm + pidx(o + j)];
     }
   }
   __syncthreads();
@@ -1677,8 +1695,10 @@ __global__ __launch_bounds__(NT) void k_invert_solve_blk(
   const int lo = off[b], hi = off[b] + sz[b];
   const int npan = (sz[b] + W - 1) / W;
   const int npmax = (sz[0] + W - 1) / W;
+  cdbl *Av = Ab[b];
+  const int ld = ldb[b];
   if (gw == 0)
-    blk_panel<RS, W>(A, nm, lo, hi, lo, min(W, sz[b]), lane, piv, stp[0][b]);
+    blk_panel<RS, W>(Av, ld, lo, hi, lo, min(W, sz[b]), lane, piv, stp[0][b]);
   __syncthreads();
   for (int P = 0; P < npmax; ++P) {
     if (P < npan) {
@@ -1687,21 +1707,22 @@ __global__ __launch_bounds__(NT) void k_invert_solve_blk(
       const int *steps = stp[P & 1][b];
       const bool ahead = P + 1 < npan;
       if (ahead && gw == 0) {
-        blk_update<RS, W>(A, nm, lo, hi, c0, w, steps, c1, c1 + w1, 1, 0, 0,
+        blk_update<RS, W>(Av, ld, lo, hi, c0, w, steps, c1, c1 + w1, 1, 0, 0,
                           lane);
-        blk_panel<RS, W>(A, nm, lo, hi, c1, w1, lane, piv,
+        blk_panel<RS, W>(Av, ld, lo, hi, c1, w1, lane, piv,
                          stp[(P + 1) & 1][b]);
       } else {
         // columns outside the panel (and the next panel, if any)
         const int nwk = ahead ? GW - 1 : GW, rank = ahead ? gw - 1 : gw;
-        blk_update<RS, W>(A, nm, lo, hi, c0, w, steps, lo + rank, hi, nwk,
+        blk_update<RS, W>(Av, ld, lo, hi, c0, w, steps, lo + rank, hi, nwk,
                           c0, ahead ? c1 + w1 : c1, lane);
       }
     }
     __syncthreads();
   }
-  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, piv, src, RQg, tbuf,
-                 qparts, diag_only, red, ainv_out);
+  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, Ab[0], ldb[0], Ab[1],
+                 ldb[1], piv, src, RQg, tbuf, qparts, diag_only, red,
+                 ainv_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1901,6 +1922,7 @@ static hipError_t launch_solve_blk(const int32_t *tasks, int n, size_t lds,
 }
 
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
+                               int64_t store_max, int rows_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
                                cdbl *ainv_out, hipStream_t s) {
@@ -1913,8 +1935,10 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                       qparts, diag_only, gscratch,
                                       (int64_t)nm_max * nm_max, ainv_out, s);
   }
+  // the blocked kernel stores the sub-blocks compactly (store_max complex
+  // values at most) and needs two rows per lane above 64 rows
   const size_t lds_blk =
-      (size_t)nm_max * nm_max * sizeof(cdbl) + 2 * (size_t)nm_max * sizeof(int);
+      (size_t)store_max * sizeof(cdbl) + 2 * (size_t)nm_max * sizeof(int);
   // register-resident elimination: grid and elements per thread by size
   const size_t lds = invert_solve_lds_bytes(nm_max);
   if (nm_max <= 16)
@@ -1922,11 +1946,11 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                          qparts, diag_only, ainv_out, s);
   // above order 16: blocked elimination in LDS with panels of 4 columns
   // (measured faster than 2, 6, 8 or 16), four waves per block up to order
-  // 48, eight above, two rows per lane above order 64
+  // 48, eight above, two rows per lane for sub-blocks above 64 rows
   if (nm_max <= 48)
     return launch_solve_blk<256, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
                                        qparts, diag_only, ainv_out, s);
-  if (nm_max <= 64)
+  if (rows_max <= 64)
     return launch_solve_blk<512, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
                                        qparts, diag_only, ainv_out, s);
   return launch_solve_blk<512, 2, 4>(tasks, n, lds_blk, items, qbuf, tbuf,

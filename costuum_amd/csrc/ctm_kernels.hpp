@@ -62,6 +62,7 @@ constexpr int kMaxOrder = 200;
 // gscratch: n * nm_max^2 complex values, used when nm_max > kMaxLdsOrder;
 // ainv_out (may be NULL): Q_c^-1 per task in the T layout.
 hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
+                               int64_t store_max, int rows_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
                                cdbl *ainv_out, hipStream_t s);
