@@ -1950,6 +1950,9 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
   if (nm_max <= 48)
     return launch_solve_blk<256, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
                                        qparts, diag_only, ainv_out, s);
+  if (rows_max <= 32)  // m = 0 blocks up to order 64
+    return launch_solve_blk<256, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
+                                       qparts, diag_only, ainv_out, s);
   if (rows_max <= 64)
     return launch_solve_blk<512, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
                                        qparts, diag_only, ainv_out, s);
