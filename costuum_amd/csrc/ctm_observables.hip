@@ -245,16 +245,20 @@ __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
       const cdbl *Ybp = Ysh + (ic * 2 + 1) * n2m;
       cdbl x0 = cmk(0., 0.), x1 = cmk(0., 0.), x2 = cmk(0., 0.),
            x3 = cmk(0., 0.);
-      for (int row = 0; row < n2m; ++row) {
-        const int h = row >= nm ? 1 : 0;
-        const int p = row - h * nm;
+      // rows p (half 0) and nm + p (half 1) share pi/tau of order p
+      for (int p = 0; p < nm; ++p) {
         const double pi = PTo[(p * n_out + o) * 2];
         const double ta = PTo[(p * n_out + o) * 2 + 1];
-        const double a = h ? ta : pi, as = h ? pi : ta;
-        rfma(x0, a, Yb[row]);
-        rfma(x1, a, Ybp[row]);
-        rfma(x2, as, Yb[row]);
-        rfma(x3, as, Ybp[row]);
+        const cdbl y0 = Yb[p], y1 = Yb[nm + p];
+        const cdbl z0 = Ybp[p], z1 = Ybp[nm + p];
+        rfma(x0, pi, y0);
+        rfma(x0, ta, y1);
+        rfma(x1, pi, z0);
+        rfma(x1, ta, z1);
+        rfma(x2, ta, y0);
+        rfma(x2, pi, y1);
+        rfma(x3, ta, z0);
+        rfma(x3, pi, z1);
       }
       Xo[(int64_t)pr * 4 + 0] = x0;
       Xo[(int64_t)pr * 4 + 1] = x1;
