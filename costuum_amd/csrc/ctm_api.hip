@@ -1286,7 +1286,7 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   CTM_HIP(ctx->d_x.ensure(nitems * (size_t)sp.n_pairs * 4 * sizeof(cdbl)));
   CTM_LAUNCH(CTM_KERNEL_OBS_X, 0., 0.,
              launch_pitau(d_ang, n_ang, nmax_all, ctx->d_pt.as<double>(), s));
-  if (obs_x_lds_bytes(nmax_all, sp.n_out, 8, 256, 1) > kObsXMaxLds) {
+  if (obs_x_lds_bytes(nmax_all, sp.n_out, 8, 256, 1, true) > kObsXMaxLds) {
     ctx->err = "too many outgoing directions for the largest order";
     return CTM_ERR_ARGUMENT;
   }

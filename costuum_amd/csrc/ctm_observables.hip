@@ -115,7 +115,7 @@ __device__ inline cdbl cfac(int nn, int n) {
 // (stored [n][direction] so neighbouring pairs read neighbouring words),
 // of the round's incoming directions and the n-factors a_n are staged in
 // LDS.
-template <int CK, int NT, int GCAP>
+template <int CK, int NT, int GCAP, bool PTG>
 __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
                                                const int32_t *item_list,
                                                const Item *items,
@@ -139,14 +139,15 @@ __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
   // PTo [nm][n_out][2], Bv [G CK][nm][2], an [nm]
   const int rows_cap = obs_x_rows_cap(nm_cap, NT, GCAP);
   cdbl *Ysh = reinterpret_cast<cdbl *>(smem);
+  // PTG: the outgoing pi/tau are read from the (L2-resident) table instead
   double *PTo = smem + 2 * (CK * 2 * rows_cap);
-  double *Bv = PTo + (int64_t)spec.n_out * nm_cap * 2;
+  double *Bv = PTo + (PTG ? 0 : (int64_t)spec.n_out * nm_cap * 2);
   double *an = Bv + (int64_t)CK * (rows_cap / 2) * 2;
   const int t = threadIdx.x;
   const int n_ang = spec.n_ang;
   const int n_out = spec.n_out;
   const double *PTm = PT + (int64_t)m * n_ang * nmax_all * 2;
-  for (int e = t; e < n_out * nm; e += NT) {
+  for (int e = t; !PTG && e < n_out * nm; e += NT) {
     const int p = e / n_out, o = e - (e / n_out) * n_out;
     const double *src =
         PTm + ((int64_t)spec.out_angle[o] * nmax_all + (nmin + p - 1)) * 2;
@@ -246,9 +247,11 @@ __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
       cdbl x0 = cmk(0., 0.), x1 = cmk(0., 0.), x2 = cmk(0., 0.),
            x3 = cmk(0., 0.);
       // rows p (half 0) and nm + p (half 1) share pi/tau of order p
+      const double *pto =
+          PTm + ((int64_t)spec.out_angle[o] * nmax_all + (nmin - 1)) * 2;
       for (int p = 0; p < nm; ++p) {
-        const double pi = PTo[(p * n_out + o) * 2];
-        const double ta = PTo[(p * n_out + o) * 2 + 1];
+        const double pi = PTG ? pto[2 * p] : PTo[(p * n_out + o) * 2];
+        const double ta = PTG ? pto[2 * p + 1] : PTo[(p * n_out + o) * 2 + 1];
         const cdbl y0 = Yb[p], y1 = Yb[nm + p];
         const cdbl z0 = Ybp[p], z1 = Ybp[nm + p];
         rfma(x0, pi, y0);
@@ -487,15 +490,16 @@ hipError_t launch_pitau(const double *ang, int n_ang, int nmax_all, double *PT,
   return hipGetLastError();
 }
 
-size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck, int nt, int gcap) {
+size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck, int nt, int gcap,
+                       bool ptg) {
   const size_t rows_cap = obs_x_rows_cap(nm_cap, nt, gcap);
   return (size_t)ck * 2 * rows_cap * sizeof(cdbl) +
-         (size_t)n_out * nm_cap * 2 * sizeof(double) +
+         (ptg ? 0 : (size_t)n_out * nm_cap * 2 * sizeof(double)) +
          (size_t)ck * (rows_cap / 2) * 2 * sizeof(double) +
          (size_t)nm_cap * sizeof(double);
 }
 
-template <int CK, int NT, int GCAP>
+template <int CK, int NT, int GCAP, bool PTG>
 static hipError_t launch_obs_x_ck(const ObsSpec &spec,
                                   const int32_t *item_list, int n_items,
                                   int nm_cap, const Item *items,
@@ -505,14 +509,14 @@ static hipError_t launch_obs_x_ck(const ObsSpec &spec,
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(k_obs_x<CK, NT, GCAP>),
+        reinterpret_cast<const void *>(k_obs_x<CK, NT, GCAP, PTG>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_obs_x<CK, NT, GCAP>), dim3(n_items), dim3(NT),
-                     obs_x_lds_bytes(nm_cap, spec.n_out, CK, NT, GCAP), s,
-                     spec,
+  hipLaunchKernelGGL((k_obs_x<CK, NT, GCAP, PTG>), dim3(n_items), dim3(NT),
+                     obs_x_lds_bytes(nm_cap, spec.n_out, CK, NT, GCAP, PTG),
+                     s, spec,
                      item_list, items, tbuf, abuf, aoff, PT, nmax_all, nm_cap,
                      X);
   return hipGetLastError();
@@ -525,18 +529,29 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
                         cdbl *X, hipStream_t s) {
   if (n_items <= 0) return hipSuccess;
   // ck 8: the largest blocks, 512 threads (two 8-direction chunks side by
-  // side) if the LDS allows, else 256
+  // side) if the LDS allows, else 256; the outgoing pi/tau are staged in
+  // LDS unless they do not fit, then read from the table (measured slower
+  // where both fit)
   if (ck == 8) {
-    if (obs_x_lds_bytes(nm_cap, spec.n_out, 8, 512, 2) <= kObsXMaxLds)
-      return launch_obs_x_ck<8, 512, 2>(spec, item_list, n_items, nm_cap,
-                                        items, tbuf, abuf, aoff, PT, nmax_all,
-                                        X, s);
-    return launch_obs_x_ck<8, 256, 1>(spec, item_list, n_items, nm_cap, items,
-                                      tbuf, abuf, aoff, PT, nmax_all, X, s);
+    if (obs_x_lds_bytes(nm_cap, spec.n_out, 8, 512, 2, false) <= kObsXMaxLds)
+      return launch_obs_x_ck<8, 512, 2, false>(spec, item_list, n_items,
+                                               nm_cap, items, tbuf, abuf,
+                                               aoff, PT, nmax_all, X, s);
+    if (obs_x_lds_bytes(nm_cap, spec.n_out, 8, 256, 1, false) <= kObsXMaxLds)
+      return launch_obs_x_ck<8, 256, 1, false>(spec, item_list, n_items,
+                                               nm_cap, items, tbuf, abuf,
+                                               aoff, PT, nmax_all, X, s);
+    return launch_obs_x_ck<8, 256, 1, true>(spec, item_list, n_items, nm_cap,
+                                            items, tbuf, abuf, aoff, PT,
+                                            nmax_all, X, s);
   }
-  return launch_obs_x_ck<4, 256, 256>(spec, item_list, n_items, nm_cap,
-                                      items, tbuf, abuf, aoff, PT, nmax_all, X,
-                                      s);
+  if (obs_x_lds_bytes(nm_cap, spec.n_out, 4, 256, 256, false) <= kObsXMaxLds)
+    return launch_obs_x_ck<4, 256, 256, false>(spec, item_list, n_items,
+                                               nm_cap, items, tbuf, abuf,
+                                               aoff, PT, nmax_all, X, s);
+  return launch_obs_x_ck<4, 256, 256, true>(spec, item_list, n_items, nm_cap,
+                                            items, tbuf, abuf, aoff, PT,
+                                            nmax_all, X, s);
 }
 
 hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,

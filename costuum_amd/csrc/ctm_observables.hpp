@@ -52,7 +52,9 @@ __host__ __device__ inline int obs_x_rows_cap(int nm_cap, int nt, int gcap) {
   const int fit = gcap * two < nt ? gcap * two : nt;
   return fit > two ? fit : two;
 }
-size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck, int nt, int gcap);
+// ptg: the outgoing pi/tau are read from the table rather than staged
+size_t obs_x_lds_bytes(int nm_cap, int n_out, int ck, int nt, int gcap,
+                       bool ptg);
 constexpr size_t kObsXMaxLds = 159 * 1024;
 // item_list: items to process (largest first); abuf/aoff (may be null):
 // full-layout ensemble T-matrices per item.

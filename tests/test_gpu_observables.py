@@ -63,3 +63,38 @@ def test_scattering_grid_matches_oracle(batch):
             Zr = O.scattering_grid(nmax, T, lam, 6, use_gl=use_gl)
             err = np.abs(Z[g] - Zr).max() / np.abs(Zr).max()
             assert err < 1e-9, (g, use_gl, err)
+
+
+def test_scattering_grid_many_directions_high_order():
+    """120 scattering angles at order 70: the outgoing pi/tau of the blocks
+    above order 32 no longer fit the workgroup's LDS and are read from the
+    table.  Z at 48 grid points against oracle queries in the same
+    directions on the GPU's own T-matrix (observables path only; the
+    oracle's grid and its queries agree to 4e-16), <= 1e-9."""
+    from costuum_amd.tmatrix import Context, observables, tmatrix_fixed
+    ctx = Context(0)
+    try:
+        rv, d, lam, mr, nmax, ng = 1.8, 1.2, 0.25, 1.4 + 0.01j, 70, 140
+        nst = 120
+        r = tmatrix_fixed([rv], [d], [lam], [mr.real], [mr.imag], [nmax],
+                          [ng], ctx=ctx)
+        print("Z grid: order %d, %d angles" % (nmax, nst), flush=True)
+        out = observables([0.3], ctx=ctx, do_absorption=False,
+                          account_for_scattering=False, n_scatter_theta=nst,
+                          scatter_gl=True, n_grains=1)
+        Z = out["Z"].cpu().numpy()[0]
+        T = r.tmatrix(0)
+        xt = O.gauss_legendre(nst)[0]
+        xp = O.gauss_legendre(2 * nst)[0]
+        rng = np.random.default_rng(3)
+        pts = [(int(i), int(j)) for i, j in
+               zip(rng.integers(0, nst, 48), rng.integers(0, 2 * nst, 48))]
+        scale = np.abs(Z).max()
+        for i, j in pts:
+            _, Zq, _ = O.scattering_query(nmax, T, lam, 0., 0., np.pi / 2, 0.,
+                                          np.arccos(xt[i]),
+                                          np.pi * xp[j] + np.pi)
+            err = np.abs(Z[i, j] - Zq).max() / scale
+            assert err < 1e-9, (i, j, err)
+    finally:
+        ctx.close()
