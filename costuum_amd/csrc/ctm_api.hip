@@ -1501,11 +1501,13 @@ int ctm_query(ctm_context *ctx, int n, const ctm_query_spec *q,
   }
   if (n == 0) return CTM_OK;
   std::vector<Query> qs(n);
+  int qmax = 1;
   for (int i = 0; i < n; ++i) {
     if (q[i].grain < 0 || q[i].grain >= ctx->n_grains) {
       ctx->err = "query grain out of range";
       return CTM_ERR_ARGUMENT;
     }
+    qmax = std::max(qmax, (int)ctx->nmax[q[i].grain]);
     if ((int)ctx->nmax[q[i].grain] >= kMaxQueryOrder) {
       ctx->err = "order too large for the query kernel";
       return CTM_ERR_ARGUMENT;
@@ -1518,7 +1520,8 @@ int ctm_query(ctm_context *ctx, int n, const ctm_query_spec *q,
   CTM_HIP(upload(ctx, ctx->d_query, qs));
   CTM_HIP(ctx->d_qout.ensure((size_t)n * kQueryOut * sizeof(double)));
   CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
-             launch_query(ctx->d_query.as<Query>(), n, ctx->d_items.as<Item>(),
+             launch_query(ctx->d_query.as<Query>(), n, qmax,
+                          ctx->d_items.as<Item>(),
                           ctx->d_gioff.as<int32_t>(), ctx->gp_saved,
                           ctx->d_t.as<cdbl>(),
                           ctx->averaged ? ctx->d_avg.as<cdbl>() : nullptr,

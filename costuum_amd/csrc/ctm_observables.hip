@@ -587,6 +587,8 @@ namespace ctm {
 // summed over m in the particle frame, then rotated back.  One workgroup
 // per query: m runs sequentially, the (n, n') double sum of each m is spread
 // over the threads.
+constexpr int kQueryMB = 8;  // orders m whose pi/tau are computed together
+
 __global__ __launch_bounds__(256) void k_query(const Query *queries,
                                                const Item *items,
                                                const int32_t *gi_off,
@@ -594,9 +596,12 @@ __global__ __launch_bounds__(256) void k_query(const Query *queries,
                                                const cdbl *tbuf,
                                                const cdbl *abuf,
                                                const int64_t *aoff,
-                                               double *out) {
+                                               int nstride, double *out) {
+  // pi/tau of both directions for kQueryMB orders m at a time (dynamic
+  // LDS): ptab[(mm * 4 + d) * nstride + n - 1], d = 0, 1 pi/tau incoming,
+  // 2, 3 pi/tau outgoing; nstride >= the largest order of the launch
+  extern __shared__ double ptab[];
   __shared__ double geo[32];
-  __shared__ double ptab[4][kMaxQueryOrder];
   __shared__ double red[4][8][2];
   const Query q = queries[blockIdx.x];
   const int g = q.grain;
@@ -674,16 +679,21 @@ __global__ __launch_bounds__(256) void k_query(const Query *queries,
   const cdbl step = cmk(cpo * cpi + spo * spi, spo * cpi - cpo * spi);
   cdbl ex = cmk(1., 0.);
   cdbl S[4] = {cmk(0., 0.), cmk(0., 0.), cmk(0., 0.), cmk(0., 0.)};
-  for (int m = 0; m <= N; ++m) {
-    const int nmin = m > 0 ? m : 1;
-    const int nm = N + 1 - nmin;
+  for (int mb = 0; mb <= N; mb += kQueryMB) {
+    // the recurrences of kQueryMB orders m and both directions in parallel
     __syncthreads();
-    if (t < 2) {
-      const int d = t;  // 0: incoming, 1: outgoing
-      pitau_dev(geo[d * 5], geo[d * 5 + 1], geo[d * 5 + 2], N, m,
-                &ptab[2 * d][0], &ptab[2 * d + 1][0], 1);
+    if (t < 2 * kQueryMB) {
+      const int mm = t >> 1, d = t & 1;  // d 0: incoming, 1: outgoing
+      if (mb + mm <= N)
+        pitau_dev(geo[d * 5], geo[d * 5 + 1], geo[d * 5 + 2], N, mb + mm,
+                  ptab + (mm * 4 + 2 * d) * nstride,
+                  ptab + (mm * 4 + 2 * d + 1) * nstride, 1);
     }
     __syncthreads();
+  for (int m = mb; m <= N && m < mb + kQueryMB; ++m) {
+    const int nmin = m > 0 ? m : 1;
+    const int nm = N + 1 - nmin;
+    const double *pt = ptab + (m - mb) * 4 * nstride;
     const double fcos = (m == 0) ? 1. : 2. * ex.x;
     const double fsin = (m == 0) ? 0. : 2. * ex.y;
     ex = cmul(ex, step);
@@ -694,8 +704,9 @@ __global__ __launch_bounds__(256) void k_query(const Query *queries,
     for (int e = t; e < nm * nm; e += 256) {
       const int pn = e / nm, pnn = e - (e / nm) * nm;
       const int n = nmin + pn, nn = nmin + pnn;
-      const double pin = ptab[0][nn - 1], tin = ptab[1][nn - 1];
-      const double pout = ptab[2][n - 1], tout = ptab[3][n - 1];
+      const double pin = pt[nn - 1], tin = pt[nstride + nn - 1];
+      const double pout = pt[2 * nstride + n - 1];
+      const double tout = pt[3 * nstride + n - 1];
       // T(h n; h' nn) for the four half blocks
       cdbl Th[2][2];
       for (int h = 0; h < 2; ++h)
@@ -736,6 +747,7 @@ __global__ __launch_bounds__(256) void k_query(const Query *queries,
         cfma(S[3], cscale(c, fcos), s3);
       }
     }
+  }
   }
   // block reduction of S
   for (int k = 0; k < 4; ++k) {
@@ -825,13 +837,23 @@ __global__ __launch_bounds__(256) void k_query(const Query *queries,
   }
 }
 
-hipError_t launch_query(const Query *q, int n, const Item *items,
+hipError_t launch_query(const Query *q, int n, int nmax, const Item *items,
                         const int32_t *gi_off, const GrainParams &gp,
                         const cdbl *tbuf, const cdbl *abuf,
                         const int64_t *aoff, double *out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_query, dim3(n), dim3(256), 0, s, q, items, gi_off, gp,
-                     tbuf, abuf, aoff, out);
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(k_query),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  const int nstride = nmax > 0 ? nmax : 1;
+  const size_t lds = (size_t)kQueryMB * 4 * nstride * sizeof(double);
+  hipLaunchKernelGGL(k_query, dim3(n), dim3(256), lds, s, q, items, gi_off,
+                     gp, tbuf, abuf, aoff, nstride, out);
   return hipGetLastError();
 }
 

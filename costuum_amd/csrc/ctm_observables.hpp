@@ -36,7 +36,8 @@ struct Query {
 constexpr int kQueryOut = 8 + 16 + 16;
 constexpr int kMaxQueryOrder = 256;
 
-hipError_t launch_query(const Query *q, int n, const Item *items,
+// nmax: the largest order among the queried grains
+hipError_t launch_query(const Query *q, int n, int nmax, const Item *items,
                         const int32_t *gi_off, const GrainParams &gp,
                         const cdbl *tbuf, const cdbl *abuf,
                         const int64_t *aoff, double *out, hipStream_t s);
