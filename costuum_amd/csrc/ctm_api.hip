@@ -88,6 +88,7 @@ struct SharedTables {
   std::vector<int32_t> gl_by_g;                 // G -> key index or -1
   std::unordered_map<uint64_t, int32_t> wig_base;  // (N, G) -> first slot
   std::vector<int32_t> wig_slot;                // key index or -1
+  std::vector<uint8_t> wig_full;  // per slot base: every m registered
   std::vector<GlKey> gl_keys;
   std::vector<WigKey> wig_keys;
   int64_t gl_size = 0, wig_size = 0;
@@ -99,6 +100,7 @@ struct SharedTables {
     gl_by_g.clear();
     wig_base.clear();
     wig_slot.clear();
+    wig_full.clear();
     gl_keys.clear();
     wig_keys.clear();
     gl_size = wig_size = 0;
@@ -130,6 +132,7 @@ struct SharedTables {
         } else {
           last_base = (int)wig_slot.size();
           wig_slot.resize(wig_slot.size() + N + 1, -1);
+          wig_full.resize(wig_slot.size(), 0);
           wig_base.emplace(key, last_base);
         }
         if (N >= (int)wig_by_n.size()) wig_by_n.resize(N + 1, -1);
@@ -140,6 +143,22 @@ struct SharedTables {
     }
     int32_t &slot = wig_slot[last_base + m];
     if (slot >= 0) return slot;
+    return new_wig(N, m, G, slot);
+  }
+  // Registers the keys of every m = 0..N of (N, G); returns the slot base:
+  // the key of m is wig_slot[base + m].
+  int wig_all(int N, int G) {
+    wig(N, 0, G);
+    const int base = last_base;
+    if (wig_full[base]) return base;
+    for (int m = 1; m <= N; ++m) {
+      int32_t &slot = wig_slot[base + m];
+      if (slot < 0) new_wig(N, m, G, slot);
+    }
+    wig_full[base] = 1;
+    return base;
+  }
+  int new_wig(int N, int m, int G, int32_t &slot) {
     const int g = gl(G);
     WigKey k{};
     k.N = N;
@@ -166,6 +185,8 @@ struct WorkList {
   // entries each, parts 0-2 in that order).
   std::vector<int32_t> tile_prefix;
   int n_part[5] = {0, 0, 0, 0, 0};
+  // algorithmic flops and bytes of each assembly part (profiler credit)
+  double f_asm[5] = {0., 0., 0., 0., 0.}, b_asm[5] = {0., 0., 0., 0., 0.};
   std::vector<int32_t> solve;        // item * 2 + class, by block order
   std::vector<int32_t> scratch_a, scratch_b, scratch_c;  // build_tasks
   int64_t table_size = 0, q_size = 0, t_size = 0;
@@ -310,6 +331,7 @@ struct ctm_context {
   std::vector<int32_t> obs_items;
   Profiler prof;
   Staging staging;
+  Staging readback;  // pinned landing area of device-to-host copies
 };
 
 #define CTM_HIP(expr)                                                   \
@@ -377,6 +399,44 @@ hipError_t upload(ctm_context *ctx, DevBuf &b, const std::vector<T> &v) {
   return stage_copy(ctx, b.p, v.data(), v.size() * sizeof(T));
 }
 
+// Device-to-host copies of k device ranges src[i] (bytes[i] each) to dst[i]
+// through pinned memory, then a drain of the stream (a pageable destination
+// makes the runtime stage the copy itself, at a fraction of the rate).
+hipError_t read_back(ctm_context *ctx, int k, void *const *dst,
+                     const void *const *src, const size_t *bytes) {
+  Staging &rb = ctx->readback;
+  size_t total = 0;
+  for (int i = 0; i < k; ++i) total += (bytes[i] + 255) & ~(size_t)255;
+  if (total > rb.cap) {
+    hipError_t e = drain(ctx);
+    if (e != hipSuccess) return e;
+    rb.release();
+    const size_t want = std::max(total, (size_t)1 << 20);
+    e = hipHostMalloc(reinterpret_cast<void **>(&rb.p), want,
+                      hipHostMallocDefault);
+    if (e != hipSuccess) {
+      rb.p = nullptr;
+      return e;
+    }
+    rb.cap = want;
+  }
+  size_t off = 0;
+  for (int i = 0; i < k; ++i) {
+    hipError_t e = hipMemcpyAsync(rb.p + off, src[i], bytes[i],
+                                  hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return e;
+    off += (bytes[i] + 255) & ~(size_t)255;
+  }
+  hipError_t e = drain(ctx);
+  if (e != hipSuccess) return e;
+  off = 0;
+  for (int i = 0; i < k; ++i) {
+    std::memcpy(dst[i], rb.p + off, bytes[i]);
+    off += (bytes[i] + 255) & ~(size_t)255;
+  }
+  return hipSuccess;
+}
+
 // Computes the grain-independent tables whose keys were added since the
 // last call.
 int flush_shared(ctm_context *ctx) {
@@ -428,23 +488,35 @@ int flush_shared(ctm_context *ctx) {
 
 // Builds the cost-sorted item order with the prefix sum of assembly tiles
 // (largest G first) and the solve task list (largest blocks first), both by
-// counting sort.
-void build_tasks(WorkList &w) {
-  const int n = (int)w.items.size();
+// counting sort, for the items i0 .. i1 - 1 (global indices).  The items are
+// read once; the sorts run over packed per-item words.
+void build_tasks(WorkList &w, int i0, int i1) {
+  const int n = i1 - i0;
   // per item: block order, an m = 0 flag and the part (micro-tile 3, 2, 1
-  // -> 0, 1, 2; packed -> 3 (m > 0) or 4 (m = 0)); then the assembly key
+  // -> 0, 1, 2; packed -> 3 (m > 0) or 4 (m = 0)); G (then the sort key)
   std::vector<int32_t> &info = w.scratch_a, &key = w.scratch_b;
   info.resize(n);
   key.resize(n);
   int gmax = 0, nmmax = 0;
-  for (int k = 0; k < 5; ++k) w.n_part[k] = 0;
+  for (int k = 0; k < 5; ++k) {
+    w.n_part[k] = 0;
+    w.f_asm[k] = w.b_asm[k] = 0.;
+  }
+  // Algorithmic work of the assembly launches (the per-block counts of the
+  // flop model F_T: 96 G nm^2 for an m > 0 block, 32 G n^2 for an m = 0
+  // order) and their compulsory HBM traffic (tables and Wigner values read
+  // once, Q/RgQ written once).
   for (int i = 0; i < n; ++i) {
-    const Item &it = w.items[i];
+    const Item &it = w.items[i0 + i];
     const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
     const int micro = assemble_micro(nm, it.m == 0);
     const int part = micro == 4 ? (it.m == 0 ? 4 : 3) : 3 - micro;
     info[i] = nm | (part << 16) | (it.m == 0 ? (1 << 30) : 0);
+    key[i] = it.G;
     w.n_part[part]++;
+    const double d = nm;
+    w.f_asm[part] += (it.m > 0 ? 96. : 32.) * it.G * d * d;
+    w.b_asm[part] += 16. * d * it.G + 64. * d * d;
     gmax = std::max(gmax, it.G);
     nmmax = std::max(nmmax, nm);
   }
@@ -453,12 +525,12 @@ void build_tasks(WorkList &w) {
   std::vector<int32_t> &cnt = w.scratch_c;
   cnt.assign(std::max(nkey, nmmax + 1) + 1, 0);
   for (int i = 0; i < n; ++i) {
-    key[i] = ((info[i] >> 16) & 7) * (gmax + 1) + (gmax - w.items[i].G);
+    key[i] = ((info[i] >> 16) & 7) * (gmax + 1) + (gmax - key[i]);
     cnt[key[i] + 1]++;
   }
   for (int g = 1; g <= nkey; ++g) cnt[g] += cnt[g - 1];
   w.order.resize(n);
-  for (int i = 0; i < n; ++i) w.order[cnt[key[i]]++] = i;
+  for (int i = 0; i < n; ++i) w.order[cnt[key[i]]++] = i0 + i;
   // tiles of 16 MT rows, twice as wide for m = 0
   w.tile_prefix.resize((size_t)n + 3);
   int32_t *pre = w.tile_prefix.data();
@@ -467,7 +539,7 @@ void build_tasks(WorkList &w) {
     const int tr = 16 * (3 - part);
     pre[0] = 0;
     for (int k = 0; k < w.n_part[part]; ++k, ++q) {
-      const int f = info[w.order[q]], nm = f & 0xffff;
+      const int f = info[w.order[q] - i0], nm = f & 0xffff;
       const int ntr = (nm + tr - 1) / tr;
       pre[k + 1] = pre[k] + ntr * ((f >> 30) ? (nm + 2 * tr - 1) / (2 * tr)
                                              : ntr);
@@ -481,8 +553,8 @@ void build_tasks(WorkList &w) {
   w.solve.resize(2 * (size_t)n);
   for (int i = 0; i < n; ++i) {
     const int q = cnt[nmmax - (info[i] & 0xffff)]++;
-    w.solve[2 * q] = 2 * i;
-    w.solve[2 * q + 1] = 2 * i + 1;
+    w.solve[2 * q] = 2 * (i0 + i);
+    w.solve[2 * q + 1] = 2 * (i0 + i) + 1;
   }
 }
 
@@ -560,32 +632,51 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
   return CTM_OK;
 }
 
-// Runs tables + assembly + inversion/solve for the current work list.
-int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
-  WorkList &w = ctx->work;
-  if (w.items.empty()) return CTM_OK;
-  int rc = flush_shared(ctx);
-  if (rc != CTM_OK) return rc;
-  hipStream_t s = ctx->stream;
-  // the tables need only the sets: launch them first and build the
-  // assembly and solve lists while they run
-  CTM_HIP(upload(ctx, ctx->d_sets, w.sets));
-  CTM_HIP(ctx->d_tables.ensure(std::max<int64_t>(w.table_size, 1) *
+// Device room for n_sets table sets holding table_size doubles: sized
+// before the first tables launch of a list, which may come before the list
+// is complete.
+int size_tables(ctm_context *ctx, size_t n_sets, int64_t table_size) {
+  CTM_HIP(ctx->d_sets.ensure(std::max<size_t>(n_sets, 1) * sizeof(TableSet)));
+  CTM_HIP(ctx->d_tables.ensure(std::max<int64_t>(table_size, 1) *
                                sizeof(double)));
-  // Algorithmic work of the launches (the per-block counts of the flop
-  // model F_T: 96 G nm^2 for an m > 0 assembly, 32 G n^2 for an m = 0
-  // order; (64/3) nm^3 and (16/3) n^3 for the parity-split solves) and the
-  // compulsory HBM traffic (tables and Wigner values read once, Q/RgQ
-  // written once by the assembly, read once by the solve, T written once).
+  return CTM_OK;
+}
+
+// Tables of the sets s0 .. s1 - 1 of the current work list (device room
+// from size_tables).  Launched ahead of the item lists: the host builds
+// those while the tables run.
+int launch_set_tables(ctm_context *ctx, const GrainParams &gp, int s0,
+                      int s1) {
+  WorkList &w = ctx->work;
+  if (s1 <= s0) return CTM_OK;
+  int rc = flush_shared(ctx);  // Gauss-Legendre nodes and Wigner values
+  if (rc != CTM_OK) return rc;
+  CTM_HIP(stage_copy(ctx, ctx->d_sets.as<TableSet>() + s0, w.sets.data() + s0,
+                     (size_t)(s1 - s0) * sizeof(TableSet)));
+  // compulsory traffic: every table value written once
   double b_tab = 0.;
-  for (const TableSet &ts : w.sets)
-    b_tab += 8. * ((double)kNodeScalars * ts.G +
-                   (double)kBesselArrays * ts.N * ts.G);
+  for (int k = s0; k < s1; ++k)
+    b_tab += 8. * ((double)kNodeScalars * w.sets[k].G +
+                   (double)kBesselArrays * w.sets[k].N * w.sets[k].G);
+  const int node0 = w.sets[s0].node0;
+  const int node1 = s1 < (int)w.sets.size() ? w.sets[s1].node0 : w.n_nodes;
   CTM_LAUNCH(CTM_KERNEL_TABLES, 0., b_tab,
-             launch_tables(ctx->d_sets.as<TableSet>(), (int)w.sets.size(),
-                           w.n_nodes, gp, ctx->d_gl.as<double>(),
-                           ctx->d_tables.as<double>(), s));
-  build_tasks(w);
+             launch_tables(ctx->d_sets.as<TableSet>() + s0, s1 - s0, node0,
+                           node1 - node0, gp, ctx->d_gl.as<double>(),
+                           ctx->d_tables.as<double>(), ctx->stream));
+  return CTM_OK;
+}
+
+// Wigner tables of new keys, assembly and inversion/solve of all items of
+// the current work list (their table sets launched before).
+int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
+  WorkList &w = ctx->work;
+  const int ni = (int)w.items.size();
+  if (ni == 0) return CTM_OK;
+  hipStream_t s = ctx->stream;
+  build_tasks(w, 0, ni);
+  int rc = flush_shared(ctx);  // Wigner values of new (N, m, G)
+  if (rc != CTM_OK) return rc;
   CTM_HIP(upload(ctx, ctx->d_items, w.items));
   CTM_HIP(upload(ctx, ctx->d_tiles, w.order));
   CTM_HIP(upload(ctx, ctx->d_gioff_tmp, w.tile_prefix));
@@ -593,22 +684,10 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   CTM_HIP(ctx->d_q.ensure(std::max<int64_t>(w.q_size, 1) * sizeof(cdbl)));
   if (want_t)
     CTM_HIP(ctx->d_t.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
-  CTM_HIP(ctx->d_qparts.ensure(std::max<size_t>(w.items.size(), 1) * 2 *
-                               kQParts * sizeof(double)));
-  // (index: part, micro-tile 3, 2, 1, packed m > 0, packed m = 0)
-  double f_asm[5] = {0., 0., 0., 0., 0.}, b_asm[5] = {0., 0., 0., 0., 0.};
-  for (const Item &it : w.items) {
-    const int nmin = it.m > 0 ? it.m : 1;
-    const int nmi = it.N + 1 - nmin;
-    const double nm = nmi;
-    const int micro = assemble_micro(nmi, it.m == 0);
-    const int k = micro == 4 ? (it.m == 0 ? 4 : 3) : 3 - micro;
-    f_asm[k] += (it.m > 0 ? 96. : 32.) * it.G * nm * nm;
-    b_asm[k] += 8. * (2. * nm * it.G) + 16. * 4. * nm * nm;
-  }
+  CTM_HIP(ctx->d_qparts.ensure((size_t)ni * 2 * kQParts * sizeof(double)));
   {
-    // one launch per part (3 x 3, 2 x 2, 1 x 1 micro-tiles), each credited
-    // with the algorithmic work of its blocks
+    // one launch per part (3 x 3, 2 x 2, 1 x 1 micro-tiles, packed m > 0,
+    // packed m = 0), each credited with the algorithmic work of its blocks
     const int32_t *ord = ctx->d_tiles.as<int32_t>();
     const int32_t *pre = ctx->d_gioff_tmp.as<int32_t>();
     const int kid[3] = {CTM_KERNEL_ASSEMBLE_WIDE, CTM_KERNEL_ASSEMBLE,
@@ -617,7 +696,7 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     for (int part = 0; part < 3; ++part) {
       const int np = w.n_part[part];
       if (np > 0)
-        CTM_LAUNCH(kid[part], f_asm[part], b_asm[part],
+        CTM_LAUNCH(kid[part], w.f_asm[part], w.b_asm[part],
                    launch_assemble(3 - part, ord, pre + po, np,
                                    w.tile_prefix[po + np],
                                    ctx->d_items.as<Item>(),
@@ -632,7 +711,7 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     for (int part = 3; part < 5; ++part) {
       const int np = w.n_part[part];
       if (np > 0)
-        CTM_LAUNCH(CTM_KERNEL_ASSEMBLE_PACK, f_asm[part], b_asm[part],
+        CTM_LAUNCH(CTM_KERNEL_ASSEMBLE_PACK, w.f_asm[part], w.b_asm[part],
                    launch_assemble_pack4(ord, np, part == 4 ? 1 : 0,
                                          ctx->d_items.as<Item>(),
                                          ctx->d_sets.as<TableSet>(),
@@ -644,6 +723,16 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     }
   }
   return run_solves(ctx, want_t, nullptr);
+}
+
+// Runs tables + assembly + inversion/solve for the current work list.
+int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
+  WorkList &w = ctx->work;
+  if (w.items.empty()) return CTM_OK;
+  int rc = size_tables(ctx, w.sets.size(), w.table_size);
+  if (rc == CTM_OK) rc = launch_set_tables(ctx, gp, 0, (int)w.sets.size());
+  if (rc != CTM_OK) return rc;
+  return run_items(ctx, gp, want_t);
 }
 
 // Rough order estimate from the largest size parameter of the spheroid
@@ -697,6 +786,7 @@ int ctm_destroy(ctm_context *ctx) {
   for (DevBuf *b : bufs) b->release();
   ctx->prof.release();
   ctx->staging.release();
+  ctx->readback.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CTM_OK;
@@ -734,14 +824,25 @@ static int run_all_m(ctm_context *ctx, const GrainParams &gp, int n,
   w.clear();
   SharedTables &sh = ctx->shared;
   ctx->grain_items_off.assign(n + 1, 0);
+  // table set g is grain g's; the Wigner values and tables run while the
+  // items are listed
+  std::vector<int32_t> &wbase = w.scratch_a;
+  wbase.resize(n);
+  for (int g = 0; g < n; ++g) {
+    w.add_set(g, (int)N[g], (int)G[g], sh.gl_keys[sh.gl((int)G[g])].off);
+    wbase[g] = sh.wig_all((int)N[g], (int)G[g]);
+  }
+  int rc = size_tables(ctx, w.sets.size(), w.table_size);
+  if (rc == CTM_OK) rc = launch_set_tables(ctx, gp, 0, n);
+  if (rc != CTM_OK) return rc;
+  size_t n_items = 0;
+  for (int g = 0; g < n; ++g) n_items += N[g] + 1;
+  w.items.reserve(n_items);
   for (int g = 0; g < n; ++g) {
     const int Ng = (int)N[g], Gg = (int)G[g];
-    const int set = w.add_set(g, Ng, Gg, sh.gl_keys[sh.gl(Gg)].off);
+    const int32_t *wk = sh.wig_slot.data() + wbase[g];
     ctx->grain_items_off[g] = (int)w.items.size();
-    for (int m = 0; m <= Ng; ++m) {
-      const int wk = sh.wig(Ng, m, Gg);
-      w.add_item(g, Ng, m, Gg, set, wk, true);
-    }
+    for (int m = 0; m <= Ng; ++m) w.add_item(g, Ng, m, Gg, g, wk[m], true);
   }
   ctx->grain_items_off[n] = (int)w.items.size();
   ctx->have_tangents = false;
@@ -758,7 +859,7 @@ static int run_all_m(ctm_context *ctx, const GrainParams &gp, int n,
     ctx->gp_saved = GrainParams{P, P + n, P + 2 * (size_t)n, P + 3 * (size_t)n,
                                 P + 4 * (size_t)n};
   }
-  int rc = run_work(ctx, gp, true);
+  rc = run_items(ctx, gp, true);
   if (rc != CTM_OK) return rc;
   ctx->grain_items.resize(w.items.size());
   for (size_t i = 0; i < w.items.size(); ++i) ctx->grain_items[i] = (int32_t)i;
@@ -812,13 +913,13 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
 
   // host copies of the sizes (to size the windows)
   std::vector<double> h_rv(n), h_d(n), h_lam(n);
-  CTM_HIP(hipMemcpyAsync(h_rv.data(), radius, n * sizeof(double),
-                         hipMemcpyDeviceToHost, s));
-  CTM_HIP(hipMemcpyAsync(h_d.data(), axis_ratio, n * sizeof(double),
-                         hipMemcpyDeviceToHost, s));
-  CTM_HIP(hipMemcpyAsync(h_lam.data(), wavelength, n * sizeof(double),
-                         hipMemcpyDeviceToHost, s));
-  CTM_HIP(drain(ctx));
+  {
+    void *dst[3] = {h_rv.data(), h_d.data(), h_lam.data()};
+    const void *src[3] = {radius, axis_ratio, wavelength};
+    const size_t bytes[3] = {n * sizeof(double), n * sizeof(double),
+                             n * sizeof(double)};
+    CTM_HIP(read_back(ctx, 3, dst, src, bytes));
+  }
 
   std::vector<ConvState> st(n);
   for (int g = 0; g < n; ++g) {
@@ -852,21 +953,56 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
   for (int g = 0; g < n; ++g) active.push_back(g);
 
   // phase 1: windows of orders until every grain has converged
+  std::vector<int32_t> wkey, win_off, win_items;
   while (!active.empty()) {
     WorkList &w = ctx->work;
     w.clear();
-    std::vector<int32_t> win_off(1, 0), win_items;
+    // The window's table sets (one per grain and order) are listed grain by
+    // grain and their tables launched every kSetsPerLaunch sets, so the GPU
+    // starts while the host lists the rest; the device room comes first,
+    // from the window's extent (sum over N of 7 G + 8 N G, G = glfac N).
+    constexpr size_t kSetsPerLaunch = 8192;
+    size_t n_sets = 0;
+    int64_t t_size = 0;
+    for (int g : active) {
+      const int64_t a = lo[g] - 1, b = hi[g];
+      if (b <= a) continue;
+      const int64_t s1 = (b * (b + 1) - a * (a + 1)) / 2;
+      const int64_t s2 =
+          (b * (b + 1) * (2 * b + 1) - a * (a + 1) * (2 * a + 1)) / 6;
+      n_sets += (size_t)(b - a);
+      t_size += glfac * ((int64_t)kNodeScalars * s1 +
+                         (int64_t)kBesselArrays * s2);
+    }
+    rc = size_tables(ctx, n_sets, t_size);
+    if (rc != CTM_OK) return rc;
+    wkey.clear();
+    win_off.assign(1, 0);
+    size_t s_done = 0;
     for (int g : active) {
       for (int N = lo[g]; N <= hi[g]; ++N) {
         const int G = glfac * N;
         const int gk = ctx->shared.gl(G);
-        const int set = w.add_set(g, N, G, ctx->shared.gl_keys[gk].off);
-        const int wk = ctx->shared.wig(N, 0, G);
-        win_items.push_back(w.add_item(g, N, 0, G, set, wk, false));
+        w.add_set(g, N, G, ctx->shared.gl_keys[gk].off);
+        wkey.push_back(ctx->shared.wig(N, 0, G));
       }
-      win_off.push_back((int32_t)win_items.size());
+      win_off.push_back((int32_t)w.sets.size());
+      if (w.sets.size() - s_done >= kSetsPerLaunch) {
+        rc = launch_set_tables(ctx, gp, (int)s_done, (int)w.sets.size());
+        if (rc != CTM_OK) return rc;
+        s_done = w.sets.size();
+      }
     }
-    rc = run_work(ctx, gp, false);
+    rc = launch_set_tables(ctx, gp, (int)s_done, (int)w.sets.size());
+    if (rc != CTM_OK) return rc;
+    // item i: the m = 0 block of set i
+    win_items.resize(w.sets.size());
+    for (size_t i = 0; i < w.sets.size(); ++i) {
+      const TableSet &ts = w.sets[i];
+      win_items[i] = w.add_item(ts.grain, ts.N, 0, ts.G, (int)i, wkey[i],
+                                false);
+    }
+    rc = run_items(ctx, gp, false);
     if (rc != CTM_OK) return rc;
     CTM_HIP(upload(ctx, ctx->d_active, active));
     CTM_HIP(upload(ctx, ctx->d_winoff, win_off));
@@ -878,9 +1014,12 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
                                ctx->d_items.as<Item>(),
                                ctx->d_qparts.as<double>(), conv->tolerance,
                                last_order, ctx->d_state.as<ConvState>(), s));
-    CTM_HIP(hipMemcpyAsync(st.data(), ctx->d_state.p, n * sizeof(ConvState),
-                           hipMemcpyDeviceToHost, s));
-    CTM_HIP(drain(ctx));
+    {
+      void *dst = st.data();
+      const void *src = ctx->d_state.p;
+      const size_t bytes = n * sizeof(ConvState);
+      CTM_HIP(read_back(ctx, 1, &dst, &src, &bytes));
+    }
     std::vector<int32_t> next;
     for (int g : active) {
       if (st[g].done) continue;

@@ -39,13 +39,22 @@ __device__ inline int bessel_start(int N, double absz) {
 // K0: Gauss-Legendre nodes.  Thread i runs the Newton iteration of root i of
 // the 2G-point rule (Lether-Wenston guess, tolerance 1e-16 relaxed x10 per
 // iteration past 100) and stores node i = -root, the first half in ascending
-// order.
-__global__ void k_gauss_legendre(const GlKey *keys, double *gl) {
+// order.  The Legendre recurrence takes its ratios j / (j + 1) from a table
+// in LDS (one FP64 division per step otherwise).
+constexpr int kGlRatios = 2048;
+
+__global__ __launch_bounds__(256) void k_gauss_legendre(const GlKey *keys,
+                                                        double *gl) {
+  __shared__ double ratio[kGlRatios];
   const GlKey key = keys[blockIdx.x];
   const int G = key.G;
   double *x = gl + key.off;
   double *w = x + G;
   const unsigned order = 2u * (unsigned)G;
+  const unsigned ntab = order < (unsigned)kGlRatios ? order : kGlRatios;
+  for (unsigned j = threadIdx.x; j < ntab; j += blockDim.x)
+    ratio[j] = j / (j + 1.);
+  __syncthreads();
   const double inv = 1.0 / order;
   const double inv2 = inv * inv;
   for (int i = threadIdx.x; i < G; i += blockDim.x) {
@@ -55,7 +64,7 @@ __global__ void k_gauss_legendre(const GlKey *keys, double *gl) {
                 cos(t);
     double tol = 1.e-16;
     int iter = 0;
-    double pinv, step;
+    double pinv, step, last = 1.;
     do {
       ++iter;
       if (iter > 100) tol *= 10.;
@@ -63,16 +72,21 @@ __global__ void k_gauss_legendre(const GlKey *keys, double *gl) {
       double p_cur = xi;
       for (unsigned j = 1; j < order; ++j) {
         const double p_old = p_prev;
+        const double r = j < ntab ? ratio[j] : j / (j + 1.);
         p_prev = p_cur;
-        p_cur = xi * p_prev + (xi * p_prev - p_old) * j / (j + 1.);
+        p_cur = xi * p_prev + (xi * p_prev - p_old) * r;
       }
       pinv = 1. / ((p_prev - xi * p_cur) * order);
       step = pinv * p_cur * (1. - xi * xi);
       xi = xi - step;
-      // A few nodes stagnate at the rounding level, where the 1e-16 test
-      // keeps iterating until the tolerance has relaxed (iteration > 100);
-      // they are converged to ~1e-16 already (|x| <= 1), so stop there.
+      // Newton converges quadratically down to the rounding level of the
+      // recurrence (a few 1e-16 relative, more near the ends of large
+      // rules), where the steps stop shrinking; the 1e-16 test alone would
+      // keep iterating there until the tolerance has relaxed past
+      // iteration 100.  Such a node is converged: stop.
+      if (iter > 3 && fabs(step) >= 0.25 * last) break;
       if (iter > 4 && fabs(step) <= 4.e-16) break;
+      last = fabs(step);
     } while (fabs(step) > tol * fabs(xi));
     x[i] = -xi;
     w[i] = 2. * pinv * pinv * (1. - xi * xi);
@@ -133,16 +147,19 @@ __global__ void k_wigner(const WigKey *keys, const double *gl, double *wig) {
 // ---------------------------------------------------------------------------
 // K1: per-(grain, order) node scalars and Bessel tables.  One thread per
 // node over all sets of the list (a thread finds its set by binary search
-// in the sets' node offsets); the ratio recurrence of j_n runs down from
+// in the sets' node offsets, which start at node_base for the first set of
+// the launch); the ratio recurrence of j_n runs down from
 // the start order and parks the ratios for n < N in the output rows before
 // the forward pass overwrites them with j_n.
 __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
-                                                int n_sets, int n_nodes,
+                                                int n_sets, int node_base,
+                                                int n_nodes,
                                                 const GrainParams gp,
                                                 const double *gl,
                                                 double *tables) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= n_nodes) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_nodes) return;
+  const int tid = node_base + t;
   int lo = 0, hi = n_sets;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -1805,7 +1822,7 @@ __global__ void k_write_orders(int n_grains, const ConvState *state,
 hipError_t launch_gauss_legendre(const GlKey *keys, int n, double *gl,
                                  hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gauss_legendre, dim3(n), dim3(128), 0, s, keys, gl);
+  hipLaunchKernelGGL(k_gauss_legendre, dim3(n), dim3(256), 0, s, keys, gl);
   return hipGetLastError();
 }
 
@@ -1816,12 +1833,12 @@ hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
   return hipGetLastError();
 }
 
-hipError_t launch_tables(const TableSet *sets, int n, int n_nodes,
-                         const GrainParams &gp, const double *gl,
+hipError_t launch_tables(const TableSet *sets, int n, int node_base,
+                         int n_nodes, const GrainParams &gp, const double *gl,
                          double *tables, hipStream_t s) {
   if (n <= 0 || n_nodes <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_tables, dim3((n_nodes + 127) / 128), dim3(128), 0, s,
-                     sets, n, n_nodes, gp, gl, tables);
+                     sets, n, node_base, n_nodes, gp, gl, tables);
   return hipGetLastError();
 }
 

@@ -18,8 +18,10 @@ hipError_t launch_gauss_legendre(const GlKey *keys, int n, double *gl,
                                  hipStream_t s);
 hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
                          double *wig, hipStream_t s);
-hipError_t launch_tables(const TableSet *sets, int n, int n_nodes,
-                         const GrainParams &gp,
+// sets: the launch's sets, whose node offsets start at node_base; n_nodes:
+// the nodes of those sets.
+hipError_t launch_tables(const TableSet *sets, int n, int node_base,
+                         int n_nodes, const GrainParams &gp,
                          const double *gl, double *tables, hipStream_t s);
 // micro = 1: 16 x 16 pair tiles (blocks up to order kSmallBlock);
 // micro = 2: 32 x 32 tiles of 2 x 2 pairs per thread; micro = 3: 48 x 48
