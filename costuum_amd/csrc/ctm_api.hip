@@ -958,10 +958,11 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     WorkList &w = ctx->work;
     w.clear();
     // The window's table sets (one per grain and order) are listed grain by
-    // grain and their tables launched every kSetsPerLaunch sets, so the GPU
-    // starts while the host lists the rest; the device room comes first,
-    // from the window's extent (sum over N of 7 G + 8 N G, G = glfac N).
-    constexpr size_t kSetsPerLaunch = 8192;
+    // grain and their tables launched every kSetsPerLaunch sets (the first
+    // launch after kFirstSets), so the GPU starts while the host lists the
+    // rest; the device room comes first, from the window's extent (sum over
+    // N of 7 G + 8 N G, G = glfac N).
+    constexpr size_t kSetsPerLaunch = 8192, kFirstSets = 2048;
     size_t n_sets = 0;
     int64_t t_size = 0;
     for (int g : active) {
@@ -976,6 +977,18 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     }
     rc = size_tables(ctx, n_sets, t_size);
     if (rc != CTM_OK) return rc;
+    {
+      // every order's quadrature and m = 0 Wigner keys before the first
+      // tables launch: one launch each computes all of them
+      int nlo = last_order, nhi = 0;
+      for (int g : active) {
+        nlo = std::min(nlo, lo[g]);
+        nhi = std::max(nhi, hi[g]);
+      }
+      for (int N = nlo; N <= nhi; ++N) ctx->shared.wig(N, 0, glfac * N);
+      rc = flush_shared(ctx);
+      if (rc != CTM_OK) return rc;
+    }
     wkey.clear();
     win_off.assign(1, 0);
     size_t s_done = 0;
@@ -987,7 +1000,7 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
         wkey.push_back(ctx->shared.wig(N, 0, G));
       }
       win_off.push_back((int32_t)w.sets.size());
-      if (w.sets.size() - s_done >= kSetsPerLaunch) {
+      if (w.sets.size() - s_done >= (s_done ? kSetsPerLaunch : kFirstSets)) {
         rc = launch_set_tables(ctx, gp, (int)s_done, (int)w.sets.size());
         if (rc != CTM_OK) return rc;
         s_done = w.sets.size();

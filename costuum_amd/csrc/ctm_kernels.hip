@@ -70,11 +70,16 @@ __global__ __launch_bounds__(256) void k_gauss_legendre(const GlKey *keys,
       if (iter > 100) tol *= 10.;
       double p_prev = 1.;
       double p_cur = xi;
-      for (unsigned j = 1; j < order; ++j) {
+#pragma unroll 8
+      for (unsigned j = 1; j < ntab; ++j) {
         const double p_old = p_prev;
-        const double r = j < ntab ? ratio[j] : j / (j + 1.);
         p_prev = p_cur;
-        p_cur = xi * p_prev + (xi * p_prev - p_old) * r;
+        p_cur = xi * p_prev + (xi * p_prev - p_old) * ratio[j];
+      }
+      for (unsigned j = ntab; j < order; ++j) {
+        const double p_old = p_prev;
+        p_prev = p_cur;
+        p_cur = xi * p_prev + (xi * p_prev - p_old) * (j / (j + 1.));
       }
       pinv = 1. / ((p_prev - xi * p_cur) * order);
       step = pinv * p_cur * (1. - xi * xi);
