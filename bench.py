@@ -31,8 +31,10 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    # defaults: 20 timed steps after 3 warm-up steps for the batch workload
+    # (about half a second timed), 1 + 1 for the grid and gradient workloads
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--grains", type=int, default=4096)
     ap.add_argument("--cpu-baseline", type=int, default=1,
@@ -50,7 +52,13 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 512 MiB five times "
                          "(reference kernel for the HBM byte counters)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    batch = a.workload == "batch"
+    if a.steps is None:
+        a.steps = 20 if batch else 1
+    if a.warmup is None:
+        a.warmup = 3 if batch else 1
+    return a
 
 
 def copy_calibration():

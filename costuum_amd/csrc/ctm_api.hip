@@ -1166,7 +1166,7 @@ int ctm_tmatrix_tangents(ctm_context *ctx, double *dqsca_host,
   const int n = ctx->n_grains;
   const int ni = (int)w.items.size();
   // tangent tables
-  CTM_HIP(ctx->d_ttables.ensure(std::max<int64_t>(w.table_size, 1) *
+  CTM_HIP(ctx->d_ttables.ensure(std::max<int64_t>(w.table_size, 1) * 2 *
                                 kTangentDirs * sizeof(double)));
   CTM_LAUNCH(CTM_KERNEL_TABLES, 0., 0.,
              launch_tables_tangent(ctx->d_sets.as<TableSet>(),

@@ -5,10 +5,13 @@
 //                nodes (theta > pi/2) of the 2G-point Gauss-Legendre rule.
 //   Wigner cache per distinct (N, m): d^n_{0m}[nm][G], d'[nm][G] for
 //                n = max(m,1)..N at those nodes (grain-independent).
-//   Table set    per (grain, order N): node scalars [7][G] (w r^2, dr/r,
-//                1/sin, 1/sin^2, 1/kr, 1/(m kr) re, im) followed by Bessel
-//                tables [8][N][G] (j, [kr j]'/kr, y, [kr y]'/kr at kr; re/im
-//                of j and [z j]'/z at z = m kr).
+//   Table set    per (grain, order N): node scalars [11][G] (w r^2, dr/r,
+//                1/sin, 1/sin^2, 1/kr, 1/(m kr) re, im, j_0(kr), y_0(kr),
+//                j_0(m kr) re, im) followed by Bessel tables [4][N][G]
+//                (j_n, y_n at kr; re/im of j_n at m kr; n = 1..N).  The
+//                derivatives [z f_n]'/z = f_{n-1} - n f_n / z are formed by
+//                the readers from neighbouring orders (bessel_kr,
+//                bessel_mkr), which halves the tables' HBM traffic.
 //   Q area       per work item (grain, N, m): four nm x nm complex blocks
 //                Q_0, Q_1, RgQ_0, RgQ_1 (parity classes 0 and 1).
 //   T area       per work item: T_0, T_1 (nm x nm complex each).
@@ -59,8 +62,36 @@ __host__ __device__ inline void rfma(cdbl &acc, double s, cdbl b) {
 }
 
 // Number of node scalars and Bessel arrays in a table set.
-constexpr int kNodeScalars = 7;
-constexpr int kBesselArrays = 8;
+constexpr int kNodeScalars = 11;
+constexpr int kBesselArrays = 4;
+
+// j_n(kr), y_n(kr) and [z j_n]'/z, [z y_n]'/z at z = kr for order n >= 1 at
+// node ig of a table set (S: node scalars, B: Bessel tables, NG = N G).
+__device__ __forceinline__ void bessel_kr(const double *S, const double *B,
+                                          int64_t NG, int G, int n, int ig,
+                                          double &j, double &dj, double &y,
+                                          double &dy) {
+  const int64_t o = (int64_t)(n - 1) * G + ig;
+  j = B[o];
+  y = B[NG + o];
+  const double jp = n > 1 ? B[o - G] : S[7 * G + ig];
+  const double yp = n > 1 ? B[NG + o - G] : S[8 * G + ig];
+  const double zinv = S[4 * G + ig];
+  dj = jp - n * zinv * j;
+  dy = yp - n * zinv * y;
+}
+
+// j_n(m kr) and [z j_n]'/z at z = m kr for order n >= 1.
+__device__ __forceinline__ void bessel_mkr(const double *S, const double *B,
+                                           int64_t NG, int G, int n, int ig,
+                                           cdbl &jm, cdbl &djm) {
+  const int64_t o = (int64_t)(n - 1) * G + ig;
+  jm = cmk(B[2 * NG + o], B[3 * NG + o]);
+  const cdbl jp = n > 1 ? cmk(B[2 * NG + o - G], B[3 * NG + o - G])
+                        : cmk(S[9 * G + ig], S[10 * G + ig]);
+  const cdbl zinv = cmk(S[5 * G + ig], S[6 * G + ig]);
+  djm = csub(jp, cmul(cscale(zinv, (double)n), jm));
+}
 
 struct GlKey {
   int32_t G;

@@ -155,7 +155,9 @@ __global__ void k_wigner(const WigKey *keys, const double *gl, double *wig) {
 // in the sets' node offsets, which start at node_base for the first set of
 // the launch); the ratio recurrence of j_n runs down from
 // the start order and parks the ratios for n < N in the output rows before
-// the forward pass overwrites them with j_n.
+// the forward pass overwrites them with j_n.  Only the functions are stored
+// (the readers form the derivatives from neighbouring orders): j_0, y_0 and
+// j_0(m kr) go with the node scalars.
 __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
                                                 int n_sets, int node_base,
                                                 int n_nodes,
@@ -181,9 +183,7 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
   double *S = tables + s.off;
   double *B = S + (int64_t)kNodeScalars * G;
   const int64_t NG = (int64_t)N * G;
-  double *bj = B, *bdj = B + NG, *by = B + 2 * NG, *bdy = B + 3 * NG;
-  double *bjmr = B + 4 * NG, *bjmi = B + 5 * NG;
-  double *bdjmr = B + 6 * NG, *bdjmi = B + 7 * NG;
+  double *bj = B, *by = B + NG, *bjmr = B + 2 * NG, *bjmi = B + 3 * NG;
 
   const double k = 2. * M_PI / lam;
   const cdbl kmr = cscale(mr, k);
@@ -225,9 +225,9 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
         if (idx - 1 < N) bj[(int64_t)(idx - 1) * G + ig] = rho;
       }
       const double j0 = sin(kr) * zinv;
+      S[7 * G + ig] = j0;
       double jprev = bj[ig] * j0;
       bj[ig] = jprev;
-      bdj[ig] = j0 - jprev * zinv;
       // the parked ratios are read in chunks of 8 ahead of the chunk's
       // stores, so the loads overlap instead of waiting one by one
       for (int i0 = 1; i0 < N; i0 += 8) {
@@ -241,7 +241,6 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
           if (i >= N) break;
           const double ji = rho8[u] * jprev;
           bj[(int64_t)i * G + ig] = ji;
-          bdj[(int64_t)i * G + ig] = jprev - (i + 1.) * zinv * ji;
           jprev = ji;
         }
       }
@@ -251,16 +250,14 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
       const double cz = cos(kr), sz = sin(kr);
       const double zi = krinv, zi2 = zi * zi, zi3 = zi2 * zi;
       double ym2 = -cz * zi2 - sz * zi;
+      S[8 * G + ig] = -cz * zi;
       by[ig] = ym2;
-      bdy[ig] = -zi * (cz + ym2);
       if (N > 1) {
         double ym1 = (-3. * zi3 + zi) * cz - 3. * zi2 * sz;
         by[G + ig] = ym1;
-        bdy[G + ig] = ym2 - 2. * zi * ym1;
         for (int i = 2; i < N; ++i) {
           const double yi = (2. * i + 1.) * zi * ym1 - ym2;
           by[(int64_t)i * G + ig] = yi;
-          bdy[(int64_t)i * G + ig] = ym1 - (i + 1.) * zi * yi;
           ym2 = ym1;
           ym1 = yi;
         }
@@ -282,12 +279,11 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
       // sin(z) = sin x cosh y + i cos x sinh y
       const cdbl sz = cmk(sin(krmr.x) * cosh(krmr.y), cos(krmr.x) * sinh(krmr.y));
       const cdbl j0 = cmul(sz, zinv);
+      S[9 * G + ig] = j0.x;
+      S[10 * G + ig] = j0.y;
       cdbl jprev = cmul(cmk(bjmr[ig], bjmi[ig]), j0);
       bjmr[ig] = jprev.x;
       bjmi[ig] = jprev.y;
-      const cdbl dj0 = csub(j0, cmul(jprev, zinv));
-      bdjmr[ig] = dj0.x;
-      bdjmi[ig] = dj0.y;
       for (int i0 = 1; i0 < N; i0 += 8) {
         cdbl rho8[8];
 #pragma unroll
@@ -303,9 +299,6 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
           const cdbl ji = cmul(rho8[u], jprev);
           bjmr[o] = ji.x;
           bjmi[o] = ji.y;
-          const cdbl dji = csub(jprev, cmul(cscale(zinv, i + 1.), ji));
-          bdjmr[o] = dji.x;
-          bdjmi[o] = dji.y;
           jprev = ji;
         }
       }
@@ -353,9 +346,8 @@ __device__ __forceinline__ void row_factor_entry(
     const int n1 = nmin + p;
     const double d1 = Wd[(int64_t)p * G + ig];
     const double dd1 = Wdd[(int64_t)p * G + ig];
-    const int64_t bo = (int64_t)(n1 - 1) * G + ig;
-    const double j1 = B[bo], dj1 = B[NG + bo];
-    const double y1 = B[2 * NG + bo], dy1 = B[3 * NG + bo];
+    double j1, dj1, y1, dy1;
+    bessel_kr(S, B, NG, G, n1, ig, j1, dj1, y1, dy1);
     const double wr2 = S[ig], dr = S[G + ig], s1 = S[2 * G + ig];
     const double s2 = S[3 * G + ig], krinv = S[4 * G + ig];
     const cdbl imkr = cmk(S[5 * G + ig], S[6 * G + ig]);
@@ -412,15 +404,14 @@ __device__ __forceinline__ void row_factor_entry(
 // Column factors (R1..R5, 10 doubles) of block column p at node ig, zero
 // beyond the block or the node range.
 __device__ __forceinline__ void col_factor_entry(
-    double *R, int p, int ig, int G, int nm, int nmin, const double *B,
-    int64_t NG, const double *Wd, const double *Wdd) {
+    double *R, int p, int ig, int G, int nm, int nmin, const double *S,
+    const double *B, int64_t NG, const double *Wd, const double *Wdd) {
   if (ig < G && p < nm) {
     const int n2 = nmin + p;
     const double d2 = Wd[(int64_t)p * G + ig];
     const double dd2 = Wdd[(int64_t)p * G + ig];
-    const int64_t bo = (int64_t)(n2 - 1) * G + ig;
-    const cdbl jm = cmk(B[4 * NG + bo], B[5 * NG + bo]);
-    const cdbl djm = cmk(B[6 * NG + bo], B[7 * NG + bo]);
+    cdbl jm, djm;
+    bessel_mkr(S, B, NG, G, n2, ig, jm, djm);
     const double nn2 = n2 * (n2 + 1.);
     const double a2 = sqrt((2. * n2 + 1.) / nn2);
     const cdbl R1 = cscale(djm, a2 * d2);
@@ -457,8 +448,8 @@ __device__ __forceinline__ void stage_factors(
     } else {
       const int tt = e - TR * CH;
       const int c = tt / CH, kk = tt % CH;
-      col_factor_entry(Rsh[kk][c], tc * cw + c, ig0 + kk, G, nm, nmin, B, NG,
-                       Wd, Wdd);
+      col_factor_entry(Rsh[kk][c], tc * cw + c, ig0 + kk, G, nm, nmin, S, B,
+                       NG, Wd, Wdd);
     }
   }
 }
@@ -804,8 +795,8 @@ __global__ __launch_bounds__(256, 2) void k_assemble_pack4(
         const int c = tt / CH, kk = tt - (tt / CH) * CH;
         const int bb = c / PR, p = c - bb * PR;
         const PackBlock &pb = blk[bb];
-        col_factor_entry(Rsh[kk][c], p, ig0 + kk, pb.G, pb.nm, pb.nmin, pb.B,
-                         pb.NG, pb.Wd, pb.Wdd);
+        col_factor_entry(Rsh[kk][c], p, ig0 + kk, pb.G, pb.nm, pb.nmin, pb.S,
+                         pb.B, pb.NG, pb.Wd, pb.Wdd);
       }
     }
     __syncthreads();

@@ -47,7 +47,20 @@ __device__ inline dz zadd(dz a, dz b) { return dz{cadd(a.v, b.v), cadd(a.d, b.d)
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Tangent tables: one block per (set, direction), one thread per node.
+// Tangent tables: one block per (set, direction), one thread per node.  A
+// set's tangent tables hold [7][G] node-scalar tangents and [8][N][G]
+// tangents of j, [z j]'/z, y, [z y]'/z at kr and re/im of j, [z j]'/z at
+// m kr; they sit at twice the set's offset in the base tables (7 G + 8 N G
+// is at most twice the base set's 11 G + 4 N G), direction dir at
+// dir * 2 * table_size.
+constexpr int kTangentScalars = 7;
+
+__device__ __forceinline__ int64_t tangent_set_off(int dir,
+                                                   int64_t table_size,
+                                                   int64_t off) {
+  return (int64_t)dir * 2 * table_size + 2 * off;
+}
+
 __global__ void k_tables_tangent(const TableSet *sets, const GrainParams gp,
                                  const double *gl, const double *tables,
                                  int64_t table_size, double *ttables) {
@@ -62,8 +75,8 @@ __global__ void k_tables_tangent(const TableSet *sets, const GrainParams gp,
   const double *wgt = cth + G;
   const double *S = tables + s.off;
   const double *B = S + (int64_t)kNodeScalars * G;
-  double *dS = ttables + (int64_t)dir * table_size + s.off;
-  double *dB = dS + (int64_t)kNodeScalars * G;
+  double *dS = ttables + tangent_set_off(dir, table_size, s.off);
+  double *dB = dS + (int64_t)kTangentScalars * G;
   const int64_t NG = (int64_t)N * G;
   const double k = 2. * M_PI / lam;
   const double d2 = d * d;
@@ -110,16 +123,16 @@ __global__ void k_tables_tangent(const TableSet *sets, const GrainParams gp,
       const double nn = n * (n + 1.);
       const int64_t o = (int64_t)i * G + ig;
       // real argument: j and y
-      const double j = B[o], dj = B[NG + o];
-      const double y = B[2 * NG + o], dy = B[3 * NG + o];
+      double j, dj, y, dy;
+      bessel_kr(S, B, NG, G, n, ig, j, dj, y, dy);
       const double q = nn * zi * zi - 1.;
       dB[o] = (dj - j * zi) * dzr;
       dB[NG + o] = (q * j - dj * zi) * dzr;
       dB[2 * NG + o] = (dy - y * zi) * dzr;
       dB[3 * NG + o] = (q * y - dy * zi) * dzr;
       // complex argument: j(m kr)
-      const cdbl jm = cmk(B[4 * NG + o], B[5 * NG + o]);
-      const cdbl djm = cmk(B[6 * NG + o], B[7 * NG + o]);
+      cdbl jm, djm;
+      bessel_mkr(S, B, NG, G, n, ig, jm, djm);
       const cdbl qm = csub(cscale(cmul(zmi, zmi), nn), cmk(1., 0.));
       const cdbl tj = cmul(csub(djm, cmul(jm, zmi)), dzm);
       const cdbl tdj = cmul(csub(cmul(qm, jm), cmul(djm, zmi)), dzm);
@@ -169,8 +182,8 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
   const int tr = lt / ntc, tc = lt - (lt / ntc) * ntc;
   const double *S = tables + ts.off;
   const double *B = S + (int64_t)kNodeScalars * G;
-  const double *dS = ttables + (int64_t)dir * table_size + ts.off;
-  const double *dB = dS + (int64_t)kNodeScalars * G;
+  const double *dS = ttables + tangent_set_off(dir, table_size, ts.off);
+  const double *dB = dS + (int64_t)kTangentScalars * G;
   const int64_t NG = (int64_t)N * G;
   const double *Wd = wig + wk.off;
   const double *Wdd = Wd + (int64_t)nm * G;
@@ -202,10 +215,12 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
           const double d1 = Wd[(int64_t)p * G + ig];
           const double dd1 = Wdd[(int64_t)p * G + ig];
           const int64_t bo = (int64_t)(n1 - 1) * G + ig;
-          const dv j1 = mkd(B[bo], dB[bo]);
-          const dv dj1 = mkd(B[NG + bo], dB[NG + bo]);
-          const dv y1 = mkd(B[2 * NG + bo], dB[2 * NG + bo]);
-          const dv dy1 = mkd(B[3 * NG + bo], dB[3 * NG + bo]);
+          double bj, bdj, by, bdy;
+          bessel_kr(S, B, NG, G, n1, ig, bj, bdj, by, bdy);
+          const dv j1 = mkd(bj, dB[bo]);
+          const dv dj1 = mkd(bdj, dB[NG + bo]);
+          const dv y1 = mkd(by, dB[2 * NG + bo]);
+          const dv dy1 = mkd(bdy, dB[3 * NG + bo]);
           const dv wr2 = mkd(S[ig], dS[ig]);
           const dv dr = mkd(S[G + ig], dS[G + ig]);
           const double s1 = S[2 * G + ig], s2 = S[3 * G + ig];
@@ -279,10 +294,12 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
           const double d2 = Wd[(int64_t)p * G + ig];
           const double dd2 = Wdd[(int64_t)p * G + ig];
           const int64_t bo = (int64_t)(n2 - 1) * G + ig;
-          const dz jm = zmk(mkd(B[4 * NG + bo], dB[4 * NG + bo]),
-                            mkd(B[5 * NG + bo], dB[5 * NG + bo]));
-          const dz djm = zmk(mkd(B[6 * NG + bo], dB[6 * NG + bo]),
-                             mkd(B[7 * NG + bo], dB[7 * NG + bo]));
+          cdbl bjm, bdjm;
+          bessel_mkr(S, B, NG, G, n2, ig, bjm, bdjm);
+          const dz jm = zmk(mkd(bjm.x, dB[4 * NG + bo]),
+                            mkd(bjm.y, dB[5 * NG + bo]));
+          const dz djm = zmk(mkd(bdjm.x, dB[6 * NG + bo]),
+                             mkd(bdjm.y, dB[7 * NG + bo]));
           const double nn2 = n2 * (n2 + 1.);
           const double a2 = sqrt((2. * n2 + 1.) / nn2);
           const dz f[5] = {zscalec(djm, a2 * d2), zscalec(djm, a2 * dd2),

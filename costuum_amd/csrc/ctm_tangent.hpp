@@ -10,7 +10,9 @@ namespace ctm {
 // Parameter directions of the tangents.
 constexpr int kTangentDirs = 4;  // R_V, d, Re m, Im m
 
-// Tangent tables: ttables[dir * table_size + layout of tables].
+// Tangent tables: direction dir of the set at base offset off starts at
+// ttables[dir * 2 * table_size + 2 * off] (2 * kTangentDirs * table_size
+// doubles in all).
 hipError_t launch_tables_tangent(const TableSet *sets, int n_sets,
                                  const GrainParams &gp, const double *gl,
                                  const double *tables, int64_t table_size,
