@@ -153,11 +153,16 @@ __global__ void k_wigner(const WigKey *keys, const double *gl, double *wig) {
 // K1: per-(grain, order) node scalars and Bessel tables.  One thread per
 // node over all sets of the list (a thread finds its set by binary search
 // in the sets' node offsets, which start at node_base for the first set of
-// the launch); the ratio recurrence of j_n runs down from
-// the start order and parks the ratios for n < N in the output rows before
-// the forward pass overwrites them with j_n.  Only the functions are stored
-// (the readers form the derivatives from neighbouring orders): j_0, y_0 and
-// j_0(m kr) go with the node scalars.
+// the launch).  j_n (at kr and at m kr): the ratio recurrence
+// r_n = j_n / j_{n-1} runs down from the start order, and the product of the
+// ratios below N gives j_{N-1} = j_0 r_1 ... r_{N-1}; from j_N and j_{N-1}
+// the three-term recurrence runs down (the direction in which j_n, the
+// minimal solution, is stable) and writes every j_n once.  Only when
+// j_{N-1} lies below the normal range (tiny arguments at high order) are
+// the ratios parked in the output rows and multiplied up instead.  Only the
+// functions are stored (the readers form the derivatives from neighbouring
+// orders): j_0, y_0 and j_0(m kr) go with the node scalars.
+constexpr double kMillerFloor = 1e-280;
 __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
                                                 int n_sets, int node_base,
                                                 int n_nodes,
@@ -214,18 +219,39 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
     S[5 * G + ig] = krmrinv.x;
     S[6 * G + ig] = krmrinv.y;
 
-    // j_n(kr): downward ratio recurrence, then forward products.
+    // j_n(kr)
+    const int ns = bessel_start(N, kr);
+    const double j0 = sin(kr) * krinv;
+    S[7 * G + ig] = j0;
+    bool parked;
     {
       const double zinv = krinv;
-      const int ns = bessel_start(N, kr);
+      double rho = kr / (2. * ns + 1.), rho_top = 0., prod = 1.;
+      for (int idx = ns - 1; idx >= 1; --idx) {
+        rho = 1. / ((2. * idx + 1.) * zinv - rho);
+        if (idx == N) rho_top = rho;
+        if (idx < N) prod *= rho;
+      }
+      double a0 = j0 * prod;  // j_{N-1}
+      parked = !(fabs(a0) >= kMillerFloor);
+      if (!parked) {
+        double a1 = a0 * rho_top;  // j_N
+        bj[(int64_t)(N - 1) * G + ig] = a1;
+        for (int n = N - 1; n >= 1; --n) {
+          bj[(int64_t)(n - 1) * G + ig] = a0;
+          const double am = (2. * n + 1.) * zinv * a0 - a1;
+          a1 = a0;
+          a0 = am;
+        }
+      }
+    }
+    if (parked) {
+      const double zinv = krinv;
       double rho = kr / (2. * ns + 1.);
-      if (ns - 1 < N) bj[(int64_t)(ns - 1) * G + ig] = rho;
       for (int idx = ns - 1; idx >= 1; --idx) {
         rho = 1. / ((2. * idx + 1.) * zinv - rho);
         if (idx - 1 < N) bj[(int64_t)(idx - 1) * G + ig] = rho;
       }
-      const double j0 = sin(kr) * zinv;
-      S[7 * G + ig] = j0;
       double jprev = bj[ig] * j0;
       bj[ig] = jprev;
       // the parked ratios are read in chunks of 8 ahead of the chunk's
@@ -263,12 +289,43 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
         }
       }
     }
-    // j_n(m kr): complex downward ratio recurrence.
+    // j_n(m kr), as j_n(kr) in complex arithmetic
+    const int nsm = bessel_start(N, sqrt(krmr.x * krmr.x + krmr.y * krmr.y));
+    // sin(z) = sin x cosh y + i cos x sinh y
+    const cdbl szm = cmk(sin(krmr.x) * cosh(krmr.y), cos(krmr.x) * sinh(krmr.y));
+    const cdbl jm0 = cmul(szm, krmrinv);
+    S[9 * G + ig] = jm0.x;
+    S[10 * G + ig] = jm0.y;
     {
       const cdbl zinv = krmrinv;
-      const int ns = bessel_start(N, sqrt(krmr.x * krmr.x + krmr.y * krmr.y));
-      cdbl rho = cscale(krmr, 1. / (2. * ns + 1.));
-      for (int idx = ns - 1; idx >= 1; --idx) {
+      cdbl rho = cscale(krmr, 1. / (2. * nsm + 1.)), rho_top = cmk(0., 0.);
+      cdbl prod = cmk(1., 0.);
+      for (int idx = nsm - 1; idx >= 1; --idx) {
+        const double f = 2. * idx + 1.;
+        rho = crcp(csub(cscale(zinv, f), rho));
+        if (idx == N) rho_top = rho;
+        if (idx < N) prod = cmul(prod, rho);
+      }
+      cdbl a0 = cmul(jm0, prod);  // j_{N-1}(m kr)
+      parked = !(fmax(fabs(a0.x), fabs(a0.y)) >= kMillerFloor);
+      if (!parked) {
+        cdbl a1 = cmul(a0, rho_top);  // j_N(m kr)
+        bjmr[(int64_t)(N - 1) * G + ig] = a1.x;
+        bjmi[(int64_t)(N - 1) * G + ig] = a1.y;
+        for (int n = N - 1; n >= 1; --n) {
+          const int64_t o = (int64_t)(n - 1) * G + ig;
+          bjmr[o] = a0.x;
+          bjmi[o] = a0.y;
+          const cdbl am = csub(cmul(cscale(zinv, 2. * n + 1.), a0), a1);
+          a1 = a0;
+          a0 = am;
+        }
+      }
+    }
+    if (parked) {
+      const cdbl zinv = krmrinv;
+      cdbl rho = cscale(krmr, 1. / (2. * nsm + 1.));
+      for (int idx = nsm - 1; idx >= 1; --idx) {
         const double f = 2. * idx + 1.;
         rho = crcp(csub(cscale(zinv, f), rho));
         if (idx - 1 < N) {
@@ -276,12 +333,7 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
           bjmi[(int64_t)(idx - 1) * G + ig] = rho.y;
         }
       }
-      // sin(z) = sin x cosh y + i cos x sinh y
-      const cdbl sz = cmk(sin(krmr.x) * cosh(krmr.y), cos(krmr.x) * sinh(krmr.y));
-      const cdbl j0 = cmul(sz, zinv);
-      S[9 * G + ig] = j0.x;
-      S[10 * G + ig] = j0.y;
-      cdbl jprev = cmul(cmk(bjmr[ig], bjmi[ig]), j0);
+      cdbl jprev = cmul(cmk(bjmr[ig], bjmi[ig]), jm0);
       bjmr[ig] = jprev.x;
       bjmi[ig] = jprev.y;
       for (int i0 = 1; i0 < N; i0 += 8) {
