@@ -219,19 +219,45 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
     S[5 * G + ig] = krmrinv.x;
     S[6 * G + ig] = krmrinv.y;
 
-    // j_n(kr)
+    // j_n(kr) and j_n(m kr): both ratio recurrences in one loop from the
+    // larger start order (a start above the needed one reaches the same
+    // ratios), so the two dependency chains overlap
     const int ns = bessel_start(N, kr);
+    const int nsm = bessel_start(N, sqrt(krmr.x * krmr.x + krmr.y * krmr.y));
+    const int ntop = ns > nsm ? ns : nsm;
     const double j0 = sin(kr) * krinv;
     S[7 * G + ig] = j0;
+    // sin(z) = sin x cosh y + i cos x sinh y
+    const cdbl szm = cmk(sin(krmr.x) * cosh(krmr.y), cos(krmr.x) * sinh(krmr.y));
+    const cdbl jm0 = cmul(szm, krmrinv);
+    S[9 * G + ig] = jm0.x;
+    S[10 * G + ig] = jm0.y;
+    double rho_top = 0., prod = 1.;
+    cdbl rhom_top = cmk(0., 0.), prodm = cmk(1., 0.);
+    {
+      const double zinv = krinv;
+      const cdbl zinvm = krmrinv;
+      double rho = kr / (2. * ntop + 1.);
+      cdbl rhom = cscale(krmr, 1. / (2. * ntop + 1.));
+      for (int idx = ntop - 1; idx >= 1; --idx) {
+        const double f = 2. * idx + 1.;
+        rho = 1. / (f * zinv - rho);
+        const cdbl den = csub(cscale(zinvm, f), rhom);
+        const double dinv = 1. / (den.x * den.x + den.y * den.y);
+        rhom = cmk(den.x * dinv, -den.y * dinv);
+        if (idx == N) {
+          rho_top = rho;
+          rhom_top = rhom;
+        }
+        if (idx < N) {
+          prod *= rho;
+          prodm = cmul(prodm, rhom);
+        }
+      }
+    }
     bool parked;
     {
       const double zinv = krinv;
-      double rho = kr / (2. * ns + 1.), rho_top = 0., prod = 1.;
-      for (int idx = ns - 1; idx >= 1; --idx) {
-        rho = 1. / ((2. * idx + 1.) * zinv - rho);
-        if (idx == N) rho_top = rho;
-        if (idx < N) prod *= rho;
-      }
       double a0 = j0 * prod;  // j_{N-1}
       parked = !(fabs(a0) >= kMillerFloor);
       if (!parked) {
@@ -290,26 +316,12 @@ __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
       }
     }
     // j_n(m kr), as j_n(kr) in complex arithmetic
-    const int nsm = bessel_start(N, sqrt(krmr.x * krmr.x + krmr.y * krmr.y));
-    // sin(z) = sin x cosh y + i cos x sinh y
-    const cdbl szm = cmk(sin(krmr.x) * cosh(krmr.y), cos(krmr.x) * sinh(krmr.y));
-    const cdbl jm0 = cmul(szm, krmrinv);
-    S[9 * G + ig] = jm0.x;
-    S[10 * G + ig] = jm0.y;
     {
       const cdbl zinv = krmrinv;
-      cdbl rho = cscale(krmr, 1. / (2. * nsm + 1.)), rho_top = cmk(0., 0.);
-      cdbl prod = cmk(1., 0.);
-      for (int idx = nsm - 1; idx >= 1; --idx) {
-        const double f = 2. * idx + 1.;
-        rho = crcp(csub(cscale(zinv, f), rho));
-        if (idx == N) rho_top = rho;
-        if (idx < N) prod = cmul(prod, rho);
-      }
-      cdbl a0 = cmul(jm0, prod);  // j_{N-1}(m kr)
+      cdbl a0 = cmul(jm0, prodm);  // j_{N-1}(m kr)
       parked = !(fmax(fabs(a0.x), fabs(a0.y)) >= kMillerFloor);
       if (!parked) {
-        cdbl a1 = cmul(a0, rho_top);  // j_N(m kr)
+        cdbl a1 = cmul(a0, rhom_top);  // j_N(m kr)
         bjmr[(int64_t)(N - 1) * G + ig] = a1.x;
         bjmi[(int64_t)(N - 1) * G + ig] = a1.y;
         for (int n = N - 1; n >= 1; --n) {
