@@ -45,7 +45,11 @@ struct DevBuf {
   hipError_t ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
     if (p) {
-      hipError_t e = hipFree(p);
+      // launches on any of the context's streams may still read the old
+      // buffer
+      hipError_t e = hipDeviceSynchronize();
+      if (e != hipSuccess) return e;
+      e = hipFree(p);
       if (e != hipSuccess) return e;
       p = nullptr;
       cap = 0;
@@ -301,9 +305,18 @@ struct Profiler {
 
 }  // namespace
 
+// Side streams that take independent launches of one phase (assembly
+// parts, solve buckets, observable buckets) beside the context's stream.
+constexpr int kLanes = 4;
+
 struct ctm_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // lane 0 is `stream`; lanes 1..kLanes-1 are side streams of the default
+  // priority, ordered against `stream` by fork_lanes / join_lanes
+  hipStream_t side[kLanes - 1] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kLanes - 1] = {};
   std::string err;
   SharedTables shared;
   WorkList work;
@@ -343,15 +356,44 @@ struct ctm_context {
     }                                                                   \
   } while (0)
 
-// Launch `call` between two profiler events when profiling is on.
-#define CTM_LAUNCH(kernel, flops, bytes, call)                          \
-  do {                                                                  \
-    const int _pi = ctx->prof.begin(kernel, ctx->stream, flops, bytes); \
-    CTM_HIP(call);                                                      \
-    ctx->prof.end(_pi, ctx->stream);                                    \
+// Launch `call` on stream `strm` between two profiler events when profiling
+// is on.
+#define CTM_LAUNCH_ON(strm, kernel, flops, bytes, call)           \
+  do {                                                            \
+    const int _pi = ctx->prof.begin(kernel, strm, flops, bytes);  \
+    CTM_HIP(call);                                                \
+    ctx->prof.end(_pi, strm);                                     \
   } while (0)
+#define CTM_LAUNCH(kernel, flops, bytes, call) \
+  CTM_LAUNCH_ON(ctx->stream, kernel, flops, bytes, call)
 
 namespace {
+
+hipStream_t lane(const ctm_context *ctx, int k) {
+  return k == 0 ? ctx->stream : ctx->side[k - 1];
+}
+
+// Orders the side streams after everything queued on the context's stream
+// so far.  The launches of a phase that follow are independent of each
+// other (disjoint outputs), so a launch's tail overlaps the next one's start.
+hipError_t fork_lanes(ctm_context *ctx) {
+  hipError_t e = hipEventRecord(ctx->ev_fork, ctx->stream);
+  for (int k = 0; e == hipSuccess && k < kLanes - 1; ++k)
+    e = hipStreamWaitEvent(ctx->side[k], ctx->ev_fork, 0);
+  return e;
+}
+
+// Orders the context's stream after everything queued on the side streams:
+// every fork_lanes is followed by join_lanes before the context's stream is
+// synchronised or its results are read.
+hipError_t join_lanes(ctm_context *ctx) {
+  hipError_t e = hipSuccess;
+  for (int k = 0; e == hipSuccess && k < kLanes - 1; ++k) {
+    e = hipEventRecord(ctx->ev_join[k], ctx->side[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->ev_join[k], 0);
+  }
+  return e;
+}
 
 // Host-to-device copy through the context's pinned staging arena: a copy
 // from pinned memory runs asynchronously, a pageable one stages through the
@@ -562,7 +604,6 @@ void build_tasks(WorkList &w, int i0, int i1) {
 // in LDS-size buckets; ainv_out (may be NULL) receives every Q_c^-1.
 static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
   WorkList &w = ctx->work;
-  hipStream_t s = ctx->stream;
   // solve tasks in LDS-size buckets, largest first; blocks above the LDS
   // capacity are inverted in global scratch, at most kGlobalSolveBytes of it
   // per launch
@@ -571,6 +612,10 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
   // together and one row per lane suffices)
   const int bounds[] = {kMaxOrder, kMaxLdsOrder, 64, 48, 32, 16};
   constexpr int kBuckets = 6;
+  // buckets side by side on the lanes; the launches of the global bucket
+  // share one scratch buffer and stay on lane 0 in order
+  const int lane_of[kBuckets] = {0, 1, 2, 3, 0, 1};
+  CTM_HIP(fork_lanes(ctx));
   size_t pos = 0;
   const int32_t *dsolve = ctx->d_solve.as<int32_t>();
   for (int b = 0; b < kBuckets && pos < w.solve.size(); ++b) {
@@ -612,15 +657,17 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
         store_max = std::max(store_max, r0 * r0 + r1 * r1);
         rows_max = std::max(rows_max, (int)r0);
       }
-      CTM_LAUNCH(CTM_KERNEL_SOLVE, fs, bs,
-                 launch_invert_solve(dsolve + pos, (int)(end - pos), nm_max,
-                                     store_max, rows_max,
-                                     ctx->d_items.as<Item>(),
-                                     ctx->d_q.as<cdbl>(),
-                                     want_t ? ctx->d_t.as<cdbl>() : nullptr,
-                                     ctx->d_qparts.as<double>(),
-                                     want_t ? 0 : 1,
-                                     ctx->d_gsolve.as<cdbl>(), ainv_out, s));
+      const hipStream_t ls = lane(ctx, lane_of[b]);
+      CTM_LAUNCH_ON(ls, CTM_KERNEL_SOLVE, fs, bs,
+                    launch_invert_solve(dsolve + pos, (int)(end - pos),
+                                        nm_max, store_max, rows_max,
+                                        ctx->d_items.as<Item>(),
+                                        ctx->d_q.as<cdbl>(),
+                                        want_t ? ctx->d_t.as<cdbl>() : nullptr,
+                                        ctx->d_qparts.as<double>(),
+                                        want_t ? 0 : 1,
+                                        ctx->d_gsolve.as<cdbl>(), ainv_out,
+                                        ls));
     }
     pos = end;
     // the global bucket may need several launches
@@ -629,6 +676,7 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
       if (it.N + 1 - (it.m > 0 ? it.m : 1) > kMaxLdsOrder) --b;
     }
   }
+  CTM_HIP(join_lanes(ctx));
   return CTM_OK;
 }
 
@@ -673,7 +721,6 @@ int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   WorkList &w = ctx->work;
   const int ni = (int)w.items.size();
   if (ni == 0) return CTM_OK;
-  hipStream_t s = ctx->stream;
   build_tasks(w, 0, ni);
   int rc = flush_shared(ctx);  // Wigner values of new (N, m, G)
   if (rc != CTM_OK) return rc;
@@ -692,35 +739,45 @@ int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     const int32_t *pre = ctx->d_gioff_tmp.as<int32_t>();
     const int kid[3] = {CTM_KERNEL_ASSEMBLE_WIDE, CTM_KERNEL_ASSEMBLE,
                         CTM_KERNEL_ASSEMBLE_SMALL};
+    // the parts write disjoint blocks.  The 3 x 3 part (most of the work,
+    // enough tiles to fill the GPU) runs alone; the others follow side by
+    // side on the lanes
+    const int lane_of[5] = {0, 1, 2, 3, 2};
     size_t po = 0;
     for (int part = 0; part < 3; ++part) {
+      if (part == 1) CTM_HIP(fork_lanes(ctx));
       const int np = w.n_part[part];
       if (np > 0)
-        CTM_LAUNCH(kid[part], w.f_asm[part], w.b_asm[part],
-                   launch_assemble(3 - part, ord, pre + po, np,
-                                   w.tile_prefix[po + np],
-                                   ctx->d_items.as<Item>(),
-                                   ctx->d_sets.as<TableSet>(),
-                                   ctx->d_wigkeys.as<WigKey>(), gp,
-                                   ctx->d_tables.as<double>(),
-                                   ctx->d_wig.as<double>(),
-                                   ctx->d_q.as<cdbl>(), s));
+        CTM_LAUNCH_ON(lane(ctx, lane_of[part]), kid[part], w.f_asm[part],
+                      w.b_asm[part],
+                      launch_assemble(3 - part, ord, pre + po, np,
+                                      w.tile_prefix[po + np],
+                                      ctx->d_items.as<Item>(),
+                                      ctx->d_sets.as<TableSet>(),
+                                      ctx->d_wigkeys.as<WigKey>(), gp,
+                                      ctx->d_tables.as<double>(),
+                                      ctx->d_wig.as<double>(),
+                                      ctx->d_q.as<cdbl>(),
+                                      lane(ctx, lane_of[part])));
       ord += np;
       po += np + 1;
     }
     for (int part = 3; part < 5; ++part) {
       const int np = w.n_part[part];
       if (np > 0)
-        CTM_LAUNCH(CTM_KERNEL_ASSEMBLE_PACK, w.f_asm[part], w.b_asm[part],
-                   launch_assemble_pack4(ord, np, part == 4 ? 1 : 0,
-                                         ctx->d_items.as<Item>(),
-                                         ctx->d_sets.as<TableSet>(),
-                                         ctx->d_wigkeys.as<WigKey>(), gp,
-                                         ctx->d_tables.as<double>(),
-                                         ctx->d_wig.as<double>(),
-                                         ctx->d_q.as<cdbl>(), s));
+        CTM_LAUNCH_ON(lane(ctx, lane_of[part]), CTM_KERNEL_ASSEMBLE_PACK,
+                      w.f_asm[part], w.b_asm[part],
+                      launch_assemble_pack4(ord, np, part == 4 ? 1 : 0,
+                                            ctx->d_items.as<Item>(),
+                                            ctx->d_sets.as<TableSet>(),
+                                            ctx->d_wigkeys.as<WigKey>(), gp,
+                                            ctx->d_tables.as<double>(),
+                                            ctx->d_wig.as<double>(),
+                                            ctx->d_q.as<cdbl>(),
+                                            lane(ctx, lane_of[part])));
       ord += np;
     }
+    CTM_HIP(join_lanes(ctx));
   }
   return run_solves(ctx, want_t, nullptr);
 }
@@ -757,8 +814,15 @@ int ctm_create(ctm_context **out, int device) {
   ctx->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  for (int k = 0; e == hipSuccess && k < kLanes - 1; ++k) {
+    e = hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking);
+    if (e == hipSuccess)
+      e = hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming);
+  }
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
   if (e != hipSuccess) {
-    delete ctx;
+    (void)ctm_destroy(ctx);
     return CTM_ERR_HIP;
   }
   *out = ctx;
@@ -769,6 +833,8 @@ int ctm_destroy(ctm_context *ctx) {
   if (!ctx) return CTM_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (hipStream_t st : ctx->side)
+    if (st) (void)hipStreamSynchronize(st);
   DevBuf *bufs[] = {&ctx->d_gl,     &ctx->d_glkeys,  &ctx->d_wig,
                     &ctx->d_wigkeys, &ctx->d_tables, &ctx->d_sets,
                     &ctx->d_items,  &ctx->d_tiles,   &ctx->d_solve,
@@ -788,6 +854,11 @@ int ctm_destroy(ctm_context *ctx) {
   ctx->staging.release();
   ctx->readback.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  for (int k = 0; k < kLanes - 1; ++k) {
+    if (ctx->side[k]) (void)hipStreamDestroy(ctx->side[k]);
+    if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
+  }
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   delete ctx;
   return CTM_OK;
 }
@@ -1457,6 +1528,8 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
     // thread, and up to order 80 two such chunks side by side fit the LDS)
     const int caps[] = {nmax_all, 80, 64, 32, 16};
     constexpr int kCaps = 5;
+    // the buckets write disjoint items: side by side on the lanes
+    CTM_HIP(fork_lanes(ctx));
     size_t pos = 0;
     for (int b = 0; b < kCaps && pos < nitems; ++b) {
       const int cap = caps[b];
@@ -1470,23 +1543,26 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
         ++end;
       }
       if (end > pos) {
-        CTM_LAUNCH(CTM_KERNEL_OBS_X, fb, 0.,
-                   launch_obs_x(sp, ctx->d_obsitems.as<int32_t>() + pos,
-                                (int)(end - pos), cap, cap > 64 ? 8 : 4,
-                                ctx->d_items.as<Item>(),
-                                ctx->obs_source >= 0
-                                    ? ctx->d_dt.as<cdbl>() +
-                                          (int64_t)ctx->obs_source *
-                                              ctx->work.t_size
-                                    : ctx->d_t.as<cdbl>(),
-                                ctx->averaged ? ctx->d_avg.as<cdbl>() : nullptr,
-                                ctx->averaged ? ctx->d_aoff.as<int64_t>()
-                                              : nullptr,
-                                ctx->d_pt.as<double>(), nmax_all,
-                                ctx->d_x.as<cdbl>(), s));
+        const hipStream_t ls = lane(ctx, b % kLanes);
+        CTM_LAUNCH_ON(ls, CTM_KERNEL_OBS_X, fb, 0.,
+                      launch_obs_x(sp, ctx->d_obsitems.as<int32_t>() + pos,
+                                   (int)(end - pos), cap, cap > 64 ? 8 : 4,
+                                   ctx->d_items.as<Item>(),
+                                   ctx->obs_source >= 0
+                                       ? ctx->d_dt.as<cdbl>() +
+                                             (int64_t)ctx->obs_source *
+                                                 ctx->work.t_size
+                                       : ctx->d_t.as<cdbl>(),
+                                   ctx->averaged ? ctx->d_avg.as<cdbl>()
+                                                 : nullptr,
+                                   ctx->averaged ? ctx->d_aoff.as<int64_t>()
+                                                 : nullptr,
+                                   ctx->d_pt.as<double>(), nmax_all,
+                                   ctx->d_x.as<cdbl>(), ls));
       }
       pos = end;
     }
+    CTM_HIP(join_lanes(ctx));
   }
   if (nth > 0 && (qext || qextpol || qextcpol || qabs || qabspol)) {
     CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
