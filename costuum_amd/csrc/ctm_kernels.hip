@@ -647,8 +647,10 @@ __global__ __launch_bounds__(256) void k_assemble(
     const GrainParams gp, const double *tables, const double *wig,
     cdbl *qbuf) {
   constexpr int TR = 16 * MT;            // rows of pairs per tile
-  // quadrature nodes per LDS stage (two blocks per CU for every MT)
-  constexpr int CH = MT == 1 ? 8 : MT == 2 ? 4 : 2;
+  // quadrature nodes per LDS stage: two blocks per CU for MT = 2, 3 (their
+  // registers allow no more); MT = 1 (120 registers) runs four blocks per
+  // CU on 28 KiB of LDS each (measured faster than 8, 6 or 2 nodes)
+  constexpr int CH = MT == 1 ? 4 : MT == 2 ? 4 : 2;
   __shared__ double Lsh[CH][TR][kLeftStride];
   __shared__ double Rsh[CH][2 * TR][kRightStride];
 
