@@ -10,9 +10,10 @@
 // K3 k_assemble        Q/RgQ class blocks, one 16x16 tile of (n1, n2)
 //                      pairs per workgroup (TMatrixResource.hpp:601-709,
 //                      927-1100)
-// K4 k_invert_solve    Gauss-Jordan inverse of Q_c in LDS and
-//                      T_c = -RgQ_c Q_c^-1 (Matrix.hpp:261-384,
-//                      TMatrixResource.hpp:712-730,1102-1119)
+// K4 k_invert_solve_*  Gauss-Jordan inverse of Q_c (registers, LDS or
+//                      global scratch) and T_c = -RgQ_c Q_c^-1
+//                      (Matrix.hpp:261-384, TMatrixResource.hpp:712-730,
+//                      1102-1119)
 // K5 k_converge        per-grain order test (TMatrixResource.hpp:743-775)
 // K5 k_full_sums       per-grain Q_sca/Q_ext over all m (TMatrix.hpp:2055-2110)
 #include "ctm_device.hpp"
@@ -923,24 +924,17 @@ __global__ __launch_bounds__(256, 2) void k_assemble_pack4(
 }
 
 // ---------------------------------------------------------------------------
-// K4: T_c = -RgQ_c Q_c^-1 for one (item, class).  Q_c is inverted in place in
-// LDS by Gauss-Jordan elimination with partial pivoting on |Q_ij| (largest
+// K4: T_c = -RgQ_c Q_c^-1 for one (item, class).  Q_c is inverted in place
+// by Gauss-Jordan elimination with partial pivoting on |Q_ij| (largest
 // modulus, first index on ties: the rule of Matrix.hpp:280-293); the column
 // exchanges of the final unpermutation (Matrix.hpp:367-383) are folded into
 // the product's column index.  For m = 0 the class block splits further into
 // the rows of even and odd p (only n1 + n2 even couples,
-// TMatrixResource.hpp:622): the two sub-blocks are stored side by side
-// (permuted index) and eliminated concurrently, one pivot each per step.
-// Each step costs two barriers: the pivot of the next column is reduced
-// while the current update is applied.  The product streams RgQ_c from HBM
+// TMatrixResource.hpp:622): the two sub-blocks are stored apart (permuted
+// index) and eliminated concurrently.  The product streams RgQ_c from HBM
 // in groups of four rows per wave.  With diag_only only the diagonal of T is
 // formed (all the convergence test needs).
-// GLOBAL: blocks of order above the LDS capacity keep Q_c in a per-block
-// slice (gstride complex values) of a global scratch buffer instead; the
-// pivot bookkeeping stays in LDS and the access pattern is unchanged
-// (512-byte row segments per half-wave), so the working set streams through
-// L2 and the Infinity Cache.
-// Shared end of both solve kernels: the column unpermutation of the
+// Shared end of the solve kernels: the column unpermutation of the
 // Gauss-Jordan inverse Ainv (held in A, LDS or global), the product
 // T_c = -RgQ_c Ainv (or only its diagonal) and the per-block sums.
 template <int NT>
@@ -1096,195 +1090,12 @@ __device__ __forceinline__ void solve_tail(
   }
 }
 
-template <int NT, bool GLOBAL>
-__global__ __launch_bounds__(NT) void k_invert_solve(
-    const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
-    double *qparts, int diag_only, cdbl *gscratch, int64_t gstride,
-    cdbl *ainv_out) {
-  constexpr int NW = NT / 64;
-  extern __shared__ double smem[];
-  __shared__ double wb_mag[NW][2];
-  __shared__ int wb_row[NW][2];
-  __shared__ double red[NW][4];
-
-  const int task = tasks[blockIdx.x];
-  const int item_index = task >> 1;
-  const int cls = task & 1;
-  const Item it = items[item_index];
-  const int m = it.m;
-  const int nmin = m > 0 ? m : 1;
-  const int nm = it.N + 1 - nmin;
-  const int64_t nm2 = (int64_t)nm * nm;
-  const int nb = (m == 0 && nm > 1) ? 2 : 1;
-  const int sz0 = (nb == 2) ? (nm + 1) / 2 : nm;
-  const int off[2] = {0, sz0};
-  const int sz[2] = {sz0, nm - sz0};
-  // permuted index -> p
-  auto pidx = [&](int i) {
-    return nb == 2 ? (i < sz0 ? 2 * i : 2 * (i - sz0) + 1) : i;
-  };
-
-  cdbl *A = GLOBAL ? gscratch + (int64_t)blockIdx.x * gstride
-                  : reinterpret_cast<cdbl *>(smem);
-  cdbl *colk = GLOBAL ? reinterpret_cast<cdbl *>(smem) : A + nm2;
-  cdbl *rowk = colk + nm;
-  cdbl *rowold = rowk + nm;
-  int *piv = reinterpret_cast<int *>(rowold + nm);
-  int *src = piv + nm;
-
-  const cdbl *Qg = qbuf + it.q_off + cls * nm2;
-  const cdbl *RQg = qbuf + it.q_off + (2 + cls) * nm2;
-  const int t = threadIdx.x;
-  const int lane = t & 63, wv = t >> 6;
-
-  for (int i = t >> 5; i < nm; i += NT >> 5) {
-    const int pi = pidx(i);
-    const int bi = i >= sz0;
-    for (int j = t & 31; j < nm; j += 32) {
-      const int bj = j >= sz0;
-      A[(int64_t)i * nm + j] =
-          (bi == bj) ? Qg[(int64_t)pi * nm + pidx(j)] : cmk(0., 0.);
-    }
-  }
-  __syncthreads();
-
-  // wave-level argmax of (mag, row): larger mag wins, lower row on ties
-  auto wave_best = [&](double &mag, int &row) {
-    for (int o = 32; o > 0; o >>= 1) {
-      const double om = __shfl_xor(mag, o);
-      const int orow = __shfl_xor(row, o);
-      if (om > mag || (om == mag && orow < row)) {
-        mag = om;
-        row = orow;
-      }
-    }
-  };
-  // pivot candidates of the first column of each sub-block
-  {
-    double bm[2] = {-1., -1.};
-    int br[2] = {nm, nm};
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      if (b >= nb) continue;
-      const int k = off[b];
-      for (int i = k + t; i < off[b] + sz[b]; i += NT) {
-        const cdbl v = A[(int64_t)i * nm + k];
-        const double mag = v.x * v.x + v.y * v.y;
-        if (mag > bm[b]) {
-          bm[b] = mag;
-          br[b] = i;
-        }
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      wave_best(bm[b], br[b]);
-      if (lane == 0) {
-        wb_mag[wv][b] = bm[b];
-        wb_row[wv][b] = br[b];
-      }
-    }
-  }
-  __syncthreads();
-
-  for (int s = 0; s < sz0; ++s) {
-    int kb[2], pb[2];
-    bool act[2];
-    cdbl inv[2];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      act[b] = b < nb && s < sz[b];
-      kb[b] = off[b] + s;
-      double best = -1.;
-      int row = nm;
-      for (int w = 0; w < NW; ++w) {
-        const double mg = wb_mag[w][b];
-        const int rw = wb_row[w][b];
-        if (mg > best || (mg == best && rw < row)) {
-          best = mg;
-          row = rw;
-        }
-      }
-      pb[b] = act[b] ? row : kb[b];
-      inv[b] = cmk(1., 0.);
-      if (act[b]) inv[b] = crcp(A[(int64_t)pb[b] * nm + kb[b]]);
-    }
-    // phase A: pivot row (scaled), old row k, column k after the swap
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      if (!act[b]) continue;
-      const int k = kb[b], p = pb[b];
-      for (int j = off[b] + t; j < off[b] + sz[b]; j += NT) {
-        rowk[j] = cmul(A[(int64_t)p * nm + j], inv[b]);
-        rowold[j] = A[(int64_t)k * nm + j];
-        const int i = j;
-        colk[i] = (i == k) ? A[(int64_t)p * nm + k]
-                           : (i == p ? A[(int64_t)k * nm + k]
-                                     : A[(int64_t)i * nm + k]);
-      }
-      if (t == 0) piv[k] = p;
-    }
-    __syncthreads();
-    // phase B: update, and candidates for the next pivot
-    double bm[2] = {-1., -1.};
-    int br[2] = {nm, nm};
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      if (!act[b]) continue;
-      const int k = kb[b], p = pb[b];
-      const int lo = off[b], hi = off[b] + sz[b];
-      for (int i = lo + (t >> 5); i < hi; i += NT >> 5) {
-        const cdbl ci = colk[i];
-        for (int j = lo + (t & 31); j < hi; j += 32) {
-          const int64_t e = (int64_t)i * nm + j;
-          cdbl v;
-          if (i == k) {
-            v = (j == k) ? inv[b] : rowk[j];
-          } else {
-            const cdbl base = (i == p) ? rowold[j] : A[e];
-            if (j == k) {
-              v = cmul(ci, inv[b]);
-              v.x = -v.x;
-              v.y = -v.y;
-            } else {
-              const cdbl g = rowk[j];
-              v.x = base.x - (ci.x * g.x - ci.y * g.y);
-              v.y = base.y - (ci.x * g.y + ci.y * g.x);
-            }
-          }
-          A[e] = v;
-          if (j == k + 1 && i > k) {
-            const double mag = v.x * v.x + v.y * v.y;
-            if (mag > bm[b] || (mag == bm[b] && i < br[b])) {
-              bm[b] = mag;
-              br[b] = i;
-            }
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      wave_best(bm[b], br[b]);
-      if (lane == 0) {
-        wb_mag[wv][b] = bm[b];
-        wb_row[wv][b] = br[b];
-      }
-    }
-    __syncthreads();
-  }
-  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, nm,
-                 nb == 2 ? A + (int64_t)sz0 * nm + sz0 : A, nm, piv, src, RQg,
-                 tbuf, qparts, diag_only, red, ainv_out);
-}
-
-// K4 for blocks up to order 48: the same elimination (pivot rule, swaps,
-// update expressions; bit-identical results) with the matrix held in
-// registers.  Thread (ty, tx) of a TRR x TCC grid owns rows ty + TRR a
-// (a < RPT) and columns tx + TCC c (c < CPT).  A step exchanges only the
-// pivot row, the old row k and column k through LDS (3 nm values) instead of
-// reading and writing the whole matrix there; the pivot value of the next
-// step travels with the (magnitude, row) reduction.  The inverse is stored
+// K4 for blocks up to order 16: the elimination one column per step with
+// the matrix held in registers.  Thread (ty, tx) of a TRR x TCC grid owns
+// rows ty + TRR a (a < RPT) and columns tx + TCC c (c < CPT).  A step
+// exchanges only the pivot row, the old row k and column k through LDS
+// (3 nm values); the pivot value of the next step travels with the
+// (magnitude, row) reduction.  The inverse is stored
 // to LDS once at the end for the product.
 template <int NT, int TCC, int RPT, int CPT>
 __global__ __launch_bounds__(NT) void k_invert_solve_reg(
@@ -1526,7 +1337,7 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
 // column precede its stores in the wave's order, so no barrier).  Look-ahead:
 // the wave that eliminates the next panel first applies this update to that
 // panel's columns, while the other waves of the sub-block update the rest;
-// one barrier separates panels.  Same pivot rule as k_invert_solve (largest
+// one barrier separates panels.  Same pivot rule as k_invert_solve_reg (largest
 // |a_ik|^2 over rows i >= k, first row on ties); results equal it to
 // rounding.  Each sub-block is stored compactly (sz x sz, one after the
 // other), so an m = 0 block holds half the LDS of its full square.
@@ -1718,10 +1529,14 @@ __device__ __forceinline__ void blk_update(cdbl *Ab, int ld, int lo, int hi,
   }
 }
 
-template <int NT, int RS, int W>
+// GLOBAL: the sub-blocks live in gscratch (gstride complex values per
+// workgroup) instead of LDS, for blocks above the LDS capacity; the panel
+// and update passes then stream the matrix through L2 once per W columns.
+template <int NT, int RS, int W, bool GLOBAL>
 __global__ __launch_bounds__(NT) void k_invert_solve_blk(
     const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
-    double *qparts, int diag_only, cdbl *ainv_out) {
+    double *qparts, int diag_only, cdbl *gscratch, int64_t gstride,
+    cdbl *ainv_out) {
   constexpr int NW = NT / 64;
   static_assert(NW >= 4, "two waves per sub-block at least");
   extern __shared__ double smem[];
@@ -1745,11 +1560,13 @@ __global__ __launch_bounds__(NT) void k_invert_solve_blk(
   };
   // the sub-blocks stored one after the other, each sz[b] x sz[b] (for
   // m = 0 half the LDS of the full block, whose cross blocks are zero)
-  cdbl *A = reinterpret_cast<cdbl *>(smem);
+  cdbl *A = GLOBAL ? gscratch + (int64_t)blockIdx.x * gstride
+                  : reinterpret_cast<cdbl *>(smem);
   cdbl *Ab[2] = {A, A + (int64_t)sz0 * sz0};
   const int ldb[2] = {sz[0], sz[1]};
   const int64_t nstore = (int64_t)sz[0] * sz[0] + (int64_t)sz[1] * sz[1];
-  int *piv = reinterpret_cast<int *>(A + nstore);
+  int *piv = GLOBAL ? reinterpret_cast<int *>(smem)
+                    : reinterpret_cast<int *>(A + nstore);
   int *src = piv + nm;
   const cdbl *Qg = qbuf + it.q_off + cls * nm2;
   const cdbl *RQg = qbuf + it.q_off + (2 + cls) * nm2;
@@ -1943,26 +1760,6 @@ size_t invert_solve_lds_bytes(int nm) {
          2 * (size_t)nm * sizeof(int);
 }
 
-template <int NT, bool GLOBAL>
-static hipError_t launch_solve_nt(const int32_t *tasks, int n, size_t lds,
-                                  const Item *items, const cdbl *qbuf,
-                                  cdbl *tbuf, double *qparts, int diag_only,
-                                  cdbl *gscratch, int64_t gstride,
-                                  cdbl *ainv_out, hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(k_invert_solve<NT, GLOBAL>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
-  hipLaunchKernelGGL((k_invert_solve<NT, GLOBAL>), dim3(n), dim3(NT), lds, s,
-                     tasks, items, qbuf, tbuf, qparts, diag_only, gscratch,
-                     gstride, ainv_out);
-  return hipGetLastError();
-}
-
 template <int NT, int TCC, int RPT, int CPT>
 static hipError_t launch_solve_reg(const int32_t *tasks, int n, size_t lds,
                                    const Item *items, const cdbl *qbuf,
@@ -1982,21 +1779,24 @@ static hipError_t launch_solve_reg(const int32_t *tasks, int n, size_t lds,
   return hipGetLastError();
 }
 
-template <int NT, int RS, int W>
+template <int NT, int RS, int W, bool GLOBAL = false>
 static hipError_t launch_solve_blk(const int32_t *tasks, int n, size_t lds,
                                    const Item *items, const cdbl *qbuf,
                                    cdbl *tbuf, double *qparts, int diag_only,
-                                   cdbl *ainv_out, hipStream_t s) {
+                                   cdbl *ainv_out, hipStream_t s,
+                                   cdbl *gscratch = nullptr,
+                                   int64_t gstride = 0) {
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(k_invert_solve_blk<NT, RS, W>),
+        reinterpret_cast<const void *>(k_invert_solve_blk<NT, RS, W, GLOBAL>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_invert_solve_blk<NT, RS, W>), dim3(n), dim3(NT), lds,
-                     s, tasks, items, qbuf, tbuf, qparts, diag_only, ainv_out);
+  hipLaunchKernelGGL((k_invert_solve_blk<NT, RS, W, GLOBAL>), dim3(n),
+                     dim3(NT), lds, s, tasks, items, qbuf, tbuf, qparts,
+                     diag_only, gscratch, gstride, ainv_out);
   return hipGetLastError();
 }
 
@@ -2007,12 +1807,14 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                cdbl *ainv_out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (nm_max > kMaxLdsOrder) {
-    // the matrix lives in gscratch (n * nm_max^2 complex values)
-    const size_t lds = invert_solve_lds_bytes(nm_max) -
-                       (size_t)nm_max * nm_max * sizeof(cdbl);
-    return launch_solve_nt<512, true>(tasks, n, lds, items, qbuf, tbuf,
-                                      qparts, diag_only, gscratch,
-                                      (int64_t)nm_max * nm_max, ainv_out, s);
+    // the matrix lives in gscratch (n * nm_max^2 complex values): blocked
+    // elimination with four rows per lane (sub-blocks up to 256 rows), the
+    // pivot bookkeeping in LDS
+    if (rows_max > 4 * 64 || store_max > (int64_t)nm_max * nm_max)
+      return hipErrorInvalidValue;
+    return launch_solve_blk<512, 4, 4, true>(
+        tasks, n, 2 * (size_t)nm_max * sizeof(int), items, qbuf, tbuf, qparts,
+        diag_only, ainv_out, s, gscratch, (int64_t)nm_max * nm_max);
   }
   // the blocked kernel stores the sub-blocks compactly (store_max complex
   // values at most) and needs two rows per lane above 64 rows
