@@ -317,6 +317,9 @@ struct ctm_context {
   hipStream_t side[kLanes - 1] = {};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[kLanes - 1] = {};
+  // end of each assembly part (run_items): a solve bucket waits only for
+  // the parts that assembled its blocks
+  hipEvent_t ev_part[5] = {};
   std::string err;
   SharedTables shared;
   WorkList work;
@@ -528,6 +531,14 @@ int flush_shared(ctm_context *ctx) {
   return CTM_OK;
 }
 
+// Assembly part of an item: micro-tile 3, 2, 1 -> 0, 1, 2; packed -> 3
+// (m > 0) or 4 (m = 0).
+inline int item_part(const Item &it) {
+  const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+  const int micro = assemble_micro(nm, it.m == 0);
+  return micro == 4 ? (it.m == 0 ? 4 : 3) : 3 - micro;
+}
+
 // Builds the cost-sorted item order with the prefix sum of assembly tiles
 // (largest G first) and the solve task list (largest blocks first), both by
 // counting sort, for the items i0 .. i1 - 1 (global indices).  The items are
@@ -551,8 +562,7 @@ void build_tasks(WorkList &w, int i0, int i1) {
   for (int i = 0; i < n; ++i) {
     const Item &it = w.items[i0 + i];
     const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-    const int micro = assemble_micro(nm, it.m == 0);
-    const int part = micro == 4 ? (it.m == 0 ? 4 : 3) : 3 - micro;
+    const int part = item_part(it);
     info[i] = nm | (part << 16) | (it.m == 0 ? (1 << 30) : 0);
     key[i] = it.G;
     w.n_part[part]++;
@@ -602,20 +612,29 @@ void build_tasks(WorkList &w, int i0, int i1) {
 
 // Inversion and solve of the current work list's blocks (Q in ctx->d_q),
 // in LDS-size buckets; ainv_out (may be NULL) receives every Q_c^-1.
-static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
+// parts: the assembly parts run_items launched (their ends recorded in
+// ev_part, the side lanes forked); 0 when Q was written on the context's
+// stream.
+static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out,
+                      unsigned parts = 0) {
   WorkList &w = ctx->work;
   // solve tasks in LDS-size buckets, largest first; blocks above the LDS
   // capacity are inverted in global scratch, at most kGlobalSolveBytes of it
   // per launch
   constexpr size_t kGlobalSolveBytes = (size_t)1 << 30;
-  // (64 splits the LDS bucket where two blocks fit a CU's 160 KiB of LDS
-  // together and one row per lane suffices)
-  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 64, 48, 32, 16};
-  constexpr int kBuckets = 6;
+  // (70: up to this order two blocks fit a CU's 160 KiB of LDS together;
+  // 64: one row per lane suffices)
+  const int bounds[] = {kMaxOrder, kMaxLdsOrder, 70, 64, 48, 32, 16};
+  constexpr int kBuckets = 7;
   // buckets side by side on the lanes; the launches of the global bucket
-  // share one scratch buffer and stay on lane 0 in order
-  const int lane_of[kBuckets] = {0, 1, 2, 3, 0, 1};
-  CTM_HIP(fork_lanes(ctx));
+  // share one scratch buffer and stay on lane 0 in order.  The blocks above
+  // order 64 (the longest chains) and those of order 33-48 come from the
+  // first assembly part (lane 0) alone, the others from the parts on lanes
+  // 1-3: orders above 70 follow the first part on lane 0, 49-64 the second
+  // part on lane 1, 65-70 and 33-48 the packed part on lane 3, up to 32 the
+  // small parts on lane 2 (measured best of the placements tried)
+  const int lane_of[kBuckets] = {0, 0, 3, 1, 3, 2, 2};
+  if (parts == 0) CTM_HIP(fork_lanes(ctx));
   size_t pos = 0;
   const int32_t *dsolve = ctx->d_solve.as<int32_t>();
   for (int b = 0; b < kBuckets && pos < w.solve.size(); ++b) {
@@ -658,6 +677,14 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out) {
         rows_max = std::max(rows_max, (int)r0);
       }
       const hipStream_t ls = lane(ctx, lane_of[b]);
+      if (parts != 0) {
+        unsigned need = 0;
+        for (size_t q = pos; q < end; ++q)
+          need |= 1u << item_part(w.items[w.solve[q] >> 1]);
+        for (int p = 0; p < 5; ++p)
+          if ((need & parts) >> p & 1u)
+            CTM_HIP(hipStreamWaitEvent(ls, ctx->ev_part[p], 0));
+      }
       CTM_LAUNCH_ON(ls, CTM_KERNEL_SOLVE, fs, bs,
                     launch_invert_solve(dsolve + pos, (int)(end - pos),
                                         nm_max, store_max, rows_max,
@@ -732,6 +759,7 @@ int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   if (want_t)
     CTM_HIP(ctx->d_t.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
   CTM_HIP(ctx->d_qparts.ensure((size_t)ni * 2 * kQParts * sizeof(double)));
+  unsigned parts = 0;
   {
     // one launch per part (3 x 3, 2 x 2, 1 x 1 micro-tiles, packed m > 0,
     // packed m = 0), each credited with the algorithmic work of its blocks
@@ -747,7 +775,8 @@ int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     for (int part = 0; part < 3; ++part) {
       if (part == 1) CTM_HIP(fork_lanes(ctx));
       const int np = w.n_part[part];
-      if (np > 0)
+      if (np > 0) {
+        parts |= 1u << part;
         CTM_LAUNCH_ON(lane(ctx, lane_of[part]), kid[part], w.f_asm[part],
                       w.b_asm[part],
                       launch_assemble(3 - part, ord, pre + po, np,
@@ -759,12 +788,15 @@ int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
                                       ctx->d_wig.as<double>(),
                                       ctx->d_q.as<cdbl>(),
                                       lane(ctx, lane_of[part])));
+        CTM_HIP(hipEventRecord(ctx->ev_part[part], lane(ctx, lane_of[part])));
+      }
       ord += np;
       po += np + 1;
     }
     for (int part = 3; part < 5; ++part) {
       const int np = w.n_part[part];
-      if (np > 0)
+      if (np > 0) {
+        parts |= 1u << part;
         CTM_LAUNCH_ON(lane(ctx, lane_of[part]), CTM_KERNEL_ASSEMBLE_PACK,
                       w.f_asm[part], w.b_asm[part],
                       launch_assemble_pack4(ord, np, part == 4 ? 1 : 0,
@@ -775,11 +807,14 @@ int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
                                             ctx->d_wig.as<double>(),
                                             ctx->d_q.as<cdbl>(),
                                             lane(ctx, lane_of[part])));
+        CTM_HIP(hipEventRecord(ctx->ev_part[part], lane(ctx, lane_of[part])));
+      }
       ord += np;
     }
-    CTM_HIP(join_lanes(ctx));
   }
-  return run_solves(ctx, want_t, nullptr);
+  // the solve buckets wait for their parts' events; run_solves joins the
+  // lanes
+  return run_solves(ctx, want_t, nullptr, parts);
 }
 
 // Runs tables + assembly + inversion/solve for the current work list.
@@ -821,6 +856,8 @@ int ctm_create(ctm_context **out, int device) {
   }
   if (e == hipSuccess)
     e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+  for (int k = 0; e == hipSuccess && k < 5; ++k)
+    e = hipEventCreateWithFlags(&ctx->ev_part[k], hipEventDisableTiming);
   if (e != hipSuccess) {
     (void)ctm_destroy(ctx);
     return CTM_ERR_HIP;
@@ -859,6 +896,8 @@ int ctm_destroy(ctm_context *ctx) {
     if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
   }
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  for (hipEvent_t ev : ctx->ev_part)
+    if (ev) (void)hipEventDestroy(ev);
   delete ctx;
   return CTM_OK;
 }
