@@ -49,6 +49,9 @@ def parse():
                          "shape grid at orders up to 160; gradients: the "
                          "1024-grain seed-1 batch with d/d(R_V, d, Re m, "
                          "Im m) of Q_ext, Q_abs, Q_sca")
+    ap.add_argument("--grid-cells", type=int, default=204,
+                    help="grid workload: (size, wavelength) cells per batch "
+                         "(each cell holds its 20 shapes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 512 MiB five times "
                          "(reference kernel for the HBM byte counters)")
@@ -159,7 +162,7 @@ def run_grid(a):
     xg, _ = np.polynomial.legendre.leggauss(8)
     theta = np.arccos(xg)
     Theta = np.radians(np.arange(0.0, 181.0, 2.0))
-    cells = 204
+    cells = a.grid_cells
     pN = np.zeros((2, 321))
     pN[:, 0] = 1.0
     pN[0, 2] = 1.5 * 0.6 - 0.5   # oblate (d > 1)

@@ -672,25 +672,33 @@ __global__ __launch_bounds__(256) void k_query(const Query *queries,
     for (int d = 0; d < 2; ++d)
       for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) geo[10 + d * 4 + i * 2 + j] = R[d][i][j];
+    // a direction along the symmetry axis (the test of pitau_dev) has
+    // pi = tau = 0 for every m but 1, so only m = 1 adds to S
+    geo[18] = (fabs(1. - fabs(pf[0][0])) <= 1.e-10 ||
+               fabs(1. - fabs(pf[1][0])) <= 1.e-10) ? 1. : 0.;
   }
   __syncthreads();
   // e^{i(phi_out - phi_in)} in the particle frame
   const double cpi = geo[3], spi = geo[4], cpo = geo[8], spo = geo[9];
   const cdbl step = cmk(cpo * cpi + spo * spi, spo * cpi - cpo * spi);
-  cdbl ex = cmk(1., 0.);
+  // orders m with a contribution: all, or only m = 1 (the terms skipped
+  // are exact zeros)
+  const bool axis = geo[18] != 0.;
+  const int m_lo = axis ? 1 : 0, m_hi = axis ? min(1, N) : N;
+  cdbl ex = axis ? step : cmk(1., 0.);  // e^{i m_lo dphi}
   cdbl S[4] = {cmk(0., 0.), cmk(0., 0.), cmk(0., 0.), cmk(0., 0.)};
-  for (int mb = 0; mb <= N; mb += kQueryMB) {
+  for (int mb = m_lo; mb <= m_hi; mb += kQueryMB) {
     // the recurrences of kQueryMB orders m and both directions in parallel
     __syncthreads();
     if (t < 2 * kQueryMB) {
       const int mm = t >> 1, d = t & 1;  // d 0: incoming, 1: outgoing
-      if (mb + mm <= N)
+      if (mb + mm <= m_hi)
         pitau_dev(geo[d * 5], geo[d * 5 + 1], geo[d * 5 + 2], N, mb + mm,
                   ptab + (mm * 4 + 2 * d) * nstride,
                   ptab + (mm * 4 + 2 * d + 1) * nstride, 1);
     }
     __syncthreads();
-  for (int m = mb; m <= N && m < mb + kQueryMB; ++m) {
+  for (int m = mb; m <= m_hi && m < mb + kQueryMB; ++m) {
     const int nmin = m > 0 ? m : 1;
     const int nm = N + 1 - nmin;
     const double *pt = ptab + (m - mb) * 4 * nstride;

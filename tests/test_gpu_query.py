@@ -46,6 +46,10 @@ def _queries(seed, n):
     q[0] = [0.0, 0.0, 0.7, 0.3, 0.7, 0.3]
     q[1] = [0.4, 0.9, 1.1, 2.0, np.pi - 1.1, 2.0 + np.pi]
     q[2] = [1.0, 0.0, 0.0, 0.0, np.pi / 2, 1.0]
+    # outgoing along the symmetry axis; incidence along it with
+    # backscattering along it (only m = 1 contributes to S there)
+    q[3] = [0.3, 0.0, 0.9, 0.4, 0.0, 0.0]
+    q[4] = [0.0, 0.0, 0.0, 0.0, np.pi, 0.0]
     return q
 
 
