@@ -1698,13 +1698,34 @@ int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
     ctx->err = "Clebsch-Gordan table too large";
     return CTM_ERR_ARGUMENT;
   }
+  // per distribution: the number of orders N < k with a non-zero
+  // coefficient.  A pair (n1, n2) couples only through |n1 - n2| <= N <=
+  // n1 + n2: pairs no such N reaches have a zero ensemble entry, written by
+  // one fill instead of a task (Mishchenko alignment: N = 0, 2 only)
+  std::vector<int32_t> ncum((size_t)n_dist * (2 * nmax + 2), 0);
+  for (int dd = 0; dd < n_dist; ++dd)
+    for (int N = 0; N <= 2 * nmax; ++N)
+      ncum[(size_t)dd * (2 * nmax + 2) + N + 1] =
+          ncum[(size_t)dd * (2 * nmax + 2) + N] +
+          (pN_host[(size_t)dd * ncoef + N] != 0. ? 1 : 0);
   std::vector<AvgTask> tasks;
   ctx->aoff.assign(ctx->final_items.size(), 0);
   int64_t asize = 0;
+  bool skipped = false;
   for (int g = 0; g < n; ++g) {
     const int Ng = (int)ctx->nmax[g];
+    const int dd = dist_of_grain ? dist_of_grain[g] : 0;
+    const int32_t *nc = dd >= 0 ? ncum.data() + (size_t)dd * (2 * nmax + 2)
+                                : nullptr;
     for (int n1 = 1; n1 <= Ng; ++n1)
-      for (int n2 = 1; n2 <= Ng; ++n2) tasks.push_back({g, n1, n2, 0});
+      for (int n2 = 1; n2 <= Ng; ++n2) {
+        // alignment disabled (dd < 0): every pair is a copy
+        if (nc && nc[n1 + n2 + 1] == nc[std::abs(n1 - n2)]) {
+          skipped = true;
+          continue;
+        }
+        tasks.push_back({g, n1, n2, 0});
+      }
     for (int m = 0; m <= Ng; ++m) {
       const int64_t nm = Ng + 1 - (m > 0 ? m : 1);
       ctx->aoff[ctx->grain_items_off[g] + m] = asize;
@@ -1722,6 +1743,9 @@ int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,
   if (dist_of_grain) CTM_HIP(upload(ctx, ctx->d_dist, dist));
   CTM_HIP(ctx->d_cg.ensure(std::max<int64_t>(cgsize, 1) * sizeof(double)));
   CTM_HIP(ctx->d_avg.ensure(std::max<int64_t>(asize, 1) * sizeof(cdbl)));
+  if (skipped)
+    CTM_LAUNCH(CTM_KERNEL_AVERAGE, 0., 16. * asize,
+               launch_zero(ctx->d_avg.as<cdbl>(), asize, s));
   CTM_LAUNCH(CTM_KERNEL_AVERAGE, 0., 0.,
              launch_cg(ctx->d_cgkeys.as<CgKey>(), (int)keys.size(),
                        ctx->d_cg.as<double>(), s));

@@ -170,6 +170,21 @@ __global__ __launch_bounds__(64) void k_average(
   }
 }
 
+// Grid-stride zero fill, two complex values per thread and pass.
+__global__ __launch_bounds__(256) void k_zero(cdbl *a, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+    a[i] = cmk(0., 0.);
+}
+
+hipError_t launch_zero(cdbl *a, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, a, n);
+  return hipGetLastError();
+}
+
 // One thread per output value; the weighted sum runs over the group in
 // order, as the reference's shape average does
 // (ShapeAveragingTask.hpp:127-166).  Consecutive threads read consecutive

@@ -25,6 +25,9 @@ hipError_t launch_average(const AvgTask *tasks, int n, const Item *items,
                           int nmax_cg, cdbl *abuf, const int64_t *aoff,
                           hipStream_t s);
 
+// a[0 .. n) = 0 (the ensemble entries of pairs no coefficient reaches)
+hipError_t launch_zero(cdbl *a, int64_t n, hipStream_t s);
+
 // out[o][k] = sum_s w[o][s] in[o][s][k] (divided by sum_s w[o][s] when
 // normalise != 0), s in order: shape averaging and size integration.
 hipError_t launch_weighted_reduce(int n_groups, int group, int64_t width,

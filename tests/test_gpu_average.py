@@ -85,3 +85,19 @@ def test_weighted_sums_shape_and_size():
     ref2 = np.einsum("s,sij->ij", w, x)[None]
     assert np.abs(got2 - ref2).max() < 1e-13 * np.abs(ref2).max()
     ctx.close()
+
+
+def test_sparse_coefficients_after_dense(batch):
+    # Davis-Greenstein coefficients reach every (n1, n2) pair; the isotropic
+    # ones only n1 = n2: the second average must leave no entry of the first
+    ctx, r = batch
+    nmax_all = max(c[4] for c in CASES)
+    r.average(O.orientation_coefficients(2, 2.0, 2 * nmax_all))
+    p = O.orientation_coefficients(0, 0., 2 * nmax_all)
+    print("average DG then isotropic", flush=True)
+    r.average(p)
+    for g, (rv, d, lam, mr, nmax, ng) in enumerate(CASES):
+        T = O.tmatrix_all(rv, d, lam, mr, nmax, ng)
+        Te = O.orientation_average(nmax, T, p)
+        got = r.averaged_tmatrix(g)
+        assert np.abs(got - Te).max() / np.abs(Te).max() < 1e-10, g
