@@ -305,15 +305,26 @@ def run_gradients(a):
               for j in range(4)]
         return r, q, dqs, dqe, dq
 
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        r, q, dqs, dqe, dq = step()
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / max(a.steps, 1)
+    def forward():
+        r = tmatrix_batch(*dg, ctx=ctx, min_order=4, max_order=100, glfac=2,
+                          tolerance=1e-4, sync=False)
+        q = observables(theta, ctx=ctx, account_for_scattering=False,
+                        n_grains=n)
+        return r, q
+
+    def timed(fn):
+        for _ in range(a.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            res = fn()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        return res, (time.perf_counter() - t0) * 1e3 / max(a.steps, 1)
+
+    _, ms_fwd = timed(forward)
+    (r, q, dqs, dqe, dq), ms = timed(step)
     st = r.status.cpu().numpy()
     out = {
         "metric": "converged T-matrices with gradients/s on 1 GPU",
@@ -327,7 +338,11 @@ def run_gradients(a):
                                "Q_ext, Q_abs at 16 angles and full-T sums "
                                "with forward-mode derivatives",
                    "grains": n, "converged": int((st == 0).sum()),
-                   "mean_dqext_dR": float(dq[0]["qext"].mean().item())},
+                   "forward_only_ms_per_step": round(ms_fwd, 3),
+                   "with_gradients_over_forward": round(ms / ms_fwd, 3),
+                   "mean_dqext_dR_converged": float(
+                       dq[0]["qext"][torch.as_tensor(st == 0, device=dev)]
+                       .mean().item())},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
