@@ -48,7 +48,10 @@ typedef struct ctm_context ctm_context;
 /* Creates a context bound to HIP device `device` with its own stream
  * (replaces the per-thread TMatrixAuxiliarySpaceManager scratch and the
  * QuickSched runner: TMatrixResource.hpp:281-434,
- * QuickSchedWrapper.hpp:302-318). */
+ * QuickSchedWrapper.hpp:302-318).  Inside a call, independent launches also
+ * run on three side streams of the context; they are joined into the
+ * context's stream before the call returns, so the context's stream orders
+ * every result. */
 int ctm_create(ctm_context **ctx, int device);
 int ctm_destroy(ctm_context *ctx);
 const char *ctm_last_error(const ctm_context *ctx);
@@ -260,7 +263,7 @@ int ctm_orientation_coefficients(ctm_real_fn p, void *arg, uint32_t order,
                                  double *coefficients);
 
 /* Kernel timing.  With profiling on, every launch of the library is
- * bracketed by two HIP events on the context's stream; ctm_profile_read
+ * bracketed by two HIP events on the stream it runs on; ctm_profile_read
  * returns, per kernel, the number of launches, the summed event time and the
  * algorithmic FLOPs and compulsory HBM bytes those launches stand for.  No
  * reference counterpart: it replaces QuickSched's per-task tic/toc log
