@@ -1304,14 +1304,16 @@ int ctm_tmatrix_tangents(ctm_context *ctx, double *dqsca_host,
                                    ctx->d_gl.as<double>(),
                                    ctx->d_tables.as<double>(), w.table_size,
                                    ctx->d_ttables.as<double>(), s));
-  // tangent assembly over 16 x 16 pair tiles of every item
+  // tangent assembly over 16 x kTangentTileCols pair tiles of every item
+  // (twice as wide for m = 0)
   std::vector<int32_t> order(ni), prefix(ni + 1, 0);
   for (int i = 0; i < ni; ++i) {
     order[i] = i;
     const Item &it = w.items[i];
     const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
     const int ntr = (nm + 15) / 16;
-    const int ntc = it.m == 0 ? (nm + 31) / 32 : ntr;
+    const int cw = it.m == 0 ? 2 * kTangentTileCols : kTangentTileCols;
+    const int ntc = (nm + cw - 1) / cw;
     prefix[i + 1] = prefix[i] + ntr * ntc;
   }
   CTM_HIP(upload(ctx, ctx->d_ttiles, order));

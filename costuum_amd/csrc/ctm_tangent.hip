@@ -145,24 +145,31 @@ __global__ void k_tables_tangent(const TableSet *sets, const GrainParams gp,
 }
 
 // ---------------------------------------------------------------------------
-// Tangent assembly: the pair tiles of the forward kernel (16 x 16, four
-// waves split by pair parity), blockIdx.y = direction.  Rows and columns
-// stage value and derivative of every factor; the accumulators hold the
-// derivatives of J_a, J_b, RgJ_a, RgJ_b (and the values for the dm term).
+// Tangent assembly: pair tiles of 16 rows and kTangentTileCols columns
+// (twice that for m = 0), four waves split by pair parity as in the forward
+// kernel, each thread one row and two columns (cc and cc + half the tile
+// width) so a row's staged factors serve two pairs.  Rows and columns stage
+// value and derivative of every factor; the accumulators hold the
+// derivatives of J_a, J_b, RgJ_a, RgJ_b.  NEEDV (the refractive-index
+// directions, blockIdx.y + 2) also accumulates their values for the
+// d(k^2 m) term; along R_V and d that term vanishes and the value products
+// are skipped.
 constexpr int kTT = 16;
 constexpr int kTCh = 4;
 constexpr int kTLS = 34;
 constexpr int kTRS = 10;
 
+template <bool NEEDV>
 __global__ __launch_bounds__(256) void k_assemble_tangent(
     const int32_t *order, const int32_t *tile_prefix, int n_order,
     const Item *items, const TableSet *sets, const WigKey *wkeys,
     const GrainParams gp, const double *tables, const double *ttables,
     int64_t table_size, const double *wig, cdbl *dq, int64_t q_size) {
+  constexpr int kTC = kTangentTileCols;
   __shared__ double Lv[kTCh][kTT][kTLS], Ld[kTCh][kTT][kTLS];
-  __shared__ double Rv[kTCh][2 * kTT][kTRS], Rd[kTCh][2 * kTT][kTRS];
+  __shared__ double Rv[kTCh][2 * kTC][kTRS], Rd[kTCh][2 * kTC][kTRS];
 
-  const int dir = blockIdx.y;
+  const int dir = blockIdx.y + (NEEDV ? 2 : 0);
   const int bid = blockIdx.x;
   int lo = 0, hi = n_order;
   while (hi - lo > 1) {
@@ -176,7 +183,7 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
   const int nmin = m > 0 ? m : 1;
   const int nm = N + 1 - nmin;
   const bool m0 = (m == 0);
-  const int cw = m0 ? 2 * kTT : kTT;
+  const int cw = m0 ? 2 * kTC : kTC;  // tile columns
   const int ntc = (nm + cw - 1) / cw;
   const int lt = bid - tile_prefix[lo];
   const int tr = lt / ntc, tc = lt - (lt / ntc) * ntc;
@@ -191,15 +198,23 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
   const int t = threadIdx.x;
   const int wv = t >> 6, lane = t & 63;
   const int rr = 2 * (lane >> 3) + (wv & 1);
+  // first column of the thread; the second is cc + cw / 2 (same parity)
   const int cc = m0 ? kTT * (wv >> 1) + 2 * (lane & 7) + (wv & 1)
                     : 2 * (lane & 7) + (wv >> 1);
+  const int half = cw / 2;
   const int p1 = tr * kTT + rr;
-  const int p2 = tc * cw + cc;
   const bool odd = !m0 && (((wv & 1) ^ (wv >> 1)) != 0);
   const int nstage = kTT * kTCh + cw * kTCh;
   const double mm = (double)m;
 
-  dz Ja{cmk(0., 0.), cmk(0., 0.)}, Jb = Ja, RJa = Ja, RJb = Ja;
+  dz Ja[2], Jb[2], RJa[2], RJb[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    Ja[c] = dz{cmk(0., 0.), cmk(0., 0.)};
+    Jb[c] = Ja[c];
+    RJa[c] = Ja[c];
+    RJb[c] = Ja[c];
+  }
 
   for (int ig0 = 0; ig0 < G; ig0 += kTCh) {
     __syncthreads();
@@ -324,44 +339,58 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
     for (int kk = 0; kk < kmax; ++kk) {
       const double *L = Lv[kk][rr] + (odd ? 16 : 0);
       const double *LD = Ld[kk][rr] + (odd ? 16 : 0);
-      const double *R = Rv[kk][cc];
-      const double *RD = Rd[kk][cc];
-      dz r[5];
-#pragma unroll
-      for (int q2 = 0; q2 < 5; ++q2)
-        r[q2] = dz{cmk(R[2 * q2], R[2 * q2 + 1]),
-                   cmk(RD[2 * q2], RD[2 * q2 + 1])};
-      auto lz = [&](int i) {
-        return dz{cmk(L[i], L[i + 1]), cmk(LD[i], LD[i + 1])};
+      // A += l r (dual): d += l.v r.d + l.d r.v, and v += l.v r.v if NEEDV
+      auto zz = [&](dz &A, int i, const dz &r) {
+        const cdbl lv = cmk(L[i], L[i + 1]), ld = cmk(LD[i], LD[i + 1]);
+        if (NEEDV) cfma(A.v, lv, r.v);
+        cfma(A.d, lv, r.d);
+        cfma(A.d, ld, r.v);
       };
-      auto lr = [&](int i) { return dv{L[i], LD[i]}; };
-      if (!odd) {
-        Ja = zadd(Ja, zadd(zmul(lz(0), r[0]),
-                           zadd(zmul(lz(2), r[1]), zmul(lz(4), r[2]))));
-        Jb = zadd(Jb, zadd(zmul(lz(6), r[3]), zmul(lz(8), r[4])));
-        RJa = zadd(RJa, zadd(zscale(r[0], lr(10)),
-                             zadd(zscale(r[1], lr(11)), zmul(lz(12), r[2]))));
-        RJb = zadd(RJb, zadd(zscale(r[3], lr(14)), zscale(r[4], lr(15))));
-      } else {
-        Ja = zadd(Ja, zadd(zmul(lz(0), r[4]), zmul(lz(2), r[3])));
-        Jb = zadd(Jb, zadd(zmul(lz(4), r[1]),
-                           zadd(zmul(lz(6), r[0]), zmul(lz(8), r[2]))));
-        RJa = zadd(RJa, zadd(zscale(r[4], lr(10)), zscale(r[3], lr(11))));
-        RJb = zadd(RJb, zadd(zscale(r[1], lr(12)),
-                             zadd(zscale(r[0], lr(13)), zmul(lz(14), r[2]))));
+      auto rz = [&](dz &A, int i, const dz &r) {
+        const double lv = L[i], ld = LD[i];
+        if (NEEDV) rfma(A.v, lv, r.v);
+        rfma(A.d, lv, r.d);
+        rfma(A.d, ld, r.v);
+      };
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const double *R = Rv[kk][cc + c * half];
+        const double *RD = Rd[kk][cc + c * half];
+        dz r[5];
+#pragma unroll
+        for (int q2 = 0; q2 < 5; ++q2)
+          r[q2] = dz{cmk(R[2 * q2], R[2 * q2 + 1]),
+                     cmk(RD[2 * q2], RD[2 * q2 + 1])};
+        if (!odd) {
+          zz(Ja[c], 0, r[0]);
+          zz(Ja[c], 2, r[1]);
+          zz(Ja[c], 4, r[2]);
+          zz(Jb[c], 6, r[3]);
+          zz(Jb[c], 8, r[4]);
+          rz(RJa[c], 10, r[0]);
+          rz(RJa[c], 11, r[1]);
+          zz(RJa[c], 12, r[2]);
+          rz(RJb[c], 14, r[3]);
+          rz(RJb[c], 15, r[4]);
+        } else {
+          zz(Ja[c], 0, r[4]);
+          zz(Ja[c], 2, r[3]);
+          zz(Jb[c], 4, r[1]);
+          zz(Jb[c], 6, r[0]);
+          zz(Jb[c], 8, r[2]);
+          rz(RJa[c], 10, r[4]);
+          rz(RJa[c], 11, r[3]);
+          rz(RJb[c], 12, r[1]);
+          rz(RJb[c], 13, r[0]);
+          zz(RJb[c], 14, r[2]);
+        }
       }
     }
   }
-  if (p1 >= nm || p2 >= nm) return;
-  if (odd) {
-    auto timesi = [](dz a) {
-      return dz{cmk(-a.v.y, a.v.x), cmk(-a.d.y, a.d.x)};
-    };
-    Ja = timesi(Ja);
-    Jb = timesi(Jb);
-    RJa = timesi(RJa);
-    RJb = timesi(RJb);
-  }
+  if (p1 >= nm) return;
+  auto timesi = [](dz a) {
+    return dz{cmk(-a.v.y, a.v.x), cmk(-a.d.y, a.d.x)};
+  };
   const double lam = gp.wavelength[it.grain];
   const double k = 2. * M_PI / lam;
   const double k2 = k * k;
@@ -371,19 +400,27 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
   auto dq_of = [&](dz a, dz b) {
     return cadd(cadd(cmul(dk2m, a.v), cmul(k2mr, a.d)), cscale(b.d, k2));
   };
-  const cdbl dQx = dq_of(Ja, Jb);
-  const cdbl dQy = dq_of(Jb, Ja);
-  const cdbl dRQx = dq_of(RJa, RJb);
-  const cdbl dRQy = dq_of(RJb, RJa);
   const int n1 = nmin + p1;
   const int cx = n1 & 1;
   const int64_t nm2 = (int64_t)nm * nm;
-  const int64_t e = (int64_t)p1 * nm + p2;
   cdbl *Q = dq + (int64_t)dir * q_size + it.q_off;
-  Q[cx * nm2 + e] = dQx;
-  Q[(1 - cx) * nm2 + e] = dQy;
-  Q[(2 + cx) * nm2 + e] = dRQx;
-  Q[(2 + 1 - cx) * nm2 + e] = dRQy;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int p2 = tc * cw + cc + c * half;
+    if (p2 >= nm) continue;
+    dz ja = Ja[c], jb = Jb[c], rja = RJa[c], rjb = RJb[c];
+    if (odd) {
+      ja = timesi(ja);
+      jb = timesi(jb);
+      rja = timesi(rja);
+      rjb = timesi(rjb);
+    }
+    const int64_t e = (int64_t)p1 * nm + p2;
+    Q[cx * nm2 + e] = dq_of(ja, jb);
+    Q[(1 - cx) * nm2 + e] = dq_of(jb, ja);
+    Q[(2 + cx) * nm2 + e] = dq_of(rja, rjb);
+    Q[(2 + 1 - cx) * nm2 + e] = dq_of(rjb, rja);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -508,9 +545,15 @@ hipError_t launch_assemble_tangent(const int32_t *order,
                                    const double *wig, cdbl *dq, int64_t q_size,
                                    hipStream_t s) {
   if (n_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_assemble_tangent, dim3(n_tiles, kTangentDirs),
-                     dim3(256), 0, s, order, tile_prefix, n_order, items, sets,
-                     wkeys, gp, tables, ttables, table_size, wig, dq, q_size);
+  // R_V and d (no value products), then Re m and Im m
+  hipLaunchKernelGGL(k_assemble_tangent<false>, dim3(n_tiles, 2), dim3(256),
+                     0, s, order, tile_prefix, n_order, items, sets, wkeys, gp,
+                     tables, ttables, table_size, wig, dq, q_size);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_assemble_tangent<true>, dim3(n_tiles, 2), dim3(256),
+                     0, s, order, tile_prefix, n_order, items, sets, wkeys, gp,
+                     tables, ttables, table_size, wig, dq, q_size);
   return hipGetLastError();
 }
 

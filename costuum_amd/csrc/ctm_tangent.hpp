@@ -9,6 +9,9 @@ namespace ctm {
 
 // Parameter directions of the tangents.
 constexpr int kTangentDirs = 4;  // R_V, d, Re m, Im m
+// Tangent assembly tiles: 16 rows x kTangentTileCols columns of pairs for
+// m > 0, twice as wide for m = 0.
+constexpr int kTangentTileCols = 32;
 
 // Tangent tables: direction dir of the set at base offset off starts at
 // ttables[dir * 2 * table_size + 2 * off] (2 * kTangentDirs * table_size
@@ -18,7 +21,8 @@ hipError_t launch_tables_tangent(const TableSet *sets, int n_sets,
                                  const double *tables, int64_t table_size,
                                  double *ttables, hipStream_t s);
 // dQ, dRgQ per direction: dq[dir * q_size + layout of Q].  order /
-// tile_prefix: every item with 16 x 16 pair tiles.
+// tile_prefix: every item with 16 x kTangentTileCols pair tiles (twice as
+// wide for m = 0).
 hipError_t launch_assemble_tangent(const int32_t *order,
                                    const int32_t *tile_prefix, int n_order,
                                    int n_tiles, const Item *items,
