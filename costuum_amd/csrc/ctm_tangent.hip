@@ -429,15 +429,22 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
 __global__ __launch_bounds__(256) void k_tangent_m(const Item *items,
                                                    const cdbl *tbuf, cdbl *dq,
                                                    int64_t q_size) {
-  const int task = blockIdx.x, dir = blockIdx.y;
+  // all four directions per thread: T(r, k) is loaded once for them
+  const int task = blockIdx.x;
   const Item it = items[task >> 1];
   const int cls = task & 1;
   const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
   const int64_t nm2 = (int64_t)nm * nm;
   const cdbl *T = tbuf + it.t_off + cls * nm2;
-  cdbl *Q = dq + (int64_t)dir * q_size + it.q_off;
-  const cdbl *dQ = Q + cls * nm2;
-  cdbl *M = Q + (2 + cls) * nm2;  // dRgQ, replaced by dRgQ + T dQ
+  // per direction: dQ, and dRgQ replaced by M = dRgQ + T dQ
+  const cdbl *dQ[kTangentDirs];
+  cdbl *M[kTangentDirs];
+#pragma unroll
+  for (int d = 0; d < kTangentDirs; ++d) {
+    cdbl *Q = dq + (int64_t)d * q_size + it.q_off;
+    dQ[d] = Q + cls * nm2;
+    M[d] = Q + (2 + cls) * nm2;
+  }
   // m = 0: only indices of equal parity couple (TMatrixResource.hpp:622);
   // the assembly leaves the other entries unwritten, so they are skipped
   // here and zero in M
@@ -445,16 +452,23 @@ __global__ __launch_bounds__(256) void k_tangent_m(const Item *items,
   for (int64_t e = threadIdx.x; e < nm2; e += blockDim.x) {
     const int r = (int)(e / nm), c = (int)(e - (int64_t)(e / nm) * nm);
     if (step == 2 && ((r ^ c) & 1)) {
-      M[e] = cmk(0., 0.);
+#pragma unroll
+      for (int d = 0; d < kTangentDirs; ++d) M[d][e] = cmk(0., 0.);
       continue;
     }
-    cdbl acc = M[e];
-    for (int k = step == 2 ? (r & 1) : 0; k < nm; k += step)
-      cfma(acc, T[(int64_t)r * nm + k], dQ[(int64_t)k * nm + c]);
-    M[e] = acc;
+    cdbl acc[kTangentDirs];
+#pragma unroll
+    for (int d = 0; d < kTangentDirs; ++d) acc[d] = M[d][e];
+    for (int k = step == 2 ? (r & 1) : 0; k < nm; k += step) {
+      const cdbl tv = T[(int64_t)r * nm + k];
+#pragma unroll
+      for (int d = 0; d < kTangentDirs; ++d)
+        cfma(acc[d], tv, dQ[d][(int64_t)k * nm + c]);
+    }
+#pragma unroll
+    for (int d = 0; d < kTangentDirs; ++d) M[d][e] = acc[d];
   }
 }
-
 __global__ __launch_bounds__(256) void k_tangent_t(const Item *items,
                                                    const cdbl *tbuf,
                                                    const cdbl *ainv,
@@ -462,48 +476,69 @@ __global__ __launch_bounds__(256) void k_tangent_t(const Item *items,
                                                    int64_t q_size, cdbl *dt,
                                                    int64_t t_size,
                                                    double *tparts) {
-  __shared__ double red[4][2];
-  const int task = blockIdx.x, dir = blockIdx.y;
+  // all four directions per thread: Q^-1(k, c) is loaded once for them
+  __shared__ double red[4][kTangentDirs][2];
+  const int task = blockIdx.x;
   const Item it = items[task >> 1];
   const int cls = task & 1;
   const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
   const int64_t nm2 = (int64_t)nm * nm;
   const cdbl *T = tbuf + it.t_off + cls * nm2;
   const cdbl *Ai = ainv + it.t_off + cls * nm2;
-  const cdbl *M = dq + (int64_t)dir * q_size + it.q_off + (2 + cls) * nm2;
-  cdbl *D = dt + (int64_t)dir * t_size + it.t_off + cls * nm2;
-  double s_sca = 0., s_ext = 0.;
+  const cdbl *M[kTangentDirs];
+  cdbl *D[kTangentDirs];
+#pragma unroll
+  for (int d = 0; d < kTangentDirs; ++d) {
+    M[d] = dq + (int64_t)d * q_size + it.q_off + (2 + cls) * nm2;
+    D[d] = dt + (int64_t)d * t_size + it.t_off + cls * nm2;
+  }
+  double s_sca[kTangentDirs], s_ext[kTangentDirs];
+#pragma unroll
+  for (int d = 0; d < kTangentDirs; ++d) s_sca[d] = s_ext[d] = 0.;
   const int step = it.m == 0 ? 2 : 1;
   for (int64_t e = threadIdx.x; e < nm2; e += blockDim.x) {
     const int r = (int)(e / nm), c = (int)(e - (int64_t)(e / nm) * nm);
-    cdbl acc = cmk(0., 0.);
+    cdbl acc[kTangentDirs];
+#pragma unroll
+    for (int d = 0; d < kTangentDirs; ++d) acc[d] = cmk(0., 0.);
     if (!(step == 2 && ((r ^ c) & 1)))
-      for (int k = step == 2 ? (r & 1) : 0; k < nm; k += step)
-        cfma(acc, M[(int64_t)r * nm + k], Ai[(int64_t)k * nm + c]);
-    const cdbl v = cmk(-acc.x, -acc.y);
-    D[e] = v;
+      for (int k = step == 2 ? (r & 1) : 0; k < nm; k += step) {
+        const cdbl av = Ai[(int64_t)k * nm + c];
+#pragma unroll
+        for (int d = 0; d < kTangentDirs; ++d)
+          cfma(acc[d], M[d][(int64_t)r * nm + k], av);
+      }
     const cdbl tv = T[e];
-    s_sca += 2. * (tv.x * v.x + tv.y * v.y);
-    if (r == c) s_ext += v.x;
+#pragma unroll
+    for (int d = 0; d < kTangentDirs; ++d) {
+      const cdbl v = cmk(-acc[d].x, -acc[d].y);
+      D[d][e] = v;
+      s_sca[d] += 2. * (tv.x * v.x + tv.y * v.y);
+      if (r == c) s_ext[d] += v.x;
+    }
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int o = 32; o > 0; o >>= 1) {
-    s_sca += __shfl_xor(s_sca, o);
-    s_ext += __shfl_xor(s_ext, o);
-  }
-  if (lane == 0) {
-    red[wv][0] = s_sca;
-    red[wv][1] = s_ext;
+#pragma unroll
+  for (int d = 0; d < kTangentDirs; ++d) {
+    for (int o = 32; o > 0; o >>= 1) {
+      s_sca[d] += __shfl_xor(s_sca[d], o);
+      s_ext[d] += __shfl_xor(s_ext[d], o);
+    }
+    if (lane == 0) {
+      red[wv][d][0] = s_sca[d];
+      red[wv][d][1] = s_ext[d];
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < kTangentDirs) {
+    const int d = threadIdx.x;
     double a = 0., b = 0.;
     for (int w = 0; w < 4; ++w) {
-      a += red[w][0];
-      b += red[w][1];
+      a += red[w][d][0];
+      b += red[w][d][1];
     }
-    tparts[((int64_t)task * kTangentDirs + dir) * 2] = a;
-    tparts[((int64_t)task * kTangentDirs + dir) * 2 + 1] = b;
+    tparts[((int64_t)task * kTangentDirs + d) * 2] = a;
+    tparts[((int64_t)task * kTangentDirs + d) * 2 + 1] = b;
   }
 }
 
@@ -562,12 +597,12 @@ hipError_t launch_tangent_solve(int n_tasks, const Item *items,
                                 int64_t q_size, cdbl *dt, int64_t t_size,
                                 double *tparts, hipStream_t s) {
   if (n_tasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tangent_m, dim3(n_tasks, kTangentDirs), dim3(256), 0,
-                     s, items, tbuf, dq, q_size);
+  hipLaunchKernelGGL(k_tangent_m, dim3(n_tasks), dim3(256), 0, s, items,
+                     tbuf, dq, q_size);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_tangent_t, dim3(n_tasks, kTangentDirs), dim3(256), 0,
-                     s, items, tbuf, ainv, dq, q_size, dt, t_size, tparts);
+  hipLaunchKernelGGL(k_tangent_t, dim3(n_tasks), dim3(256), 0, s, items,
+                     tbuf, ainv, dq, q_size, dt, t_size, tparts);
   return hipGetLastError();
 }
 
