@@ -339,18 +339,21 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
     for (int kk = 0; kk < kmax; ++kk) {
       const double *L = Lv[kk][rr] + (odd ? 16 : 0);
       const double *LD = Ld[kk][rr] + (odd ? 16 : 0);
-      // A += l r (dual): d += l.v r.d + l.d r.v, and v += l.v r.v if NEEDV
-      auto zz = [&](dz &A, int i, const dz &r) {
-        const cdbl lv = cmk(L[i], L[i + 1]), ld = cmk(LD[i], LD[i + 1]);
+      // A += l r (dual): d += l.v r.d + l.d r.v, and v += l.v r.v if NEEDV.
+      // Along Re m and Im m (NEEDV) a row factor has a derivative only
+      // where it carries 1/(m kr) (A3, A3r, B3, B3r: the factors of R3);
+      // the other l.d are zero and are not read (LD = 0 skips them).
+      auto zz = [&](dz &A, int i, const dz &r, bool ld_used) {
+        const cdbl lv = cmk(L[i], L[i + 1]);
         if (NEEDV) cfma(A.v, lv, r.v);
         cfma(A.d, lv, r.d);
-        cfma(A.d, ld, r.v);
+        if (!NEEDV || ld_used) cfma(A.d, cmk(LD[i], LD[i + 1]), r.v);
       };
       auto rz = [&](dz &A, int i, const dz &r) {
-        const double lv = L[i], ld = LD[i];
+        const double lv = L[i];
         if (NEEDV) rfma(A.v, lv, r.v);
         rfma(A.d, lv, r.d);
-        rfma(A.d, ld, r.v);
+        if (!NEEDV) rfma(A.d, LD[i], r.v);
       };
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -362,27 +365,27 @@ __global__ __launch_bounds__(256) void k_assemble_tangent(
           r[q2] = dz{cmk(R[2 * q2], R[2 * q2 + 1]),
                      cmk(RD[2 * q2], RD[2 * q2 + 1])};
         if (!odd) {
-          zz(Ja[c], 0, r[0]);
-          zz(Ja[c], 2, r[1]);
-          zz(Ja[c], 4, r[2]);
-          zz(Jb[c], 6, r[3]);
-          zz(Jb[c], 8, r[4]);
+          zz(Ja[c], 0, r[0], false);
+          zz(Ja[c], 2, r[1], false);
+          zz(Ja[c], 4, r[2], true);    // A3
+          zz(Jb[c], 6, r[3], false);
+          zz(Jb[c], 8, r[4], false);
           rz(RJa[c], 10, r[0]);
           rz(RJa[c], 11, r[1]);
-          zz(RJa[c], 12, r[2]);
+          zz(RJa[c], 12, r[2], true);  // A3r
           rz(RJb[c], 14, r[3]);
           rz(RJb[c], 15, r[4]);
         } else {
-          zz(Ja[c], 0, r[4]);
-          zz(Ja[c], 2, r[3]);
-          zz(Jb[c], 4, r[1]);
-          zz(Jb[c], 6, r[0]);
-          zz(Jb[c], 8, r[2]);
+          zz(Ja[c], 0, r[4], false);
+          zz(Ja[c], 2, r[3], false);
+          zz(Jb[c], 4, r[1], false);
+          zz(Jb[c], 6, r[0], false);
+          zz(Jb[c], 8, r[2], true);    // B3
           rz(RJa[c], 10, r[4]);
           rz(RJa[c], 11, r[3]);
           rz(RJb[c], 12, r[1]);
           rz(RJb[c], 13, r[0]);
-          zz(RJb[c], 14, r[2]);
+          zz(RJb[c], 14, r[2], true);  // B3r
         }
       }
     }
