@@ -1328,9 +1328,11 @@ int ctm_tmatrix_tangents(ctm_context *ctx, double *dqsca_host,
                  ctx->gp_saved, ctx->d_tables.as<double>(),
                  ctx->d_ttables.as<double>(), w.table_size,
                  ctx->d_wig.as<double>(), ctx->d_dq.as<cdbl>(), w.q_size, s));
-  // Q_c^-1 of every block (the solve is repeated; T comes out unchanged)
+  // Q_c^-1 of every block: the elimination is repeated with only the
+  // diagonal of the product formed (T is left as it is; the per-block sums
+  // it overwrites were consumed by the batch's Q_sca/Q_ext)
   CTM_HIP(ctx->d_ainv.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
-  int rc = run_solves(ctx, true, ctx->d_ainv.as<cdbl>());
+  int rc = run_solves(ctx, false, ctx->d_ainv.as<cdbl>());
   if (rc != CTM_OK) return rc;
   CTM_HIP(ctx->d_dt.ensure(std::max<int64_t>(w.t_size, 1) * kTangentDirs *
                            sizeof(cdbl)));
