@@ -545,6 +545,13 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
                                             items, tbuf, abuf, aoff, PT,
                                             nmax_all, X, s);
   }
+  // blocks up to order 16 on 128 threads: more workgroups per CU for the
+  // many small items (measured faster than 256 and 64)
+  if (nm_cap <= 16 &&
+      obs_x_lds_bytes(nm_cap, spec.n_out, 4, 128, 128, false) <= kObsXMaxLds)
+    return launch_obs_x_ck<4, 128, 128, false>(spec, item_list, n_items,
+                                               nm_cap, items, tbuf, abuf,
+                                               aoff, PT, nmax_all, X, s);
   if (obs_x_lds_bytes(nm_cap, spec.n_out, 4, 256, 256, false) <= kObsXMaxLds)
     return launch_obs_x_ck<4, 256, 256, false>(spec, item_list, n_items,
                                                nm_cap, items, tbuf, abuf,
