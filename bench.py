@@ -43,12 +43,15 @@ def parse():
                     help="CPU sample = every k-th grain in size order")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="batch",
-                    choices=["batch", "grid", "gradients"],
+                    choices=["batch", "grid", "gradients", "single"],
                     help="batch: the 4096-grain seed-0 batch (default); "
                          "grid: the 61 440 T-matrix size x wavelength x "
                          "shape grid at orders up to 160; gradients: the "
                          "1024-grain seed-1 batch with d/d(R_V, d, Re m, "
-                         "Im m) of Q_ext, Q_abs, Q_sca")
+                         "Im m) of Q_ext, Q_abs, Q_sca; single: one "
+                         "grain through the calculator path (prolate d = "
+                         "0.5 and oblate d = 2), isotropic Q at 30 deg, Z at "
+                         "91 angles")
     ap.add_argument("--grid-cells", type=int, default=204,
                     help="grid workload: (size, wavelength) cells per batch "
                          "(each cell holds its 20 shapes)")
@@ -58,7 +61,7 @@ def parse():
     a = ap.parse_args()
     batch = a.workload == "batch"
     if a.steps is None:
-        a.steps = 20 if batch else 1
+        a.steps = 20 if batch else (10 if a.workload == "single" else 1)
     if a.warmup is None:
         a.warmup = 3 if batch else 1
     return a
@@ -348,16 +351,68 @@ def run_gradients(a):
     print(json.dumps(out), flush=True)
 
 
+def run_single(a):
+    """The single-grain case through the reference's module surface:
+    R_V = 0.2 um, lambda = 0.55 um, m = 1.5 + 0.02i, for the prolate (d =
+    0.5) and oblate (d = 2) reading of axis ratio 2; TMatrixCalculator
+    convergence (tolerance 1e-4), isotropic ensemble, Q_ext and Q_abs at 30
+    deg, Z at 91 scattering angles 0..180 deg.  A step is both grains."""
+    import math
+    import numpy as np
+    from costuum_amd import CosTuuM as C
+    rv, lam, mr = 0.2, 0.55, 1.5 + 0.02j
+    th = math.radians(30.0)
+    Theta = np.radians(np.arange(0, 181, 2.0))
+    area = math.pi * rv * rv
+
+    def step():
+        out = []
+        for d in (0.5, 2.0):
+            tm = C.TMatrix(rv, d, lam, mr, cos2beta=1.0 / 3.0)
+            qabs = tm.get_absorption_cross_section(th)[0] / area
+            qext = tm.get_extinction_matrix(th, 0.0)[0, 0] / area
+            Z = tm.get_scattering_matrix(0.0, 0.0, Theta, 0.0)
+            out.append((tm.get_nmax(), tm.get_ngauss(), qext, qabs,
+                        float(Z[45, 0, 0])))
+            tm.close()
+        return out
+
+    for _ in range(a.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = step()
+    ms = (time.perf_counter() - t0) * 1e3 / max(a.steps, 1)
+    out = {
+        "metric": "single grains through the calculator path per second",
+        "value": round(2 / (ms / 1e3), 2),
+        "unit": "converged T-matrices/s (with Q and Z at 91 angles)",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "single grain R_V 0.2 um, lambda 0.55 um, "
+                               "m 1.5+0.02i, d 0.5 and 2 (one step = both)",
+                   "grains": [{"d": d, "nmax": int(r[0]), "ngauss": int(r[1]),
+                               "qext_30deg": r[2], "qabs_30deg": r[3],
+                               "z11_90deg": r[4]}
+                              for d, r in zip((0.5, 2.0), res)]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
         copy_calibration()
         return
-    if a.workload in ("grid", "gradients"):
+    if a.workload in ("grid", "gradients", "single"):
         if a.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
             sys.exit(2)
-        (run_grid if a.workload == "grid" else run_gradients)(a)
+        {"grid": run_grid, "gradients": run_gradients,
+         "single": run_single}[a.workload](a)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU only: grains shard across GPUs "
