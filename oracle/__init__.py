@@ -332,3 +332,43 @@ def scattering_query(nmax, T, wavelength, alpha, beta, th_in, ph_in, th_out,
                             ph_in, th_out, ph_out, _p(out))
     S = (out[0:8:2] + 1j * out[1:8:2]).reshape(2, 2)
     return S, out[8:24].reshape(4, 4), out[24:40].reshape(4, 4)
+
+
+def batch_task_obs_ext(rv, d, lam, mr_re, mr_im, theta, wtheta, ngauss=20,
+                       min_order=4, max_order=100, glfac=2, tol=1e-4,
+                       nthreads=0, nmax_forced=None, per_angle=False):
+    """batch_task_obs plus the full-T sums and, with per_angle, Q_ext and
+    Q_abs at every incidence angle.  With nmax_forced the convergence loop
+    is skipped and every grain is computed at its given order.  Returns a
+    dict of numpy arrays."""
+    L = _olib()
+    if not getattr(L, "_btox", False):
+        L.ctmo_batch_task_obs_ext.argtypes = [
+            _I, _PD, _PD, _PD, _PD, _PD, _U, _U, _U, _D, _I, _U, _PD, _PD, _U,
+            _PU, _PU, _PI, _PD, _PD, _PD, _PD, _PD, _PD]
+        L._btox = True
+    rv, d, lam, mr_re, mr_im, theta, wtheta = [
+        np.ascontiguousarray(a, dtype=np.float64)
+        for a in (rv, d, lam, mr_re, mr_im, theta, wtheta)]
+    n = len(rv)
+    nt = len(theta)
+    out = dict(nmax=np.zeros(n, dtype=np.uint32),
+               status=np.zeros(n, dtype=np.int32),
+               qext_avg=np.zeros(n), qabs_avg=np.zeros(n),
+               qsca_sum=np.zeros(n), qext_sum=np.zeros(n))
+    if per_angle:
+        out["qext_theta"] = np.zeros((n, nt))
+        out["qabs_theta"] = np.zeros((n, nt))
+    forced = None
+    if nmax_forced is not None:
+        nf = np.ascontiguousarray(nmax_forced, dtype=np.uint32)
+        forced = nf.ctypes.data_as(_PU)
+    pa = (lambda k: _p(out[k]) if per_angle else None)
+    L.ctmo_batch_task_obs_ext(
+        n, _p(rv), _p(d), _p(lam), _p(mr_re), _p(mr_im), min_order,
+        max_order, glfac, tol, nthreads, nt, _p(theta), _p(wtheta), ngauss,
+        forced, out["nmax"].ctypes.data_as(_PU),
+        out["status"].ctypes.data_as(_PI), _p(out["qext_avg"]),
+        _p(out["qabs_avg"]), _p(out["qsca_sum"]), _p(out["qext_sum"]),
+        pa("qext_theta"), pa("qabs_theta"))
+    return out

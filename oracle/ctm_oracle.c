@@ -1338,27 +1338,43 @@ void ctmo_orientation_average(unsigned nmax, const double *Tind,
  * ctmo_batch_task, then Q_ext and Q_abs (with the scattered-light
  * quadrature of ngauss x ngauss incoming directions) at ntheta incidence
  * angles, averaged with weights wtheta; parallel over (grain, m) blocks and
- * over (grain, angle) pairs. */
-void ctmo_batch_task_obs(int n, const double *rv, const double *d,
-                         const double *lambda, const double *mr_re,
-                         const double *mr_im, unsigned min_order,
-                         unsigned max_order, unsigned glfac, double tol,
-                         int nthreads, unsigned ntheta, const double *theta,
-                         const double *wtheta, unsigned ngauss,
-                         unsigned *nmax_out, int *status_out,
-                         double *qext_avg, double *qabs_avg) {
+ * over (grain, angle) pairs.  Optional outputs (NULL to skip): the full-T
+ * sums Q_sca, Q_ext (TMatrix.hpp:2055-2110) and the per-angle Q_ext, Q_abs
+ * ([n][ntheta]).  With nmax_forced (not NULL) no convergence loop runs:
+ * every grain is computed at nmax_forced[g], ngauss = glfac nmax_forced[g]
+ * (status 0), as for the sensitivity of a converged grain to its inputs. */
+void ctmo_batch_task_obs_ext(int n, const double *rv, const double *d,
+                             const double *lambda, const double *mr_re,
+                             const double *mr_im, unsigned min_order,
+                             unsigned max_order, unsigned glfac, double tol,
+                             int nthreads, unsigned ntheta,
+                             const double *theta, const double *wtheta,
+                             unsigned ngauss, const unsigned *nmax_forced,
+                             unsigned *nmax_out, int *status_out,
+                             double *qext_avg, double *qabs_avg,
+                             double *qsca_sum, double *qext_sum,
+                             double *qext_theta, double *qabs_theta) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
   unsigned *ng = (unsigned *)malloc(sizeof(unsigned) * (n ? n : 1));
+  if (nmax_forced) {
+    for (int g = 0; g < n; ++g) {
+      nmax_out[g] = nmax_forced[g];
+      ng[g] = glfac * nmax_forced[g];
+      status_out[g] = 0;
+    }
+  } else {
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
-  for (int g = 0; g < n; ++g) {
-    double dqs, dqe;
-    status_out[g] = ctmo_converge_task(rv[g], d[g], lambda[g], mr_re[g],
-                                       mr_im[g], min_order, max_order, glfac,
-                                       tol, &nmax_out[g], &ng[g], &dqs, &dqe);
+    for (int g = 0; g < n; ++g) {
+      double dqs, dqe;
+      status_out[g] = ctmo_converge_task(rv[g], d[g], lambda[g], mr_re[g],
+                                         mr_im[g], min_order, max_order,
+                                         glfac, tol, &nmax_out[g], &ng[g],
+                                         &dqs, &dqe);
+    }
   }
   tables_t *tabs = (tables_t *)malloc(sizeof(tables_t) * (n ? n : 1));
   cplx **Ts = (cplx **)malloc(sizeof(cplx *) * (n ? n : 1));
@@ -1409,6 +1425,18 @@ void ctmo_batch_task_obs(int n, const double *rv, const double *d,
     }
     qext_avg[g] = se;
     qabs_avg[g] = sa;
+    if (qsca_sum || qext_sum) {
+      double s, e;
+      ctmo_qcoeffs(nmax_out[g], (const double *)Ts[g], &s, &e);
+      if (qsca_sum) qsca_sum[g] = s;
+      if (qext_sum) qext_sum[g] = e;
+    }
+    if (qext_theta)
+      memcpy(qext_theta + (size_t)g * ntheta, qe + (size_t)g * ntheta,
+             sizeof(double) * ntheta);
+    if (qabs_theta)
+      memcpy(qabs_theta + (size_t)g * ntheta, qa + (size_t)g * ntheta,
+             sizeof(double) * ntheta);
     tables_free(&tabs[g]);
     free(Ts[g]);
   }
@@ -1418,6 +1446,20 @@ void ctmo_batch_task_obs(int n, const double *rv, const double *d,
   free(tabs);
   free(Ts);
   free(ng);
+}
+
+void ctmo_batch_task_obs(int n, const double *rv, const double *d,
+                         const double *lambda, const double *mr_re,
+                         const double *mr_im, unsigned min_order,
+                         unsigned max_order, unsigned glfac, double tol,
+                         int nthreads, unsigned ntheta, const double *theta,
+                         const double *wtheta, unsigned ngauss,
+                         unsigned *nmax_out, int *status_out,
+                         double *qext_avg, double *qabs_avg) {
+  ctmo_batch_task_obs_ext(n, rv, d, lambda, mr_re, mr_im, min_order,
+                          max_order, glfac, tol, nthreads, ntheta, theta,
+                          wtheta, ngauss, NULL, nmax_out, status_out,
+                          qext_avg, qabs_avg, NULL, NULL, NULL, NULL);
 }
 
 /* a26: S, Z and K for particle orientation (alpha, beta) and laboratory

@@ -123,6 +123,20 @@ void ctmo_batch_task_obs(int n, const double *rv, const double *axis_ratio,
                          unsigned *nmax_out, int *status_out,
                          double *qext_avg, double *qabs_avg);
 
+/* ctmo_batch_task_obs with optional full-T sums and per-angle values
+ * ([n][ntheta]); with nmax_forced, no convergence loop (orders forced). */
+void ctmo_batch_task_obs_ext(int n, const double *rv, const double *axis_ratio,
+                             const double *wavelength, const double *mr_re,
+                             const double *mr_im, unsigned min_order,
+                             unsigned max_order, unsigned glfac,
+                             double tolerance, int nthreads, unsigned ntheta,
+                             const double *theta, const double *wtheta,
+                             unsigned ngauss, const unsigned *nmax_forced,
+                             unsigned *nmax_out, int *status_out,
+                             double *qext_avg, double *qabs_avg,
+                             double *qsca_sum, double *qext_sum,
+                             double *qext_theta, double *qabs_theta);
+
 /* a23: ExtinctionCoefficientTask.hpp:626-653 (per incidence angle). */
 void ctmo_extinction(unsigned nmax, const double *T, double wavelength,
                      double rv, unsigned ntheta, const double *theta,
