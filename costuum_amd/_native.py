@@ -10,6 +10,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcostuum_amd.so")
 
 CTM_OK = 0
+CTM_ERR_ARGUMENT = -1
+CTM_ERR_HIP = -2
+CTM_ERR_STATE = -3
+CTM_ERR_MEMORY = -4
 CTM_CONVERGED = 0
 CTM_NOT_CONVERGED = 1
 CTM_PRECISION_FP64 = 0
@@ -25,7 +29,7 @@ EXPORTS = (
     "ctm_query", "ctm_m0_sums", "ctm_gauss_legendre", "ctm_rv_over_rs",
     "ctm_adaptive_quadrature", "ctm_orientation_coefficients",
     "ctm_weighted_reduce", "ctm_tmatrix_tangents", "ctm_tangent_get",
-    "ctm_observables_select", "ctm_vjp",
+    "ctm_observables_select", "ctm_vjp", "ctm_observables_noalloc",
 )
 
 # double f(double x, void *arg): the host callback type (ctm_real_fn)
@@ -119,6 +123,10 @@ def lib():
     L.ctm_observables.argtypes = [vp, ctypes.POINTER(CtmObservableSpec),
                                   dp, dp, dp, dp, dp, dp]
     L.ctm_observables.restype = ctypes.c_int
+    L.ctm_observables_noalloc.argtypes = [vp,
+                                          ctypes.POINTER(CtmObservableSpec),
+                                          dp, dp, dp, dp, dp, dp]
+    L.ctm_observables_noalloc.restype = ctypes.c_int
     L.ctm_orientation_average.argtypes = [vp, ctypes.c_int, ctypes.c_int,
                                           dp, dp]
     L.ctm_orientation_average.restype = ctypes.c_int

@@ -21,6 +21,7 @@
 #include <climits>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <unordered_map>
@@ -309,6 +310,34 @@ struct Profiler {
 // parts, solve buckets, observable buckets) beside the context's stream.
 constexpr int kLanes = 4;
 
+// Fixed device scratch of ctm_observables_noalloc: 64 MiB per device, held
+// from the library's first use on that device until the process ends.  In
+// host code `scratch` names a host-side stand-in, not device memory: the
+// device address of each GPU's copy comes from hipGetSymbolAddress (taken
+// once per device, with that device current, in ctm_create) and only that
+// address is handed to kernels and copies.  Every call on a device shares
+// that device's array.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device `device`.
+static hipError_t scratch_address(int device, char **ptr) {
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<char *>(p);
+  }
+  *ptr = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
 struct ctm_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -348,6 +377,7 @@ struct ctm_context {
   Profiler prof;
   Staging staging;
   Staging readback;  // pinned landing area of device-to-host copies
+  char *scratch = nullptr;  // this device's fixed scratch (device address)
 };
 
 #define CTM_HIP(expr)                                                   \
@@ -881,6 +911,7 @@ int ctm_create(ctm_context **out, int device) {
     e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
   for (int k = 0; e == hipSuccess && k < 5; ++k)
     e = hipEventCreateWithFlags(&ctx->ev_part[k], hipEventDisableTiming);
+  if (e == hipSuccess) e = scratch_address(device, &ctx->scratch);
   if (e != hipSuccess) {
     (void)ctm_destroy(ctx);
     return CTM_ERR_HIP;
@@ -1417,10 +1448,27 @@ static void push_angle(std::vector<double> &ang, double c) {
   ang.push_back(sn != 0. ? 1. / sn : 9000.);
 }
 
-int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
-                    double *qext, double *qextpol, double *qextcpol,
-                    double *qabs, double *qabspol, double *Z) {
+// Launch `call` on stream `strm`, between profiler events unless `quiet`
+// (the no-allocation path takes no events from the profiler's pool).
+#define CTM_LAUNCH_Q(quiet, strm, kernel, flops, bytes, call)    \
+  do {                                                          \
+    if (quiet) {                                                \
+      CTM_HIP(call);                                            \
+    } else {                                                    \
+      CTM_LAUNCH_ON(strm, kernel, flops, bytes, call);          \
+    }                                                           \
+  } while (0)
+
+// The observables of the context's T-matrices.  Their scratch (direction
+// tables, the pi/tau table, the bilinear forms X, the item order) comes
+// from the context's growable buffers, or, when `fixed` is not null, from
+// the fixed device range fixed[0 .. fixed_cap) with no allocation at all.
+static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
+                            double *qext, double *qextpol, double *qextcpol,
+                            double *qabs, double *qabspol, double *Z,
+                            char *fixed, size_t fixed_cap) {
   if (!ctx || !spec) return CTM_ERR_ARGUMENT;
+  const bool quiet = fixed != nullptr;
   if (ctx->final_items.empty()) {
     ctx->err = "no T-matrices in the context";
     return CTM_ERR_STATE;
@@ -1546,9 +1594,56 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   const size_t o_pw = put(pw.data(), pw.size() * 8);
   const size_t o_sc = put(sc.data(), sc.size() * 8);
   const size_t o_ss = put(ss.data(), ss.size() * 8);
-  CTM_HIP(ctx->d_spec.ensure(blob.size()));
-  CTM_HIP(stage_copy(ctx, ctx->d_spec.p, blob.data(), blob.size()));
-  char *base = ctx->d_spec.as<char>();
+
+  int nmax_all = 1, nm_max = 1;
+  for (int g = 0; g < n; ++g) nmax_all = std::max(nmax_all, (int)ctx->nmax[g]);
+  nm_max = nmax_all;
+  const size_t nitems = ctx->final_items.size();
+  if (obs_x_lds_bytes(nmax_all, sp.n_out, 8, 256, 1, true) > kObsXMaxLds) {
+    ctx->err = "too many outgoing directions for the largest order";
+    return CTM_ERR_ARGUMENT;
+  }
+  const size_t pt_bytes =
+      (size_t)(nmax_all + 1) * n_ang * nmax_all * 2 * sizeof(double);
+  const size_t x_bytes = nitems * (size_t)sp.n_pairs * 4 * sizeof(cdbl);
+  const size_t li_bytes = std::max<size_t>(nitems, 1) * sizeof(int32_t);
+  char *base, *d_pt, *d_x, *d_li;
+  if (fixed) {
+    // one bump pass over the fixed range, 256-byte aligned parts
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      const size_t o = off;
+      off += (bytes + 255) & ~(size_t)255;
+      return o;
+    };
+    const size_t f_spec = take(blob.size()), f_pt = take(pt_bytes),
+                 f_x = take(x_bytes), f_li = take(li_bytes);
+    if (off > fixed_cap) {
+      ctx->err = "ctm_observables_noalloc: these observables need " +
+                 std::to_string(off) + " bytes of scratch, more than its " +
+                 std::to_string(fixed_cap) + " (fewer grains, angles or "
+                 "quadrature directions per call, or ctm_observables)";
+      return CTM_ERR_ARGUMENT;
+    }
+    base = fixed + f_spec;
+    d_pt = fixed + f_pt;
+    d_x = fixed + f_x;
+    d_li = fixed + f_li;
+    // a pageable source: the runtime stages it; the call drains its stream
+    // before `blob` goes out of scope
+    CTM_HIP(hipMemcpyAsync(base, blob.data(), blob.size(),
+                           hipMemcpyHostToDevice, s));
+  } else {
+    CTM_HIP(ctx->d_spec.ensure(blob.size()));
+    CTM_HIP(stage_copy(ctx, ctx->d_spec.p, blob.data(), blob.size()));
+    CTM_HIP(ctx->d_pt.ensure(pt_bytes));
+    CTM_HIP(ctx->d_x.ensure(x_bytes));
+    CTM_HIP(ctx->d_obsitems.ensure(li_bytes));
+    base = ctx->d_spec.as<char>();
+    d_pt = ctx->d_pt.as<char>();
+    d_x = ctx->d_x.as<char>();
+    d_li = ctx->d_obsitems.as<char>();
+  }
   const double *d_ang = reinterpret_cast<const double *>(base + o_ang);
   sp.in_angle = reinterpret_cast<const int32_t *>(base + o_in);
   sp.out_angle = reinterpret_cast<const int32_t *>(base + o_out);
@@ -1563,19 +1658,12 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
   sp.sphi_cos = reinterpret_cast<const double *>(base + o_sc);
   sp.sphi_sin = reinterpret_cast<const double *>(base + o_ss);
 
-  int nmax_all = 1, nm_max = 1;
-  for (int g = 0; g < n; ++g) nmax_all = std::max(nmax_all, (int)ctx->nmax[g]);
-  nm_max = nmax_all;
-  const size_t nitems = ctx->final_items.size();
-  CTM_HIP(ctx->d_pt.ensure((size_t)(nmax_all + 1) * n_ang * nmax_all * 2 *
-                           sizeof(double)));
-  CTM_HIP(ctx->d_x.ensure(nitems * (size_t)sp.n_pairs * 4 * sizeof(cdbl)));
-  CTM_LAUNCH(CTM_KERNEL_OBS_X, 0., 0.,
-             launch_pitau(d_ang, n_ang, nmax_all, ctx->d_pt.as<double>(), s));
-  if (obs_x_lds_bytes(nmax_all, sp.n_out, 8, 256, 1, true) > kObsXMaxLds) {
-    ctx->err = "too many outgoing directions for the largest order";
-    return CTM_ERR_ARGUMENT;
-  }
+  (void)nm_max;
+  double *PT = reinterpret_cast<double *>(d_pt);
+  cdbl *X = reinterpret_cast<cdbl *>(d_x);
+  int32_t *LI = reinterpret_cast<int32_t *>(d_li);
+  CTM_LAUNCH_Q(quiet, s, CTM_KERNEL_OBS_X, 0., 0.,
+               launch_pitau(d_ang, n_ang, nmax_all, PT, s));
   // items by block order (largest first), launched in LDS-size buckets
   {
     std::vector<int32_t> &lst = ctx->obs_items;
@@ -1586,7 +1674,11 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
     for (int q = 1; q <= nmax_all + 1; ++q) cnt[q] += cnt[q - 1];
     for (size_t i = 0; i < nitems; ++i)
       lst[cnt[nmax_all - nm_of(ctx->final_items[i])]++] = (int32_t)i;
-    CTM_HIP(upload(ctx, ctx->d_obsitems, lst));
+    if (fixed)
+      CTM_HIP(hipMemcpyAsync(LI, lst.data(), nitems * sizeof(int32_t),
+                             hipMemcpyHostToDevice, s));
+    else
+      CTM_HIP(stage_copy(ctx, LI, lst.data(), nitems * sizeof(int32_t)));
     // (80 and 64: the blocks above 64 take 8 incoming directions per
     // thread, and up to order 80 two such chunks side by side fit the LDS)
     const int caps[] = {nmax_all, 80, 64, 32, 16};
@@ -1607,8 +1699,8 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
       }
       if (end > pos) {
         const hipStream_t ls = lane(ctx, b % kLanes);
-        CTM_LAUNCH_ON(ls, CTM_KERNEL_OBS_X, fb, 0.,
-                      launch_obs_x(sp, ctx->d_obsitems.as<int32_t>() + pos,
+        CTM_LAUNCH_Q(quiet, ls, CTM_KERNEL_OBS_X, fb, 0.,
+                      launch_obs_x(sp, LI + pos,
                                    (int)(end - pos), cap, cap > 64 ? 8 : 4,
                                    ctx->d_items.as<Item>(),
                                    ctx->obs_source >= 0
@@ -1620,17 +1712,16 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
                                                  : nullptr,
                                    ctx->averaged ? ctx->d_aoff.as<int64_t>()
                                                  : nullptr,
-                                   ctx->d_pt.as<double>(), nmax_all,
-                                   ctx->d_x.as<cdbl>(), ls));
+                                   PT, nmax_all, X, ls));
       }
       pos = end;
     }
     CTM_HIP(join_lanes(ctx));
   }
   if (nth > 0 && (qext || qextpol || qextcpol || qabs || qabspol)) {
-    CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
+    CTM_LAUNCH_Q(quiet, s, CTM_KERNEL_OBS_SUM, 0., 0.,
                launch_obs_q(sp, n, ctx->d_gioff.as<int32_t>(), ctx->gp_saved,
-                            ctx->d_x.as<cdbl>(),
+                            X,
                             spec->do_extinction ? qext : nullptr,
                             spec->do_extinction ? qextpol : nullptr,
                             spec->do_extinction ? qextcpol : nullptr,
@@ -1638,13 +1729,33 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
                             spec->do_absorption ? qabspol : nullptr, s));
   }
   if (nst > 0 && Z) {
-    CTM_LAUNCH(CTM_KERNEL_OBS_SUM, 0., 0.,
+    CTM_LAUNCH_Q(quiet, s, CTM_KERNEL_OBS_SUM, 0., 0.,
                launch_obs_z(sp, n, ctx->d_gioff.as<int32_t>(), ctx->gp_saved,
-                            ctx->d_x.as<cdbl>(), Z, s));
+                            X, Z, s));
   }
   // the spec blob and the host vectors must outlive the queued copies
   CTM_HIP(drain(ctx));
   return CTM_OK;
+}
+
+int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
+                    double *qext, double *qextpol, double *qextcpol,
+                    double *qabs, double *qabspol, double *Z) {
+  return observables_impl(ctx, spec, qext, qextpol, qextcpol, qabs, qabspol,
+                          Z, nullptr, 0);
+}
+
+int ctm_observables_noalloc(ctm_context *ctx,
+                            const ctm_observable_spec *spec, double *qext,
+                            double *qextpol, double *qextcpol, double *qabs,
+                            double *qabspol, double *Z) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (!ctx->scratch) {
+    ctx->err = "no fixed scratch on this device";
+    return CTM_ERR_STATE;
+  }
+  return observables_impl(ctx, spec, qext, qextpol, qextcpol, qabs, qabspol,
+                          Z, ctx->scratch, kNoAllocScratchBytes);
 }
 
 int ctm_orientation_average(ctm_context *ctx, int n_dist, int ncoef,

@@ -145,6 +145,25 @@ int ctm_observables(ctm_context *ctx, const ctm_observable_spec *spec,
                     double *qext, double *qextpol, double *qextcpol,
                     double *qabs, double *qabspol, double *Z);
 
+/* ctm_observables with no allocation of any kind on the device: no
+ * hipMalloc, hipFree or allocating library routine, and no profiler events.
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  Its scratch (direction tables, the pi/tau table, the
+ * bilinear forms and the item order) lives in a fixed 64 MiB device array
+ * of the library, one per GPU, held until the process ends; a call whose
+ * scratch would exceed it returns CTM_ERR_ARGUMENT with the size needed
+ * (X takes n_items * n_pairs * 64 bytes: n_items = sum over grains of
+ * nmax + 1, n_pairs = n_theta (1 + ngauss) + n_scatter_theta).  The
+ * outputs are the caller's, as for ctm_observables, and agree with it bit
+ * for bit.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (several contexts or threads) race, and the caller
+ * must serialise them. */
+int ctm_observables_noalloc(ctm_context *ctx,
+                            const ctm_observable_spec *spec, double *qext,
+                            double *qextpol, double *qextcpol, double *qabs,
+                            double *qabspol, double *Z);
+
 /* Replaces, for the observable calls, the context's T-matrices by their
  * ensemble averages over an axially symmetric orientation distribution with
  * Legendre coefficients p_N (Mishchenko 1991 eq. 3.27;
