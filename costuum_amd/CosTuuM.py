@@ -644,6 +644,34 @@ def _reduce_groups(ctx, x, group, weights):
     return out.reshape((n // group,) + tuple(x.shape[1:]))
 
 
+def _memory_batches(inter, shapes, min_order, max_order, n_pairs, budget,
+                    device):
+    """Consecutive ranges of interactions (each with all its shapes) whose
+    estimated device memory (ctm_grain_memory_estimate per grain) stays
+    within min(budget, 80 % of the free device memory), at most
+    _GRAINS_PER_BATCH grains each: the maximum_memory_size semantics of the
+    reference's task resources (TaskManager.hpp:101-348,718-760) applied to
+    the batches the GPU holds at once."""
+    free, _ = torch.cuda.mem_get_info(device)
+    cap = min(budget, 0.8 * free)
+    est = _lib().ctm_grain_memory_estimate
+    per = max(1, _GRAINS_PER_BATCH // max(len(shapes), 1))
+    out, i0 = [], 0
+    while i0 < len(inter):
+        i1, used = i0, 0.0
+        while i1 < len(inter) and i1 - i0 < per:
+            a, lam, _ = inter[i1]
+            b = sum(est(float(a), float(d), float(lam), min_order, max_order,
+                        n_pairs) for d in shapes)
+            if i1 > i0 and used + b > cap:
+                break
+            used += b
+            i1 += 1
+        out.append((i0, i1))
+        i0 = i1
+    return out
+
+
 def get_table(types, sizes, wavelengths, thetas, shape_distribution,
               alignment_distribution, dust_properties, minimum_order=10,
               maximum_order=100, gauss_legendre_factor=2, tolerance=1e-4,
@@ -661,9 +689,11 @@ def get_table(types, sizes, wavelengths, thetas, shape_distribution,
     (Qabs, Qabspol)? + (Qext, Qextpol, Qextcpol)?] and/or [ncomp, nsize,
     nlambda, ntheta, 2 ntheta, 4, 4] (both, as a tuple, when both are asked
     for).  Sizes and wavelengths are sorted ascending first
-    (TaskManager.hpp:457-459).  maximum_memory_size and number_of_threads
-    have no GPU meaning and are accepted for compatibility; the QuickSched
-    logs are replaced by a per-kernel timing log written to
+    (TaskManager.hpp:457-459).  maximum_memory_size bounds the device memory
+    of each batch of grains (with 80 % of the free device memory); the grid
+    streams through the GPU in as many batches as that takes.
+    number_of_threads has no GPU meaning and is accepted for compatibility;
+    the QuickSched logs are replaced by a per-kernel timing log written to
     quicksched_task_log when it is given."""
     comps = _as_1d(types, np.int64)
     size_arr = np.sort(_as_1d(sizes, np.float64))
@@ -693,10 +723,15 @@ def get_table(types, sizes, wavelengths, thetas, shape_distribution,
     ctx = Context(0)
     if quicksched_task_log:
         ctx.profile(True)
-    per_batch = max(1, _GRAINS_PER_BATCH // max(nshape, 1))
+    chunks = _memory_batches(inter, shapes, int(minimum_order),
+                             int(maximum_order),
+                             nth * (1 + (int(number_of_quadrature_angles)
+                                         if account_for_scattering else 0)) +
+                             (nth if do_scattering else 0),
+                             float(maximum_memory_size), ctx.device)
     try:
-        for i0 in range(0, ninter, per_batch):
-            chunk = inter[i0:i0 + per_batch]
+        for i0, i1 in chunks:
+            chunk = inter[i0:i1]
             rv = np.repeat([c[0] for c in chunk], nshape)
             lam = np.repeat([c[1] for c in chunk], nshape)
             mre = np.repeat([c[2].real for c in chunk], nshape)

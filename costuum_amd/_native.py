@@ -30,6 +30,7 @@ EXPORTS = (
     "ctm_adaptive_quadrature", "ctm_orientation_coefficients",
     "ctm_weighted_reduce", "ctm_tmatrix_tangents", "ctm_tangent_get",
     "ctm_observables_select", "ctm_vjp", "ctm_observables_noalloc",
+    "ctm_tmatrix_set", "ctm_grain_memory_estimate",
 )
 
 # double f(double x, void *arg): the host callback type (ctm_real_fn)
@@ -131,6 +132,12 @@ def lib():
                                           dp, dp]
     L.ctm_orientation_average.restype = ctypes.c_int
     L.ctm_average_get.argtypes = [vp, ctypes.c_int, dp, ctypes.c_size_t]
+    L.ctm_tmatrix_set.argtypes = [vp, ctypes.c_int, dp, ctypes.c_size_t]
+    L.ctm_tmatrix_set.restype = ctypes.c_int
+    L.ctm_grain_memory_estimate.argtypes = [ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_int32]
+    L.ctm_grain_memory_estimate.restype = ctypes.c_double
     L.ctm_average_get.restype = ctypes.c_int
     L.ctm_query.argtypes = [vp, ctypes.c_int, ctypes.POINTER(CtmQuery), dp]
     L.ctm_query.restype = ctypes.c_int

@@ -1895,6 +1895,42 @@ int ctm_weighted_reduce(ctm_context *ctx, int n_groups, int group,
   return CTM_OK;
 }
 
+int ctm_tmatrix_set(ctm_context *ctx, int g, const double *T_host,
+                    size_t count) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (ctx->final_items.empty()) {
+    ctx->err = "no T-matrices in the context";
+    return CTM_ERR_STATE;
+  }
+  if (ctx->obs_source >= 0) {
+    ctx->err = "a tangent is selected for the observables";
+    return CTM_ERR_STATE;
+  }
+  const size_t need = ctm_tmatrix_size(ctx, g);
+  if (need == 0 || !T_host || count != need) {
+    ctx->err = "invalid grain index or element count";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (!ctx->averaged) {
+    // the full-layout copy of every grain's T-matrix (alignment disabled)
+    int nmax = 1;
+    for (int i = 0; i < ctx->n_grains; ++i)
+      nmax = std::max(nmax, (int)ctx->nmax[i]);
+    std::vector<double> pn(2 * (size_t)nmax + 1, 0.);
+    std::vector<int32_t> dist(ctx->n_grains, -1);
+    const int rc = ctm_orientation_average(ctx, 1, 2 * nmax + 1, pn.data(),
+                                           dist.data());
+    if (rc != CTM_OK) return rc;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  const int64_t o = ctx->aoff[ctx->grain_items_off[g]];
+  CTM_HIP(hipMemcpyAsync(ctx->d_avg.as<cdbl>() + o, T_host,
+                         need * sizeof(cdbl), hipMemcpyHostToDevice,
+                         ctx->stream));
+  CTM_HIP(drain(ctx));
+  return CTM_OK;
+}
+
 int ctm_average_get(ctm_context *ctx, int g, double *T_host, size_t capacity) {
   if (!ctx) return CTM_ERR_ARGUMENT;
   if (!ctx->averaged) {

@@ -3,6 +3,7 @@
 // a spheroid and the Legendre coefficients of orientation distributions.
 // Nothing here touches the GPU; the Python module and the C++ set-up code
 // call these while building a batch.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -161,6 +162,28 @@ int ctm_orientation_coefficients(ctm_real_fn p, void *arg, uint32_t order,
     ok = ok && reached;
   }
   return ok ? CTM_OK : CTM_NOT_CONVERGED;
+}
+
+double ctm_grain_memory_estimate(double radius, double axis_ratio,
+                                 double wavelength, uint32_t min_order,
+                                 uint32_t max_order, int32_t n_pairs) {
+  if (!(radius > 0.) || !(axis_ratio > 0.) || !(wavelength > 0.)) return 0.;
+  // order estimate from the largest semi-axis (a = R_V d^(2/3), b = R_V
+  // d^(-1/3)), with a margin for the convergence windows
+  const double amax = radius * std::max(std::pow(axis_ratio, 2. / 3.),
+                                        std::pow(axis_ratio, -1. / 3.));
+  const double x = 2. * M_PI * amax / wavelength;
+  double N = std::ceil(x + 4.05 * std::cbrt(x)) + 8.;
+  N = std::max(N, (double)min_order + 1.);
+  N = std::min(N, std::max((double)max_order - 1., 1.));
+  // sum over the m blocks of nm^2 (m = 0: N, m > 0: N + 1 - m)
+  const double s2 = N * N + N * (N + 1.) * (2. * N + 1.) / 6.;
+  // Q and RgQ (4 s2), T (2 s2), ensemble T (4 s2) complex values; Bessel
+  // tables of the window orders (16 n^2 doubles each, summed up to N);
+  // the observables' bilinear forms (4 complex values per direction pair
+  // and (grain, m) block)
+  return 16. * 10. * s2 + 8. * 16. * N * (N + 1.) * (2. * N + 1.) / 6. +
+         64. * (N + 1.) * std::max(n_pairs, 0);
 }
 
 }  // extern "C"

@@ -194,6 +194,15 @@ int ctm_weighted_reduce(ctm_context *ctx, int n_groups, int group,
 int ctm_average_get(ctm_context *ctx, int grain, double *T_host,
                     size_t capacity);
 
+/* Replaces grain g's T-matrix, as seen by the observable and query calls,
+ * by T_host (count = ctm_tmatrix_size complex values in the reference block
+ * layout): TMatrix::set_element (TMatrix.hpp:1062-1080) applied to a whole
+ * T-matrix.  The values go to the context's full-layout (ensemble) copy;
+ * if the context holds no ensemble T-matrices yet, every grain's T-matrix
+ * is first copied there unchanged, as for a disabled alignment. */
+int ctm_tmatrix_set(ctm_context *ctx, int grain, const double *T_host,
+                    size_t count);
+
 /* S (2x2 complex), Z (4x4) and K (4x4) of grain g for a particle orientation
  * (alpha, beta) and laboratory incoming/outgoing directions, from the
  * context's T-matrices (ensemble ones after ctm_orientation_average):
@@ -280,6 +289,18 @@ int ctm_adaptive_quadrature(ctm_real_fn f, void *arg, double a, double b,
 int ctm_orientation_coefficients(ctm_real_fn p, void *arg, uint32_t order,
                                  double abs_err, double rel_err,
                                  double *coefficients);
+
+/* Estimated device bytes one grain takes in the batched path (its
+ * T-matrix blocks, Q/RgQ, ensemble copy, the Bessel tables of its
+ * convergence orders and, for n_pairs observable direction pairs, their
+ * bilinear forms), from an order estimate x + 4.05 x^(1/3) + 8 on the
+ * largest semi-axis clamped to [min_order + 1, max_order - 1].  Callers
+ * size batches with it to honour a memory budget, as the reference's
+ * maximum_memory_usage does for its task resources
+ * (TaskManager.hpp:101-348,718-760).  Host only. */
+double ctm_grain_memory_estimate(double radius, double axis_ratio,
+                                 double wavelength, uint32_t min_order,
+                                 uint32_t max_order, int32_t n_pairs);
 
 /* Kernel timing.  With profiling on, every launch of the library is
  * bracketed by two HIP events on the stream it runs on; ctm_profile_read

@@ -11,6 +11,14 @@
  *            get_extinction_coefficient    TMatrix.hpp:2055-2110
  *   TMatrix::get_scattering_matrix,
  *            get_extinction_matrix         TMatrix.hpp:1891-2010
+ *   TMatrix::set_element                   TMatrix.hpp:1062-1080
+ *   TMatrix::get_forward_scattering_matrix TMatrix.hpp:1109,1246,1306
+ *   TMatrix::get_reduced_extinction_matrix TMatrix.hpp:2034-2048
+ *   TMatrix::get_absorption_cross_sections,
+ *            get_absorption_cross_section,
+ *            get_average_absorption_cross_section
+ *                                          TMatrix.hpp:2133-2287
+ *   Matrix<T> (row-major, operator()(i, j)) Matrix.hpp:40-120
  *
  * Every TMatrix owns a library context on HIP device 0 that holds its
  * T-matrix on the device; the caller owns the returned pointers, as with the
@@ -33,6 +41,27 @@
 #include <vector>
 
 #include "costuum_amd.h"
+
+// The reference's floating-point type (Configuration.hpp: double).
+#ifndef COSTUUM_AMD_NO_FLOAT_TYPE
+typedef double float_type;
+#endif
+
+// Dense row-major matrix with the reference's accessors (Matrix.hpp).
+template <typename T>
+class Matrix {
+ public:
+  Matrix(size_t rows = 0, size_t cols = 0)
+      : _rows(rows), _cols(cols), _v(rows * cols, T(0)) {}
+  T &operator()(size_t i, size_t j) { return _v[i * _cols + j]; }
+  const T &operator()(size_t i, size_t j) const { return _v[i * _cols + j]; }
+  size_t get_number_of_rows() const { return _rows; }
+  size_t get_number_of_columns() const { return _cols; }
+
+ private:
+  size_t _rows, _cols;
+  std::vector<T> _v;
+};
 
 namespace costuum_amd_shim {
 
@@ -86,10 +115,31 @@ struct Session {
   double *out(int j) { return dev + (size_t)(5 + j) * cap; }
 };
 
-// Row-major 4 x 4 real matrix; Z(i, j) as with the reference's Matrix.
-struct Matrix4 {
-  double v[16];
-  double operator()(int i, int j) const { return v[4 * i + j]; }
+// 4 x 4 real matrices (Z, K) are returned as the reference's Matrix.
+typedef ::Matrix<double> Matrix4;
+
+// Device array of n doubles, freed on scope exit.
+struct DeviceDoubles {
+  double *p = nullptr;
+  explicit DeviceDoubles(size_t n) {
+    check_hip(hipMalloc(reinterpret_cast<void **>(&p),
+                        std::max<size_t>(n, 1) * sizeof(double)),
+              "hipMalloc");
+    check_hip(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(double)),
+              "hipMemset");
+  }
+  ~DeviceDoubles() {
+    if (p) (void)hipFree(p);
+  }
+  DeviceDoubles(const DeviceDoubles &) = delete;
+  DeviceDoubles &operator=(const DeviceDoubles &) = delete;
+  std::vector<double> host(size_t n) const {
+    std::vector<double> h(n);
+    check_hip(hipMemcpy(h.data(), p, n * sizeof(double),
+                        hipMemcpyDeviceToHost),
+              "hipMemcpy");
+    return h;
+  }
 };
 
 }  // namespace costuum_amd_shim
@@ -115,6 +165,16 @@ class TMatrix {
     const size_t nm = _nmax + 1 - (m1 > 0 ? m1 : 1);
     const size_t p1 = n1 - (m1 > 0 ? m1 : 1), p2 = n2 - (m2 > 0 ? m2 : 1);
     return _host[_block[m1] + (i1 * nm + p1) * 2 * nm + i2 * nm + p2];
+  }
+
+  // Sets T^{(i1 i2)}_{n1 n2 m1 m2} (TMatrix.hpp:1062-1080); later queries
+  // and cross sections use the new values.
+  void set_element(uint_fast8_t i1, uint_fast32_t n1, uint_fast32_t m1,
+                   uint_fast8_t i2, uint_fast32_t n2, uint_fast32_t m2,
+                   const std::complex<double> &value) {
+    const std::complex<double> &e = (*this)(i1, n1, m1, i2, n2, m2);
+    const_cast<std::complex<double> &>(e) = value;
+    _dirty = true;
   }
 
   // Sums over the whole T-matrix (TMatrix.hpp:2055-2110), unnormalised:
@@ -144,6 +204,122 @@ class TMatrix {
     return q;
   }
 
+  // S (2 x 2 complex) for a particle orientation (alpha, beta) and
+  // incoming/outgoing directions (TMatrix.hpp:1306-1482), for the particle
+  // frame (TMatrix.hpp:1246-1285) and from the sines and cosines of the
+  // particle-frame angles (TMatrix.hpp:1109-1229; 1 / sin theta is not
+  // needed, a zero sine being handled as the reference does).
+  Matrix<std::complex<double>> get_forward_scattering_matrix(
+      double alpha, double beta, double theta_in, double phi_in,
+      double theta_out, double phi_out) const {
+    double out[40];
+    raw_query(alpha, beta, theta_in, phi_in, theta_out, phi_out, out);
+    Matrix<std::complex<double>> S(2, 2);
+    for (int i = 0; i < 4; ++i)
+      S(i / 2, i % 2) = std::complex<double>(out[2 * i], out[2 * i + 1]);
+    return S;
+  }
+  Matrix<std::complex<double>> get_forward_scattering_matrix(
+      double theta_in, double phi_in, double theta_out,
+      double phi_out) const {
+    return get_forward_scattering_matrix(0., 0., theta_in, phi_in, theta_out,
+                                         phi_out);
+  }
+  Matrix<std::complex<double>> get_forward_scattering_matrix(
+      double costheta_in, double sintheta_in, double /*sintheta_in_inv*/,
+      double cosphi_in, double sinphi_in, double costheta_out,
+      double sintheta_out, double /*sintheta_out_inv*/, double cosphi_out,
+      double sinphi_out) const {
+    return get_forward_scattering_matrix(
+        std::atan2(sintheta_in, costheta_in), std::atan2(sinphi_in, cosphi_in),
+        std::atan2(sintheta_out, costheta_out),
+        std::atan2(sinphi_out, cosphi_out));
+  }
+
+  // [C_ext, C_pol, C_cpol] = (2 pi / k) [Im(S11 + S22), Im(S11 - S22),
+  // Re(S22 - S11)] for the forward direction (TMatrix.hpp:2034-2048).
+  void get_reduced_extinction_matrix(double alpha, double beta,
+                                     double theta_in, double phi_in,
+                                     double output[3]) const {
+    Matrix<std::complex<double>> S = get_forward_scattering_matrix(
+        alpha, beta, theta_in, phi_in, theta_in, phi_in);
+    const double pre = _lam;  // 2 pi / k
+    output[0] = pre * (S(0, 0) + S(1, 1)).imag();
+    output[1] = pre * (S(0, 0) - S(1, 1)).imag();
+    output[2] = pre * (S(1, 1) - S(0, 0)).real();
+  }
+
+  // C_abs (result[0]) and C_abspol (result[1]) for incidence theta in the
+  // particle frame, minus the scattered light over the given quadrature of
+  // incoming directions (TMatrix.hpp:2133-2185).
+  void get_absorption_cross_sections(
+      double theta, uint_fast32_t ngauss, bool do_Cabs, bool do_Cabspol,
+      const std::vector<double> &thetaGL,
+      const std::vector<double> &costhetaweightsGL,
+      const std::vector<double> &phiGL,
+      const std::vector<double> &phiweightsGL, double *result) const {
+    const size_t nq = 1 + (size_t)ngauss * ngauss;
+    std::vector<ctm_query_spec> q(nq);
+    q[0] = ctm_query_spec{0, 0, 0., 0., theta, 0., theta, 0.};
+    for (uint_fast32_t i = 0; i < ngauss; ++i)
+      for (uint_fast32_t j = 0; j < ngauss; ++j)
+        q[1 + (size_t)i * ngauss + j] =
+            ctm_query_spec{0, 0, 0., 0., thetaGL[i], phiGL[j], theta, 0.};
+    std::vector<double> out(40 * nq);
+    sync_device();
+    costuum_amd_shim::check(ctm_query(_s->ctx, (int)nq, q.data(), out.data()),
+                            "ctm_query", _s->ctx);
+    auto S = [&](size_t k, int e) {
+      return std::complex<double>(out[40 * k + 2 * e],
+                                  out[40 * k + 2 * e + 1]);
+    };
+    const double pre = _lam;
+    if (do_Cabs) result[0] = pre * (S(0, 0) + S(0, 3)).imag();
+    if (do_Cabspol) result[1] = pre * (S(0, 0) - S(0, 3)).imag();
+    for (uint_fast32_t i = 0; i < ngauss; ++i)
+      for (uint_fast32_t j = 0; j < ngauss; ++j) {
+        const size_t k = 1 + (size_t)i * ngauss + j;
+        const double w = costhetaweightsGL[i] * phiweightsGL[j];
+        const double a = std::norm(S(k, 0)) + std::norm(S(k, 1));
+        const double b = std::norm(S(k, 2)) + std::norm(S(k, 3));
+        if (do_Cabs) result[0] -= 0.5 * (a + b) * w;
+        if (do_Cabspol) result[1] -= 0.5 * (a - b) * w;
+      }
+  }
+  // C_abs, C_abspol pairs for ntheta angles with an ngauss x ngauss
+  // Gauss-Legendre quadrature (TMatrix.hpp:2194-2215).
+  void get_absorption_cross_sections(const double *theta,
+                                     uint_fast32_t ntheta,
+                                     uint_fast32_t ngauss,
+                                     double *Cabs) const {
+    std::vector<double> a(ntheta), p(ntheta);
+    cabs(theta, ntheta, ngauss, a.data(), p.data());
+    for (uint_fast32_t i = 0; i < ntheta; ++i) {
+      Cabs[2 * i] = a[i];
+      Cabs[2 * i + 1] = p[i];
+    }
+  }
+  // C_abs only (TMatrix.hpp:2229-2250).
+  void get_absorption_cross_section(const double *theta, uint_fast32_t ntheta,
+                                    uint_fast32_t ngauss,
+                                    double *Cabs) const {
+    std::vector<double> p(ntheta);
+    cabs(theta, ntheta, ngauss, Cabs, p.data());
+  }
+  // C_abs averaged over incidence directions with the same Gauss-Legendre
+  // rule (TMatrix.hpp:2261-2287).
+  double get_average_absorption_cross_section(uint_fast32_t ngauss) const {
+    std::vector<double> x(ngauss), w(ngauss), th(ngauss), c(ngauss);
+    costuum_amd_shim::check(
+        ctm_gauss_legendre((uint32_t)ngauss, -1., 1., x.data(), w.data()),
+        "ctm_gauss_legendre", nullptr);
+    for (uint_fast32_t i = 0; i < ngauss; ++i) th[i] = std::acos(x[i]);
+    get_absorption_cross_section(th.data(), ngauss, ngauss, c.data());
+    double r = 0.;
+    for (uint_fast32_t i = 0; i < ngauss; ++i) r += c[i] * w[i];
+    return 0.5 * r;
+  }
+
   // Z for a particle orientation (alpha, beta) and incoming/outgoing
   // directions (radians), TMatrix.hpp:1891-1952.
   costuum_amd_shim::Matrix4 get_scattering_matrix(
@@ -166,8 +342,53 @@ class TMatrix {
   double _rv = 0., _d = 1., _lam = 1.;  // equal-volume radius, d, lambda
   std::complex<double> _mr;
   bool _averaged = false;
+  bool _dirty = false;  // set_element changed _host after the last upload
   mutable std::vector<std::complex<double>> _host;
   mutable std::vector<size_t> _block;  // start of block m in _host
+
+  // Uploads elements changed by set_element before any device query.
+  void sync_device() const {
+    if (!_dirty) return;
+    std::vector<double> buf(2 * _host.size());
+    for (size_t i = 0; i < _host.size(); ++i) {
+      buf[2 * i] = _host[i].real();
+      buf[2 * i + 1] = _host[i].imag();
+    }
+    costuum_amd_shim::check(
+        ctm_tmatrix_set(_s->ctx, 0, buf.data(), _host.size()),
+        "ctm_tmatrix_set", _s->ctx);
+    const_cast<TMatrix *>(this)->_dirty = false;
+    const_cast<TMatrix *>(this)->_averaged = true;
+  }
+
+  void raw_query(double alpha, double beta, double theta_in, double phi_in,
+                 double theta_out, double phi_out, double *out) const {
+    sync_device();
+    ctm_query_spec q{0, 0, alpha, beta, theta_in, phi_in, theta_out, phi_out};
+    costuum_amd_shim::check(ctm_query(_s->ctx, 1, &q, out), "ctm_query",
+                            _s->ctx);
+  }
+
+  // C_abs and C_abspol at ntheta angles: Q_abs, Q_abspol of the library
+  // (scattered light subtracted with an ngauss x ngauss Gauss-Legendre
+  // quadrature) times pi R_V^2.
+  void cabs(const double *theta, uint_fast32_t ntheta, uint_fast32_t ngauss,
+            double *a, double *p) const {
+    if (ntheta == 0) return;
+    sync_device();
+    costuum_amd_shim::DeviceDoubles qa(ntheta), qp(ntheta);
+    ctm_observable_spec spec{(int32_t)ntheta, theta, 0, 1, 1, (int32_t)ngauss,
+                             0, 1};
+    costuum_amd_shim::check(ctm_observables(_s->ctx, &spec, nullptr, nullptr,
+                                            nullptr, qa.p, qp.p, nullptr),
+                            "ctm_observables", _s->ctx);
+    const std::vector<double> ha = qa.host(ntheta), hp = qp.host(ntheta);
+    const double area = M_PI * _rv * _rv;
+    for (uint_fast32_t i = 0; i < ntheta; ++i) {
+      a[i] = ha[i] * area;
+      p[i] = hp[i] * area;
+    }
+  }
 
   void load() const {
     if (!_host.empty()) return;
@@ -191,12 +412,10 @@ class TMatrix {
   costuum_amd_shim::Matrix4 query(double alpha, double beta, double theta_in,
                                   double phi_in, double theta_out,
                                   double phi_out, int at) const {
-    ctm_query_spec q{0, 0, alpha, beta, theta_in, phi_in, theta_out, phi_out};
     double out[40];
-    costuum_amd_shim::check(ctm_query(_s->ctx, 1, &q, out), "ctm_query",
-                            _s->ctx);
-    costuum_amd_shim::Matrix4 r;
-    for (int i = 0; i < 16; ++i) r.v[i] = out[at + i];
+    raw_query(alpha, beta, theta_in, phi_in, theta_out, phi_out, out);
+    costuum_amd_shim::Matrix4 r(4, 4);
+    for (int i = 0; i < 16; ++i) r(i / 4, i % 4) = out[at + i];
     return r;
   }
 };
@@ -270,6 +489,13 @@ class TMatrixCalculator {
   static TMatrix *apply_orientation_distribution(const TMatrix &T,
                                                  const Distribution &dist) {
     using namespace costuum_amd_shim;
+    // the library averages the T-matrix it computes for (R_V, d, lambda,
+    // m, nmax, ngauss): an ensemble or element-wise edited input has no
+    // such description
+    if (T._dirty || T._averaged)
+      fail("apply_orientation_distribution needs a T-matrix from "
+           "calculate_TMatrix (not an ensemble or edited one)",
+           nullptr);
     auto s = std::make_shared<Session>(1);
     s->put(T._rv, T._d, T._lam, T._mr, 1);
     TMatrix *A = fixed(s, T._rv, T._d, T._lam, T._mr, T._nmax, T._ngauss);

@@ -6,6 +6,7 @@ import math
 import os
 
 import numpy as np
+import torch
 import pytest
 
 import oracle as O
@@ -192,6 +193,32 @@ def test_get_table_matches_oracle():
     for k in range(5):
         assert relmax(q[..., k], rq[..., k]) < 1e-8, k
     assert relmax(z, rz) < 1e-8
+
+
+def test_get_table_memory_budget_streams_batches():
+    """maximum_memory_size bounds each batch: with a budget below one
+    interaction's estimate every (composition, size, wavelength) becomes its
+    own batch, and the table is the same as in one batch."""
+    from costuum_amd import CosTuuM as C
+    shape = C.DraineHensleyShapeDistribution(3)
+    align = C.SizeBasedAlignmentDistribution(0.1, C.MISHCHENKO_ALIGNMENT, 50)
+    dust = C.CustomDustProperties(lambda lam, a, t: complex(1.5, 0.03))
+    args = ([C.SILICON], [0.25, 0.08, 0.15], [1.1, 0.55], [0.3, 1.2], shape,
+            align, dust)
+    kw = dict(minimum_order=4, maximum_order=60, do_absorption=True,
+              do_extinction=True, account_for_scattering=True,
+              number_of_quadrature_angles=8)
+    inter = [(a, lam, 1.5 + 0.03j) for a in (0.08, 0.15, 0.25)
+             for lam in (0.55, 1.1)]
+    shapes = [shape.get_shape(i) for i in range(3)]
+    dev = torch.device("cuda", 0)
+    assert len(C._memory_batches(inter, shapes, 4, 60, 18, 1.0, dev)) == 6
+    assert len(C._memory_batches(inter, shapes, 4, 60, 18, 1e10, dev)) == 1
+    print("get_table with a one-byte budget", flush=True)
+    small = C.get_table(*args, maximum_memory_size=1, **kw)
+    big = C.get_table(*args, **kw)
+    assert small.shape == big.shape == (3, 2, 2, 5)
+    assert np.abs(small - big).max() <= 1e-13 * np.abs(big).max()
 
 
 def test_get_table_disabled_alignment_and_squeeze():
