@@ -35,7 +35,7 @@ def parse():
     # (about half a second timed), 1 + 1 for the grid and gradient workloads
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--grains", type=int, default=4096)
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the CPU oracle on a sample (rank 0, N=1)")

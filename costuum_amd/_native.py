@@ -30,7 +30,7 @@ EXPORTS = (
     "ctm_adaptive_quadrature", "ctm_orientation_coefficients",
     "ctm_weighted_reduce", "ctm_tmatrix_tangents", "ctm_tangent_get",
     "ctm_observables_select", "ctm_vjp", "ctm_observables_noalloc",
-    "ctm_tmatrix_set", "ctm_grain_memory_estimate",
+    "ctm_tmatrix_set", "ctm_grain_memory_estimate", "ctm_precision_stats",
 )
 
 # double f(double x, void *arg): the host callback type (ctm_real_fn)
@@ -48,7 +48,7 @@ QUERY_OUT = 40
 
 KERNELS = ("gauss_legendre", "wigner", "tables", "assemble", "solve",
            "converge", "sums", "obs_x", "obs_sum", "average",
-           "assemble_small", "assemble_wide", "assemble_pack")
+           "assemble_small", "assemble_wide", "assemble_pack", "refine")
 
 
 class CtmObservableSpec(ctypes.Structure):
@@ -138,6 +138,9 @@ def lib():
                                             ctypes.c_double, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_int32]
     L.ctm_grain_memory_estimate.restype = ctypes.c_double
+    L.ctm_precision_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int64)]
+    L.ctm_precision_stats.restype = ctypes.c_int
     L.ctm_average_get.restype = ctypes.c_int
     L.ctm_query.argtypes = [vp, ctypes.c_int, ctypes.POINTER(CtmQuery), dp]
     L.ctm_query.restype = ctypes.c_int

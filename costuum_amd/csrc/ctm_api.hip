@@ -35,6 +35,7 @@
 #include "ctm_average.hpp"
 #include "ctm_host.hpp"
 #include "ctm_tangent.hpp"
+#include "ctm_mixed.hpp"
 
 using namespace ctm;
 
@@ -357,7 +358,14 @@ struct ctm_context {
       d_winitems, d_gioff, d_gitems, d_gioff_tmp, d_params, d_ang, d_pt,
       d_x, d_spec, d_cgkeys, d_cg, d_cgidx, d_avgtasks, d_avg, d_aoff, d_pn,
       d_dist, d_obsitems, d_query, d_qout, d_gsolve, d_ttables, d_dq, d_ainv,
-      d_dt, d_tparts, d_dsum, d_ttiles, d_tprefix, d_wigfresh;
+      d_dt, d_tparts, d_dsum, d_ttiles, d_tprefix, d_wigfresh, d_x32, d_xexp,
+      d_rbuf, d_flags, d_fallback;
+  // mixed precision for the current batch call, and its block counts since
+  // the batch started (blocks through the FP32 elimination, blocks re-solved
+  // in FP64 after the refinement stalled)
+  bool mixed = false;
+  int64_t mixed_blocks = 0, mixed_fallback = 0;
+  std::vector<int32_t> fallback;
   // Results of the last batch (host).
   int n_grains = 0;
   std::vector<uint32_t> nmax, ngauss;
@@ -640,14 +648,20 @@ void build_tasks(WorkList &w, int i0, int i1) {
   }
 }
 
-// Inversion and solve of the current work list's blocks (Q in ctx->d_q),
-// in LDS-size buckets; ainv_out (may be NULL) receives every Q_c^-1.
-// parts: the assembly parts run_items launched (their ends recorded in
-// ev_part, the side lanes forked); 0 when Q was written on the context's
-// stream.
-static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out,
-                      unsigned parts = 0) {
+// Inversion and solve of the given tasks (item << 1 | class, block order
+// descending; host list `tasks`, device copy `dtasks`) of the current work
+// list (Q in ctx->d_q), in LDS-size buckets; ainv_out (may be NULL)
+// receives every Q_c^-1.  parts: the assembly parts run_items launched
+// (their ends recorded in ev_part, the side lanes forked); 0 when Q was
+// written on the context's stream.  With `mixed` the buckets of orders
+// kMixedMinOrder..kMixedMaxOrder take the mixed-precision solve (FP32
+// elimination, FP64 refinement), whose per-task flags go to
+// ctx->d_flags at the tasks' positions; *n_mixed counts those tasks.
+static int solve_list(ctm_context *ctx, const std::vector<int32_t> &tasks,
+                      const int32_t *dtasks, bool want_t, cdbl *ainv_out,
+                      unsigned parts, bool mixed, size_t *n_mixed) {
   WorkList &w = ctx->work;
+  if (n_mixed) *n_mixed = 0;
   // solve tasks in LDS-size buckets, largest first; blocks above the LDS
   // capacity are inverted in global scratch, at most kGlobalSolveBytes of it
   // per launch
@@ -666,15 +680,14 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out,
   const int lane_of[kBuckets] = {0, 0, 3, 1, 3, 2, 2};
   if (parts == 0) CTM_HIP(fork_lanes(ctx));
   size_t pos = 0;
-  const int32_t *dsolve = ctx->d_solve.as<int32_t>();
-  for (int b = 0; b < kBuckets && pos < w.solve.size(); ++b) {
+  for (int b = 0; b < kBuckets && pos < tasks.size(); ++b) {
     const int hi = bounds[b];
     const int lo = (b < kBuckets - 1) ? bounds[b + 1] : 0;
     size_t end = pos;
     int nm_max = 0;
-    size_t cap = w.solve.size();
-    while (end < w.solve.size() && end - pos < cap) {
-      const Item &it = w.items[w.solve[end] >> 1];
+    size_t cap = tasks.size();
+    while (end < tasks.size() && end - pos < cap) {
+      const Item &it = w.items[tasks[end] >> 1];
       const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
       if (nm <= lo) break;
       if (nm > hi) {
@@ -695,7 +708,7 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out,
       int64_t store_max = 0;
       int rows_max = 0;
       for (size_t q = pos; q < end; ++q) {
-        const Item &it = w.items[w.solve[q] >> 1];
+        const Item &it = w.items[tasks[q] >> 1];
         const int nmi = it.N + 1 - (it.m > 0 ? it.m : 1);
         const double nm = nmi;
         fs += (it.m > 0 ? 32. / 3. : 8. / 3.) * nm * nm * nm;
@@ -710,31 +723,97 @@ static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out,
       if (parts != 0) {
         unsigned need = 0;
         for (size_t q = pos; q < end; ++q)
-          need |= 1u << item_part(w.items[w.solve[q] >> 1]);
+          need |= 1u << item_part(w.items[tasks[q] >> 1]);
         for (int p = 0; p < 5; ++p)
           if ((need & parts) >> p & 1u)
             CTM_HIP(hipStreamWaitEvent(ls, ctx->ev_part[p], 0));
       }
-      CTM_LAUNCH_ON(ls, CTM_KERNEL_SOLVE, fs, bs,
-                    launch_invert_solve(dsolve + pos, (int)(end - pos),
-                                        nm_max, store_max, rows_max,
-                                        ctx->d_items.as<Item>(),
-                                        ctx->d_q.as<cdbl>(),
-                                        want_t ? ctx->d_t.as<cdbl>() : nullptr,
-                                        ctx->d_qparts.as<double>(),
-                                        want_t ? 0 : 1,
-                                        ctx->d_gsolve.as<cdbl>(), ainv_out,
-                                        ls));
+      const bool mix = mixed && want_t && !ainv_out &&
+                       lo >= kMixedMinOrder - 1 && nm_max <= kMixedMaxOrder;
+      if (mix) {
+        CTM_LAUNCH_ON(ls, CTM_KERNEL_SOLVE, fs, bs,
+                      launch_invert32(dtasks + pos, (int)(end - pos), nm_max,
+                                      store_max, rows_max,
+                                      ctx->d_items.as<Item>(),
+                                      ctx->d_q.as<cdbl>(),
+                                      ctx->d_x32.as<cflt>(),
+                                      ctx->d_xexp.as<int32_t>(), ls));
+        CTM_LAUNCH_ON(ls, CTM_KERNEL_REFINE, 0., 0.,
+                      launch_refine(dtasks + pos, (int)(end - pos), nm_max,
+                                    ctx->d_items.as<Item>(),
+                                    ctx->d_q.as<cdbl>(),
+                                    ctx->d_x32.as<cflt>(),
+                                    ctx->d_xexp.as<int32_t>(),
+                                    ctx->d_t.as<cdbl>(),
+                                    ctx->d_rbuf.as<cdbl>(),
+                                    ctx->d_qparts.as<double>(),
+                                    ctx->d_flags.as<int32_t>() + pos, ls));
+        if (n_mixed) *n_mixed += end - pos;
+      } else {
+        CTM_LAUNCH_ON(ls, CTM_KERNEL_SOLVE, fs, bs,
+                      launch_invert_solve(dtasks + pos, (int)(end - pos),
+                                          nm_max, store_max, rows_max,
+                                          ctx->d_items.as<Item>(),
+                                          ctx->d_q.as<cdbl>(),
+                                          want_t ? ctx->d_t.as<cdbl>()
+                                                 : nullptr,
+                                          ctx->d_qparts.as<double>(),
+                                          want_t ? 0 : 1,
+                                          ctx->d_gsolve.as<cdbl>(), ainv_out,
+                                          ls));
+        if (mixed && want_t && ctx->d_flags.p)
+          CTM_HIP(hipMemsetAsync(ctx->d_flags.as<int32_t>() + pos, 0,
+                                 (end - pos) * sizeof(int32_t), ls));
+      }
     }
     pos = end;
     // the global bucket may need several launches
-    if (b == 0 && pos < w.solve.size()) {
-      const Item &it = w.items[w.solve[pos] >> 1];
+    if (b == 0 && pos < tasks.size()) {
+      const Item &it = w.items[tasks[pos] >> 1];
       if (it.N + 1 - (it.m > 0 ? it.m : 1) > kMaxLdsOrder) --b;
     }
   }
   CTM_HIP(join_lanes(ctx));
   return CTM_OK;
+}
+
+// Inversion and solve of the current work list's blocks; in mixed
+// precision (ctx->mixed, T wanted) the tasks whose refinement did not
+// converge are solved again in FP64.
+static int run_solves(ctm_context *ctx, bool want_t, cdbl *ainv_out,
+                      unsigned parts = 0) {
+  WorkList &w = ctx->work;
+  const bool mixed = ctx->mixed && want_t && !ainv_out;
+  if (mixed) {
+    const size_t nt = w.solve.size();
+    CTM_HIP(ctx->d_x32.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cflt)));
+    CTM_HIP(ctx->d_xexp.ensure(std::max<int64_t>(w.t_size, 1) *
+                               sizeof(int32_t)));
+    CTM_HIP(ctx->d_rbuf.ensure(std::max<int64_t>(w.t_size, 1) *
+                               sizeof(cdbl)));
+    CTM_HIP(ctx->d_flags.ensure(std::max<size_t>(nt, 1) * sizeof(int32_t)));
+  }
+  size_t n_mixed = 0;
+  int rc = solve_list(ctx, w.solve, ctx->d_solve.as<int32_t>(), want_t,
+                      ainv_out, parts, mixed, &n_mixed);
+  if (rc != CTM_OK || !mixed) return rc;
+  ctx->mixed_blocks += (int64_t)n_mixed;
+  if (n_mixed == 0) return CTM_OK;
+  // tasks whose refinement stalled: the FP64 solve, in the same order
+  std::vector<int32_t> flags(w.solve.size());
+  void *dst = flags.data();
+  const void *src = ctx->d_flags.p;
+  const size_t bytes = flags.size() * sizeof(int32_t);
+  CTM_HIP(read_back(ctx, 1, &dst, &src, &bytes));
+  std::vector<int32_t> &fb = ctx->fallback;
+  fb.clear();
+  for (size_t q = 0; q < w.solve.size(); ++q)
+    if (flags[q]) fb.push_back(w.solve[q]);
+  ctx->mixed_fallback += (int64_t)fb.size();
+  if (fb.empty()) return CTM_OK;
+  CTM_HIP(upload(ctx, ctx->d_fallback, fb));
+  return solve_list(ctx, fb, ctx->d_fallback.as<int32_t>(), true, nullptr, 0,
+                    false, nullptr);
 }
 
 // Device room for n_sets table sets holding table_size doubles: sized
@@ -939,7 +1018,9 @@ int ctm_destroy(ctm_context *ctx) {
                     &ctx->d_obsitems, &ctx->d_query, &ctx->d_qout,
                     &ctx->d_gsolve, &ctx->d_ttables, &ctx->d_dq,
                     &ctx->d_ainv, &ctx->d_dt, &ctx->d_tparts, &ctx->d_dsum,
-                    &ctx->d_ttiles, &ctx->d_tprefix, &ctx->d_wigfresh};
+                    &ctx->d_ttiles, &ctx->d_tprefix, &ctx->d_wigfresh,
+                    &ctx->d_x32, &ctx->d_xexp, &ctx->d_rbuf, &ctx->d_flags,
+                    &ctx->d_fallback};
   for (DevBuf *b : bufs) b->release();
   ctx->prof.release();
   ctx->staging.release();
@@ -1052,9 +1133,9 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     ctx->err = "invalid convergence parameters";
     return CTM_ERR_ARGUMENT;
   }
-  if (conv->precision != CTM_PRECISION_FP64) {
-    ctx->err = "only CTM_PRECISION_FP64 is built: the Q assembly needs FP64 "
-               "for 1e-6 Q_ext, and FP64 runs at the matrix-core rate here";
+  if (conv->precision != CTM_PRECISION_FP64 &&
+      conv->precision != CTM_PRECISION_MIXED) {
+    ctx->err = "unknown precision";
     return CTM_ERR_ARGUMENT;
   }
   if ((int)conv->max_order - 1 > kMaxOrder) {
@@ -1069,6 +1150,8 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
   ctx->shared.clear();
   ctx->n_grains = n;
   ctx->final_items.clear();
+  ctx->mixed = conv->precision == CTM_PRECISION_MIXED;
+  ctx->mixed_blocks = ctx->mixed_fallback = 0;
   if (n == 0) return CTM_OK;
   GrainParams gp{radius, axis_ratio, wavelength, mr_re, mr_im};
   const int min_order = (int)conv->min_order;
@@ -1231,8 +1314,8 @@ int ctm_tmatrix_fixed(ctm_context *ctx, int n, const double *radius,
     ctx->err = "missing forced orders";
     return CTM_ERR_ARGUMENT;
   }
-  if (precision != CTM_PRECISION_FP64) {
-    ctx->err = "only CTM_PRECISION_FP64 is available for forced orders";
+  if (precision != CTM_PRECISION_FP64 && precision != CTM_PRECISION_MIXED) {
+    ctx->err = "unknown precision";
     return CTM_ERR_ARGUMENT;
   }
   for (int g = 0; g < n; ++g) {
@@ -1251,6 +1334,8 @@ int ctm_tmatrix_fixed(ctm_context *ctx, int n, const double *radius,
   ctx->nmax.assign(nmax_host, nmax_host + n);
   ctx->ngauss.assign(ngauss_host, ngauss_host + n);
   ctx->status.assign(n, 0);
+  ctx->mixed = precision == CTM_PRECISION_MIXED;
+  ctx->mixed_blocks = ctx->mixed_fallback = 0;
   if (n == 0) return CTM_OK;
   GrainParams gp{radius, axis_ratio, wavelength, mr_re, mr_im};
   return run_all_m(ctx, gp, n, ctx->nmax, ctx->ngauss, qsca_out, qext_out);
@@ -2023,6 +2108,7 @@ int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
   ctx->obs_source = -1;
   ctx->n_grains = 0;
   ctx->shared.clear();
+  ctx->mixed = false;
   if (n == 0) return CTM_OK;
   GrainParams gp{radius, axis_ratio, wavelength, mr_re, mr_im};
   WorkList &w = ctx->work;
@@ -2044,6 +2130,14 @@ int ctm_m0_sums(ctm_context *ctx, int n, const double *radius,
     qsca_host[g] = p0[0] + p1[0];
     qext_host[g] = p0[1] + p1[1];
   }
+  return CTM_OK;
+}
+
+int ctm_precision_stats(ctm_context *ctx, int64_t *blocks_mixed,
+                        int64_t *blocks_fallback) {
+  if (!ctx) return CTM_ERR_ARGUMENT;
+  if (blocks_mixed) *blocks_mixed = ctx->mixed_blocks;
+  if (blocks_fallback) *blocks_fallback = ctx->mixed_fallback;
   return CTM_OK;
 }
 

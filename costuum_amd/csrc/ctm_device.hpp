@@ -61,6 +61,26 @@ __host__ __device__ inline void rfma(cdbl &acc, double s, cdbl b) {
   acc.y = fma(s, b.y, acc.y);
 }
 
+// Single-precision complex (the FP32 elimination of the mixed path).
+struct cflt {
+  float x, y;
+};
+
+__host__ __device__ inline cflt cmkf(float a, float b) { return {a, b}; }
+__host__ __device__ inline cflt cmul(cflt a, cflt b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__host__ __device__ inline cflt crcp(cflt a) {
+  const float d = a.x * a.x + a.y * a.y;
+  return {a.x / d, -a.y / d};
+}
+__host__ __device__ inline void cfma(cflt &acc, cflt a, cflt b) {
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(a.y, b.x, acc.y);
+}
+
 // Number of node scalars and Bessel arrays in a table set.
 constexpr int kNodeScalars = 11;
 constexpr int kBesselArrays = 4;

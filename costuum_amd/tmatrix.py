@@ -63,6 +63,16 @@ class Context:
     def synchronize(self):
         self.check(self._lib.ctm_synchronize(self._h), "ctm_synchronize")
 
+    def precision_stats(self):
+        """(blocks through the FP32 elimination, blocks re-solved in FP64)
+        of the last batch call."""
+        a = ctypes.c_int64()
+        b = ctypes.c_int64()
+        self.check(self._lib.ctm_precision_stats(self._h, ctypes.byref(a),
+                                                 ctypes.byref(b)),
+                   "ctm_precision_stats")
+        return a.value, b.value
+
     def stream_handle(self):
         return self._lib.ctm_stream(self._h)
 

@@ -34,12 +34,14 @@ extern "C" {
 #define CTM_CONVERGED 0
 #define CTM_NOT_CONVERGED 1 /* last order kept, as the task path does */
 
-/* Arithmetic of the batched path.  Only CTM_PRECISION_FP64 is built:
- * CTM_PRECISION_MIXED (FP32 elimination with FP64 refinement) is reserved
- * and rejected with CTM_ERR_ARGUMENT.  The Q/RgQ assembly must stay FP64 to
- * reach 1e-6 in Q_ext (rounding Q to FP32 is amplified by cond(Q)), and on
- * this GPU FP64 runs at the same rate on the vector and matrix units, so a
- * mixed solve could only touch the elimination, which is latency-bound. */
+/* Arithmetic of the batched path.  CTM_PRECISION_FP64: every step in FP64,
+ * as the reference (Configuration.hpp.in:62).  CTM_PRECISION_MIXED: the
+ * blocks of order 17..99 of the all-m phase are eliminated in FP32 (after a
+ * power-of-two equilibration) and T = -RgQ Q^-1 is recovered by FP64
+ * iterative refinement against the FP64 Q and RgQ; a block whose
+ * refinement stalls is solved again in FP64.  The Q assembly, the Bessel
+ * recurrences, the convergence windows (orders and their tests) and the
+ * other blocks stay FP64. */
 #define CTM_PRECISION_FP64 0
 #define CTM_PRECISION_MIXED 1
 
@@ -321,7 +323,8 @@ double ctm_grain_memory_estimate(double radius, double axis_ratio,
 #define CTM_KERNEL_ASSEMBLE_SMALL 10 /* blocks of order <= 16 */
 #define CTM_KERNEL_ASSEMBLE_WIDE 11  /* blocks on 48-row tiles */
 #define CTM_KERNEL_ASSEMBLE_PACK 12  /* blocks of order 9-24, four per group */
-#define CTM_KERNEL_COUNT 13
+#define CTM_KERNEL_REFINE 13  /* FP64 refinement of the mixed solve */
+#define CTM_KERNEL_COUNT 14
 
 typedef struct ctm_kernel_stats {
   uint64_t launches;
@@ -329,6 +332,12 @@ typedef struct ctm_kernel_stats {
   double flops;
   double bytes;
 } ctm_kernel_stats;
+
+/* Blocks of the last batch call that went through the FP32 elimination
+ * (CTM_PRECISION_MIXED) and those re-solved in FP64 because their
+ * refinement stalled.  No reference counterpart. */
+int ctm_precision_stats(ctm_context *ctx, int64_t *blocks_mixed,
+                        int64_t *blocks_fallback);
 
 int ctm_profile_enable(ctm_context *ctx, int on);
 int ctm_profile_reset(ctm_context *ctx);

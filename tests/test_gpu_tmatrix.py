@@ -158,8 +158,20 @@ def test_large_grains_converge_like_oracle(ctx):
     assert np.abs(r.tmatrix(1) - Tref).max() / scale < bar
 
 
-def test_mixed_precision_is_rejected(ctx):
-    from costuum_amd.tmatrix import CtmError, tmatrix_batch
-    with pytest.raises(CtmError):
-        tmatrix_batch([0.1], [1.5], [0.5], [1.5], [0.01], precision="mixed",
-                      ctx=ctx)
+def test_unknown_precision_is_rejected(ctx):
+    import ctypes
+    import torch
+    from costuum_amd import _native
+    dev = torch.device("cuda", 0)
+    a = [torch.tensor([v], dtype=torch.float64, device=dev)
+         for v in (0.1, 1.5, 0.5, 1.5, 0.01)]
+    out = torch.zeros(4, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    conv = _native.CtmConvergence(4, 100, 2, 7, 1e-4)
+    rc = ctx._lib.ctm_tmatrix_batch(
+        ctx.handle, 1, *[ctypes.c_void_p(t.data_ptr()) for t in a],
+        ctypes.byref(conv), ctypes.c_void_p(out.data_ptr()),
+        ctypes.c_void_p(out.data_ptr() + 8),
+        ctypes.c_void_p(out.data_ptr() + 16),
+        ctypes.c_void_p(out.data_ptr() + 24))
+    assert rc == _native.CTM_ERR_ARGUMENT
