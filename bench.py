@@ -11,8 +11,16 @@ convergence loop, all m blocks at the converged order, the per-grain
 Q_sca/Q_ext sums, and Q_ext and Q_abs (minus the scattered light, 20 x 20
 quadrature) at 32 Gauss-Legendre incidence angles, averaged over them.
 
-Prints one JSON line.  ``value`` counts the grains whose convergence test
-passed, per second of the timed steps.
+Both arithmetic paths are timed, each over its own steps: plain FP64 and
+mixed (FP32 elimination with FP64 refinement).  A grain counts as converged
+only when its convergence test passed, its order equals the oracle's and
+its averaged Q_ext is within 1e-6 of the oracle's (the all-grain oracle
+fixture tests/golden/oracle_batch_seed0_n4096.npz); ``value`` is the faster
+path's count per second, and the line carries both paths, their error
+distributions against the oracle, the mixed/plain ratio and the plain FP64
+path run one grain per call.
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -35,7 +43,12 @@ def parse():
     # (about half a second timed), 1 + 1 for the grid and gradient workloads
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="both",
+                    choices=["both", "fp64", "mixed"],
+                    help="arithmetic path(s) timed on the batch workload")
+    ap.add_argument("--per-grain", type=int, default=64,
+                    help="grains of the per-grain (one call each) FP64 "
+                         "comparator, every k-th in size order")
     ap.add_argument("--grains", type=int, default=4096)
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the CPU oracle on a sample (rank 0, N=1)")
@@ -122,7 +135,7 @@ def cpu_baseline(g, stride, threads, theta, wtheta):
     order = np.argsort(x)
     sel = order[stride // 2::stride]
     if threads <= 0:
-        threads = min(os.cpu_count() or 1, 16)
+        threads = len(os.sched_getaffinity(0)) or os.cpu_count() or 1
     args = [g[k][sel] for k in ("radius", "axis_ratio", "wavelength",
                                  "mr_re", "mr_im")]
     t0 = time.perf_counter()
@@ -133,12 +146,80 @@ def cpu_baseline(g, stride, threads, theta, wtheta):
     dt = time.perf_counter() - t0
     return ({"value": float((status == 0).sum() / dt),
              "unit": "converged T-matrices/s", "cores": int(threads),
-             "kind": "port",
+             "kind": "port", "host": host_info(),
              "sample": "%d of 4096 grains (every %dth in size-parameter "
                        "order, largest N %d), T-matrix plus Q_ext/Q_abs at "
                        "32 angles, %.2f s wall on %d threads"
                        % (len(sel), stride, int(nmax.max()), dt, threads)},
             sel, nmax, qe, qa)
+
+
+def host_info():
+    """nproc, the CPUs this process may use, and the CPU model (lscpu)."""
+    import subprocess
+    info = {"nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+            elif line.startswith("Socket(s):"):
+                info["sockets"] = line.split(":", 1)[1].strip()
+    except Exception:  # lscpu missing: the counts above still stand
+        pass
+    return info
+
+
+def flops_per_tmatrix(N, min_order=4, glfac=2):
+    """Algorithmic FLOPs of one converged T-matrix of order N (complex
+    multiply-add = 8 flops): the all-m assembly and parity-split solves plus
+    the m = 0 passes of the convergence orders min_order..N."""
+    import numpy as np
+    N = int(N)
+    G = glfac * N
+    nm = np.arange(N, 0, -1, dtype=np.float64)  # m = 1..N
+    f = np.sum(96. * G * nm * nm + 64. / 3. * nm ** 3)
+    n = np.arange(min_order, N + 1, dtype=np.float64)
+    f += np.sum(32. * glfac * n * n * n + 16. / 3. * n ** 3)
+    return float(f)
+
+
+def load_fixture():
+    """The oracle on all 4096 grains (tests/golden/make_batch_fixture.py),
+    or None when it has not been generated."""
+    import numpy as np
+    p = os.path.join(ROOT, "tests", "golden", "oracle_batch_seed0_n4096.npz")
+    if not os.path.exists(p):
+        return None
+    with np.load(p) as f:
+        return {k: f[k].copy() for k in f.files}
+
+
+def verify(fx, nmax, status, qext, qabs):
+    """Per-grain agreement with the oracle fixture: same order, converged,
+    and the relative errors of the averaged Q_ext and Q_abs; the parity bar
+    of a grain is max(1e-8, 50 x the oracle's change under one ulp of R_V)."""
+    import numpy as np
+    same = (status == 0) & (fx["status"] == 0) & (nmax == fx["nmax"])
+    ee = np.abs(qext - fx["qext_avg"]) / np.abs(fx["qext_avg"])
+    ea = np.abs(qabs - fx["qabs_avg"]) / np.abs(fx["qabs_avg"])
+    se = np.abs(fx["qext_avg_ulp"] - fx["qext_avg"]) / np.abs(fx["qext_avg"])
+    sa = np.abs(fx["qabs_avg_ulp"] - fx["qabs_avg"]) / np.abs(fx["qabs_avg"])
+    bar = np.maximum(1e-8, 50. * np.maximum(se, sa))
+    ok6 = same & (ee <= 1e-6)
+    ok8 = same & (ee <= bar) & (ea <= bar)
+    sel = same
+    return {"grains_same_order_converged": int(same.sum()),
+            "within_1e-6_qext": int(ok6.sum()),
+            "within_parity_bar": int(ok8.sum()),
+            "max_rel_err_qext": float(ee[sel].max()),
+            "p99_rel_err_qext": float(np.percentile(ee[sel], 99)),
+            "max_rel_err_qabs": float(ea[sel].max()),
+            "p99_rel_err_qabs": float(np.percentile(ea[sel], 99)),
+            "other_order": [int(i) for i in np.flatnonzero(
+                (status == 0) & (nmax != fx["nmax"]))]}, ok6
 
 
 def run_grid(a):
@@ -427,46 +508,100 @@ def main():
     g = make_inputs.load(0, 4096)
     if a.grains != 4096:
         g = {k: v[:a.grains] for k, v in g.items()}
+    n = len(g["radius"])
     ctx = Context(0)
     dev = ctx.device
     keys = ("radius", "axis_ratio", "wavelength", "mr_re", "mr_im")
     dg = [torch.as_tensor(g[k], dtype=torch.float64, device=dev) for k in keys]
-    conv = dict(min_order=4, max_order=100, glfac=2, tolerance=1e-4,
-                precision=a.precision)
 
     # 32 Gauss-Legendre incidence angles (cos theta nodes), alignment off
     xg, wg = np.polynomial.legendre.leggauss(32)
     theta = np.arccos(xg)
     wgt = torch.as_tensor(0.5 * wg, dtype=torch.float64, device=dev)
-    n = len(g["radius"])
 
-    def step():
-        r = tmatrix_batch(*dg, ctx=ctx, sync=False, **conv)
+    def step(prec):
+        r = tmatrix_batch(*dg, ctx=ctx, sync=False, min_order=4,
+                          max_order=100, glfac=2, tolerance=1e-4,
+                          precision=prec)
         obs = observables(theta, ctx=ctx, ngauss=20, n_grains=n)
-        qext = obs["qext"] @ wgt
-        qabs = obs["qabs"] @ wgt
-        return r, qext, qabs
+        return r, obs["qext"] @ wgt, obs["qabs"] @ wgt
 
-    res = None
-    for _ in range(a.warmup):
-        res, qext_avg, qabs_avg = step()
-    ctx.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        res, qext_avg, qabs_avg = step()
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    stats = ctx.profile_stats()
-    ms_per_step = (t1 - t0) * 1e3 / max(a.steps, 1)
-    status = res.status.cpu().numpy()
-    nconv = int((status == 0).sum())
-    value = nconv / (ms_per_step / 1e3)
+    def timed(prec):
+        for _ in range(a.warmup):
+            step(prec)
+        ctx.profile(True)  # enables and resets the counters
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            res = step(prec)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        stats = ctx.profile_stats()
+        ctx.profile(False)
+        r, qe, qa = res
+        nmax, _, status = r.orders()
+        return {"ms": (t1 - t0) * 1e3 / max(a.steps, 1), "stats": stats,
+                "nmax": nmax, "status": status,
+                "qext": qe.cpu().numpy(), "qabs": qa.cpu().numpy(),
+                "blocks": ctx.precision_stats()}
 
-    qext_np = qext_avg.cpu().numpy()
-    qabs_np = qabs_avg.cpu().numpy()
-    symbol, pname, summary = dominant_kernel(stats)
+    precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    runs = {p: timed(p) for p in precs}
+    fx = load_fixture() if n == 4096 else None
+
+    # the plain FP64 path one grain per call (the caller that does not
+    # batch): every k-th grain in size order, each call synchronous
+    per_grain = None
+    if a.per_grain > 0:
+        x = 2 * np.pi * g["radius"] / g["wavelength"]
+        order = np.argsort(x)
+        stride = max(1, n // a.per_grain)
+        sel = order[stride // 2::stride]
+        single = [[t[i:i + 1] for t in dg] for i in sel]
+        step_one = (lambda args: (tmatrix_batch(
+            *args, ctx=ctx, min_order=4, max_order=100, glfac=2,
+            tolerance=1e-4), observables(theta, ctx=ctx, ngauss=20,
+                                         n_grains=1)))
+        step_one(single[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nconv1 = 0
+        for args in single:
+            r1, _ = step_one(args)
+            nconv1 += int(r1.status.cpu().item() == 0)
+        torch.cuda.synchronize()
+        dt1 = time.perf_counter() - t0
+        per_grain = {"value": round(nconv1 / dt1, 2),
+                     "unit": "converged T-matrices/s",
+                     "sample": "%d grains (every %dth in size-parameter "
+                               "order), one ctm_tmatrix_batch + "
+                               "ctm_observables call each, FP64, %.3f s"
+                               % (len(sel), stride, dt1)}
+
+    summary = {}
+    for p, rr in runs.items():
+        nconv = int((rr["status"] == 0).sum())
+        entry = {"ms_per_step": round(rr["ms"], 3), "converged": nconv,
+                 "blocks_fp32_elimination": int(rr["blocks"][0]),
+                 "blocks_fp64_fallback": int(rr["blocks"][1])}
+        if fx is not None:
+            ver, ok6 = verify(fx, rr["nmax"], rr["status"], rr["qext"],
+                              rr["qabs"])
+            entry.update(ver)
+            entry["value"] = round(int(ok6.sum()) / (rr["ms"] / 1e3), 2)
+        else:
+            entry["value"] = round(nconv / (rr["ms"] / 1e3), 2)
+            entry["unverified"] = "no oracle fixture: convergence test only"
+        summary[p] = entry
+    best = max(summary, key=lambda p: summary[p]["value"])
+    rr = runs[best]
+    stats = rr["stats"]
+    ms_per_step = rr["ms"]
+
+    # rooflines: the assembly kernel with the most algorithmic work (FP64
+    # compute bound), the solve and observable families, and the whole step
+    symbol, pname, summ = dominant_kernel(stats)
     launches, ms, flops, nbytes = stats[pname]
     avg_ms = ms / max(launches, 1)
     achieved = flops / (ms / 1e3) / 1e12 if ms > 0 else 0.0
@@ -474,72 +609,76 @@ def main():
                 "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS,
                                                   4),
-                "traffic": measured_traffic(summary) if a.grains == 4096
-                else None,
+                "traffic": measured_traffic(summ) if n == 4096 else None,
                 "avg_launch_ms": round(avg_ms, 4),
                 "launches": int(launches),
                 "algorithmic_flops_per_launch": flops / max(launches, 1),
                 "algorithmic_bytes_per_launch": nbytes / max(launches, 1)}
+    families = {}
+    for fam in ("solve", "refine", "obs_x", "obs_sum", "assemble_small",
+                "assemble_pack", "assemble", "tables"):
+        l, fm, ff, fb = stats.get(fam, (0, 0., 0., 0.))
+        if l == 0:
+            continue
+        families[fam] = {"launches": int(l), "ms_per_step": round(
+            fm / max(a.steps, 1), 3)}
+        if ff > 0:
+            tf = ff / (fm / 1e3) / 1e12
+            families[fam].update({"tflops": round(tf, 3), "frac_fp64": round(
+                tf / FP64_PEAK_TFLOPS, 4)})
+        if fb > 0:
+            families[fam]["gbs"] = round(fb / (fm / 1e3) / 1e9, 1)
+    roofline["families"] = families
+    nref = fx["nmax"] if fx is not None else rr["nmax"]
+    cref = (fx["status"] == 0) if fx is not None else (rr["status"] == 0)
+    step_flops = sum(flops_per_tmatrix(int(v)) for v in nref[cref])
+    roofline["step"] = {
+        "algorithmic_flops": step_flops,
+        "achieved_tflops": round(step_flops / (ms_per_step / 1e3) / 1e12, 3),
+        "frac": round(step_flops / (ms_per_step / 1e3) / 1e12 /
+                      FP64_PEAK_TFLOPS, 4),
+        "note": "sum over converged grains of the F_T(N) model at the "
+                "oracle's orders, per step time"}
     kernels = {k: {"launches": int(v[0]), "ms": round(v[1], 3),
                    "tflops": round(v[2] / (v[1] / 1e3) / 1e12, 3)
                    if v[1] > 0 and v[2] > 0 else None}
                for k, v in stats.items()}
     cpu = None
-    parity = None
     if a.cpu_baseline:
         cpu, sel, n_ref, qe_ref, qa_ref = cpu_baseline(
             g, a.cpu_stride, a.cpu_threads, theta, 0.5 * wg)
-        nmax_gpu = res.nmax.cpu().numpy()[sel]
-        # grains whose convergence test never passed keep the last order,
-        # where the EBCM system is numerically meaningless (as in the
-        # reference): only converged grains count
-        same = (nmax_gpu == n_ref) & (status[sel] == 0)
-        ee = np.abs(qext_np[sel] - qe_ref) / np.abs(qe_ref)
-        ea = np.abs(qabs_np[sel] - qa_ref) / np.abs(qa_ref)
-        # converged grains whose order differs from the oracle's (deep
-        # Rayleigh grains: the reference accepts an order only when the
-        # Q_sca change is strictly positive, and there that change is
-        # rounding noise), with their Q errors at the two orders
-        xs = 2 * np.pi * g["radius"][sel] / g["wavelength"][sel]
-        other = np.nonzero((status[sel] == 0) & (nmax_gpu != n_ref))[0]
-        parity = {"grains": int(len(sel)),
-                  "converged_same_nmax": int(same.sum()),
-                  "max_rel_err_qext": float(ee[same].max()),
-                  "max_rel_err_qabs": float(ea[same].max()),
-                  "p99_rel_err_qext": float(np.percentile(ee[same], 99)),
-                  "p99_rel_err_qabs": float(np.percentile(ea[same], 99)),
-                  "other_nmax": [{"grain": int(sel[k]),
-                                  "x": float(xs[k]),
-                                  "nmax_gpu": int(nmax_gpu[k]),
-                                  "nmax_oracle": int(n_ref[k]),
-                                  "rel_err_qext": float(ee[k].max()),
-                                  "rel_err_qabs": float(ea[k].max())}
-                                 for k in other]}
+    plain = summary.get("fp64", {}).get("value")
+    mixed = summary.get("mixed", {}).get("value")
     out = {
         "metric": "converged T-matrices/s on 1 GPU, batched mixed vs plain "
-                  "FP64, at <=1e-6 rel Q_ext err",
-        "value": round(value, 2),
+                  "FP64, at \u22641e-6 rel Q_ext err",
+        "value": summary[best]["value"],
         "unit": "converged T-matrices/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if best == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "batched plain %s: %d spheroids, seed 0, "
-                               "task-path convergence (orders 4..99, "
-                               "ngauss = 2 order, tol 1e-4)"
-                               % (a.precision.upper(), len(g["radius"])),
-                   "grains": len(g["radius"]),
-                   "converged": nconv,
-                   "nmax_max": int(res.nmax.max().item()),
-                   "orientations": 32, "absorption_quadrature": "20x20",
-                   "mean_qext_converged": float(np.mean(qext_np[status == 0])),
-                   "mean_qabs_converged": float(np.mean(qabs_np[status == 0])),
-                   "precision": a.precision},
+        "config": {"workload": "batched %d spheroids, seed 0, task-path "
+                               "convergence (orders 4..99, ngauss = 2 order, "
+                               "tol 1e-4), Q_ext/Q_abs (20x20 scattered "
+                               "light) at 32 GL angles" % n,
+                   "grains": n, "path_in_value": best,
+                   "counted": "converged, same order as the oracle, "
+                              "averaged Q_ext within 1e-6 of the oracle"
+                              if fx is not None else
+                              "converged (no oracle fixture)",
+                   "orientations": 32, "absorption_quadrature": "20x20"},
+        "paths": summary,
+        "mixed_over_plain": round(mixed / plain, 4)
+        if plain and mixed else None,
+        "per_grain_fp64_gpu": per_grain,
+        "batched_over_per_grain": round(summary[best]["value"] /
+                                        per_grain["value"], 1)
+        if per_grain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "parity_vs_oracle": parity,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
