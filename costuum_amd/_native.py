@@ -31,6 +31,7 @@ EXPORTS = (
     "ctm_weighted_reduce", "ctm_tmatrix_tangents", "ctm_tangent_get",
     "ctm_observables_select", "ctm_vjp", "ctm_observables_noalloc",
     "ctm_tmatrix_set", "ctm_grain_memory_estimate", "ctm_precision_stats",
+    "ctm_observables_derivative",
 )
 
 # double f(double x, void *arg): the host callback type (ctm_real_fn)
@@ -141,6 +142,10 @@ def lib():
     L.ctm_precision_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64)]
     L.ctm_precision_stats.restype = ctypes.c_int
+    L.ctm_observables_derivative.argtypes = [
+        vp, ctypes.c_int, ctypes.POINTER(CtmObservableSpec), dp, dp, dp, dp,
+        dp, dp]
+    L.ctm_observables_derivative.restype = ctypes.c_int
     L.ctm_average_get.restype = ctypes.c_int
     L.ctm_query.argtypes = [vp, ctypes.c_int, ctypes.POINTER(CtmQuery), dp]
     L.ctm_query.restype = ctypes.c_int

@@ -339,6 +339,9 @@ static hipError_t scratch_address(int device, char **ptr) {
   return hipSuccess;
 }
 
+// Observable source of ctm_observables_derivative: T +- s dT in d_tpm.
+constexpr int kObsSourcePM = -2;
+
 struct ctm_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -365,6 +368,9 @@ struct ctm_context {
   // in FP64 after the refinement stalled)
   bool mixed = false;
   int64_t mixed_blocks = 0, mixed_fallback = 0;
+  // T +- s dT of ctm_observables_derivative (T layout), per-grain scales s,
+  // and the two evaluations of the observables
+  DevBuf d_tpm, d_pmscale, d_pmbuf;
   std::vector<int32_t> fallback;
   // Results of the last batch (host).
   int n_grains = 0;
@@ -1020,7 +1026,8 @@ int ctm_destroy(ctm_context *ctx) {
                     &ctx->d_ainv, &ctx->d_dt, &ctx->d_tparts, &ctx->d_dsum,
                     &ctx->d_ttiles, &ctx->d_tprefix, &ctx->d_wigfresh,
                     &ctx->d_x32, &ctx->d_xexp, &ctx->d_rbuf, &ctx->d_flags,
-                    &ctx->d_fallback};
+                    &ctx->d_fallback, &ctx->d_tpm, &ctx->d_pmscale,
+                    &ctx->d_pmbuf};
   for (DevBuf *b : bufs) b->release();
   ctx->prof.release();
   ctx->staging.release();
@@ -1507,6 +1514,80 @@ int ctm_observables_select(ctm_context *ctx, int which) {
   return CTM_OK;
 }
 
+static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
+                            double *qext, double *qextpol, double *qextcpol,
+                            double *qabs, double *qabspol, double *Z,
+                            char *fixed, size_t fixed_cap);
+
+int ctm_observables_derivative(ctm_context *ctx, int dir,
+                               const ctm_observable_spec *spec, double *dqext,
+                               double *dqextpol, double *dqextcpol,
+                               double *dqabs, double *dqabspol, double *dZ) {
+  if (!ctx || !spec) return CTM_ERR_ARGUMENT;
+  if (!ctx->have_tangents || dir < 0 || dir >= kTangentDirs) {
+    ctx->err = "no tangents computed, or direction out of range";
+    return CTM_ERR_STATE;
+  }
+  if (ctx->averaged) {
+    ctx->err = "the context holds ensemble T-matrices: derivatives apply "
+               "to the batch's own T-matrices";
+    return CTM_ERR_STATE;
+  }
+  if (ctx->obs_source != -1) {
+    ctx->err = "a tangent is selected for the observables";
+    return CTM_ERR_STATE;
+  }
+  const int n = ctx->n_grains;
+  const int nth = spec->n_theta, nst = spec->n_scatter_theta;
+  if (nth < 0 || nst < 0 || (nth > 0 && !spec->theta) ||
+      (spec->account_for_scattering && spec->ngauss < 1)) {
+    ctx->err = "invalid observable specification";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const WorkList &w = ctx->work;
+  const int64_t qn = (int64_t)n * nth;
+  const int64_t zw = (int64_t)nst * 2 * nst * 16, zn = (int64_t)n * zw;
+  const int64_t block = 5 * qn + zn;
+  CTM_HIP(ctx->d_tpm.ensure(std::max<int64_t>(w.t_size, 1) * sizeof(cdbl)));
+  CTM_HIP(ctx->d_pmscale.ensure(std::max(n, 1) * sizeof(double)));
+  CTM_HIP(ctx->d_pmbuf.ensure(std::max<int64_t>(2 * block, 1) *
+                              sizeof(double)));
+  const cdbl *dT = ctx->d_dt.as<cdbl>() + (int64_t)dir * w.t_size;
+  double *scale = ctx->d_pmscale.as<double>();
+  CTM_HIP(launch_pm_scale(n, ctx->d_gioff.as<int32_t>(),
+                          ctx->d_items.as<Item>(), ctx->d_t.as<cdbl>(), dT,
+                          scale, s));
+  double *buf = ctx->d_pmbuf.as<double>();
+  const int nitems = (int)ctx->final_items.size();
+  for (int k = 0; k < 2; ++k) {
+    double *b = buf + k * block;
+    CTM_HIP(launch_pm_form(nitems, ctx->d_items.as<Item>(),
+                           ctx->d_t.as<cdbl>(), dT, scale, k ? -1. : 1.,
+                           ctx->d_tpm.as<cdbl>(), s));
+    ctx->obs_source = kObsSourcePM;
+    const int rc = observables_impl(ctx, spec, b, b + qn, b + 2 * qn,
+                                    b + 3 * qn, b + 4 * qn,
+                                    nst > 0 ? b + 5 * qn : nullptr, nullptr,
+                                    0);
+    ctx->obs_source = -1;
+    if (rc != CTM_OK) return rc;
+  }
+  double *outs[5] = {dqext, dqextpol, dqextcpol, dqabs, dqabspol};
+  for (int q = 0; q < 5; ++q) {
+    const bool on = q < 3 ? spec->do_extinction : spec->do_absorption;
+    if (on && outs[q] && nth > 0)
+      CTM_HIP(launch_pm_combine(qn, nth, buf + q * qn, buf + block + q * qn,
+                                scale, outs[q], s));
+  }
+  if (nst > 0 && dZ)
+    CTM_HIP(launch_pm_combine(zn, zw, buf + 5 * qn, buf + block + 5 * qn,
+                              scale, dZ, s));
+  CTM_HIP(drain(ctx));
+  return CTM_OK;
+}
+
 int ctm_vjp(ctm_context *ctx, const double *cot_qsca, const double *cot_qext,
             double *grad) {
   if (!ctx || !grad) return CTM_ERR_ARGUMENT;
@@ -1792,6 +1873,8 @@ static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
                                        ? ctx->d_dt.as<cdbl>() +
                                              (int64_t)ctx->obs_source *
                                                  ctx->work.t_size
+                                   : ctx->obs_source == kObsSourcePM
+                                       ? ctx->d_tpm.as<cdbl>()
                                        : ctx->d_t.as<cdbl>(),
                                    ctx->averaged ? ctx->d_avg.as<cdbl>()
                                                  : nullptr,

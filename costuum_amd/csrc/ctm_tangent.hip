@@ -564,6 +564,102 @@ __global__ void k_tangent_sums(int n_grains, const int32_t *gi_off,
   dqext[idx] = e;
 }
 
+// Derivatives of the observables that are quadratic in T (Z, and Q_abs
+// with the scattered light): S is linear in T and every observable is at
+// most quadratic in S, so O(T + s dT) - O(T - s dT) = 2 s dO exactly (the
+// quadratic terms cancel) for any s.  Per grain, s = max |T| / max |dT|
+// keeps both evaluations of the size of O(T), so the difference loses no
+// more than a few ulps of it.
+__global__ __launch_bounds__(256) void k_pm_scale(const int32_t *gi_off,
+                                                  const Item *items,
+                                                  const cdbl *tbuf,
+                                                  const cdbl *dt,
+                                                  double *scale) {
+  const int g = blockIdx.x;
+  double mt = 0., md = 0.;
+  for (int i = gi_off[g]; i < gi_off[g + 1]; ++i) {
+    const Item it = items[i];
+    const int64_t nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+    for (int64_t e = threadIdx.x; e < 2 * nm * nm; e += 256) {
+      const cdbl a = tbuf[it.t_off + e], b = dt[it.t_off + e];
+      mt = fmax(mt, fmax(fabs(a.x), fabs(a.y)));
+      md = fmax(md, fmax(fabs(b.x), fabs(b.y)));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mt = fmax(mt, __shfl_xor(mt, o));
+    md = fmax(md, __shfl_xor(md, o));
+  }
+  __shared__ double red[4][2];
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6][0] = mt;
+    red[threadIdx.x >> 6][1] = md;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      mt = fmax(mt, red[w][0]);
+      md = fmax(md, red[w][1]);
+    }
+    mt = fmax(mt, red[0][0]);
+    md = fmax(md, red[0][1]);
+    scale[g] = (mt > 0. && md > 0.) ? mt / md : 1.;
+  }
+}
+
+// out = T + sign s_g dT over the T areas of the grains' items (one
+// workgroup per item).
+__global__ __launch_bounds__(256) void k_pm_form(const Item *items,
+                                                 const cdbl *tbuf,
+                                                 const cdbl *dt,
+                                                 const double *scale,
+                                                 double sign, cdbl *out) {
+  const Item it = items[blockIdx.x];
+  const int64_t nm = it.N + 1 - (it.m > 0 ? it.m : 1);
+  const double s = sign * scale[it.grain];
+  for (int64_t e = threadIdx.x; e < 2 * nm * nm; e += 256) {
+    const cdbl a = tbuf[it.t_off + e], b = dt[it.t_off + e];
+    out[it.t_off + e] = cmk(fma(s, b.x, a.x), fma(s, b.y, a.y));
+  }
+}
+
+// out = (plus - minus) / (2 s_g) for an array of n_grains rows of `width`
+// values.
+__global__ void k_pm_combine(int64_t n, int64_t width, const double *plus,
+                             const double *minus, const double *scale,
+                             double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (plus[i] - minus[i]) / (2. * scale[i / width]);
+}
+
+hipError_t launch_pm_scale(int n_grains, const int32_t *gi_off,
+                           const Item *items, const cdbl *tbuf, const cdbl *dt,
+                           double *scale, hipStream_t s) {
+  if (n_grains <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pm_scale, dim3(n_grains), dim3(256), 0, s, gi_off,
+                     items, tbuf, dt, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_pm_form(int n_items, const Item *items, const cdbl *tbuf,
+                          const cdbl *dt, const double *scale, double sign,
+                          cdbl *out, hipStream_t s) {
+  if (n_items <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pm_form, dim3(n_items), dim3(256), 0, s, items, tbuf,
+                     dt, scale, sign, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pm_combine(int64_t n, int64_t width, const double *plus,
+                             const double *minus, const double *scale,
+                             double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pm_combine, dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, s, n, width, plus, minus, scale, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_tables_tangent(const TableSet *sets, int n_sets,
                                  const GrainParams &gp, const double *gl,
                                  const double *tables, int64_t table_size,

@@ -37,6 +37,19 @@ hipError_t launch_tangent_solve(int n_tasks, const Item *items,
                                 const cdbl *tbuf, const cdbl *ainv, cdbl *dq,
                                 int64_t q_size, cdbl *dt, int64_t t_size,
                                 double *tparts, hipStream_t s);
+// Directional derivatives of observables quadratic in T by the exact
+// symmetric difference O(T + s dT) - O(T - s dT) = 2 s dO: per-grain scales
+// s = max |T| / max |dT| (gi_off: the grains' item ranges), T +- s dT into
+// `out` (T layout), and (plus - minus) / (2 s) per row of `width` values.
+hipError_t launch_pm_scale(int n_grains, const int32_t *gi_off,
+                           const Item *items, const cdbl *tbuf, const cdbl *dt,
+                           double *scale, hipStream_t s);
+hipError_t launch_pm_form(int n_items, const Item *items, const cdbl *tbuf,
+                          const cdbl *dt, const double *scale, double sign,
+                          cdbl *out, hipStream_t s);
+hipError_t launch_pm_combine(int64_t n, int64_t width, const double *plus,
+                             const double *minus, const double *scale,
+                             double *out, hipStream_t s);
 // Per-grain tangents of the full-T sums (f_m = 1 for m = 0, else 2).
 hipError_t launch_tangent_sums(int n_grains, const int32_t *gi_off,
                                const double *tparts, double *dqsca,

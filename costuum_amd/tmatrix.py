@@ -384,6 +384,96 @@ def vjp(res, cot_qsca=None, cot_qext=None):
     return out
 
 
+def observables_derivative(theta, direction, ctx, n_grains, radius=None,
+                           do_extinction=True, do_absorption=True,
+                           account_for_scattering=True, ngauss=20,
+                           n_scatter_theta=0, scatter_gl=True):
+    """d/dp of every observable of ``observables`` (p = PARAMS[direction],
+    after ``tangents``), including the absorption with the scattered-light
+    correction and the scattering matrix Z (ctm_observables_derivative: the
+    exact symmetric difference of the quadratic forms).  For p = R_V the Q
+    values get the -2 Q / R_V of their 1 / (pi R_V^2) normalisation here
+    (radius per grain is required then); Z is not normalised.  Returns a
+    dict of device tensors like ``observables``."""
+    dev = ctx.device
+    th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    n, nt = n_grains, len(th)
+    out = {k: torch.zeros((n, nt), dtype=torch.float64, device=dev)
+           for k in ("qext", "qextpol", "qextcpol", "qabs", "qabspol")}
+    Z = None
+    if n_scatter_theta > 0:
+        Z = torch.zeros((n, n_scatter_theta, 2 * n_scatter_theta, 4, 4),
+                        dtype=torch.float64, device=dev)
+        out["Z"] = Z
+    torch.cuda.current_stream(dev).synchronize()
+    spec = _native.CtmObservableSpec(nt, th.ctypes.data, int(do_extinction),
+                                     int(do_absorption),
+                                     int(account_for_scattering), ngauss,
+                                     n_scatter_theta, int(scatter_gl))
+    ctx.check(ctx._lib.ctm_observables_derivative(
+        ctx.handle, int(direction), ctypes.byref(spec), _ptr(out["qext"]),
+        _ptr(out["qextpol"]), _ptr(out["qextcpol"]), _ptr(out["qabs"]),
+        _ptr(out["qabspol"]), _ptr(Z)), "ctm_observables_derivative")
+    if direction == 0:
+        if radius is None:
+            raise ValueError("radius is required for the R_V direction")
+        q = observables(th, ctx=ctx, do_extinction=do_extinction,
+                        do_absorption=do_absorption,
+                        account_for_scattering=account_for_scattering,
+                        ngauss=ngauss, n_grains=n)
+        rv = torch.as_tensor(np.asarray(radius, dtype=np.float64),
+                             device=dev).reshape(-1, 1)
+        for k in ("qext", "qextpol", "qextcpol", "qabs", "qabspol"):
+            out[k] = out[k] - 2.0 * q[k] / rv
+    return out
+
+
+class _Observables(torch.autograd.Function):
+    """(radius, axis_ratio, mr_re, mr_im) -> (Q_ext, Q_abs, Q_sca) at the
+    incidence angles theta (Q_abs with the ngauss x ngauss scattered-light
+    correction, Q_sca = Q_ext - Q_abs) at frozen orders.  The forward pass
+    also forms the per-grain Jacobian [n, n_theta, 2, 4] with the library's
+    forward-mode tangents (four directions), from which backward forms any
+    vector-Jacobian product."""
+
+    @staticmethod
+    def forward(fctx, radius, axis_ratio, mr_re, mr_im, wavelength, nmax,
+                ngauss, theta, ngauss_abs, ctx):
+        res = tmatrix_fixed(radius.detach(), axis_ratio.detach(),
+                            wavelength, mr_re.detach(), mr_im.detach(), nmax,
+                            ngauss, ctx=ctx)
+        n = res.n
+        q = observables(theta, ctx=ctx, ngauss=ngauss_abs, n_grains=n)
+        tangents(res)
+        rv = radius.detach().cpu().numpy()
+        cols = []
+        for j in range(4):
+            d = observables_derivative(theta, j, ctx, n, radius=rv,
+                                       ngauss=ngauss_abs)
+            cols.append(torch.stack([d["qext"], d["qabs"]], dim=-1))
+        fctx.save_for_backward(torch.stack(cols, dim=-1))
+        return q["qext"], q["qabs"], q["qext"] - q["qabs"]
+
+    @staticmethod
+    def backward(fctx, g_ext, g_abs, g_sca):
+        J, = fctx.saved_tensors
+        ge = (g_ext + g_sca).unsqueeze(-1)
+        ga = (g_abs - g_sca).unsqueeze(-1)
+        grad = (ge * J[:, :, 0, :] + ga * J[:, :, 1, :]).sum(dim=1)
+        return (grad[:, 0], grad[:, 1], grad[:, 2], grad[:, 3], None, None,
+                None, None, None, None)
+
+
+def observables_autograd(radius, axis_ratio, wavelength, mr_re, mr_im, nmax,
+                         ngauss, theta, ngauss_abs=20, ctx=None):
+    """Differentiable (Q_ext, Q_abs, Q_sca) [n, n_theta] at the incidence
+    angles theta (torch autograd over radius, axis_ratio, mr_re, mr_im) at
+    forced orders."""
+    ctx = ctx or default_context()
+    return _Observables.apply(radius, axis_ratio, mr_re, mr_im, wavelength,
+                              nmax, ngauss, theta, ngauss_abs, ctx)
+
+
 class _TMatrixSums(torch.autograd.Function):
     """(radius, axis_ratio, mr_re, mr_im) -> (Q_sca, Q_ext) full-T sums at
     frozen orders, differentiable through the library's tangents."""

@@ -253,6 +253,22 @@ int ctm_tangent_get(ctm_context *ctx, int grain, int dir, double *dT_host,
  * the cross section divided by pi R_V^2 (the caller adds -2 Q / R_V for
  * p = R_V). */
 int ctm_observables_select(ctm_context *ctx, int which);
+/* Derivatives along p_dir (after ctm_tmatrix_tangents) of every observable
+ * of ctm_observables, including those quadratic in T: the absorption with
+ * the scattered-light correction (AbsorptionCoefficientTask.hpp:742-773) and
+ * the scattering matrix Z.  S is linear in T and each observable at most
+ * quadratic in S, so O(T + s dT) - O(T - s dT) = 2 s dO exactly; the library
+ * evaluates the observables at T +- s dT with a per-grain s = max |T| /
+ * max |dT| and returns the difference / (2 s).  Same specification and
+ * output layout as ctm_observables; Q derivatives are at fixed
+ * normalisation (for p = R_V the caller adds -2 Q / R_V, Z is not
+ * normalised).  No reference counterpart (the reference has no
+ * derivatives).  The batch's own T-matrices only (no ensemble). */
+int ctm_observables_derivative(ctm_context *ctx, int dir,
+                               const ctm_observable_spec *spec, double *dqext,
+                               double *dqextpol, double *dqextcpol,
+                               double *dqabs, double *dqabspol, double *dZ);
+
 /* Vector-Jacobian product of the full-T sums for per-grain cotangents
  * (host, may be NULL): grad[g][p] = cot_qsca[g] dQ_sca/dp +
  * cot_qext[g] dQ_ext/dp, after ctm_tmatrix_tangents. */

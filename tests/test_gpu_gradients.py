@@ -216,3 +216,103 @@ def test_seed1_batch_gradients_match_oracle(ctx):
                 got = d[j][key][k].cpu().numpy()
                 err = np.abs(got - fd[c]).max() / np.abs(fd[c]).max()
                 assert err < 1e-5, (gi, n, j, key, err)
+
+
+def _richardson(f, p0, j, h):
+    """(4 D(h) - D(2h)) / 3 with D the central difference of f along
+    parameter j at relative step h."""
+    def central(hh):
+        p1, p2 = p0.copy(), p0.copy()
+        p1[j] += hh * abs(p0[j])
+        p2[j] -= hh * abs(p0[j])
+        return (f(p1) - f(p2)) / (2 * hh * abs(p0[j]))
+    return (4. * central(h) - central(2 * h)) / 3.
+
+
+def test_quadratic_observable_derivatives(ctx):
+    """d/dp of Q_abs with the 12 x 12 scattered-light correction and of Z
+    (ctm_observables_derivative) against Richardson-extrapolated central
+    differences of the GPU forward observables (1e-6 of the largest value);
+    the linear outputs equal the tangent-selected path; one grain's dQ_abs
+    against the oracle's differences (1e-6)."""
+    from costuum_amd.tmatrix import (observables, observables_derivative,
+                                     observables_tangent, tangents)
+    theta = np.array([0.3, 1.1, 2.4])
+    n = len(CASES)
+    kw = dict(ngauss=12, n_scatter_theta=4)
+
+    def forward(p_all):
+        cases = [(p[0], p[1], c[2], complex(p[2], p[3]), c[4], c[5])
+                 for p, c in zip(p_all, CASES)]
+        _fixed(ctx, cases)
+        q = observables(theta, ctx=ctx, n_grains=n, **kw)
+        return {k: v.cpu().numpy() for k, v in q.items()}
+
+    p0 = np.array([_params(c) for c in CASES])
+    res = _fixed(ctx, CASES)
+    tangents(res)
+    rv = [c[0] for c in CASES]
+    got = [observables_derivative(theta, j, ctx, n, radius=rv, **kw)
+           for j in range(4)]
+    lin = [observables_tangent(theta, j, ctx, n, radius=rv) for j in range(4)]
+    for j in range(4):
+        for k in ("qext", "qextpol", "qextcpol"):
+            a = got[j][k].cpu().numpy()
+            b = lin[j][k].cpu().numpy()
+            assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max(), (j, k)
+    for g in range(n):
+        for j in range(4):
+            def f(p, key, g=g):
+                pa = p0.copy()
+                pa[g] = p
+                return forward(pa)[key][g]
+            for key in ("qabs", "Z"):
+                fd = _richardson(lambda p: f(p, key), p0[g].copy(), j, 1e-5)
+                d = got[j][key][g].cpu().numpy()
+                err = np.abs(d - fd).max() / np.abs(fd).max()
+                assert err < 1e-6, (g, j, key, err)
+    # one grain against the oracle: dQ_abs with the scattered light
+    rv0, d0, lam, mr, nmax, ng = CASES[0]
+
+    def oq(p):
+        T = O.tmatrix_all(p[0], p[1], lam, complex(p[2], p[3]), nmax, ng)
+        qa, _ = O.absorption(nmax, T, lam, p[0], theta, ngauss=12)
+        return qa
+    for j in range(4):
+        fd = _richardson(oq, p0[0].copy(), j, 1e-5)
+        d = got[j]["qabs"][0].cpu().numpy()
+        assert np.abs(d - fd).max() < 1e-6 * np.abs(fd).max(), j
+
+
+def test_autograd_of_observables(ctx):
+    """torch autograd through Q_ext, Q_abs (scattered light included) and
+    Q_sca at orientations against the oracle's Richardson differences of the
+    same loss."""
+    import torch
+    from costuum_amd.tmatrix import observables_autograd
+    dev = ctx.device
+    c = CASES[:2]
+    theta = np.array([0.4, 1.3])
+    leaves = [torch.tensor([_params(x)[j] for x in c], dtype=torch.float64,
+                           device=dev, requires_grad=True) for j in range(4)]
+    qe, qa, qs = observables_autograd(leaves[0], leaves[1],
+                                      [x[2] for x in c], leaves[2],
+                                      leaves[3], [x[4] for x in c],
+                                      [x[5] for x in c], theta,
+                                      ngauss_abs=10, ctx=ctx)
+    w = torch.tensor([1.0, -0.5], dtype=torch.float64, device=dev)
+    loss = ((qe + 2.0 * qa - 0.5 * qs) @ w).sum()
+    loss.backward()
+    for g, case in enumerate(c):
+        rv, dd, lam, mr, n, ng = case
+
+        def fl(p):
+            T = O.tmatrix_all(p[0], p[1], lam, complex(p[2], p[3]), n, ng)
+            e, _, _ = O.extinction(n, T, lam, p[0], theta)
+            a, _ = O.absorption(n, T, lam, p[0], theta, ngauss=10)
+            return float(np.dot(e + 2.0 * a - 0.5 * (e - a),
+                                [1.0, -0.5]))
+        for j in range(4):
+            fd = _richardson(fl, _params(case), j, 1e-5)
+            got = leaves[j].grad[g].item()
+            assert abs(got - fd) < 1e-6 * max(abs(fd), 1e-3), (g, j, got, fd)
