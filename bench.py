@@ -360,15 +360,16 @@ def run_grid(a):
 
 def run_gradients(a):
     """The 1024-grain seed-1 batch: converged T-matrices, Q_ext and Q_abs
-    (without the scattered-light term, the part linear in T) at 16
-    Gauss-Legendre incidence angles, the full-T Q_sca/Q_ext sums, and their
-    derivatives with respect to (R_V, d, Re m, Im m) at the converged orders
-    (forward mode).  One step does all of it."""
+    (with the 20 x 20 scattered-light correction; Q_sca = Q_ext - Q_abs) at
+    16 Gauss-Legendre incidence angles, the full-T Q_sca/Q_ext sums, and
+    their derivatives with respect to (R_V, d, Re m, Im m) at the converged
+    orders (forward-mode tangents; the quadratic scattered-light term by the
+    exact symmetric difference).  One step does all of it."""
     import numpy as np
     import torch
     import make_inputs
     from costuum_amd.tmatrix import (Context, observables,
-                                     observables_tangent, tangents,
+                                     observables_derivative, tangents,
                                      tmatrix_batch)
     g = make_inputs.load(1, 1024)
     ctx = Context(0)
@@ -382,18 +383,16 @@ def run_gradients(a):
     def step():
         r = tmatrix_batch(*dg, ctx=ctx, min_order=4, max_order=100, glfac=2,
                           tolerance=1e-4, sync=False)
-        q = observables(theta, ctx=ctx, account_for_scattering=False,
-                        n_grains=n)
+        q = observables(theta, ctx=ctx, ngauss=20, n_grains=n)
         dqs, dqe = tangents(r)
-        dq = [observables_tangent(theta, j, ctx, n, radius=g["radius"])
-              for j in range(4)]
+        dq = [observables_derivative(theta, j, ctx, n, radius=g["radius"],
+                                     ngauss=20) for j in range(4)]
         return r, q, dqs, dqe, dq
 
     def forward():
         r = tmatrix_batch(*dg, ctx=ctx, min_order=4, max_order=100, glfac=2,
                           tolerance=1e-4, sync=False)
-        q = observables(theta, ctx=ctx, account_for_scattering=False,
-                        n_grains=n)
+        q = observables(theta, ctx=ctx, ngauss=20, n_grains=n)
         return r, q
 
     def timed(fn):
@@ -419,8 +418,9 @@ def run_gradients(a):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "1024 spheroids, seed 1, orders 4..99; "
-                               "Q_ext, Q_abs at 16 angles and full-T sums "
-                               "with forward-mode derivatives",
+                               "Q_ext, Q_abs (20x20 scattered light), Q_sca "
+                               "at 16 angles and full-T sums with their "
+                               "derivatives along (R_V, d, Re m, Im m)",
                    "grains": n, "converged": int((st == 0).sum()),
                    "forward_only_ms_per_step": round(ms_fwd, 3),
                    "with_gradients_over_forward": round(ms / ms_fwd, 3),

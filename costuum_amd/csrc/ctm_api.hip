@@ -36,6 +36,7 @@
 #include "ctm_host.hpp"
 #include "ctm_tangent.hpp"
 #include "ctm_mixed.hpp"
+#include "ctm_gj.hpp"
 
 using namespace ctm;
 
@@ -722,7 +723,13 @@ static int solve_list(ctm_context *ctx, const std::vector<int32_t> &tasks,
         // m = 0: two parity sub-blocks of (nm + 1) / 2 and nm / 2 rows
         const int64_t r0 = it.m == 0 && nmi > 1 ? (nmi + 1) / 2 : nmi;
         const int64_t r1 = nmi - r0;
-        store_max = std::max(store_max, r0 * r0 + r1 * r1);
+        // LDS sub-blocks have odd row strides (blk_lds_stride); the global
+        // scratch of the blocks above kMaxLdsOrder is unpadded
+        store_max = std::max(store_max,
+                             nm_max > kMaxLdsOrder
+                                 ? r0 * r0 + r1 * r1
+                                 : r0 * blk_lds_stride((int)r0) +
+                                       r1 * blk_lds_stride((int)r1));
         rows_max = std::max(rows_max, (int)r0);
       }
       const hipStream_t ls = lane(ctx, lane_of[b]);

@@ -8,6 +8,9 @@
 
 namespace ctm {
 
+// Row stride (elements) of an LDS sub-block of n rows: n rounded up to odd.
+__host__ __device__ inline int blk_lds_stride(int n) { return n | 1; }
+
 template <typename E>
 __device__ __forceinline__ E ezero();
 template <>

@@ -1359,9 +1359,14 @@ __global__ __launch_bounds__(NT) void k_invert_solve_blk(
   // m = 0 half the LDS of the full block, whose cross blocks are zero)
   cdbl *A = GLOBAL ? gscratch + (int64_t)blockIdx.x * gstride
                   : reinterpret_cast<cdbl *>(smem);
-  cdbl *Ab[2] = {A, A + (int64_t)sz0 * sz0};
-  const int ldb[2] = {sz[0], sz[1]};
-  const int64_t nstore = (int64_t)sz[0] * sz[0] + (int64_t)sz[1] * sz[1];
+  // rows of an LDS sub-block padded to an odd number of elements: the
+  // column accesses of the panel and update steps (lane i at row i) then
+  // fall in distinct bank groups (an even stride that is a multiple of 16
+  // elements puts every lane in one)
+  const int ldb[2] = {GLOBAL ? sz[0] : blk_lds_stride(sz[0]),
+                      GLOBAL ? sz[1] : blk_lds_stride(sz[1])};
+  cdbl *Ab[2] = {A, A + (int64_t)sz0 * ldb[0]};
+  const int64_t nstore = (int64_t)sz[0] * ldb[0] + (int64_t)sz[1] * ldb[1];
   int *piv = GLOBAL ? reinterpret_cast<int *>(smem)
                     : reinterpret_cast<int *>(A + nstore);
   int *src = piv + nm;
@@ -1377,7 +1382,7 @@ __global__ __launch_bounds__(NT) void k_invert_solve_blk(
     for (int i = t >> 5; i < n; i += NT >> 5) {
       const int pi = pidx(o + i);
       for (int j = t & 31; j < n; j += 32)
-        Ab[bb][(int64_t)i * n + j] = Qg[(int64_t)pi * nm + pidx(o + j)];
+        Ab[bb][(int64_t)i * ldb[bb] + j] = Qg[(int64_t)pi * nm + pidx(o + j)];
     }
   }
   __syncthreads();

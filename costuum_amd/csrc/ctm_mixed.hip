@@ -67,9 +67,10 @@ __global__ __launch_bounds__(NT) void k_invert32_blk(const int32_t *tasks,
     return nb == 2 ? (i < sz0 ? 2 * i : 2 * (i - sz0) + 1) : i;
   };
   cflt *A = reinterpret_cast<cflt *>(smem);
-  cflt *Ab[2] = {A, A + (int64_t)sz0 * sz0};
-  const int ldb[2] = {sz[0], sz[1]};
-  const int64_t nstore = (int64_t)sz[0] * sz[0] + (int64_t)sz[1] * sz[1];
+  // odd row strides: conflict-free column accesses (as in the FP64 solve)
+  const int ldb[2] = {blk_lds_stride(sz[0]), blk_lds_stride(sz[1])};
+  cflt *Ab[2] = {A, A + (int64_t)sz0 * ldb[0]};
+  const int64_t nstore = (int64_t)sz[0] * ldb[0] + (int64_t)sz[1] * ldb[1];
   int *piv = reinterpret_cast<int *>(A + nstore);
   int *src = piv + nm;
   int *er = src + nm;  // row exponents (local index)
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(NT) void k_invert32_blk(const int32_t *tasks,
       for (int j = t & 31; j < n; j += 32) {
         const cdbl v = Qg[pi * nm + pidx(o + j)];
         const int e = er[o + i] + ec[o + j];
-        Ab[bb][(int64_t)i * n + j] =
+        Ab[bb][(int64_t)i * ldb[bb] + j] =
             cmkf((float)ldexp(v.x, e), (float)ldexp(v.y, e));
       }
     }
@@ -431,7 +432,7 @@ hipError_t launch_invert32(const int32_t *tasks, int n, int nm_max,
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (nm_max < 1 || nm_max > kMixedMaxOrder || rows_max > 2 * 64 ||
-      store_max > (int64_t)nm_max * nm_max)
+      store_max > (int64_t)nm_max * blk_lds_stride(nm_max))
     return hipErrorInvalidValue;
   const size_t lds = invert32_lds_bytes(store_max, nm_max);
   if (nm_max <= 48 || rows_max <= 32)
