@@ -13,17 +13,28 @@ convergence (orders 4..99, ngauss = 2 order, tolerance 1e-4) and records:
                           with the radius moved up by one ulp: the oracle's
                           own sensitivity to one rounding of its inputs,
                           the parity bar for ill-conditioned grains
+  nmax_ulp                the converged order with the radius moved up by
+                          one ulp
+  dqs, dqe                the relative changes of the m = 0 sums at the
+                          converged order (TMatrixResource.hpp:743-775)
+
+A grain's converged order is fixed by FP64 arithmetic only when nmax_ulp
+equals nmax and dqs is above rounding noise; otherwise a change of rounding
+(another summation order) can move it.
 
 Writes oracle_batch_seed0_n4096.npz beside this file.  Chunks are
 checkpointed in oracle_batch_seed0_n4096.partial.npz so an interrupted run
 resumes; the partial file is removed at the end.
 
-    python tests/golden/make_batch_fixture.py [--threads 8]
+    python tests/golden/make_batch_fixture.py [--threads 8] [--orders-only]
+
+--orders-only adds nmax_ulp, dqs and dqe to an existing fixture.
 """
 import argparse
 import os
 import sys
 import time
+from multiprocessing.pool import ThreadPool
 
 import numpy as np
 
@@ -42,12 +53,42 @@ FIELDS = ("nmax", "status", "qsca_sum", "qext_sum", "qext_avg", "qabs_avg",
           "qsca_sum_ulp", "qext_sum_ulp", "qext_avg_ulp", "qabs_avg_ulp")
 
 
+def add_orders(g, out, threads):
+    """nmax_ulp, dqs and dqe of every grain (convergence loop only)."""
+    keys = list(zip(*[g[k] for k in KEYS]))
+
+    def one(args):
+        rv, d, lam, mre, mim = args
+        mr = complex(mre, mim)
+        st, n, _, dqs, dqe = oracle.converge_task(rv, d, lam, mr, 4, 100, 2,
+                                                  1e-4)
+        _, nu, _, _, _ = oracle.converge_task(np.nextafter(rv, np.inf), d,
+                                              lam, mr, 4, 100, 2, 1e-4)
+        return n, nu, dqs, dqe
+
+    with ThreadPool(threads) as pool:
+        r = pool.map(one, keys, chunksize=1)
+    n = np.array([x[0] for x in r])
+    assert np.array_equal(n, out["nmax"]), "convergence differs from batch"
+    out["nmax_ulp"] = np.array([x[1] for x in r], dtype=np.uint16)
+    out["dqs"] = np.array([x[2] for x in r])
+    out["dqe"] = np.array([x[3] for x in r])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--chunk", type=int, default=128)
+    ap.add_argument("--orders-only", action="store_true")
     a = ap.parse_args()
     g = make_inputs.load(0, 4096)
+    if a.orders_only:
+        with np.load(OUT) as f:
+            out = {k: f[k].copy() for k in f.files}
+        add_orders(g, out, a.threads)
+        np.savez_compressed(OUT, **out)
+        print("added orders to", OUT, flush=True)
+        return
     n = len(g["radius"])
     xg, wg = np.polynomial.legendre.leggauss(32)
     theta = np.arccos(xg)
@@ -91,6 +132,7 @@ def main():
     out = {k: res[k] for k in FIELDS}
     out["nmax"] = res["nmax"].astype(np.uint16)
     out["status"] = res["status"].astype(np.int8)
+    add_orders(g, out, a.threads)
     np.savez_compressed(OUT, **out)
     os.remove(PARTIAL)
     print("wrote", OUT, flush=True)
