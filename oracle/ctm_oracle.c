@@ -1308,8 +1308,12 @@ void ctmo_orientation_average(unsigned nmax, const double *Tind,
         const unsigned hi = n1 + n2;
         const unsigned M = (n1 < n2) ? n1 : n2;
         for (unsigned N = lo; N <= hi; ++N) {
-          cg_m0_family((int)n1, (int)n2, (int)N, C, scratch);
           const double p = pN[N];
+          /* a zero coefficient adds exact zeros: skip its CG family */
+          if (p == 0.) {
+            continue;
+          }
+          cg_m0_family((int)n1, (int)n2, (int)N, C, scratch);
           for (unsigned m1 = 0; m1 <= M; ++m1) {
             const double f = (m1 == 0) ? 1. : 2.;
             const int sg = ((m1 % 2 == 0) ? 1 : -1) * msign;
