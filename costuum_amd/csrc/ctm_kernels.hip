@@ -937,8 +937,13 @@ __global__ __launch_bounds__(256, 2) void k_assemble_pack4(
 // formed (all the convergence test needs).
 // Shared end of the solve kernels: the column unpermutation of the
 // Gauss-Jordan inverse Ainv (held in A, LDS or global), the product
-// T_c = -RgQ_c Ainv (or only its diagonal) and the per-block sums.
-template <int NT>
+// T_c = -RgQ_c Ainv (or only its diagonal) and the per-block sums.  RgQ is
+// read from HBM ahead of its use: a wave loads its four rows of the product
+// at once, coalesced (lane l holds column lo + l + 64 h of each row, KH
+// slots h), and broadcasts element k by readlane, so the memory latency is
+// paid once per row group instead of once per k; the diagonal loads its
+// row in batches of 16.
+template <int NT, int KH>
 __device__ __forceinline__ void solve_tail(
     const Item &it, int task, int cls, int nm, int nb, int sz0,
     const int *off, const int *sz, const cdbl *A0, int ld0, const cdbl *A1,
@@ -996,8 +1001,15 @@ __device__ __forceinline__ void solve_tail(
       const int sc = src[i];
       const int bi = (nb == 2 && i >= sz0) ? 1 : 0;
       cdbl acc = cmk(0., 0.);
-      for (int k = klo; k < khi; ++k)
-        cfma(acc, rq[pidx(k)], aget(bi, k, sc));
+      for (int kb = klo; kb < khi; kb += 16) {
+        cdbl v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          v[u] = kb + u < khi ? rq[pidx(kb + u)] : cmk(0., 0.);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (kb + u < khi) cfma(acc, v[u], aget(bi, kb + u, sc));
+      }
       const double w = 2. * (nmin + pi) + 1.;
       const double a2 = acc.x * acc.x + acc.y * acc.y;
       s_diag2 += w * a2;
@@ -1028,22 +1040,34 @@ __device__ __forceinline__ void solve_tail(
       for (int rg = lo + 4 * wv; rg < hi; rg += 4 * NW) {
         cdbl acc[4][2];
         int prow[4];
+        cdbl rq[4][KH];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           acc[q][0] = cmk(0., 0.);
           acc[q][1] = cmk(0., 0.);
           const int r = rg + q < hi ? rg + q : rg;
           prow[q] = pidx(r);
-        }
-        for (int k = lo; k < hi; ++k) {
-          const int pk = pidx(k);
-          const cdbl a0 = aget(b, k, s0);
-          const cdbl a1 = aget(b, k, s1);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const cdbl bq = RQg[(int64_t)prow[q] * nm + pk];
-            cfma(acc[q][0], bq, a0);
-            cfma(acc[q][1], bq, a1);
+          for (int h = 0; h < KH; ++h) {
+            const int k = lo + lane + 64 * h;
+            rq[q][h] = k < hi ? RQg[(int64_t)prow[q] * nm + pidx(k)]
+                              : cmk(0., 0.);
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < KH; ++h) {
+          const int kb = lo + 64 * h;
+          if (kb >= hi) break;
+          const int ke = min(hi, kb + 64);
+          for (int k = kb; k < ke; ++k) {
+            const cdbl a0 = aget(b, k, s0);
+            const cdbl a1 = aget(b, k, s1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const cdbl bq = readlane_e(rq[q][h], k - kb);
+              cfma(acc[q][0], bq, a0);
+              cfma(acc[q][1], bq, a1);
+            }
           }
         }
 #pragma unroll
@@ -1153,23 +1177,39 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
     }
   }
 
+  // wave's best candidate (largest magnitude, then lowest row): a DPP
+  // maximum of the magnitude key (bits of |a|^2 + 1, 0 = no candidate),
+  // then of the complemented row among the lanes holding that key
   auto wave_best = [&](double &mag, int &row, cdbl &val) {
-    for (int o = 32; o > 0; o >>= 1) {
-      const double om = __shfl_xor(mag, o);
-      const int orow = __shfl_xor(row, o);
-      const double ox = __shfl_xor(val.x, o);
-      const double oy = __shfl_xor(val.y, o);
-      if (om > mag || (om == mag && orow < row)) {
-        mag = om;
-        row = orow;
-        val = cmk(ox, oy);
-      }
+    const unsigned long long key =
+        mag >= 0. ? (unsigned long long)__double_as_longlong(mag) + 1ull
+                  : 0ull;
+    const unsigned long long mx = wave_max_u64(key);
+    if (mx == 0ull) {
+      mag = -1.;
+      row = nm;
+      val = cmk(0., 0.);
+      return;
     }
+    const unsigned long long rk =
+        key == mx ? (unsigned long long)(0xffffffffu - (unsigned)row) : 0ull;
+    const unsigned long long rmx = wave_max_u64(rk);
+    const unsigned long long hit = __ballot(key == mx && rk == rmx);
+    const int src_lane = (int)__builtin_ctzll(hit);
+    mag = __longlong_as_double((long long)(mx - 1ull));
+    row = (int)(0xffffffffu - (unsigned)rmx);
+    val = readlane_e(val, src_lane);
   };
   auto publish = [&](double *bm, int *br, cdbl *bv) {
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      wave_best(bm[b], br[b], bv[b]);
+      if (b >= nb) {
+        bm[b] = -1.;
+        br[b] = nm;
+        bv[b] = cmk(0., 0.);
+      } else {
+        wave_best(bm[b], br[b], bv[b]);
+      }
       if (lane == 0) {
         wb_mag[wv][b] = bm[b];
         wb_row[wv][b] = br[b];
@@ -1320,7 +1360,7 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
     }
   }
   __syncthreads();
-  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, A, nm,
+  solve_tail<NT, 1>(it, task, cls, nm, nb, sz0, off, sz, A, nm,
                  nb == 2 ? A + (int64_t)sz0 * nm + sz0 : A, nm, piv, src, RQg,
                  tbuf, qparts, diag_only, red, ainv_out);
 }
@@ -1418,7 +1458,8 @@ __global__ __launch_bounds__(NT) void k_invert_solve_blk(
     }
     __syncthreads();
   }
-  solve_tail<NT>(it, task, cls, nm, nb, sz0, off, sz, Ab[0], ldb[0], Ab[1],
+  solve_tail<NT, GLOBAL ? 4 : 2>(it, task, cls, nm, nb, sz0, off, sz, Ab[0],
+                                 ldb[0], Ab[1],
                  ldb[1], piv, src, RQg, tbuf, qparts, diag_only, red,
                  ainv_out);
 }
