@@ -115,6 +115,8 @@ __device__ inline cdbl cfac(int nn, int n) {
 // (stored [n][direction] so neighbouring pairs read neighbouring words),
 // of the round's incoming directions and the n-factors a_n are staged in
 // LDS.
+constexpr int kObsXQ = 8;  // elements of a T row per batch of loads
+
 template <int CK, int NT, int GCAP, bool PTG>
 __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
                                                const int32_t *item_list,
@@ -202,28 +204,43 @@ __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
       const int cls = (h + n) & 1;
       const int nh2 = A ? 2 : 1;
       for (int hh = 0; hh < nh2; ++hh) {
-        for (int q = 0; q < nm; ++q) {
-          const int nn = nmin + q;
-          const int h2 = A ? hh : ((cls + nn) & 1);
-          cdbl tv = A ? A[(int64_t)row * n2m + h2 * nm + q]
-                      : T[cls * nm2 + (int64_t)p * nm + q];
-          // c_{nn n} T = i^(nn - n - 1) a_nn a_n T
-          const double f = a_n * an[q];
-          const int e = (nn - n - 1) & 3;
-          cdbl cm;
-          if (e == 0) cm = cmk(f * tv.x, f * tv.y);
-          else if (e == 1) cm = cmk(-f * tv.y, f * tv.x);
-          else if (e == 2) cm = cmk(-f * tv.x, -f * tv.y);
-          else cm = cmk(f * tv.y, -f * tv.x);
+        for (int qb = 0; qb < nm; qb += kObsXQ) {
+          // the row's next kObsXQ elements of T (one cache line), loaded
+          // together: one memory latency per batch
+          cdbl tq[kObsXQ];
 #pragma unroll
-          for (int ic = 0; ic < CK; ++ic) {
-            const double bp = Bg[(ic * nm + q) * 2];
-            const double bt = Bg[(ic * nm + q) * 2 + 1];
-            // b = (pi, tau) over h2, b' = (tau, pi)
-            const double b0 = h2 ? bt : bp;
-            const double b1 = h2 ? bp : bt;
-            rfma(acc[ic][0], b0, cm);
-            rfma(acc[ic][1], b1, cm);
+          for (int u = 0; u < kObsXQ; ++u) {
+            const int q = min(qb + u, nm - 1);
+            const int h2 = A ? hh : ((cls + nmin + q) & 1);
+            tq[u] = A ? A[(int64_t)row * n2m + h2 * nm + q]
+                      : T[cls * nm2 + (int64_t)p * nm + q];
+          }
+#pragma unroll
+          for (int u = 0; u < kObsXQ; ++u) {
+            const int q = qb + u;
+            if (q >= nm) break;
+            const int nn = nmin + q;
+            const int h2 = A ? hh : ((cls + nn) & 1);
+            const cdbl tv = tq[u];
+            // c_{nn n} T = i^e a_nn a_n T, e = nn - n - 1 mod 4: odd e
+            // swaps the parts, e = 1, 2 negate the real part and e = 2, 3
+            // the imaginary part
+            const double f = a_n * an[q];
+            const int e = (nn - n - 1) & 3;
+            const double u0 = (e & 1) ? tv.y : tv.x;
+            const double u1 = (e & 1) ? tv.x : tv.y;
+            const cdbl cm = cmk(((e + 1) & 2) ? -f * u0 : f * u0,
+                                (e & 2) ? -f * u1 : f * u1);
+#pragma unroll
+            for (int ic = 0; ic < CK; ++ic) {
+              const double bp = Bg[(ic * nm + q) * 2];
+              const double bt = Bg[(ic * nm + q) * 2 + 1];
+              // b = (pi, tau) over h2, b' = (tau, pi)
+              const double b0 = h2 ? bt : bp;
+              const double b1 = h2 ? bp : bt;
+              rfma(acc[ic][0], b0, cm);
+              rfma(acc[ic][1], b1, cm);
+            }
           }
         }
       }
@@ -277,6 +294,7 @@ __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
 // S(phi) = (1/k) sum_m g_m(-phi) X_m(j, i).
 constexpr int kObsQJ = 4;
 constexpr int kObsQPhi = 2;  // azimuths per thread (measured best of 2, 4, 5, 10)
+constexpr int kObsQM = 4;    // m values per batch of X loads
 
 __global__ __launch_bounds__(256) void k_obs_q(
     const ObsSpec spec, const int32_t *gi_off, const GrainParams gp,
@@ -347,26 +365,30 @@ __global__ __launch_bounds__(256) void k_obs_q(
       }
       // e^{i m dphi} by recurrence, as the reference
       const cdbl *Xb = X + ((int64_t)i0 * spec.n_pairs + pr) * 4;
-      // X of the next m is loaded while the current one is summed
-      cdbl xn[4];
+      // X of kObsQM consecutive m loaded together (one memory latency per
+      // batch)
+      for (int mb = 0; mb < nq; mb += kObsQM) {
+        cdbl xv[kObsQM][4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) xn[c] = nq > 0 ? Xb[c] : cmk(0., 0.);
-      for (int m = 0; m < nq; ++m) {
-        cdbl xv[4];
+        for (int d = 0; d < kObsQM; ++d)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          xv[c] = xn[c];
-          if (m + 1 < nq) xn[c] = Xb[(m + 1) * xs + c];
-        }
+          for (int c = 0; c < 4; ++c)
+            xv[d][c] = mb + d < nq ? Xb[(int64_t)(mb + d) * xs + c]
+                                   : cmk(0., 0.);
 #pragma unroll
-        for (int u = 0; u < kObsQPhi; ++u) {
-          const double fc = (m == 0) ? 1. : 2. * ex[u].x;
-          const double fs = (m == 0) ? 0. : 2. * ex[u].y;
-          ex[u] = cmul(ex[u], st[u]);
-          rfma(S[u][0], fc, xv[0]);
-          rfma(S[u][3], fc, xv[3]);
-          rfma(S[u][1], fs, xv[1]);
-          rfma(S[u][2], fs, xv[2]);
+        for (int d = 0; d < kObsQM; ++d) {
+          const int m = mb + d;
+          if (m >= nq) break;
+#pragma unroll
+          for (int u = 0; u < kObsQPhi; ++u) {
+            const double fc = (m == 0) ? 1. : 2. * ex[u].x;
+            const double fs = (m == 0) ? 0. : 2. * ex[u].y;
+            ex[u] = cmul(ex[u], st[u]);
+            rfma(S[u][0], fc, xv[d][0]);
+            rfma(S[u][3], fc, xv[d][3]);
+            rfma(S[u][1], fs, xv[d][1]);
+            rfma(S[u][2], fs, xv[d][2]);
+          }
         }
       }
       double a = 0., b = 0.;
