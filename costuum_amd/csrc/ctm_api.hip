@@ -903,13 +903,15 @@ int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     const int32_t *pre = ctx->d_gioff_tmp.as<int32_t>();
     const int kid[3] = {CTM_KERNEL_ASSEMBLE_WIDE, CTM_KERNEL_ASSEMBLE,
                         CTM_KERNEL_ASSEMBLE_SMALL};
-    // the parts write disjoint blocks and run side by side on the lanes
-    // (measured 18.25 -> 17.97 ms per batch step against running the 3 x 3
-    // part alone first)
+    // the parts write disjoint blocks.  The 3 x 3 part (most of the work,
+    // enough tiles to fill the GPU) runs alone, so that its launches measure
+    // the kernel and not the GPU it shares; the others follow side by side
+    // on the lanes (all parts side by side measured 18.25 -> 17.97 ms per
+    // batch step, with the 3 x 3 launches then sharing the GPU)
     const int lane_of[5] = {0, 1, 2, 3, 2};
     size_t po = 0;
-    CTM_HIP(fork_lanes(ctx));
     for (int part = 0; part < 3; ++part) {
+      if (part == 1) CTM_HIP(fork_lanes(ctx));
       const int np = w.n_part[part];
       if (np > 0) {
         parts |= 1u << part;
