@@ -17,7 +17,9 @@ shape-averaged Z.
     cell (lambda index 5, radius index 47): x = 82.5, orders above 100
     cell (lambda index 10, radius index 40): x = 22.2, orders about 35
 
-Writes oracle_grid_cells.npz beside this file.
+Writes oracle_grid_cells.npz beside this file.  Each finished shape is
+checkpointed in oracle_grid_cells.partial.npz, so an interrupted run
+resumes; the partial file is removed at the end.
 
     python tests/golden/make_grid_fixture.py [--processes 8]
 """
@@ -37,11 +39,13 @@ import make_inputs  # noqa: E402
 
 CELLS = ((5, 47), (10, 40))
 OUT = os.path.join(HERE, "oracle_grid_cells.npz")
+PARTIAL = os.path.join(HERE, "oracle_grid_cells.partial.npz")
+FIELDS = ("n", "st", "qe", "qa", "Z", "qe2", "qa2")
 
 
 def one_shape(args):
     import oracle as O
-    rv, d, lam, mr = args
+    idx, rv, d, lam, mr = args
     theta = np.arccos(np.polynomial.legendre.leggauss(8)[0])
     Theta = np.radians(np.arange(0.0, 181.0, 2.0))
     st, n, ng, _, _ = O.converge_task(rv, d, lam, mr, 4, 160, 2, 1e-4)
@@ -59,7 +63,7 @@ def one_shape(args):
     Ta2 = O.orientation_average(n, O.tmatrix_all(rv2, d, lam, mr, n, ng), pN)
     qe2, _, _ = O.extinction(n, Ta2, lam, rv2, theta)
     qa2, _ = O.absorption(n, Ta2, lam, rv2, theta, ngauss=20)
-    return n, st, qe, qa, Z, qe2, qa2
+    return idx, (n, st, qe, qa, Z, qe2, qa2)
 
 
 def main():
@@ -73,8 +77,23 @@ def main():
     for il, ia in CELLS:
         mr = complex(make_inputs.lorentz_index(lam[il]))
         jobs += [(rad[ia], dk, lam[il], mr) for dk in d]
+    done = {}
+    if os.path.exists(PARTIAL):
+        with np.load(PARTIAL) as f:
+            for key in f.files:
+                i, fld = key.split("_", 1)
+                done.setdefault(int(i[1:]), {})[fld] = f[key].copy()
+    todo = [(i,) + jobs[i] for i in range(len(jobs)) if i not in done]
+    # largest size parameters first, so the long shapes start early
+    todo.sort(key=lambda j: -j[1] / j[3])
     with Pool(a.processes) as pool:
-        res = pool.map(one_shape, jobs, chunksize=1)
+        for i, r in pool.imap_unordered(one_shape, todo, chunksize=1):
+            done[i] = dict(zip(FIELDS, [np.asarray(v) for v in r]))
+            np.savez(PARTIAL, **{"i%d_%s" % (k, f): v for k, rr in
+                                 done.items() for f, v in rr.items()})
+            print("shape %d done (%d/%d), order %d" % (
+                i, len(done), len(jobs), int(done[i]["n"])), flush=True)
+    res = [tuple(done[i][f] for f in FIELDS) for i in range(len(jobs))]
     ns = len(d)
     for c in range(len(CELLS)):
         r = res[c * ns:(c + 1) * ns]
@@ -90,6 +109,7 @@ def main():
               out["nmax_%d" % c].max(), "status", out["status_%d" % c].max(),
               flush=True)
     np.savez_compressed(OUT, **out)
+    os.remove(PARTIAL)
     print("wrote", OUT, flush=True)
 
 
