@@ -46,6 +46,15 @@ def parse():
     ap.add_argument("--precision", default="both",
                     choices=["both", "fp64", "mixed"],
                     help="arithmetic path(s) timed on the batch workload")
+    ap.add_argument("--driver", default="plan", choices=["plan", "host"],
+                    help="batch workload: plan = work lists built on the "
+                         "GPU, no host round trip inside a step "
+                         "(ctm_plan_run); host = lists built on the CPU "
+                         "after each convergence round (ctm_tmatrix_batch "
+                         "+ ctm_observables).  The other driver's FP64 step "
+                         "time is reported beside it")
+    ap.add_argument("--compare-driver", type=int, default=1,
+                    help="also time the other driver's FP64 step")
     ap.add_argument("--per-grain", type=int, default=64,
                     help="grains of the per-grain (one call each) FP64 "
                          "comparator, every k-th in size order")
@@ -510,7 +519,8 @@ def main():
     import numpy as np
     import torch
     import make_inputs
-    from costuum_amd.tmatrix import Context, observables, tmatrix_batch
+    from costuum_amd.tmatrix import (BatchPlan, Context, observables,
+                                     tmatrix_batch)
 
     g = make_inputs.load(0, 4096)
     if a.grains != 4096:
@@ -526,14 +536,24 @@ def main():
     theta = np.arccos(xg)
     wgt = torch.as_tensor(0.5 * wg, dtype=torch.float64, device=dev)
 
-    def step(prec):
+    def host_step(prec):
         r = tmatrix_batch(*dg, ctx=ctx, sync=False, min_order=4,
                           max_order=100, glfac=2, tolerance=1e-4,
                           precision=prec)
         obs = observables(theta, ctx=ctx, ngauss=20, n_grains=n)
         return r, obs["qext"] @ wgt, obs["qabs"] @ wgt
 
-    def timed(prec):
+    def timed(prec, driver):
+        if driver == "plan":
+            plan = BatchPlan(*dg, min_order=4, max_order=100, glfac=2,
+                             tolerance=1e-4, precision=prec, theta=theta,
+                             ngauss=20, ctx=ctx)
+
+            def step(_):
+                plan.run()
+                return plan, plan.obs["qext"] @ wgt, plan.obs["qabs"] @ wgt
+        else:
+            step = host_step
         for _ in range(a.warmup):
             step(prec)
         ctx.profile(True)  # enables and resets the counters
@@ -547,14 +567,31 @@ def main():
         stats = ctx.profile_stats()
         ctx.profile(False)
         r, qe, qa = res
-        nmax, _, status = r.orders()
-        return {"ms": (t1 - t0) * 1e3 / max(a.steps, 1), "stats": stats,
-                "nmax": nmax, "status": status,
-                "qext": qe.cpu().numpy(), "qabs": qa.cpu().numpy(),
-                "blocks": ctx.precision_stats()}
+        if driver == "plan":
+            nmax = r.nmax.cpu().numpy().astype(np.uint32)
+            status = r.status.cpu().numpy()
+            info = r.info()
+            blocks = (info["mixed_blocks"], info["mixed_fallback"])
+            extra = {"plan_reruns": info["reruns"],
+                     "plan_pool_bytes": info["pool_bytes"],
+                     "window_round_orders": info["window_items"]}
+        else:
+            nmax, _, status = r.orders()
+            blocks = ctx.precision_stats()
+            extra = {}
+        out = {"ms": (t1 - t0) * 1e3 / max(a.steps, 1), "stats": stats,
+               "nmax": nmax, "status": status,
+               "qext": qe.cpu().numpy(), "qabs": qa.cpu().numpy(),
+               "blocks": blocks, "extra": extra}
+        if driver == "plan":
+            plan.close()
+        return out
 
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
-    runs = {p: timed(p) for p in precs}
+    runs = {p: timed(p, a.driver) for p in precs}
+    # the other driver's FP64 step, for comparison (same work, same results)
+    other = "host" if a.driver == "plan" else "plan"
+    other_run = timed("fp64", other) if a.compare_driver else None
     fx = load_fixture() if n == 4096 else None
 
     # the plain FP64 path one grain per call (the caller that does not
@@ -592,6 +629,7 @@ def main():
         entry = {"ms_per_step": round(rr["ms"], 3), "converged": nconv,
                  "blocks_fp32_elimination": int(rr["blocks"][0]),
                  "blocks_fp64_fallback": int(rr["blocks"][1])}
+        entry.update(rr["extra"])
         if fx is not None:
             ver, ok6 = verify(fx, rr["nmax"], rr["status"], rr["qext"],
                               rr["qabs"])
@@ -678,6 +716,17 @@ def main():
                               "converged (no oracle fixture)",
                    "orientations": 32, "absorption_quadrature": "20x20"},
         "paths": summary,
+        "driver": a.driver,
+        "other_driver_fp64": {
+            "driver": other, "ms_per_step": round(other_run["ms"], 3),
+            "same_orders": bool(np.array_equal(
+                np.asarray(other_run["nmax"], dtype=np.int64),
+                np.asarray(runs.get("fp64", other_run)["nmax"],
+                           dtype=np.int64))),
+            "max_rel_diff_qext": float(np.max(
+                np.abs(other_run["qext"] - runs.get("fp64", other_run)["qext"])
+                / np.maximum(np.abs(other_run["qext"]), 1e-300)))}
+        if other_run else None,
         "mixed_over_plain": round(mixed / plain, 4)
         if plain and mixed else None,
         "per_grain_fp64_gpu": per_grain,

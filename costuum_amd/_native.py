@@ -31,7 +31,8 @@ EXPORTS = (
     "ctm_weighted_reduce", "ctm_tmatrix_tangents", "ctm_tangent_get",
     "ctm_observables_select", "ctm_vjp", "ctm_observables_noalloc",
     "ctm_tmatrix_set", "ctm_grain_memory_estimate", "ctm_precision_stats",
-    "ctm_observables_derivative",
+    "ctm_observables_derivative", "ctm_plan_create", "ctm_plan_run",
+    "ctm_plan_info_get", "ctm_plan_destroy",
 )
 
 # double f(double x, void *arg): the host callback type (ctm_real_fn)
@@ -76,6 +77,25 @@ class CtmConvergence(ctypes.Structure):
                 ("glfac", ctypes.c_uint32),
                 ("precision", ctypes.c_uint32),
                 ("tolerance", ctypes.c_double)]
+
+
+class CtmPlanOutputs(ctypes.Structure):
+    _fields_ = [("nmax", ctypes.c_void_p), ("status", ctypes.c_void_p),
+                ("qsca", ctypes.c_void_p), ("qext", ctypes.c_void_p),
+                ("obs_qext", ctypes.c_void_p),
+                ("obs_qextpol", ctypes.c_void_p),
+                ("obs_qextcpol", ctypes.c_void_p),
+                ("obs_qabs", ctypes.c_void_p),
+                ("obs_qabspol", ctypes.c_void_p)]
+
+
+class CtmPlanInfo(ctypes.Structure):
+    _fields_ = [("runs", ctypes.c_int64), ("reruns", ctypes.c_int64),
+                ("mixed_blocks", ctypes.c_int64),
+                ("mixed_fallback", ctypes.c_int64),
+                ("pool_bytes", ctypes.c_int64),
+                ("allm_items", ctypes.c_int32),
+                ("window_items", ctypes.c_int32 * 3)]
 
 
 _lib = None
@@ -180,5 +200,17 @@ def lib():
                                                ctypes.c_double,
                                                ctypes.c_double, dp]
     L.ctm_orientation_coefficients.restype = ctypes.c_int
+    L.ctm_plan_create.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp, dp,
+                                  ctypes.POINTER(CtmConvergence),
+                                  ctypes.POINTER(CtmObservableSpec),
+                                  ctypes.POINTER(CtmPlanOutputs),
+                                  ctypes.POINTER(vp)]
+    L.ctm_plan_create.restype = ctypes.c_int
+    L.ctm_plan_run.argtypes = [vp]
+    L.ctm_plan_run.restype = ctypes.c_int
+    L.ctm_plan_info_get.argtypes = [vp, ctypes.POINTER(CtmPlanInfo)]
+    L.ctm_plan_info_get.restype = ctypes.c_int
+    L.ctm_plan_destroy.argtypes = [vp]
+    L.ctm_plan_destroy.restype = ctypes.c_int
     _lib = L
     return L

@@ -8,8 +8,10 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcostuum_amd.so")
 SOURCES = ["csrc/ctm_kernels.hip", "csrc/ctm_observables.hip",
            "csrc/ctm_average.hip", "csrc/ctm_api.hip",
-           "csrc/ctm_host.hip", "csrc/ctm_tangent.hip", "csrc/ctm_mixed.hip"]
+           "csrc/ctm_host.hip", "csrc/ctm_tangent.hip", "csrc/ctm_mixed.hip",
+           "csrc/ctm_plan.hip"]
 HEADERS = ["csrc/ctm_device.hpp", "csrc/ctm_kernels.hpp", "csrc/ctm_gj.hpp",
+           "csrc/ctm_plan.hpp",
            "csrc/ctm_mixed.hpp",
            "csrc/ctm_observables.hpp", "csrc/ctm_average.hpp",
            os.path.join("..", "include", "costuum_amd.h")]

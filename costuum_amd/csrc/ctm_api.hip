@@ -21,6 +21,7 @@
 #include <climits>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -37,6 +38,7 @@
 #include "ctm_tangent.hpp"
 #include "ctm_mixed.hpp"
 #include "ctm_gj.hpp"
+#include "ctm_plan.hpp"
 
 using namespace ctm;
 
@@ -396,6 +398,7 @@ struct ctm_context {
   Staging staging;
   Staging readback;  // pinned landing area of device-to-host copies
   char *scratch = nullptr;  // this device's fixed scratch (device address)
+  int plans = 0;            // live ctm_plan objects on this context
 };
 
 #define CTM_HIP(expr)                                                   \
@@ -907,8 +910,10 @@ int run_items(ctm_context *ctx, const GrainParams &gp, bool want_t) {
     // enough tiles to fill the GPU) runs alone, so that its launches measure
     // the kernel and not the GPU it shares; the others follow side by side
     // on the lanes (all parts side by side measured 18.25 -> 17.97 ms per
-    // batch step, with the 3 x 3 launches then sharing the GPU)
-    const int lane_of[5] = {0, 1, 2, 3, 2};
+    // batch step, with the 3 x 3 launches then sharing the GPU).  Window
+    // lists hold only m = 0 blocks, so the packed m > 0 part is empty and
+    // its lane takes the packed m = 0 part
+    const int lane_of[5] = {0, 1, 2, 3, want_t ? 2 : 3};
     size_t po = 0;
     for (int part = 0; part < 3; ++part) {
       if (part == 1) CTM_HIP(fork_lanes(ctx));
@@ -965,38 +970,6 @@ int run_work(ctm_context *ctx, const GrainParams &gp, bool want_t) {
   return run_items(ctx, gp, want_t);
 }
 
-// Rough order estimate from the largest size parameter of the spheroid
-// (TMatrixCalculator.hpp:83-85 uses x + 4.05 x^(1/3) of the equal-volume
-// sphere); only used to size the first window of orders.
-int order_estimate(double rv, double d, double lam) {
-  const double rmax = rv * std::max(std::cbrt(d), 1. / std::cbrt(d * d));
-  const double x = 2. * M_PI * rmax / lam;
-  return (int)(x + 4.05 * std::cbrt(x)) + 2;
-}
-
-// Last order of a grain's first window.  Small grains (estimate below 10)
-// take the estimate + 3 + estimate / 8.  Larger ones take a model of the
-// converged order in the size parameter x of the largest semi-axis, Re m
-// and the shape: a least-squares fit of the converged orders of a seeded
-// 4096-grain spheroid batch (tolerance 1e-4, 2 nodes per order; x up to 63,
-// Re m 1.3-2, d 0.5-2).  Its residual (converged minus model, 5th-95th
-// percentile) is -1.9 .. 1.5 orders on an independent 1024-grain batch and
-// -3.5 .. 2.4 on a 61 440-grain size x wavelength x shape grid (x up to
-// 126), against -6 .. 5 for the estimate alone; the window ends 3 + 4 %
-// above the model.  Only the amount of work depends on it: the
-// convergence decision is made order by order as the reference makes it.
-int first_window_end(double rv, double d, double lam, double mre) {
-  const int est = order_estimate(rv, d, lam);
-  if (est < 10) return est + 3 + est / 8;
-  const double rmax = rv * std::max(std::cbrt(d), 1. / std::cbrt(d * d));
-  const double x = 2. * M_PI * rmax / lam;
-  const double cx = std::cbrt(x), ld = std::fabs(std::log(d));
-  const double model = 0.2213 + 0.6409 * x + 2.4758 * cx +
-                       0.4839 * (mre - 1.) * x + 0.7758 * ld * x -
-                       0.1053 * ld * cx;
-  return (int)std::ceil(1.04 * model + 3.);
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1032,6 +1005,10 @@ int ctm_create(ctm_context **out, int device) {
 
 int ctm_destroy(ctm_context *ctx) {
   if (!ctx) return CTM_OK;
+  if (ctx->plans > 0) {
+    ctx->err = "destroy the context's plans first";
+    return CTM_ERR_STATE;
+  }
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (hipStream_t st : ctx->side)
@@ -1313,9 +1290,8 @@ int ctm_tmatrix_batch(ctm_context *ctx, int n, const double *radius,
     std::vector<int32_t> next;
     for (int g : active) {
       if (st[g].done) continue;
-      const int width = std::max(8, (hi[g] + 1) / 2);
       lo[g] = hi[g] + 1;
-      hi[g] = std::min(last_order, hi[g] + width);
+      hi[g] = next_window_end(hi[g], last_order, false);
       next.push_back(g);
     }
     active.swap(next);
@@ -1655,33 +1631,36 @@ static void push_angle(std::vector<double> &ang, double c) {
 // tables, the pi/tau table, the bilinear forms X, the item order) comes
 // from the context's growable buffers, or, when `fixed` is not null, from
 // the fixed device range fixed[0 .. fixed_cap) with no allocation at all.
-static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
-                            double *qext, double *qextpol, double *qextcpol,
-                            double *qabs, double *qabspol, double *Z,
-                            char *fixed, size_t fixed_cap) {
-  if (!ctx || !spec) return CTM_ERR_ARGUMENT;
-  const bool quiet = fixed != nullptr;
-  if (ctx->final_items.empty()) {
-    ctx->err = "no T-matrices in the context";
-    return CTM_ERR_STATE;
+// Direction table, incoming/outgoing lists and direction pairs of an
+// observable specification (validated by the caller), and the device image
+// of its arrays (blob; bind() points the spec at a device copy).
+struct ObsLayout {
+  ObsSpec sp{};
+  std::vector<char> blob;
+  int n_ang = 0;
+  size_t o_ang = 0, o_in = 0, o_out = 0, o_pin = 0, o_pout = 0, o_byin = 0,
+         o_inoff = 0, o_tw = 0, o_pc = 0, o_ps = 0, o_pw = 0, o_sc = 0,
+         o_ss = 0;
+  void bind(const char *base) {
+    sp.in_angle = reinterpret_cast<const int32_t *>(base + o_in);
+    sp.out_angle = reinterpret_cast<const int32_t *>(base + o_out);
+    sp.pair_in = reinterpret_cast<const int32_t *>(base + o_pin);
+    sp.pair_out = reinterpret_cast<const int32_t *>(base + o_pout);
+    sp.pair_by_in = reinterpret_cast<const int32_t *>(base + o_byin);
+    sp.in_pair_off = reinterpret_cast<const int32_t *>(base + o_inoff);
+    sp.theta_w = reinterpret_cast<const double *>(base + o_tw);
+    sp.phi_cos = reinterpret_cast<const double *>(base + o_pc);
+    sp.phi_sin = reinterpret_cast<const double *>(base + o_ps);
+    sp.phi_w = reinterpret_cast<const double *>(base + o_pw);
+    sp.sphi_cos = reinterpret_cast<const double *>(base + o_sc);
+    sp.sphi_sin = reinterpret_cast<const double *>(base + o_ss);
   }
+};
+
+static void obs_layout(const ctm_observable_spec *spec, ObsLayout &o) {
   const int nth = spec->n_theta;
   const int ng = spec->account_for_scattering ? spec->ngauss : 0;
   const int nst = spec->n_scatter_theta;
-  if (nth < 0 || ng < 0 || nst < 0 || (nth > 0 && !spec->theta) ||
-      (spec->account_for_scattering && spec->ngauss < 1)) {
-    ctx->err = "invalid observable specification";
-    return CTM_ERR_ARGUMENT;
-  }
-  if (ctx->obs_source >= 0 &&
-      (spec->account_for_scattering || nst > 0 || ctx->averaged)) {
-    ctx->err = "with a tangent selected only observables linear in T "
-               "(no scattered-light correction, no Z, no averaging)";
-    return CTM_ERR_ARGUMENT;
-  }
-  CTM_HIP(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const int n = ctx->n_grains;
   // direction table: outgoing theta list, GL incoming, scattering grid,
   // and the scattering incidence theta = pi/2
   std::vector<double> ang;
@@ -1720,7 +1699,8 @@ static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
     push_angle(ang, std::cos(M_PI_2));
   }
   const int n_ang = (int)ang.size() / 3;
-  if (n_ang == 0) return CTM_OK;
+  o.n_ang = n_ang;
+  if (n_ang == 0) return;
   // incoming / outgoing lists and pairs
   std::vector<int32_t> in_l, out_l, p_in, p_out;
   for (int j = 0; j < nth; ++j) {
@@ -1732,7 +1712,8 @@ static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
     in_l.push_back(nth + ng + nst);
     for (int i = 0; i < nst; ++i) out_l.push_back(nth + ng + i);
   }
-  ObsSpec sp{};
+  ObsSpec &sp = o.sp;
+  sp = ObsSpec{};
   sp.fwd_pair0 = 0;
   for (int j = 0; j < nth; ++j) {
     p_out.push_back(j);
@@ -1766,27 +1747,62 @@ static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
   sp.n_stheta = nst;
   sp.n_sphi = nsp;
   // device copies of the spec arrays (one buffer)
-  std::vector<char> blob;
+  std::vector<char> &blob = o.blob;
+  blob.clear();
   auto put = [&](const void *src, size_t bytes) {
     size_t off = (blob.size() + 255) & ~(size_t)255;
     blob.resize(off + bytes);
     if (bytes) std::memcpy(blob.data() + off, src, bytes);
     return off;
   };
-  const size_t o_ang = put(ang.data(), ang.size() * sizeof(double));
-  const size_t o_in = put(in_l.data(), in_l.size() * 4);
-  const size_t o_out = put(out_l.data(), out_l.size() * 4);
-  const size_t o_pin = put(p_in.data(), p_in.size() * 4);
-  const size_t o_pout = put(p_out.data(), p_out.size() * 4);
-  const size_t o_byin = put(by_in.data(), by_in.size() * 4);
-  const size_t o_inoff = put(in_off.data(), in_off.size() * 4);
-  const size_t o_tw = put(gw.data(), gw.size() * 8);
-  const size_t o_pc = put(pc.data(), pc.size() * 8);
-  const size_t o_ps = put(ps.data(), ps.size() * 8);
-  const size_t o_pw = put(pw.data(), pw.size() * 8);
-  const size_t o_sc = put(sc.data(), sc.size() * 8);
-  const size_t o_ss = put(ss.data(), ss.size() * 8);
+  o.o_ang = put(ang.data(), ang.size() * sizeof(double));
+  o.o_in = put(in_l.data(), in_l.size() * 4);
+  o.o_out = put(out_l.data(), out_l.size() * 4);
+  o.o_pin = put(p_in.data(), p_in.size() * 4);
+  o.o_pout = put(p_out.data(), p_out.size() * 4);
+  o.o_byin = put(by_in.data(), by_in.size() * 4);
+  o.o_inoff = put(in_off.data(), in_off.size() * 4);
+  o.o_tw = put(gw.data(), gw.size() * 8);
+  o.o_pc = put(pc.data(), pc.size() * 8);
+  o.o_ps = put(ps.data(), ps.size() * 8);
+  o.o_pw = put(pw.data(), pw.size() * 8);
+  o.o_sc = put(sc.data(), sc.size() * 8);
+  o.o_ss = put(ss.data(), ss.size() * 8);
+}
 
+static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
+                            double *qext, double *qextpol, double *qextcpol,
+                            double *qabs, double *qabspol, double *Z,
+                            char *fixed, size_t fixed_cap) {
+  if (!ctx || !spec) return CTM_ERR_ARGUMENT;
+  const bool quiet = fixed != nullptr;
+  if (ctx->final_items.empty()) {
+    ctx->err = "no T-matrices in the context";
+    return CTM_ERR_STATE;
+  }
+  const int nth = spec->n_theta;
+  const int ng = spec->account_for_scattering ? spec->ngauss : 0;
+  const int nst = spec->n_scatter_theta;
+  if (nth < 0 || ng < 0 || nst < 0 || (nth > 0 && !spec->theta) ||
+      (spec->account_for_scattering && spec->ngauss < 1)) {
+    ctx->err = "invalid observable specification";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (ctx->obs_source >= 0 &&
+      (spec->account_for_scattering || nst > 0 || ctx->averaged)) {
+    ctx->err = "with a tangent selected only observables linear in T "
+               "(no scattered-light correction, no Z, no averaging)";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int n = ctx->n_grains;
+  ObsLayout lay;
+  obs_layout(spec, lay);
+  const int n_ang = lay.n_ang;
+  if (n_ang == 0) return CTM_OK;
+  ObsSpec &sp = lay.sp;
+  const std::vector<char> &blob = lay.blob;
   int nmax_all = 1, nm_max = 1;
   for (int g = 0; g < n; ++g) nmax_all = std::max(nmax_all, (int)ctx->nmax[g]);
   nm_max = nmax_all;
@@ -1836,19 +1852,8 @@ static int observables_impl(ctm_context *ctx, const ctm_observable_spec *spec,
     d_x = ctx->d_x.as<char>();
     d_li = ctx->d_obsitems.as<char>();
   }
-  const double *d_ang = reinterpret_cast<const double *>(base + o_ang);
-  sp.in_angle = reinterpret_cast<const int32_t *>(base + o_in);
-  sp.out_angle = reinterpret_cast<const int32_t *>(base + o_out);
-  sp.pair_in = reinterpret_cast<const int32_t *>(base + o_pin);
-  sp.pair_out = reinterpret_cast<const int32_t *>(base + o_pout);
-  sp.pair_by_in = reinterpret_cast<const int32_t *>(base + o_byin);
-  sp.in_pair_off = reinterpret_cast<const int32_t *>(base + o_inoff);
-  sp.theta_w = reinterpret_cast<const double *>(base + o_tw);
-  sp.phi_cos = reinterpret_cast<const double *>(base + o_pc);
-  sp.phi_sin = reinterpret_cast<const double *>(base + o_ps);
-  sp.phi_w = reinterpret_cast<const double *>(base + o_pw);
-  sp.sphi_cos = reinterpret_cast<const double *>(base + o_sc);
-  sp.sphi_sin = reinterpret_cast<const double *>(base + o_ss);
+  const double *d_ang = reinterpret_cast<const double *>(base + lay.o_ang);
+  lay.bind(base);
 
   (void)nm_max;
   double *PT = reinterpret_cast<double *>(d_pt);
@@ -2283,6 +2288,796 @@ int ctm_batch_orders(ctm_context *ctx, uint32_t *nmax_host,
     if (ngauss_host) ngauss_host[g] = ctx->ngauss[g];
     if (status_host) status_host[g] = ctx->status[g];
   }
+  return CTM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batch plan: the batch step (convergence rounds, all-m blocks, per-grain
+// sums and the averaged Q_ext / Q_abs) with every work list built on the
+// device (ctm_plan.hip), so a run enqueues a fixed sequence of launches and
+// the host reads nothing back until the step has finished.  The
+// convergence runs in three rounds: the first window of every grain
+// (first_window_end), then half its order more, then every order up to the
+// last; each order is tested as the reference tests it, so the converged
+// orders are the host path's.  Data pools (tables, Q, T, the observables'
+// X) are sized at creation from the first windows; a run that needs more
+// flags it on the device (its lists stay empty, nothing is written out of
+// bounds), and ctm_plan_run grows the pools to the sizes the device
+// reported and runs again.
+
+struct ctm_plan {
+  ctm_context *ctx = nullptr;
+  int n = 0;
+  GrainParams gp{};
+  int min_order = 0, last_order = 0, glfac = 0;
+  double tol = 0.;
+  bool mixed = false;
+  ctm_plan_outputs out{};
+  bool want_x = false;
+  ObsLayout lay;
+  // grain-independent tables: Gauss-Legendre nodes of every G = glfac N and
+  // Wigner values of every (N, m) for N = min_order .. last_order
+  DevBuf d_gl, d_glkeys, d_wig, d_wigkeys, d_gloff, d_wigbase, d_spec, d_pt;
+  // lists and per-grain state
+  DevBuf d_sets, d_items, d_ord, d_pre, d_sol, d_obsl, d_flags, d_fb, d_hist,
+      d_cnt, d_goff, d_gitem, d_gobs, d_lo, d_hi, d_state, d_phase, d_status,
+      d_iota, d_gcap;
+  // data pools
+  DevBuf d_tables, d_q, d_t, d_x, d_qparts, d_x32, d_xexp, d_rbuf;
+  PlanCaps caps{};
+  PlanLists L{};
+  int cus = 256;
+  // grid bound of every (phase, list): one workgroup per tile, group of
+  // four blocks, task or item (one thread per table node); the first run
+  // fits them to the measured list sizes
+  int32_t gcap[kPlanPhases][kPlanLists]{};
+  bool gcap_fitted = false;
+  // end of each all-m solve bucket (the observables of its block orders
+  // start after it)
+  hipEvent_t ev_solve[kPlanBuckets] = {};
+  // after a run
+  PlanStatus st{};
+  PlanPhase ph[kPlanPhases]{};
+  int64_t runs = 0, reruns = 0;
+  // profiler records of the current run: the work of a launch is known
+  // only from the device lists, so it is credited after the run
+  struct Rec {
+    int kernel;
+    hipEvent_t a, b;
+    int phase, kind, idx;  // kind: 1 part, 2 bucket, 3 fallback, 4 obs, 5 tables
+  };
+  std::vector<Rec> recs;
+};
+
+namespace {
+
+#define PLAN_HIP(expr)                                                  \
+  do {                                                                  \
+    hipError_t _e = (expr);                                             \
+    if (_e != hipSuccess) {                                             \
+      p->ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);  \
+      return CTM_ERR_HIP;                                               \
+    }                                                                   \
+  } while (0)
+
+// Launch `call` on `strm`, between two profiler events when the context
+// profiles.
+#define PLAN_LAUNCH(strm, kernel, phase_, kind_, idx_, call)                  \
+  do {                                                                        \
+    ctm_context *_c = p->ctx;                                                 \
+    const bool _rec = _c->prof.on;                                            \
+    ctm_plan::Rec _r{kernel, nullptr, nullptr, phase_, kind_, idx_};          \
+    if (_rec) {                                                               \
+      _r.a = _c->prof.take();                                                 \
+      _r.b = _c->prof.take();                                                 \
+      if (_r.a && _r.b) PLAN_HIP(hipEventRecord(_r.a, strm));                 \
+    }                                                                         \
+    PLAN_HIP(call);                                                           \
+    if (_rec && _r.a && _r.b) {                                               \
+      PLAN_HIP(hipEventRecord(_r.b, strm));                                   \
+      p->recs.push_back(_r);                                                  \
+    }                                                                         \
+  } while (0)
+
+// Sizes of the solve launch of bucket b: largest block order, LDS store of
+// the blocked elimination and rows of its largest sub-block (m = 0 blocks
+// split in two by parity).
+void plan_solve_shape(int nm_max, bool m0_only, int64_t *store, int *rows) {
+  if (m0_only && nm_max > 1) {
+    const int r0 = (nm_max + 1) / 2, r1 = nm_max - r0;
+    *store = (int64_t)r0 * blk_lds_stride(r0) + (int64_t)r1 * blk_lds_stride(r1);
+    *rows = r0;
+  } else {
+    *store = (int64_t)nm_max * blk_lds_stride(nm_max);
+    *rows = nm_max;
+  }
+}
+
+// k_obs_x of the all-m blocks, one launch per observable bucket.  With
+// `solved` (the solve buckets' ends recorded in ev_solve and the lanes not
+// yet joined) a bucket starts on a lane once the solve buckets that hold
+// its block orders have finished, beside the remaining solves; otherwise
+// (all T blocks complete) the buckets run side by side.
+int plan_obs_x(ctm_plan *p, const bool *solved, const int *lane_of_bucket) {
+  ctm_context *ctx = p->ctx;
+  const PlanLists &L = p->L;
+  const int ph = kPlanPhases - 1;
+  PlanPhase *dph = p->d_phase.as<PlanPhase>() + ph;
+  const int64_t cap = p->caps.items;
+  for (int b = 0; b < kPlanObsBuckets; ++b) {
+    const int capn = plan_obs_hi(b, p->last_order);
+    const int lo_b =
+        b + 1 < kPlanObsBuckets ? plan_obs_hi(b + 1, p->last_order) : 0;
+    if (capn <= lo_b) continue;
+    hipStream_t ls = lane(ctx, b % kLanes);
+    if (solved) {
+      // the solve buckets holding orders lo_b + 1 .. capn: the bucket runs
+      // on the lane of the last of them and waits for the others
+      int last = -1;
+      for (int sb = 0; sb < kPlanBuckets; ++sb) {
+        const int shi = plan_bucket_hi(sb);
+        const int slo = sb + 1 < kPlanBuckets ? plan_bucket_hi(sb + 1) : 0;
+        if (shi > lo_b && slo < capn && solved[sb]) last = sb;
+      }
+      if (last >= 0) ls = lane(ctx, lane_of_bucket[last]);
+      for (int sb = 0; sb < kPlanBuckets; ++sb) {
+        const int shi = plan_bucket_hi(sb);
+        const int slo = sb + 1 < kPlanBuckets ? plan_bucket_hi(sb + 1) : 0;
+        if (sb != last && shi > lo_b && slo < capn && solved[sb])
+          PLAN_HIP(hipStreamWaitEvent(ls, p->ev_solve[sb], 0));
+      }
+    }
+    PLAN_LAUNCH(ls, CTM_KERNEL_OBS_X, ph, 4, b,
+                launch_obs_x(p->lay.sp, L.obsl + b * cap, 0, capn,
+                             capn > 64 ? 8 : 4, L.items, p->d_t.as<cdbl>(),
+                             nullptr, nullptr, p->d_pt.as<double>(),
+                             p->last_order, p->d_x.as<cdbl>(), ls,
+                             &dph->obs[b][0],
+                             std::max(1, p->gcap[ph][plan_list_obs(b)])));
+  }
+  return CTM_OK;
+}
+
+// Tables, assembly and solves of phase `ph` (lists built on the device);
+// in the all-m phase of the FP64 path with observables, also their k_obs_x.
+int plan_phase(ctm_plan *p, int ph, bool want_t) {
+  ctm_context *ctx = p->ctx;
+  const PlanLists &L = p->L;
+  PlanPhase *dph = p->d_phase.as<PlanPhase>() + ph;
+  const int64_t cap = p->caps.items;
+  const GrainParams &gp = p->gp;
+  hipStream_t s = ctx->stream;
+  const int32_t *gc = p->gcap[ph];
+  PLAN_LAUNCH(s, CTM_KERNEL_TABLES, ph, 5, 0,
+              launch_tables(L.sets, 0, 0, 0, gp, p->d_gl.as<double>(),
+                            p->d_tables.as<double>(), s, &dph->sets_nodes[0],
+                            std::max(1, (gc[0] + 127) / 128)));
+  // assembly parts: the 3 x 3 part alone, then the others side by side
+  const int kid[3] = {CTM_KERNEL_ASSEMBLE_WIDE, CTM_KERNEL_ASSEMBLE,
+                      CTM_KERNEL_ASSEMBLE_SMALL};
+  // (window lists hold only m = 0 blocks: the packed m = 0 part takes the
+  // lane of the empty packed m > 0 part)
+  const int lane_of_part[5] = {0, 1, 2, 3, want_t ? 2 : 3};
+  for (int part = 0; part < kPlanParts; ++part) {
+    if (part == 1) PLAN_HIP(fork_lanes(ctx));
+    const hipStream_t ls = lane(ctx, lane_of_part[part]);
+    const int32_t *ord = L.ord + part * cap;
+    const int grid = std::max(1, gc[plan_list_part(part)]);
+    if (part < 3) {
+      PLAN_LAUNCH(ls, kid[part], ph, 1, part,
+                  launch_assemble(3 - part, ord, L.pre + part * (cap + 1), 0,
+                                  0, L.items, L.sets,
+                                  p->d_wigkeys.as<WigKey>(), gp,
+                                  p->d_tables.as<double>(),
+                                  p->d_wig.as<double>(), p->d_q.as<cdbl>(),
+                                  ls, &dph->part[part][0], grid));
+    } else {
+      PLAN_LAUNCH(ls, CTM_KERNEL_ASSEMBLE_PACK, ph, 1, part,
+                  launch_assemble_pack4(ord, 0, part == 4 ? 1 : 0, L.items,
+                                        L.sets, p->d_wigkeys.as<WigKey>(), gp,
+                                        p->d_tables.as<double>(),
+                                        p->d_wig.as<double>(),
+                                        p->d_q.as<cdbl>(), ls,
+                                        &dph->part[part][0], grid));
+    }
+    PLAN_HIP(hipEventRecord(ctx->ev_part[part], ls));
+  }
+  // solve buckets, placed and ordered as the host path places them
+  const int lane_of_bucket[kPlanBuckets] = {0, 3, 1, 3, 2, 2};
+  const bool m0_only = !want_t;
+  const bool mix = p->mixed && want_t;
+  bool after_big = false;
+  bool solved[kPlanBuckets] = {};
+  for (int b = 0; b < kPlanBuckets; ++b) {
+    const int lo_b = b + 1 < kPlanBuckets ? plan_bucket_hi(b + 1) : 0;
+    if (p->last_order <= lo_b) continue;  // no block of these orders
+    const int nm_max = std::min(plan_bucket_hi(b), p->last_order);
+    int64_t store;
+    int rows;
+    plan_solve_shape(nm_max, m0_only, &store, &rows);
+    const hipStream_t ls = lane(ctx, lane_of_bucket[b]);
+    if (after_big) PLAN_HIP(hipStreamWaitEvent(ls, ctx->ev_bucket, 0));
+    // the parts that can hold blocks of these orders
+    unsigned need = 0;
+    for (int nm = lo_b + 1; nm <= nm_max; ++nm) {
+      need |= 1u << block_part(nm, true);
+      if (!m0_only) need |= 1u << block_part(nm, false);
+    }
+    for (int part = 0; part < kPlanParts; ++part)
+      if (need >> part & 1u)
+        PLAN_HIP(hipStreamWaitEvent(ls, ctx->ev_part[part], 0));
+    const int32_t *tasks = L.sol + b * 2 * cap;
+    const int32_t *dn = &dph->bucket[b][0];
+    const int grid = std::max(1, gc[plan_list_bucket(b)]);
+    const bool mixb = mix && lo_b >= kMixedMinOrder - 1 &&
+                      nm_max <= kMixedMaxOrder;
+    if (mixb) {
+      PLAN_LAUNCH(ls, CTM_KERNEL_SOLVE, ph, 2, b,
+                  launch_invert32(tasks, 0, nm_max, store, rows, L.items,
+                                  p->d_q.as<cdbl>(), p->d_x32.as<cflt>(),
+                                  p->d_xexp.as<int32_t>(), ls, dn, grid));
+      PLAN_LAUNCH(ls, CTM_KERNEL_REFINE, ph, 0, b,
+                  launch_refine(tasks, 0, nm_max, L.items, p->d_q.as<cdbl>(),
+                                p->d_x32.as<cflt>(), p->d_xexp.as<int32_t>(),
+                                p->d_t.as<cdbl>(), p->d_rbuf.as<cdbl>(),
+                                p->d_qparts.as<double>(),
+                                L.flags + b * 2 * cap, ls, dn, grid));
+    } else {
+      PLAN_LAUNCH(ls, CTM_KERNEL_SOLVE, ph, 2, b,
+                  launch_invert_solve(tasks, 0, nm_max, store, rows, L.items,
+                                      p->d_q.as<cdbl>(),
+                                      want_t ? p->d_t.as<cdbl>() : nullptr,
+                                      p->d_qparts.as<double>(),
+                                      want_t ? 0 : 1, nullptr, nullptr, ls,
+                                      dn, grid));
+    }
+    if (want_t) {
+      PLAN_HIP(hipEventRecord(p->ev_solve[b], ls));
+      solved[b] = true;
+    }
+    if (b == 2 && want_t) {
+      // as the host path: the buckets of smaller blocks follow the blocks
+      // of order 49..64
+      PLAN_HIP(hipEventRecord(ctx->ev_bucket, ls));
+      after_big = true;
+    }
+  }
+  if (want_t && !mix && p->want_x) {
+    const int rc = plan_obs_x(p, solved, lane_of_bucket);
+    if (rc != CTM_OK) return rc;
+  }
+  PLAN_HIP(join_lanes(ctx));
+  if (mix) {
+    // blocks whose refinement stalled: FP64 solve of the compacted lists
+    int nmb = 0;
+    for (int b = 0; b < kPlanBuckets; ++b) {
+      const int lo_b = b + 1 < kPlanBuckets ? plan_bucket_hi(b + 1) : 0;
+      if (lo_b >= kMixedMinOrder - 1) nmb = b + 1;
+    }
+    PLAN_HIP(launch_plan_fallback(L, ph, nmb, s));
+    PLAN_HIP(fork_lanes(ctx));
+    for (int b = 0; b < nmb; ++b) {
+      const int lo_b = plan_bucket_hi(b + 1);
+      if (p->last_order <= lo_b) continue;
+      const int nm_max = std::min(plan_bucket_hi(b), p->last_order);
+      int64_t store;
+      int rows;
+      plan_solve_shape(nm_max, false, &store, &rows);
+      const hipStream_t ls = lane(ctx, lane_of_bucket[b]);
+      PLAN_LAUNCH(ls, CTM_KERNEL_SOLVE, ph, 3, b,
+                  launch_invert_solve(L.fb + b * 2 * cap, 0, nm_max, store,
+                                      rows, L.items, p->d_q.as<cdbl>(),
+                                      p->d_t.as<cdbl>(),
+                                      p->d_qparts.as<double>(), 0, nullptr,
+                                      nullptr, ls, &dph->fallback[b][0],
+                                      std::max(1,
+                                               gc[plan_list_bucket(b)])));
+    }
+    PLAN_HIP(join_lanes(ctx));
+  }
+  return CTM_OK;
+}
+
+// Enqueues one run of the plan on the context's stream (lanes joined at
+// the end).
+int plan_enqueue(ctm_plan *p) {
+  ctm_context *ctx = p->ctx;
+  const PlanLists &L = p->L;
+  hipStream_t s = ctx->stream;
+  const int grid_items = p->cus * 8;
+  PLAN_HIP(launch_plan_first(L, p->gp, s));
+  for (int r = 0; r < 3; ++r) {
+    PLAN_HIP(launch_plan_build(L, kPlanWindow, r, false, grid_items, s));
+    int rc = plan_phase(p, r, false);
+    if (rc != CTM_OK) return rc;
+    PLAN_LAUNCH(s, CTM_KERNEL_CONVERGE, r, 0, 0,
+                launch_plan_converge(L, r, p->d_qparts.as<double>(), p->tol,
+                                     s));
+    if (r < 2) PLAN_HIP(launch_plan_next(L, r == 1, s));
+  }
+  const int ph = 3;
+  PLAN_HIP(launch_plan_build(L, kPlanAllM, ph, p->want_x, grid_items, s));
+  int rc = plan_phase(p, ph, true);
+  if (rc != CTM_OK) return rc;
+  PLAN_LAUNCH(s, CTM_KERNEL_SUMS, ph, 0, 0,
+              launch_full_sums(p->n, L.gitem, p->d_iota.as<int32_t>(),
+                               p->d_qparts.as<double>(), p->out.qsca,
+                               p->out.qext, s));
+  PLAN_HIP(launch_write_orders(p->n, L.state, p->out.nmax, p->out.status, s));
+  if (p->want_x) {
+    if (p->mixed) {
+      // after the FP64 re-solves of the mixed path: every T block is final
+      PLAN_HIP(fork_lanes(ctx));
+      rc = plan_obs_x(p, nullptr, nullptr);
+      if (rc != CTM_OK) return rc;
+      PLAN_HIP(join_lanes(ctx));
+    }
+    const bool ext = p->out.obs_qext || p->out.obs_qextpol ||
+                     p->out.obs_qextcpol;
+    const bool ab = p->out.obs_qabs || p->out.obs_qabspol;
+    if (ext || ab)
+      PLAN_LAUNCH(s, CTM_KERNEL_OBS_SUM, ph, 0, 0,
+                  launch_obs_q(p->lay.sp, p->n, L.gobs, p->gp,
+                               p->d_x.as<cdbl>(), p->out.obs_qext,
+                               p->out.obs_qextpol, p->out.obs_qextcpol,
+                               p->out.obs_qabs, p->out.obs_qabspol, s));
+  }
+  return CTM_OK;
+}
+
+// (Re)allocates the data pools for the current capacities and refreshes
+// the device list descriptor.
+int plan_alloc(ctm_plan *p) {
+  const PlanCaps &c = p->caps;
+  PLAN_HIP(p->d_tables.ensure(std::max<int64_t>(c.tables, 1) * sizeof(double)));
+  PLAN_HIP(p->d_q.ensure(std::max<int64_t>(c.q, 1) * sizeof(cdbl)));
+  PLAN_HIP(p->d_t.ensure(std::max<int64_t>(c.t, 1) * sizeof(cdbl)));
+  if (p->want_x)
+    PLAN_HIP(p->d_x.ensure(std::max<int64_t>(c.x, 1) * sizeof(cdbl)));
+  if (p->mixed) {
+    PLAN_HIP(p->d_x32.ensure(std::max<int64_t>(c.t, 1) * sizeof(cflt)));
+    PLAN_HIP(p->d_xexp.ensure(std::max<int64_t>(c.t, 1) * sizeof(int32_t)));
+    PLAN_HIP(p->d_rbuf.ensure(std::max<int64_t>(c.t, 1) * sizeof(cdbl)));
+  }
+  // the pools may have been reallocated: the capacities the device checks
+  // against are the requested ones (DevBuf rounds up)
+  p->L.caps = c;
+  PLAN_HIP(p->d_gcap.ensure(sizeof(p->gcap)));
+  PLAN_HIP(hipMemcpy(p->d_gcap.p, p->gcap, sizeof(p->gcap),
+                     hipMemcpyHostToDevice));
+  p->L.gcap = p->d_gcap.as<int32_t>();
+  return CTM_OK;
+}
+
+// Grid bound from a list size: a quarter more, at least 64.
+int32_t plan_grid_of(int64_t units) {
+  return (int32_t)std::min<int64_t>(units + units / 4 + 64, 0x7fffffff);
+}
+
+// First grid bounds: the lists of the first window round (the device uses
+// the same windows), the all-m blocks with every grain at its first
+// window's last order (most grains converge inside it), and an eighth of
+// the first round's for the later rounds.
+void plan_first_grids(ctm_plan *p, const std::vector<int> &hi1) {
+  int64_t c[kPlanPhases][kPlanLists] = {};
+  const int glfac = p->glfac;
+  auto add = [&](int ph, int nm, bool m0) {
+    const int part = block_part(nm, m0);
+    if (part < 3) {
+      const int tr = 16 * (3 - part);
+      const int ntr = (nm + tr - 1) / tr;
+      c[ph][plan_list_part(part)] +=
+          (int64_t)ntr * (m0 ? (nm + 2 * tr - 1) / (2 * tr) : ntr);
+    } else {
+      c[ph][plan_list_part(part)] += 1;  // blocks; groups of four below
+    }
+    c[ph][plan_list_bucket(plan_bucket_of(nm))] += 2;
+    if (ph == 3) c[ph][plan_list_obs(plan_obs_of(nm))] += 1;
+  };
+  for (size_t g = 0; g < hi1.size(); ++g) {
+    for (int N = p->min_order; N <= hi1[g]; ++N) {
+      c[0][0] += glfac * N;
+      add(0, N, true);
+    }
+    const int N = hi1[g];
+    c[3][0] += glfac * N;
+    for (int m = 0; m <= N; ++m) add(3, N + 1 - (m > 0 ? m : 1), m == 0);
+  }
+  for (int ph : {0, 3})
+    for (int q = 3; q < 5; ++q)
+      c[ph][plan_list_part(q)] = (c[ph][plan_list_part(q)] + 3) / 4;
+  for (int k = 0; k < kPlanLists; ++k) {
+    p->gcap[0][k] = plan_grid_of(c[0][k]);
+    p->gcap[3][k] = plan_grid_of(c[3][k]);
+    p->gcap[1][k] = p->gcap[2][k] = plan_grid_of(c[0][k] / 8);
+  }
+}
+
+// Waits for the run, reads the status and phase records back and credits
+// the profiler records with the work the device lists hold.
+int plan_finish(ctm_plan *p) {
+  ctm_context *ctx = p->ctx;
+  {
+    void *dst[2] = {&p->st, p->ph};
+    const void *src[2] = {p->d_status.p, p->d_phase.p};
+    const size_t bytes[2] = {sizeof(PlanStatus),
+                             kPlanPhases * sizeof(PlanPhase)};
+    if (read_back(ctx, 2, dst, src, bytes) != hipSuccess) {
+      ctx->err = "plan: status read-back failed";
+      return CTM_ERR_HIP;
+    }
+  }
+  for (const ctm_plan::Rec &r : p->recs) {
+    float ms = 0.f;
+    const bool ok = hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess;
+    const PlanPhase &P = p->ph[r.phase];
+    double f = 0., b = 0.;
+    bool work = true;
+    switch (r.kind) {
+      case 1:
+        f = P.part_flops[r.idx];
+        b = P.part_bytes[r.idx];
+        work = P.part[r.idx][0] > 0;
+        break;
+      case 2:
+        f = P.bucket_flops[r.idx];
+        b = P.bucket_bytes[r.idx];
+        work = P.bucket[r.idx][0] > 0;
+        break;
+      case 3:
+        work = P.fallback[r.idx][0] > 0;
+        if (work && P.bucket[r.idx][0] > 0) {
+          const double share =
+              (double)P.fallback[r.idx][0] / P.bucket[r.idx][0];
+          f = share * P.bucket_flops[r.idx];
+          b = share * P.bucket_bytes[r.idx];
+        }
+        break;
+      case 4:
+        f = P.obs_flops[r.idx];
+        work = P.obs[r.idx][0] > 0;
+        break;
+      case 5:
+        b = 8. * (double)P.tables;
+        work = P.sets_nodes[0] > 0;
+        break;
+      default:
+        break;
+    }
+    if (ok && work && p->st.overflow == 0) {
+      ctx->prof.launches[r.kernel] += 1;
+      ctx->prof.ms[r.kernel] += ms;
+      ctx->prof.flops[r.kernel] += f;
+      ctx->prof.bytes[r.kernel] += b;
+    }
+    ctx->prof.pool.push_back(r.a);
+    ctx->prof.pool.push_back(r.b);
+  }
+  p->recs.clear();
+  return CTM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ctm_plan_create(ctm_context *ctx, int n, const double *radius,
+                    const double *axis_ratio, const double *wavelength,
+                    const double *mr_re, const double *mr_im,
+                    const ctm_convergence *conv,
+                    const ctm_observable_spec *obs,
+                    const ctm_plan_outputs *outputs, ctm_plan **out) {
+  if (!out) return CTM_ERR_ARGUMENT;
+  *out = nullptr;
+  int rc = check_inputs(ctx, n, radius, axis_ratio, wavelength, mr_re, mr_im);
+  if (rc != CTM_OK) return rc;
+  if (n == 0 || !outputs) {
+    ctx->err = "plan: no grains or no outputs";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (!conv || conv->min_order < 1 || conv->max_order <= conv->min_order + 1 ||
+      conv->glfac < 1 || !(conv->tolerance > 0.) ||
+      (conv->precision != CTM_PRECISION_FP64 &&
+       conv->precision != CTM_PRECISION_MIXED)) {
+    ctx->err = "invalid convergence parameters";
+    return CTM_ERR_ARGUMENT;
+  }
+  if ((int)conv->max_order - 1 > kMaxLdsOrder) {
+    ctx->err = "plan: the last order must be at most 99 (blocks above order "
+               "99 are solved in global scratch; use ctm_tmatrix_batch)";
+    return CTM_ERR_ARGUMENT;
+  }
+  if (obs && (obs->n_scatter_theta != 0 || obs->n_theta < 1 ||
+              !obs->theta ||
+              (obs->account_for_scattering && obs->ngauss < 1))) {
+    ctx->err = "plan: observables are Q_ext / Q_abs at n_theta >= 1 angles "
+               "(no scattering-matrix grid)";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  CTM_HIP(drain(ctx));
+  std::unique_ptr<ctm_plan> p(new ctm_plan());
+  p->ctx = ctx;
+  p->n = n;
+  p->gp = GrainParams{radius, axis_ratio, wavelength, mr_re, mr_im};
+  p->min_order = (int)conv->min_order;
+  p->last_order = (int)conv->max_order - 1;
+  p->glfac = (int)conv->glfac;
+  p->tol = conv->tolerance;
+  p->mixed = conv->precision == CTM_PRECISION_MIXED;
+  p->out = *outputs;
+  {
+    hipDeviceProp_t prop;
+    CTM_HIP(hipGetDeviceProperties(&prop, ctx->device));
+    p->cus = std::max(1, prop.multiProcessorCount);
+  }
+  for (hipEvent_t &e : p->ev_solve)
+    CTM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const int lo_n = p->min_order, hi_n = p->last_order, glfac = p->glfac;
+  // grain-independent tables of every order the plan can reach
+  std::vector<GlKey> glk;
+  std::vector<int64_t> gloff(hi_n + 1, 0);
+  int64_t gl_size = 0;
+  for (int N = lo_n; N <= hi_n; ++N) {
+    GlKey k{};
+    k.G = glfac * N;
+    k.off = gl_size;
+    gl_size += 2 * (int64_t)k.G;
+    gloff[N] = k.off;
+    glk.push_back(k);
+  }
+  std::vector<WigKey> wk;
+  std::vector<int32_t> wbase(hi_n + 1, 0);
+  int64_t wig_size = 0;
+  for (int N = lo_n; N <= hi_n; ++N) {
+    wbase[N] = (int32_t)wk.size();
+    for (int m = 0; m <= N; ++m) {
+      WigKey k{};
+      k.N = N;
+      k.m = m;
+      k.G = glfac * N;
+      k.gl_off = gloff[N];
+      k.off = wig_size;
+      wig_size += 2 * (int64_t)(N + 1 - (m > 0 ? m : 1)) * k.G;
+      wk.push_back(k);
+    }
+  }
+  ctm_plan *pp = p.get();
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
+    CTM_HIP(b.ensure(std::max<size_t>(bytes, 1)));
+    CTM_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return CTM_OK;
+  };
+  hipStream_t s = ctx->stream;
+  if ((rc = up(pp->d_glkeys, glk.data(), glk.size() * sizeof(GlKey))) ||
+      (rc = up(pp->d_wigkeys, wk.data(), wk.size() * sizeof(WigKey))) ||
+      (rc = up(pp->d_gloff, gloff.data(), gloff.size() * sizeof(int64_t))) ||
+      (rc = up(pp->d_wigbase, wbase.data(), wbase.size() * sizeof(int32_t))))
+    return rc;
+  CTM_HIP(pp->d_gl.ensure(gl_size * sizeof(double)));
+  CTM_HIP(pp->d_wig.ensure(wig_size * sizeof(double)));
+  CTM_HIP(launch_gauss_legendre(pp->d_glkeys.as<GlKey>(), (int)glk.size(),
+                                pp->d_gl.as<double>(), s));
+  CTM_HIP(launch_wigner(pp->d_wigkeys.as<WigKey>(), (int)wk.size(),
+                        pp->d_gl.as<double>(), pp->d_wig.as<double>(), s));
+  // observables: the direction layout and the pi/tau table up to the last
+  // order (both fixed for the plan)
+  if (obs) {
+    pp->want_x = true;
+    obs_layout(obs, pp->lay);
+    if (obs_x_lds_bytes(hi_n, pp->lay.sp.n_out, 8, 256, 1, true) >
+        kObsXMaxLds) {
+      ctx->err = "plan: too many outgoing directions for the last order";
+      return CTM_ERR_ARGUMENT;
+    }
+    if ((rc = up(pp->d_spec, pp->lay.blob.data(), pp->lay.blob.size())))
+      return rc;
+    pp->lay.bind(pp->d_spec.as<char>());
+    const int n_ang = pp->lay.n_ang;
+    CTM_HIP(pp->d_pt.ensure((size_t)(hi_n + 1) * n_ang * hi_n * 2 *
+                            sizeof(double)));
+    CTM_HIP(launch_pitau(
+        reinterpret_cast<const double *>(pp->d_spec.as<char>() +
+                                         pp->lay.o_ang),
+        n_ang, hi_n, pp->d_pt.as<double>(), s));
+    if (!obs->do_extinction) {
+      pp->out.obs_qext = pp->out.obs_qextpol = pp->out.obs_qextcpol = nullptr;
+    }
+    if (!obs->do_absorption) {
+      pp->out.obs_qabs = pp->out.obs_qabspol = nullptr;
+    }
+  } else {
+    pp->out.obs_qext = pp->out.obs_qextpol = pp->out.obs_qextcpol = nullptr;
+    pp->out.obs_qabs = pp->out.obs_qabspol = nullptr;
+  }
+  // lists (fixed capacity: every grain at every order)
+  const int64_t cap = (int64_t)n * (hi_n + 1);
+  pp->caps.items = cap;
+  const int gmax = glfac * hi_n;
+  CTM_HIP(pp->d_sets.ensure(cap * sizeof(TableSet)));
+  CTM_HIP(pp->d_items.ensure(cap * sizeof(Item)));
+  CTM_HIP(pp->d_ord.ensure(kPlanParts * cap * sizeof(int32_t)));
+  CTM_HIP(pp->d_pre.ensure(3 * (cap + 1) * sizeof(int32_t)));
+  CTM_HIP(pp->d_sol.ensure(kPlanBuckets * 2 * cap * sizeof(int32_t)));
+  CTM_HIP(pp->d_obsl.ensure(kPlanObsBuckets * cap * sizeof(int32_t)));
+  CTM_HIP(pp->d_flags.ensure(kPlanBuckets * 2 * cap * sizeof(int32_t)));
+  CTM_HIP(pp->d_fb.ensure(kPlanBuckets * 2 * cap * sizeof(int32_t)));
+  CTM_HIP(pp->d_hist.ensure(plan_hist_ints(gmax, hi_n) * sizeof(int32_t)));
+  CTM_HIP(pp->d_cnt.ensure((size_t)n * 6 * sizeof(int64_t)));
+  CTM_HIP(pp->d_goff.ensure((size_t)(n + 1) * 6 * sizeof(int64_t)));
+  CTM_HIP(pp->d_gitem.ensure((size_t)(n + 1) * sizeof(int32_t)));
+  CTM_HIP(pp->d_gobs.ensure((size_t)(n + 1) * sizeof(int32_t)));
+  CTM_HIP(pp->d_lo.ensure((size_t)n * sizeof(int32_t)));
+  CTM_HIP(pp->d_hi.ensure((size_t)n * sizeof(int32_t)));
+  CTM_HIP(pp->d_state.ensure((size_t)n * sizeof(ConvState)));
+  CTM_HIP(pp->d_phase.ensure(kPlanPhases * sizeof(PlanPhase)));
+  CTM_HIP(pp->d_status.ensure(sizeof(PlanStatus)));
+  CTM_HIP(pp->d_iota.ensure(cap * sizeof(int32_t)));
+  CTM_HIP(pp->d_qparts.ensure(cap * 2 * kQParts * sizeof(double)));
+  CTM_HIP(launch_plan_iota(pp->d_iota.as<int32_t>(), cap, s));
+  // first pools from the first windows (host restatement of the device
+  // rule; a wrong guess only costs one more run)
+  {
+    std::vector<double> h[4];
+    const double *src[4] = {radius, axis_ratio, wavelength, mr_re};
+    for (int a = 0; a < 4; ++a) {
+      h[a].resize(n);
+      CTM_HIP(hipMemcpy(h[a].data(), src[a], n * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    }
+    auto s1 = [](int64_t k) { return k > 0 ? k * (k + 1) / 2 : 0; };
+    auto s2 = [](int64_t k) { return k > 0 ? k * (k + 1) * (2 * k + 1) / 6 : 0; };
+    int64_t tab_w = 0, q_w = 0, tab_a = 0, q_a = 0, items_a = 0;
+    std::vector<int> hi1(n);
+    for (int g = 0; g < n; ++g) {
+      const int end = first_window_end(h[0][g], h[1][g], h[2][g], h[3][g]);
+      const int hi = std::min(hi_n, std::max(lo_n + 1, end));
+      hi1[g] = hi;
+      tab_w += glfac * ((int64_t)kNodeScalars * (s1(hi) - s1(lo_n - 1)) +
+                        (int64_t)kBesselArrays * (s2(hi) - s2(lo_n - 1)));
+      q_w += 4 * (s2(hi) - s2(lo_n - 1));
+      const int64_t N = hi, G = glfac * N;
+      tab_a += (int64_t)kNodeScalars * G + (int64_t)kBesselArrays * N * G;
+      q_a += 4 * (N * N + s2(N));
+      items_a += N + 1;
+    }
+    auto pad = [](int64_t v) { return v + v / 8 + (1 << 20); };
+    pp->caps.tables = pad(std::max(tab_w, tab_a));
+    pp->caps.q = pad(std::max(q_w, q_a));
+    pp->caps.t = pad(q_a / 2);
+    pp->caps.x = pp->want_x
+                     ? pad(items_a * (int64_t)pp->lay.sp.n_pairs * 4)
+                     : 0;
+    plan_first_grids(pp, hi1);
+  }
+  PlanLists &L = pp->L;
+  L.sets = pp->d_sets.as<TableSet>();
+  L.items = pp->d_items.as<Item>();
+  L.ord = pp->d_ord.as<int32_t>();
+  L.pre = pp->d_pre.as<int32_t>();
+  L.sol = pp->d_sol.as<int32_t>();
+  L.obsl = pp->d_obsl.as<int32_t>();
+  L.flags = pp->d_flags.as<int32_t>();
+  L.fb = pp->d_fb.as<int32_t>();
+  L.hist = pp->d_hist.as<int32_t>();
+  L.cnt = pp->d_cnt.as<int64_t>();
+  L.goff = pp->d_goff.as<int64_t>();
+  L.gitem = pp->d_gitem.as<int32_t>();
+  L.gobs = pp->d_gobs.as<int32_t>();
+  L.lo = pp->d_lo.as<int32_t>();
+  L.hi = pp->d_hi.as<int32_t>();
+  L.state = pp->d_state.as<ConvState>();
+  L.gl_off = pp->d_gloff.as<int64_t>();
+  L.wig_base = pp->d_wigbase.as<int32_t>();
+  L.phase = pp->d_phase.as<PlanPhase>();
+  L.status = pp->d_status.as<PlanStatus>();
+  L.n = n;
+  L.glfac = glfac;
+  L.min_order = lo_n;
+  L.last_order = hi_n;
+  L.gmax = gmax;
+  L.n_in = pp->want_x ? pp->lay.sp.n_in : 0;
+  L.n_pairs = pp->want_x ? pp->lay.sp.n_pairs : 0;
+  if ((rc = plan_alloc(pp)) != CTM_OK) return rc;
+  CTM_HIP(hipStreamSynchronize(s));
+  ctx->plans += 1;
+  *out = p.release();
+  return CTM_OK;
+}
+
+int ctm_plan_run(ctm_plan *p) {
+  if (!p) return CTM_ERR_ARGUMENT;
+  ctm_context *ctx = p->ctx;
+  CTM_HIP(hipSetDevice(ctx->device));
+  ctx->err.clear();
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    int rc = plan_enqueue(p);
+    if (rc == CTM_OK) rc = plan_finish(p);
+    if (rc != CTM_OK) return rc;
+    p->runs += 1;
+    if (p->st.overflow == 0) {
+      // the last batch's block counts, as after ctm_tmatrix_batch
+      ctx->mixed_blocks = p->st.n_mixed;
+      ctx->mixed_fallback = p->st.n_fallback;
+      if (!p->gcap_fitted) {
+        // grids fitted to the lists of this batch (a later batch whose
+        // lists outgrow them is run again with larger grids)
+        for (int ph = 0; ph < kPlanPhases; ++ph)
+          for (int k = 0; k < kPlanLists; ++k)
+            p->gcap[ph][k] = plan_grid_of(p->st.count[ph][k]);
+        p->gcap_fitted = true;
+        rc = plan_alloc(p);
+        if (rc != CTM_OK) return rc;
+      }
+      return CTM_OK;
+    }
+    if (p->st.overflow & kPlanOverflowGrid) {
+      for (int ph = 0; ph < kPlanPhases; ++ph)
+        for (int k = 0; k < kPlanLists; ++k)
+          if (p->st.count[ph][k] > p->gcap[ph][k])
+            p->gcap[ph][k] = plan_grid_of(p->st.count[ph][k]);
+    }
+    // grow the pools to what the device reported and run again
+    if (p->st.overflow & kPlanOverflowNodes) {
+      ctx->err = "plan: more than 2^31 quadrature nodes in one phase";
+      return CTM_ERR_MEMORY;
+    }
+    auto grow = [](int64_t cur, int64_t need) {
+      return need > cur ? need + need / 4 : cur;
+    };
+    p->caps.tables = grow(p->caps.tables, p->st.need_tables);
+    p->caps.q = grow(p->caps.q, p->st.need_q);
+    p->caps.t = grow(p->caps.t, p->st.need_t);
+    p->caps.x = grow(p->caps.x, p->st.need_x);
+    rc = plan_alloc(p);
+    if (rc != CTM_OK) return rc;
+    p->reruns += 1;
+  }
+  ctx->err = "plan: the memory pools did not settle";
+  return CTM_ERR_MEMORY;
+}
+
+int ctm_plan_info_get(ctm_plan *p, ctm_plan_info *info) {
+  if (!p || !info) return CTM_ERR_ARGUMENT;
+  ctm_plan_info i{};
+  i.runs = p->runs;
+  i.reruns = p->reruns;
+  i.mixed_blocks = p->st.n_mixed;
+  i.mixed_fallback = p->st.n_fallback;
+  i.pool_bytes = (int64_t)(p->d_tables.cap + p->d_q.cap + p->d_t.cap +
+                           p->d_x.cap + p->d_x32.cap + p->d_xexp.cap +
+                           p->d_rbuf.cap);
+  for (int r = 0; r < 3; ++r) i.window_items[r] = p->ph[r].n_items;
+  i.allm_items = p->ph[3].n_items;
+  *info = i;
+  return CTM_OK;
+}
+
+int ctm_plan_destroy(ctm_plan *p) {
+  if (!p) return CTM_OK;
+  ctm_context *ctx = p->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)drain(ctx);
+  DevBuf *bufs[] = {&p->d_gl, &p->d_glkeys, &p->d_wig, &p->d_wigkeys,
+                    &p->d_gloff, &p->d_wigbase, &p->d_spec, &p->d_pt,
+                    &p->d_sets, &p->d_items, &p->d_ord, &p->d_pre,
+                    &p->d_sol, &p->d_obsl, &p->d_flags, &p->d_fb,
+                    &p->d_hist, &p->d_cnt, &p->d_goff, &p->d_gitem,
+                    &p->d_gobs, &p->d_lo, &p->d_hi, &p->d_state,
+                    &p->d_phase, &p->d_status, &p->d_iota, &p->d_tables,
+                    &p->d_q, &p->d_t, &p->d_x, &p->d_qparts, &p->d_x32,
+                    &p->d_xexp, &p->d_rbuf, &p->d_gcap};
+  for (DevBuf *b : bufs) b->release();
+  for (hipEvent_t e : p->ev_solve)
+    if (e) (void)hipEventDestroy(e);
+  for (const ctm_plan::Rec &r : p->recs) {
+    ctx->prof.pool.push_back(r.a);
+    ctx->prof.pool.push_back(r.b);
+  }
+  ctx->plans -= 1;
+  delete p;
   return CTM_OK;
 }
 

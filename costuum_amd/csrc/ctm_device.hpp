@@ -161,4 +161,46 @@ struct ConvState {
   int32_t pad[3];
 };
 
+// The reference's sequential convergence test (TMatrixResource.hpp:743-775)
+// over the m = 0 items q0 .. q1 - 1 of one grain in ascending order (their
+// indices win_items[q], or q itself when win_items is NULL), continuing from
+// the grain's state st; stops at the first order that passes or at
+// last_order.
+__device__ inline void converge_items(ConvState &st, int q0, int q1,
+                                      const int32_t *win_items,
+                                      const Item *items,
+                                      const double *qparts, double tol,
+                                      int last_order) {
+  for (int q = q0; q < q1; ++q) {
+    const int ii = win_items ? win_items[q] : q;
+    const Item it = items[ii];
+    const double *p0 = qparts + (int64_t)(2 * ii) * kQParts;
+    const double *p1 = p0 + kQParts;
+    const double qs = p0[0] + p1[0];
+    const double qe = p0[1] + p1[1];
+    if (st.first) {
+      st.first = 0;
+      st.dqs = -1.;
+      st.dqe = -1.;
+    } else {
+      st.dqs = fabs((st.old_s - qs) / qs);
+      st.dqe = fabs((st.old_e - qe) / qe);
+    }
+    st.old_s = qs;
+    st.old_e = qe;
+    st.nmax = it.N;
+    st.item = ii;
+    if (st.dqs > 0. && st.dqs <= tol && st.dqe <= tol) {
+      st.done = 1;
+      st.status = 0;
+      break;
+    }
+    if (it.N >= last_order) {
+      st.done = 1;
+      st.status = 1;
+      break;
+    }
+  }
+}
+
 }  // namespace ctm

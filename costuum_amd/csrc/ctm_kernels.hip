@@ -165,12 +165,19 @@ __global__ void k_wigner(const WigKey *keys, const double *gl, double *wig) {
 // functions are stored (the readers form the derivatives from neighbouring
 // orders): j_0, y_0 and j_0(m kr) go with the node scalars.
 constexpr double kMillerFloor = 1e-280;
+// dn (device list counts, may be NULL): the set and node counts are read
+// from dn[0], dn[1] (the grid covers at least the nodes).
 __global__ __launch_bounds__(128) void k_tables(const TableSet *sets,
                                                 int n_sets, int node_base,
                                                 int n_nodes,
                                                 const GrainParams gp,
                                                 const double *gl,
-                                                double *tables) {
+                                                double *tables,
+                                                const int32_t *dn) {
+  if (dn) {
+    n_sets = dn[0];
+    n_nodes = dn[1];
+  }
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_nodes) return;
   const int tid = node_base + t;
@@ -642,12 +649,15 @@ __device__ __forceinline__ void store_pairs(
   }
 }
 
+// One workgroup per tile.  dn (device list counts, may be NULL): the item
+// and tile counts are read from dn[0], dn[1]; workgroups beyond the tiles
+// return at once.
 template <int MT>
 __global__ __launch_bounds__(256) void k_assemble(
     const int32_t *order, const int32_t *tile_prefix, int n_order,
     const Item *items, const TableSet *sets, const WigKey *wkeys,
     const GrainParams gp, const double *tables, const double *wig,
-    cdbl *qbuf) {
+    cdbl *qbuf, const int32_t *dn) {
   constexpr int TR = 16 * MT;            // rows of pairs per tile
   // quadrature nodes per LDS stage: two blocks per CU for MT = 2, 3 (their
   // registers allow no more); MT = 1 (120 registers) runs four blocks per
@@ -657,6 +667,10 @@ __global__ __launch_bounds__(256) void k_assemble(
   __shared__ double Rsh[CH][2 * TR][kRightStride];
 
   const int bid = blockIdx.x;
+  if (dn) {
+    if (bid >= dn[1]) return;
+    n_order = dn[0];
+  }
   int lo = 0, hi = n_order;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -794,14 +808,20 @@ struct PackBlock {
   int nm, nmin, G, m, item, pad;
 };
 
+// dn (may be NULL): the block count is read from dn[0]; workgroups beyond
+// its groups of four return at once.
 __global__ __launch_bounds__(256, 2) void k_assemble_pack4(
     const int32_t *order, int n_order, int m0, const Item *items,
     const TableSet *sets, const WigKey *wkeys, const GrainParams gp,
-    const double *tables, const double *wig, cdbl *qbuf) {
+    const double *tables, const double *wig, cdbl *qbuf, const int32_t *dn) {
   constexpr int MT = 3, CH = 2, NB = 4, PR = kPackRows;
   __shared__ double Lsh[CH][NB * PR][kLeftStride];
   __shared__ double Rsh[CH][NB * PR][kRightStride];
   __shared__ PackBlock blk[NB];
+  if (dn) {
+    n_order = dn[0];
+    if ((int)blockIdx.x * NB >= n_order) return;
+  }
   const int t = threadIdx.x;
   if (t < NB) {
     const int q = blockIdx.x * NB + t;
@@ -1122,10 +1142,12 @@ __device__ __forceinline__ void solve_tail(
 // (3 nm values); the pivot value of the next step travels with the
 // (magnitude, row) reduction.  The inverse is stored
 // to LDS once at the end for the product.
+// One workgroup per task.  dn (may be NULL): the task count is read from
+// dn[0]; workgroups beyond it return at once.
 template <int NT, int TCC, int RPT, int CPT>
 __global__ __launch_bounds__(NT) void k_invert_solve_reg(
     const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
-    double *qparts, int diag_only, cdbl *ainv_out) {
+    double *qparts, int diag_only, cdbl *ainv_out, const int32_t *dn) {
   constexpr int NW = NT / 64;
   constexpr int TRR = NT / TCC;
   extern __shared__ double smem[];
@@ -1134,6 +1156,7 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
   __shared__ cdbl wb_val[NW][2];
   __shared__ double red[NW][4];
 
+  if (dn && (int)blockIdx.x >= dn[0]) return;
   const int task = tasks[blockIdx.x];
   const int item_index = task >> 1;
   const int cls = task & 1;
@@ -1369,17 +1392,20 @@ __global__ __launch_bounds__(NT) void k_invert_solve_reg(
 // GLOBAL: the sub-blocks live in gscratch (gstride complex values per
 // workgroup) instead of LDS, for blocks above the LDS capacity; the panel
 // and update passes then stream the matrix through L2 once per W columns.
+// dn (may be NULL): the task count is read from dn[0]; workgroups beyond
+// it return at once.
 template <int NT, int RS, int W, bool GLOBAL>
 __global__ __launch_bounds__(NT) void k_invert_solve_blk(
     const int32_t *tasks, const Item *items, const cdbl *qbuf, cdbl *tbuf,
     double *qparts, int diag_only, cdbl *gscratch, int64_t gstride,
-    cdbl *ainv_out) {
+    cdbl *ainv_out, const int32_t *dn) {
   constexpr int NW = NT / 64;
   static_assert(NW >= 4, "two waves per sub-block at least");
   extern __shared__ double smem[];
   __shared__ int stp[2][2][W];
   __shared__ double red[NW][4];
 
+  if (dn && (int)blockIdx.x >= dn[0]) return;
   const int task = tasks[blockIdx.x];
   const int item_index = task >> 1;
   const int cls = task & 1;
@@ -1477,36 +1503,8 @@ __global__ void k_converge(const int32_t *active, int n_active,
   const int g = active[a];
   ConvState st = state[g];
   if (st.done) return;
-  for (int q = win_off[a]; q < win_off[a + 1]; ++q) {
-    const int ii = win_items[q];
-    const Item it = items[ii];
-    const double *p0 = qparts + (int64_t)(2 * ii) * kQParts;
-    const double *p1 = p0 + kQParts;
-    const double qs = p0[0] + p1[0];
-    const double qe = p0[1] + p1[1];
-    if (st.first) {
-      st.first = 0;
-      st.dqs = -1.;
-      st.dqe = -1.;
-    } else {
-      st.dqs = fabs((st.old_s - qs) / qs);
-      st.dqe = fabs((st.old_e - qe) / qe);
-    }
-    st.old_s = qs;
-    st.old_e = qe;
-    st.nmax = it.N;
-    st.item = ii;
-    if (st.dqs > 0. && st.dqs <= tol && st.dqe <= tol) {
-      st.done = 1;
-      st.status = 0;
-      break;
-    }
-    if (it.N >= last_order) {
-      st.done = 1;
-      st.status = 1;
-      break;
-    }
-  }
+  converge_items(st, win_off[a], win_off[a + 1], win_items, items, qparts,
+                 tol, last_order);
   state[g] = st;
 }
 
@@ -1557,10 +1555,12 @@ hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
 
 hipError_t launch_tables(const TableSet *sets, int n, int node_base,
                          int n_nodes, const GrainParams &gp, const double *gl,
-                         double *tables, hipStream_t s) {
-  if (n <= 0 || n_nodes <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tables, dim3((n_nodes + 127) / 128), dim3(128), 0, s,
-                     sets, n, node_base, n_nodes, gp, gl, tables);
+                         double *tables, hipStream_t s, const int32_t *dn,
+                         int grid) {
+  if (!dn && (n <= 0 || n_nodes <= 0)) return hipSuccess;
+  hipLaunchKernelGGL(k_tables, dim3(dn ? grid : (n_nodes + 127) / 128),
+                     dim3(128), 0, s, sets, n, node_base, n_nodes, gp, gl,
+                     tables, dn);
   return hipGetLastError();
 }
 
@@ -1569,20 +1569,22 @@ hipError_t launch_assemble(int micro, const int32_t *order,
                            int n_tiles, const Item *items,
                            const TableSet *sets, const WigKey *wkeys,
                            const GrainParams &gp, const double *tables,
-                           const double *wig, cdbl *qbuf, hipStream_t s) {
-  if (n_tiles <= 0) return hipSuccess;
+                           const double *wig, cdbl *qbuf, hipStream_t s,
+                           const int32_t *dn, int grid) {
+  if (!dn && n_tiles <= 0) return hipSuccess;
+  const dim3 g(dn ? grid : n_tiles);
   if (micro == 3)
-    hipLaunchKernelGGL(k_assemble<3>, dim3(n_tiles), dim3(256), 0, s, order,
-                       tile_prefix, n_order, items, sets, wkeys, gp, tables,
-                       wig, qbuf);
+    hipLaunchKernelGGL(k_assemble<3>, g, dim3(256), 0, s, order, tile_prefix,
+                       n_order, items, sets, wkeys, gp, tables, wig, qbuf,
+                       dn);
   else if (micro == 2)
-    hipLaunchKernelGGL(k_assemble<2>, dim3(n_tiles), dim3(256), 0, s, order,
-                       tile_prefix, n_order, items, sets, wkeys, gp, tables,
-                       wig, qbuf);
+    hipLaunchKernelGGL(k_assemble<2>, g, dim3(256), 0, s, order, tile_prefix,
+                       n_order, items, sets, wkeys, gp, tables, wig, qbuf,
+                       dn);
   else
-    hipLaunchKernelGGL(k_assemble<1>, dim3(n_tiles), dim3(256), 0, s, order,
-                       tile_prefix, n_order, items, sets, wkeys, gp, tables,
-                       wig, qbuf);
+    hipLaunchKernelGGL(k_assemble<1>, g, dim3(256), 0, s, order, tile_prefix,
+                       n_order, items, sets, wkeys, gp, tables, wig, qbuf,
+                       dn);
   return hipGetLastError();
 }
 
@@ -1590,11 +1592,12 @@ hipError_t launch_assemble_pack4(const int32_t *order, int n_order, int m0,
                                  const Item *items, const TableSet *sets,
                                  const WigKey *wkeys, const GrainParams &gp,
                                  const double *tables, const double *wig,
-                                 cdbl *qbuf, hipStream_t s) {
-  if (n_order <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_assemble_pack4, dim3((n_order + 3) / 4), dim3(256), 0,
-                     s, order, n_order, m0, items, sets, wkeys, gp, tables,
-                     wig, qbuf);
+                                 cdbl *qbuf, hipStream_t s,
+                                 const int32_t *dn, int grid) {
+  if (!dn && n_order <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_assemble_pack4, dim3(dn ? grid : (n_order + 3) / 4),
+                     dim3(256), 0, s, order, n_order, m0, items, sets, wkeys,
+                     gp, tables, wig, qbuf, dn);
   return hipGetLastError();
 }
 
@@ -1607,7 +1610,8 @@ template <int NT, int TCC, int RPT, int CPT>
 static hipError_t launch_solve_reg(const int32_t *tasks, int n, size_t lds,
                                    const Item *items, const cdbl *qbuf,
                                    cdbl *tbuf, double *qparts, int diag_only,
-                                   cdbl *ainv_out, hipStream_t s) {
+                                   cdbl *ainv_out, hipStream_t s,
+                                   const int32_t *dn, int grid) {
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
@@ -1616,9 +1620,9 @@ static hipError_t launch_solve_reg(const int32_t *tasks, int n, size_t lds,
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_invert_solve_reg<NT, TCC, RPT, CPT>), dim3(n),
-                     dim3(NT), lds, s, tasks, items, qbuf, tbuf, qparts,
-                     diag_only, ainv_out);
+  hipLaunchKernelGGL((k_invert_solve_reg<NT, TCC, RPT, CPT>),
+                     dim3(dn ? grid : n), dim3(NT), lds, s, tasks, items,
+                     qbuf, tbuf, qparts, diag_only, ainv_out, dn);
   return hipGetLastError();
 }
 
@@ -1627,6 +1631,7 @@ static hipError_t launch_solve_blk(const int32_t *tasks, int n, size_t lds,
                                    const Item *items, const cdbl *qbuf,
                                    cdbl *tbuf, double *qparts, int diag_only,
                                    cdbl *ainv_out, hipStream_t s,
+                                   const int32_t *dn, int grid,
                                    cdbl *gscratch = nullptr,
                                    int64_t gstride = 0) {
   static bool attr_set = false;
@@ -1637,9 +1642,10 @@ static hipError_t launch_solve_blk(const int32_t *tasks, int n, size_t lds,
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_invert_solve_blk<NT, RS, W, GLOBAL>), dim3(n),
-                     dim3(NT), lds, s, tasks, items, qbuf, tbuf, qparts,
-                     diag_only, gscratch, gstride, ainv_out);
+  hipLaunchKernelGGL((k_invert_solve_blk<NT, RS, W, GLOBAL>),
+                     dim3(dn ? grid : n), dim3(NT), lds, s, tasks, items,
+                     qbuf, tbuf, qparts, diag_only, gscratch, gstride,
+                     ainv_out, dn);
   return hipGetLastError();
 }
 
@@ -1647,8 +1653,9 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                int64_t store_max, int rows_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
-                               cdbl *ainv_out, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
+                               cdbl *ainv_out, hipStream_t s,
+                               const int32_t *dn, int grid) {
+  if (!dn && n <= 0) return hipSuccess;
   if (nm_max > kMaxLdsOrder) {
     // the matrix lives in gscratch (n * nm_max^2 complex values): blocked
     // elimination with four rows per lane (sub-blocks up to 256 rows), the
@@ -1657,7 +1664,8 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
       return hipErrorInvalidValue;
     return launch_solve_blk<512, 4, 4, true>(
         tasks, n, 2 * (size_t)nm_max * sizeof(int), items, qbuf, tbuf, qparts,
-        diag_only, ainv_out, s, gscratch, (int64_t)nm_max * nm_max);
+        diag_only, ainv_out, s, dn, grid, gscratch,
+        (int64_t)nm_max * nm_max);
   }
   // the blocked kernel stores the sub-blocks compactly (store_max complex
   // values at most) and needs two rows per lane above 64 rows
@@ -1667,21 +1675,26 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
   const size_t lds = invert_solve_lds_bytes(nm_max);
   if (nm_max <= 16)
     return launch_solve_reg<64, 8, 2, 2>(tasks, n, lds, items, qbuf, tbuf,
-                                         qparts, diag_only, ainv_out, s);
+                                         qparts, diag_only, ainv_out, s, dn,
+                                       grid);
   // above order 16: blocked elimination in LDS with panels of 4 columns
   // (measured faster than 2, 6, 8 or 16), four waves per block up to order
   // 48, eight above, two rows per lane for sub-blocks above 64 rows
   if (nm_max <= 48)
     return launch_solve_blk<256, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
-                                       qparts, diag_only, ainv_out, s);
+                                       qparts, diag_only, ainv_out, s, dn,
+                                       grid);
   if (rows_max <= 32)  // m = 0 blocks up to order 64
     return launch_solve_blk<256, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
-                                       qparts, diag_only, ainv_out, s);
+                                       qparts, diag_only, ainv_out, s, dn,
+                                       grid);
   if (rows_max <= 64)
     return launch_solve_blk<512, 1, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
-                                       qparts, diag_only, ainv_out, s);
+                                       qparts, diag_only, ainv_out, s, dn,
+                                       grid);
   return launch_solve_blk<512, 2, 4>(tasks, n, lds_blk, items, qbuf, tbuf,
-                                     qparts, diag_only, ainv_out, s);
+                                     qparts, diag_only, ainv_out, s, dn,
+                                     grid);
 }
 
 hipError_t launch_converge(const int32_t *active, int n_active,

@@ -18,11 +18,20 @@ hipError_t launch_gauss_legendre(const GlKey *keys, int n, double *gl,
                                  hipStream_t s);
 hipError_t launch_wigner(const WigKey *keys, int n, const double *gl,
                          double *wig, hipStream_t s);
+// Device counts: every launcher of a per-item, per-tile or per-task kernel
+// takes an optional `dn` (device pointer to the list's counts, as each
+// kernel documents) and a grid size `grid` that bounds the list (the
+// caller guarantees the count fits); the counts passed by value are then
+// ignored and the workgroups (threads for k_tables) beyond the list return
+// at once.  Lists built on the device run without the host knowing their
+// sizes.
+//
 // sets: the launch's sets, whose node offsets start at node_base; n_nodes:
-// the nodes of those sets.
+// the nodes of those sets (dn: [n_sets, n_nodes]).
 hipError_t launch_tables(const TableSet *sets, int n, int node_base,
                          int n_nodes, const GrainParams &gp,
-                         const double *gl, double *tables, hipStream_t s);
+                         const double *gl, double *tables, hipStream_t s,
+                         const int32_t *dn = nullptr, int grid = 0);
 // micro = 1: 16 x 16 pair tiles (blocks up to order kSmallBlock);
 // micro = 2: 32 x 32 tiles of 2 x 2 pairs per thread; micro = 3: 48 x 48
 // tiles of 3 x 3 pairs per thread.
@@ -35,13 +44,14 @@ hipError_t launch_assemble_pack4(const int32_t *order, int n_order, int m0,
                                  const Item *items, const TableSet *sets,
                                  const WigKey *wkeys, const GrainParams &gp,
                                  const double *tables, const double *wig,
-                                 cdbl *qbuf, hipStream_t s);
+                                 cdbl *qbuf, hipStream_t s,
+                                 const int32_t *dn = nullptr, int grid = 0);
 // Micro-tile of a block of order nm: 4 (four blocks per workgroup on
 // kPackRows-row tiles) from kPackMin to kPackRows, 1 below, else 3 (48-row
 // tiles) where their padded pair count, at half the cost per pair of
 // 32-row tiles (measured: the 3 x 3 micro-tile reads fewer LDS operands per
 // pair), is smaller, else 2.  (m = 0 tiles are twice as wide.)
-inline int assemble_micro(int nm, bool m0) {
+__host__ __device__ inline int assemble_micro(int nm, bool m0) {
   if (nm >= kPackMin && nm <= kPackRows) return 4;
   if (nm <= kSmallBlock) return 1;
   const long r32 = (nm + 31) / 32, c32 = m0 ? (nm + 63) / 64 : r32;
@@ -54,7 +64,8 @@ hipError_t launch_assemble(int micro, const int32_t *order,
                            int n_tiles, const Item *items,
                            const TableSet *sets, const WigKey *wkeys,
                            const GrainParams &gp, const double *tables,
-                           const double *wig, cdbl *qbuf, hipStream_t s);
+                           const double *wig, cdbl *qbuf, hipStream_t s,
+                           const int32_t *dn = nullptr, int grid = 0);
 size_t invert_solve_lds_bytes(int nm);
 // Largest block order the LDS-resident inversion holds.
 constexpr int kMaxLdsOrder = 99;
@@ -67,7 +78,8 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                int64_t store_max, int rows_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
-                               cdbl *ainv_out, hipStream_t s);
+                               cdbl *ainv_out, hipStream_t s,
+                               const int32_t *dn = nullptr, int grid = 0);
 hipError_t launch_converge(const int32_t *active, int n_active,
                            const int32_t *win_off, const int32_t *win_items,
                            const Item *items, const double *qparts, double tol,

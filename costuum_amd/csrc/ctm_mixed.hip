@@ -41,17 +41,21 @@ __device__ inline double wave_max_f64(double v) {
   return v;
 }
 
+// One workgroup per task.  dn (may be NULL): the task count is read from
+// dn[0]; workgroups beyond it return at once.
 template <int NT, int RS, int W>
 __global__ __launch_bounds__(NT) void k_invert32_blk(const int32_t *tasks,
                                                      const Item *items,
                                                      const cdbl *qbuf,
                                                      cflt *xbuf,
-                                                     int32_t *xexp) {
+                                                     int32_t *xexp,
+                                                     const int32_t *dn) {
   constexpr int NW = NT / 64;
   static_assert(NW >= 4, "two waves per sub-block at least");
   extern __shared__ double smem[];
   __shared__ int stp[2][2][W];
 
+  if (dn && (int)blockIdx.x >= dn[0]) return;
   const int task = tasks[blockIdx.x];
   const int cls = task & 1;
   const Item it = items[task >> 1];
@@ -245,11 +249,13 @@ __device__ __forceinline__ double block_max(double v, double *red) {
   return m;
 }
 
+// One workgroup per task; flags[i] belongs to tasks[i].  dn (may be NULL):
+// the task count is read from dn[0]; workgroups beyond it return at once.
 template <int NT, int TX, int RY, int RX>
 __global__ __launch_bounds__(NT) void k_refine(
     const int32_t *tasks, const Item *items, const cdbl *qbuf,
     const cflt *xbuf, const int32_t *xexp, cdbl *tbuf, cdbl *rbuf,
-    double *qparts, int32_t *flags) {
+    double *qparts, int32_t *flags, const int32_t *dn) {
   constexpr int TY = NT / TX, MR = TY * RY, MC = TX * RX, KC = 8;
   constexpr int NW = NT / 64;
   __shared__ cdbl As[KC][MR + 1];
@@ -258,6 +264,7 @@ __global__ __launch_bounds__(NT) void k_refine(
   __shared__ double red[NW];
   __shared__ double red4[NW][4];
 
+  if (dn && (int)blockIdx.x >= dn[0]) return;
   const int task = tasks[blockIdx.x];
   const int cls = task & 1;
   const Item it = items[task >> 1];
@@ -395,7 +402,8 @@ __global__ __launch_bounds__(NT) void k_refine(
 template <int NT, int RS, int W>
 hipError_t launch_inv32(const int32_t *tasks, int n, size_t lds,
                         const Item *items, const cdbl *qbuf, cflt *xbuf,
-                        int32_t *xexp, hipStream_t s) {
+                        int32_t *xexp, hipStream_t s, const int32_t *dn,
+                        int grid) {
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
@@ -404,8 +412,8 @@ hipError_t launch_inv32(const int32_t *tasks, int n, size_t lds,
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_invert32_blk<NT, RS, W>), dim3(n), dim3(NT), lds, s,
-                     tasks, items, qbuf, xbuf, xexp);
+  hipLaunchKernelGGL((k_invert32_blk<NT, RS, W>), dim3(dn ? grid : n),
+                     dim3(NT), lds, s, tasks, items, qbuf, xbuf, xexp, dn);
   return hipGetLastError();
 }
 
@@ -413,10 +421,11 @@ template <int NT, int TX, int RY, int RX>
 hipError_t launch_ref(const int32_t *tasks, int n, const Item *items,
                       const cdbl *qbuf, const cflt *xbuf,
                       const int32_t *xexp, cdbl *tbuf, cdbl *rbuf,
-                      double *qparts, int32_t *flags, hipStream_t s) {
-  hipLaunchKernelGGL((k_refine<NT, TX, RY, RX>), dim3(n), dim3(NT), 0, s,
-                     tasks, items, qbuf, xbuf, xexp, tbuf, rbuf, qparts,
-                     flags);
+                      double *qparts, int32_t *flags, hipStream_t s,
+                      const int32_t *dn, int grid) {
+  hipLaunchKernelGGL((k_refine<NT, TX, RY, RX>), dim3(dn ? grid : n),
+                     dim3(NT), 0, s, tasks, items, qbuf, xbuf, xexp, tbuf,
+                     rbuf, qparts, flags, dn);
   return hipGetLastError();
 }
 
@@ -429,34 +438,37 @@ size_t invert32_lds_bytes(int64_t store_max, int nm_max) {
 hipError_t launch_invert32(const int32_t *tasks, int n, int nm_max,
                            int64_t store_max, int rows_max, const Item *items,
                            const cdbl *qbuf, cflt *xbuf, int32_t *xexp,
-                           hipStream_t s) {
-  if (n <= 0) return hipSuccess;
+                           hipStream_t s, const int32_t *dn, int grid) {
+  if (!dn && n <= 0) return hipSuccess;
   if (nm_max < 1 || nm_max > kMixedMaxOrder || rows_max > 2 * 64 ||
       store_max > (int64_t)nm_max * blk_lds_stride(nm_max))
     return hipErrorInvalidValue;
   const size_t lds = invert32_lds_bytes(store_max, nm_max);
   if (nm_max <= 48 || rows_max <= 32)
-    return launch_inv32<256, 1, 4>(tasks, n, lds, items, qbuf, xbuf, xexp, s);
+    return launch_inv32<256, 1, 4>(tasks, n, lds, items, qbuf, xbuf, xexp, s,
+                                   dn, grid);
   if (rows_max <= 64)
-    return launch_inv32<512, 1, 4>(tasks, n, lds, items, qbuf, xbuf, xexp, s);
-  return launch_inv32<512, 2, 4>(tasks, n, lds, items, qbuf, xbuf, xexp, s);
+    return launch_inv32<512, 1, 4>(tasks, n, lds, items, qbuf, xbuf, xexp, s,
+                                   dn, grid);
+  return launch_inv32<512, 2, 4>(tasks, n, lds, items, qbuf, xbuf, xexp, s,
+                                 dn, grid);
 }
 
 hipError_t launch_refine(const int32_t *tasks, int n, int nm_max,
                          const Item *items, const cdbl *qbuf,
                          const cflt *xbuf, const int32_t *xexp, cdbl *tbuf,
                          cdbl *rbuf, double *qparts, int32_t *flags,
-                         hipStream_t s) {
-  if (n <= 0) return hipSuccess;
+                         hipStream_t s, const int32_t *dn, int grid) {
+  if (!dn && n <= 0) return hipSuccess;
   if (nm_max < 1 || nm_max > 128) return hipErrorInvalidValue;
   if (nm_max <= 32)
     return launch_ref<256, 16, 2, 2>(tasks, n, items, qbuf, xbuf, xexp, tbuf,
-                                     rbuf, qparts, flags, s);
+                                     rbuf, qparts, flags, s, dn, grid);
   if (nm_max <= 64)
     return launch_ref<256, 16, 4, 4>(tasks, n, items, qbuf, xbuf, xexp, tbuf,
-                                     rbuf, qparts, flags, s);
+                                     rbuf, qparts, flags, s, dn, grid);
   return launch_ref<1024, 32, 4, 4>(tasks, n, items, qbuf, xbuf, xexp, tbuf,
-                                    rbuf, qparts, flags, s);
+                                    rbuf, qparts, flags, s, dn, grid);
 }
 
 }  // namespace ctm

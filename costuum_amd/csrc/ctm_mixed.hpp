@@ -33,7 +33,8 @@ size_t invert32_lds_bytes(int64_t store_max, int nm_max);
 hipError_t launch_invert32(const int32_t *tasks, int n, int nm_max,
                            int64_t store_max, int rows_max, const Item *items,
                            const cdbl *qbuf, cflt *xbuf, int32_t *xexp,
-                           hipStream_t s);
+                           hipStream_t s, const int32_t *dn = nullptr,
+                           int grid = 0);
 
 // FP64 refinement: T = -RgQ X, then sweeps T += (-RgQ - T Q) X until the
 // predicted error of T is below kRefineTol of its largest entry.  Writes T
@@ -44,6 +45,7 @@ hipError_t launch_refine(const int32_t *tasks, int n, int nm_max,
                          const Item *items, const cdbl *qbuf,
                          const cflt *xbuf, const int32_t *xexp, cdbl *tbuf,
                          cdbl *rbuf, double *qparts, int32_t *flags,
-                         hipStream_t s);
+                         hipStream_t s, const int32_t *dn = nullptr,
+                         int grid = 0);
 
 }  // namespace ctm

@@ -117,6 +117,8 @@ __device__ inline cdbl cfac(int nn, int n) {
 // LDS.
 constexpr int kObsXQ = 8;  // elements of a T row per batch of loads
 
+// One workgroup per listed item.  dn (may be NULL): the item count is read
+// from dn[0]; workgroups beyond it return at once.
 template <int CK, int NT, int GCAP, bool PTG>
 __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
                                                const int32_t *item_list,
@@ -126,8 +128,9 @@ __global__ __launch_bounds__(NT) void k_obs_x(const ObsSpec spec,
                                                const int64_t *aoff,
                                                const double *PT,
                                                int nmax_all, int nm_cap,
-                                               cdbl *X) {
+                                               cdbl *X, const int32_t *dn) {
   extern __shared__ double smem[];
+  if (dn && (int)blockIdx.x >= dn[0]) return;
   const int ii = item_list[blockIdx.x];
   const Item it = items[ii];
   const int m = it.m;
@@ -527,7 +530,8 @@ static hipError_t launch_obs_x_ck(const ObsSpec &spec,
                                   int nm_cap, const Item *items,
                                   const cdbl *tbuf, const cdbl *abuf,
                                   const int64_t *aoff, const double *PT,
-                                  int nmax_all, cdbl *X, hipStream_t s) {
+                                  int nmax_all, cdbl *X, hipStream_t s,
+                                  const int32_t *dn, int grid) {
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(
@@ -536,11 +540,12 @@ static hipError_t launch_obs_x_ck(const ObsSpec &spec,
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_obs_x<CK, NT, GCAP, PTG>), dim3(n_items), dim3(NT),
+  hipLaunchKernelGGL((k_obs_x<CK, NT, GCAP, PTG>), dim3(dn ? grid : n_items),
+                     dim3(NT),
                      obs_x_lds_bytes(nm_cap, spec.n_out, CK, NT, GCAP, PTG),
                      s, spec,
                      item_list, items, tbuf, abuf, aoff, PT, nmax_all, nm_cap,
-                     X);
+                     X, dn);
   return hipGetLastError();
 }
 
@@ -548,8 +553,9 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
                         int n_items, int nm_cap, int ck, const Item *items,
                         const cdbl *tbuf, const cdbl *abuf,
                         const int64_t *aoff, const double *PT, int nmax_all,
-                        cdbl *X, hipStream_t s) {
-  if (n_items <= 0) return hipSuccess;
+                        cdbl *X, hipStream_t s, const int32_t *dn,
+                        int grid) {
+  if (!dn && n_items <= 0) return hipSuccess;
   // ck 8: the largest blocks, 512 threads (two 8-direction chunks side by
   // side) if the LDS allows, else 256; the outgoing pi/tau are staged in
   // LDS unless they do not fit, then read from the table (measured slower
@@ -558,14 +564,16 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
     if (obs_x_lds_bytes(nm_cap, spec.n_out, 8, 512, 2, false) <= kObsXMaxLds)
       return launch_obs_x_ck<8, 512, 2, false>(spec, item_list, n_items,
                                                nm_cap, items, tbuf, abuf,
-                                               aoff, PT, nmax_all, X, s);
+                                               aoff, PT, nmax_all, X, s, dn,
+                                               grid);
     if (obs_x_lds_bytes(nm_cap, spec.n_out, 8, 256, 1, false) <= kObsXMaxLds)
       return launch_obs_x_ck<8, 256, 1, false>(spec, item_list, n_items,
                                                nm_cap, items, tbuf, abuf,
-                                               aoff, PT, nmax_all, X, s);
+                                               aoff, PT, nmax_all, X, s, dn,
+                                               grid);
     return launch_obs_x_ck<8, 256, 1, true>(spec, item_list, n_items, nm_cap,
                                             items, tbuf, abuf, aoff, PT,
-                                            nmax_all, X, s);
+                                            nmax_all, X, s, dn, grid);
   }
   // blocks up to order 16 on 128 threads: more workgroups per CU for the
   // many small items (measured faster than 256 and 64)
@@ -573,14 +581,16 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
       obs_x_lds_bytes(nm_cap, spec.n_out, 4, 128, 128, false) <= kObsXMaxLds)
     return launch_obs_x_ck<4, 128, 128, false>(spec, item_list, n_items,
                                                nm_cap, items, tbuf, abuf,
-                                               aoff, PT, nmax_all, X, s);
+                                               aoff, PT, nmax_all, X, s, dn,
+                                               grid);
   if (obs_x_lds_bytes(nm_cap, spec.n_out, 4, 256, 256, false) <= kObsXMaxLds)
     return launch_obs_x_ck<4, 256, 256, false>(spec, item_list, n_items,
                                                nm_cap, items, tbuf, abuf,
-                                               aoff, PT, nmax_all, X, s);
+                                               aoff, PT, nmax_all, X, s, dn,
+                                               grid);
   return launch_obs_x_ck<4, 256, 256, true>(spec, item_list, n_items, nm_cap,
                                             items, tbuf, abuf, aoff, PT,
-                                            nmax_all, X, s);
+                                            nmax_all, X, s, dn, grid);
 }
 
 hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,

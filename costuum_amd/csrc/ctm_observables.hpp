@@ -63,7 +63,8 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
                         int n_items, int nm_cap, int ck, const Item *items,
                         const cdbl *tbuf, const cdbl *abuf,
                         const int64_t *aoff, const double *PT, int nmax_all,
-                        cdbl *X, hipStream_t s);
+                        cdbl *X, hipStream_t s, const int32_t *dn = nullptr,
+                        int grid = 0);
 hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,
                         const int32_t *gi_off, const GrainParams &gp,
                         const cdbl *X, double *qext, double *qextpol,

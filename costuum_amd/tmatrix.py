@@ -233,6 +233,93 @@ def tmatrix_batch(radius, axis_ratio, wavelength, mr_re, mr_im, min_order=4,
     return BatchResult(ctx, n, nmax, status, qsca, qext)
 
 
+class BatchPlan:
+    """A fixed batch whose whole step -- convergence rounds, all-m blocks,
+    full-T sums and (optionally) Q_ext / Q_abs at the angles `theta` -- runs
+    with its work lists built on the GPU (ctm_plan_*): no host round trip
+    between the phases.
+
+    The inputs are copied to device tensors owned by the plan
+    (``self.inputs``: radius, axis_ratio, wavelength, mr_re, mr_im); a run
+    reads their current contents, so a new batch of the same size is run by
+    writing into them.  Outputs are device tensors: ``nmax``, ``status``,
+    ``qsca``, ``qext`` and, with theta, ``obs`` (qext, qextpol, qextcpol,
+    qabs, qabspol [n][n_theta])."""
+
+    def __init__(self, radius, axis_ratio, wavelength, mr_re, mr_im,
+                 min_order=4, max_order=100, glfac=2, tolerance=1e-4,
+                 precision="fp64", theta=None, ngauss=20,
+                 account_for_scattering=True, ctx=None):
+        self.ctx = ctx or default_context()
+        dev = self.ctx.device
+        self.inputs = [_dev(a, dev) for a in (radius, axis_ratio, wavelength,
+                                               mr_re, mr_im)]
+        n = self.inputs[0].numel()
+        if any(a.numel() != n for a in self.inputs):
+            raise ValueError("grain arrays differ in length")
+        self.n = n
+        self.nmax = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.status = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.qsca = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.qext = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.obs = None
+        spec = None
+        if theta is not None:
+            th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+            self._theta = th
+            self.obs = {k: torch.zeros((n, len(th)), dtype=torch.float64,
+                                       device=dev)
+                        for k in ("qext", "qextpol", "qextcpol", "qabs",
+                                  "qabspol")}
+            spec = _native.CtmObservableSpec(len(th), th.ctypes.data, 1, 1,
+                                             int(account_for_scattering),
+                                             ngauss, 0, 0)
+        o = self.obs or {}
+        outs = _native.CtmPlanOutputs(
+            self.nmax.data_ptr(), self.status.data_ptr(),
+            self.qsca.data_ptr(), self.qext.data_ptr(),
+            *[o[k].data_ptr() if k in o else None
+              for k in ("qext", "qextpol", "qextcpol", "qabs", "qabspol")])
+        conv = _native.CtmConvergence(min_order, max_order, glfac,
+                                      PRECISIONS[precision], tolerance)
+        torch.cuda.current_stream(dev).synchronize()
+        h = ctypes.c_void_p()
+        rc = self.ctx._lib.ctm_plan_create(
+            self.ctx.handle, n, *[_ptr(a) for a in self.inputs],
+            ctypes.byref(conv), ctypes.byref(spec) if spec else None,
+            ctypes.byref(outs), ctypes.byref(h))
+        self.ctx.check(rc, "ctm_plan_create")
+        self._h = h
+
+    def run(self):
+        """One step; returns when the outputs are complete."""
+        self.ctx.check(self.ctx._lib.ctm_plan_run(self._h), "ctm_plan_run")
+        return self
+
+    def info(self):
+        i = _native.CtmPlanInfo()
+        self.ctx.check(self.ctx._lib.ctm_plan_info_get(self._h,
+                                                       ctypes.byref(i)),
+                       "ctm_plan_info_get")
+        return {"runs": i.runs, "reruns": i.reruns,
+                "mixed_blocks": i.mixed_blocks,
+                "mixed_fallback": i.mixed_fallback,
+                "pool_bytes": i.pool_bytes,
+                "allm_items": i.allm_items,
+                "window_items": list(i.window_items)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx._lib.ctm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def tmatrix_fixed(radius, axis_ratio, wavelength, mr_re, mr_im, nmax, ngauss,
                   precision="fp64", ctx=None):
     """All m blocks at forced (nmax, ngauss) per grain."""

@@ -359,6 +359,61 @@ int ctm_profile_enable(ctm_context *ctx, int on);
 int ctm_profile_reset(ctm_context *ctx);
 int ctm_profile_read(ctm_context *ctx, int kernel, ctm_kernel_stats *out);
 
+/*
+ * Batch plan: the task path for a fixed batch (the whole of
+ * TaskManager::generate_tasks + QuickSched::execute_tasks for converged
+ * T-matrices and their averaged Q_ext / Q_abs, TaskManager.hpp:445-455,
+ * 1146-1197, 1314-1327; QuickSchedWrapper.hpp:375-389) with the scheduler's
+ * work queues built on the GPU: the convergence rounds, the all-m blocks
+ * and the observables of the batch are listed by device kernels from the
+ * device's convergence state, so a run is a fixed sequence of launches that
+ * the host enqueues at once, with no read-back until the run has finished
+ * (the host path reads the convergence state back after every round and
+ * lists the next one on the CPU).  Results equal
+ * ctm_tmatrix_batch + ctm_observables on the same inputs (same orders,
+ * same sums).
+ *
+ * Inputs are device arrays (n_grains doubles each, as ctm_tmatrix_batch)
+ * bound at creation; a run reads their current contents.  The last order
+ * (conv->max_order - 1) is at most 99.  obs (may be NULL): Q_ext / Q_abs
+ * at n_theta >= 1 angles, no scattering-matrix grid.  Outputs are device
+ * arrays bound at creation (any may be NULL): nmax, status, qsca, qext per
+ * grain; obs_* [n_grains][n_theta] as ctm_observables.  Destroy plans
+ * before their context.  No reference counterpart as an entry point.
+ */
+typedef struct ctm_plan ctm_plan;
+typedef struct ctm_plan_outputs {
+  uint32_t *nmax;
+  int32_t *status;
+  double *qsca;
+  double *qext;
+  double *obs_qext;
+  double *obs_qextpol;
+  double *obs_qextcpol;
+  double *obs_qabs;
+  double *obs_qabspol;
+} ctm_plan_outputs;
+int ctm_plan_create(ctm_context *ctx, int n_grains, const double *radius,
+                    const double *axis_ratio, const double *wavelength,
+                    const double *mr_re, const double *mr_im,
+                    const ctm_convergence *conv,
+                    const ctm_observable_spec *obs,
+                    const ctm_plan_outputs *outputs, ctm_plan **plan);
+/* One run; returns when the outputs are complete.  A run whose device
+ * lists outgrew the plan's memory pools is repeated with larger pools. */
+int ctm_plan_run(ctm_plan *plan);
+typedef struct ctm_plan_info {
+  int64_t runs;            /* runs so far, repeats included */
+  int64_t reruns;          /* repeats after a pool had to grow */
+  int64_t mixed_blocks;    /* last run: blocks through the FP32 elimination */
+  int64_t mixed_fallback;  /* last run: of those, re-solved in FP64 */
+  int64_t pool_bytes;      /* device bytes of the data pools */
+  int32_t allm_items;      /* last run: (grain, m) blocks */
+  int32_t window_items[3]; /* last run: m = 0 orders of each window round */
+} ctm_plan_info;
+int ctm_plan_info_get(ctm_plan *plan, ctm_plan_info *info);
+int ctm_plan_destroy(ctm_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,143 @@
+"""The batch plan (work lists built on the GPU, no host round trip between
+the convergence rounds, the all-m blocks and the observables) against the
+host-driven path (ctm_tmatrix_batch + ctm_observables) on the same inputs.
+
+Both paths run the same kernels on the same blocks, so the converged orders
+and statuses must be identical and the values equal to rounding (the plan
+changes only which workgroup takes a block and the window schedule; the
+test bar is 1e-13 relative, and the maximum difference is printed).  Cases:
+a 48-grain sample spread over the seed-0 batch (orders 5 to 99) in FP64 and
+in mixed precision, run three times in a row on the same plan (outputs
+cleared in between, every run checked); and a plan created for small
+grains and then fed larger ones, which outgrow its first memory pools (the
+run is repeated with grown pools and must still equal the host path)."""
+import numpy as np
+import pytest
+
+import make_inputs
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("radius", "axis_ratio", "wavelength", "mr_re", "mr_im")
+
+
+def _sample(lo, hi, k):
+    g = make_inputs.load(0, 4096)
+    x = 2 * np.pi * g["radius"] / g["wavelength"]
+    order = np.argsort(x)
+    sel = order[np.linspace(lo, hi, k).astype(int)]
+    return {key: g[key][sel] for key in KEYS}
+
+
+def _spread():
+    # 32 grains over the size parameter, 16 of the 400 largest
+    a = _sample(0, 3687, 32)
+    b = _sample(3688, 4095, 16)
+    return {k: np.concatenate([a[k], b[k]]) for k in KEYS}
+
+
+THETA = np.arccos(np.polynomial.legendre.leggauss(8)[0])
+
+
+def _host(g, precision, ctx):
+    from costuum_amd.tmatrix import observables, tmatrix_batch
+    n = len(g["radius"])
+    r = tmatrix_batch(*[g[k] for k in KEYS], ctx=ctx, min_order=4,
+                      max_order=100, glfac=2, tolerance=1e-4,
+                      precision=precision)
+    nmax, _, status = r.orders()
+    obs = observables(THETA, ctx=ctx, ngauss=20, n_grains=n)
+    return dict(nmax=nmax.astype(np.int64), status=status.astype(np.int64),
+                qsca=r.qsca.cpu().numpy(), qext=r.qext.cpu().numpy(),
+                oqext=obs["qext"].cpu().numpy(),
+                oqabs=obs["qabs"].cpu().numpy(),
+                oqabspol=obs["qabspol"].cpu().numpy())
+
+
+def _plan_out(p):
+    return dict(nmax=p.nmax.cpu().numpy().astype(np.int64),
+                status=p.status.cpu().numpy().astype(np.int64),
+                qsca=p.qsca.cpu().numpy(), qext=p.qext.cpu().numpy(),
+                oqext=p.obs["qext"].cpu().numpy(),
+                oqabs=p.obs["qabs"].cpu().numpy(),
+                oqabspol=p.obs["qabspol"].cpu().numpy())
+
+
+def _compare(a, b, what):
+    assert np.array_equal(a["nmax"], b["nmax"]), (what, a["nmax"], b["nmax"])
+    assert np.array_equal(a["status"], b["status"]), what
+    worst = 0.
+    for k in ("qsca", "qext", "oqext", "oqabs", "oqabspol"):
+        x, y = a[k], b[k]
+        den = np.maximum(np.abs(x) + np.abs(y), 1e-300)
+        rel = np.abs(x - y) / den
+        worst = max(worst, float(rel.max()))
+        assert rel.max() <= 1e-13, (what, k, rel.max(), int(rel.argmax()))
+    print("%s: orders identical, max rel diff %.2e" % (what, worst),
+          flush=True)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_plan_equals_host_path(precision):
+    from costuum_amd.tmatrix import BatchPlan, Context
+    g = _spread()
+    ctx = Context(0)
+    try:
+        print("48-grain spread, %s: host path" % precision, flush=True)
+        ref = _host(g, precision, ctx)
+        print("48-grain spread, %s: plan" % precision, flush=True)
+        p = BatchPlan(*[g[k] for k in KEYS], min_order=4, max_order=100,
+                      glfac=2, tolerance=1e-4, precision=precision,
+                      theta=THETA, ngauss=20, ctx=ctx)
+        try:
+            p.run()
+            _compare(ref, _plan_out(p), "%s run 0" % precision)
+            info = p.info()
+            print("plan info", info, flush=True)
+            assert info["allm_items"] == int(np.sum(ref["nmax"] + 1))
+            if precision == "mixed":
+                assert info["mixed_blocks"] > 0
+            for rep in (1, 2):
+                p.nmax.zero_()
+                p.qext.zero_()
+                p.obs["qabs"].zero_()
+                import torch
+                torch.cuda.synchronize()
+                p.run()
+                _compare(ref, _plan_out(p), "%s run %d" % (precision, rep))
+        finally:
+            p.close()
+    finally:
+        ctx.close()
+
+
+def test_plan_pools_grow():
+    from costuum_amd.tmatrix import BatchPlan, Context
+    small = _sample(0, 2000, 24)     # orders up to about 8
+    large = _sample(3900, 4095, 24)  # orders 30 and above
+    ctx = Context(0)
+    try:
+        print("plan created on 24 small grains, run on 24 large ones",
+              flush=True)
+        p = BatchPlan(*[small[k] for k in KEYS], min_order=4, max_order=100,
+                      glfac=2, tolerance=1e-4, theta=THETA, ngauss=20,
+                      ctx=ctx)
+        try:
+            p.run()
+            first = p.info()
+            _compare(_host(small, "fp64", ctx), _plan_out(p), "small grains")
+            for i, k in enumerate(KEYS):
+                p.inputs[i].copy_(p.inputs[i].new_tensor(large[k]))
+            import torch
+            torch.cuda.synchronize()
+            p.run()
+            info = p.info()
+            print("pools", first["pool_bytes"], "->", info["pool_bytes"],
+                  "reruns", info["reruns"], flush=True)
+            assert info["reruns"] > first["reruns"]
+            assert info["pool_bytes"] > first["pool_bytes"]
+            _compare(_host(large, "fp64", ctx), _plan_out(p), "large grains")
+        finally:
+            p.close()
+    finally:
+        ctx.close()
