@@ -53,6 +53,10 @@ def parse():
                          "after each convergence round (ctm_tmatrix_batch "
                          "+ ctm_observables).  The other driver's FP64 step "
                          "time is reported beside it")
+    ap.add_argument("--tune", type=int, default=1,
+                    help="plan driver: autotune the plan on the batch "
+                         "before the warm-up (ctm_plan_tune; results do "
+                         "not depend on it)")
     ap.add_argument("--compare-driver", type=int, default=1,
                     help="also time the other driver's FP64 step")
     ap.add_argument("--per-grain", type=int, default=64,
@@ -548,6 +552,7 @@ def main():
             plan = BatchPlan(*dg, min_order=4, max_order=100, glfac=2,
                              tolerance=1e-4, precision=prec, theta=theta,
                              ngauss=20, ctx=ctx)
+            tuning = plan.tune(3) if a.tune else None
 
             def step(_):
                 plan.run()
@@ -574,7 +579,8 @@ def main():
             blocks = (info["mixed_blocks"], info["mixed_fallback"])
             extra = {"plan_reruns": info["reruns"],
                      "plan_pool_bytes": info["pool_bytes"],
-                     "window_round_orders": info["window_items"]}
+                     "window_round_orders": info["window_items"],
+                     "tuning": tuning}
         else:
             nmax, _, status = r.orders()
             blocks = ctx.precision_stats()

@@ -32,7 +32,7 @@ EXPORTS = (
     "ctm_observables_select", "ctm_vjp", "ctm_observables_noalloc",
     "ctm_tmatrix_set", "ctm_grain_memory_estimate", "ctm_precision_stats",
     "ctm_observables_derivative", "ctm_plan_create", "ctm_plan_run",
-    "ctm_plan_info_get", "ctm_plan_destroy",
+    "ctm_plan_info_get", "ctm_plan_destroy", "ctm_plan_tune",
 )
 
 # double f(double x, void *arg): the host callback type (ctm_real_fn)
@@ -96,6 +96,16 @@ class CtmPlanInfo(ctypes.Structure):
                 ("pool_bytes", ctypes.c_int64),
                 ("allm_items", ctypes.c_int32),
                 ("window_items", ctypes.c_int32 * 3)]
+
+
+class CtmPlanTuning(ctypes.Structure):
+    _fields_ = [("window_scale", ctypes.c_double),
+                ("window_add", ctypes.c_double),
+                ("micro_ratio", ctypes.c_double),
+                ("ms_default", ctypes.c_double),
+                ("ms_chosen", ctypes.c_double),
+                ("trials", ctypes.c_int32),
+                ("runs_per_trial", ctypes.c_int32)]
 
 
 _lib = None
@@ -210,6 +220,9 @@ def lib():
     L.ctm_plan_run.restype = ctypes.c_int
     L.ctm_plan_info_get.argtypes = [vp, ctypes.POINTER(CtmPlanInfo)]
     L.ctm_plan_info_get.restype = ctypes.c_int
+    L.ctm_plan_tune.argtypes = [vp, ctypes.c_int,
+                                ctypes.POINTER(CtmPlanTuning)]
+    L.ctm_plan_tune.restype = ctypes.c_int
     L.ctm_plan_destroy.argtypes = [vp]
     L.ctm_plan_destroy.restype = ctypes.c_int
     _lib = L

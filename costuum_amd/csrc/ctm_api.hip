@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <climits>
@@ -2337,6 +2338,7 @@ struct ctm_plan {
   // end of each all-m solve bucket (the observables of its block orders
   // start after it)
   hipEvent_t ev_solve[kPlanBuckets] = {};
+  std::vector<int> hi1;  // host estimate of the first windows
   // after a run
   PlanStatus st{};
   PlanPhase ph[kPlanPhases]{};
@@ -2502,8 +2504,8 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
     // the parts that can hold blocks of these orders
     unsigned need = 0;
     for (int nm = lo_b + 1; nm <= nm_max; ++nm) {
-      need |= 1u << block_part(nm, true);
-      if (!m0_only) need |= 1u << block_part(nm, false);
+      need |= 1u << block_part(nm, true, p->L.micro_ratio);
+      if (!m0_only) need |= 1u << block_part(nm, false, p->L.micro_ratio);
     }
     for (int part = 0; part < kPlanParts; ++part)
       if (need >> part & 1u)
@@ -2664,7 +2666,7 @@ void plan_first_grids(ctm_plan *p, const std::vector<int> &hi1) {
   int64_t c[kPlanPhases][kPlanLists] = {};
   const int glfac = p->glfac;
   auto add = [&](int ph, int nm, bool m0) {
-    const int part = block_part(nm, m0);
+    const int part = block_part(nm, m0, p->L.micro_ratio);
     if (part < 3) {
       const int tr = 16 * (3 - part);
       const int ntr = (nm + tr - 1) / tr;
@@ -2950,7 +2952,7 @@ int ctm_plan_create(ctm_context *ctx, int n, const double *radius,
     pp->caps.x = pp->want_x
                      ? pad(items_a * (int64_t)pp->lay.sp.n_pairs * 4)
                      : 0;
-    plan_first_grids(pp, hi1);
+    pp->hi1 = std::move(hi1);
   }
   PlanLists &L = pp->L;
   L.sets = pp->d_sets.as<TableSet>();
@@ -2980,6 +2982,10 @@ int ctm_plan_create(ctm_context *ctx, int n, const double *radius,
   L.gmax = gmax;
   L.n_in = pp->want_x ? pp->lay.sp.n_in : 0;
   L.n_pairs = pp->want_x ? pp->lay.sp.n_pairs : 0;
+  L.win_scale = kWindowScale;
+  L.win_add = kWindowAdd;
+  L.micro_ratio = kMicroRatio;
+  plan_first_grids(pp, pp->hi1);
   if ((rc = plan_alloc(pp)) != CTM_OK) return rc;
   CTM_HIP(hipStreamSynchronize(s));
   ctx->plans += 1;
@@ -3037,6 +3043,82 @@ int ctm_plan_run(ctm_plan *p) {
   }
   ctx->err = "plan: the memory pools did not settle";
   return CTM_ERR_MEMORY;
+}
+
+// Autotuner: whole runs of the plan under candidate values of its
+// tunables, by coordinate descent (first-window margin added, its scale,
+// the tile-shape cost ratio), each candidate timed as the median wall time
+// of `runs` runs after one untimed run (which also absorbs any pool or grid
+// growth); the fastest values are kept.  The tunables change only how the
+// work is split into rounds and tiles: every candidate computes the same
+// orders and values.
+int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
+  if (!p || runs < 1) return CTM_ERR_ARGUMENT;
+  ctm_context *ctx = p->ctx;
+  const bool prof = ctx->prof.on;
+  ctx->prof.on = false;
+  auto measure = [&](double *ms) -> int {
+    int rc = ctm_plan_run(p);
+    if (rc != CTM_OK) return rc;
+    std::vector<double> t;
+    for (int r = 0; r < runs; ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = ctm_plan_run(p);
+      if (rc != CTM_OK) return rc;
+      t.push_back(std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - t0)
+                      .count());
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return CTM_OK;
+  };
+  double *knob[3] = {&p->L.win_add, &p->L.win_scale, &p->L.micro_ratio};
+  const std::vector<double> cand[3] = {{1., 2., 3., 4., 5.},
+                                       {1.0, 1.04, 1.08},
+                                       {0.35, 0.5, 0.65, 0.8}};
+  ctm_plan_tuning res{};
+  double best = 0.;
+  int rc = measure(&best);
+  if (rc != CTM_OK) {
+    ctx->prof.on = prof;
+    return rc;
+  }
+  res.ms_default = best;
+  res.trials = 1;
+  for (int k = 0; k < 3; ++k) {
+    const double keep = *knob[k];
+    double pick = keep;
+    for (double v : cand[k]) {
+      if (v == keep) continue;
+      *knob[k] = v;
+      double ms = 0.;
+      rc = measure(&ms);
+      if (rc != CTM_OK) {
+        *knob[k] = keep;
+        ctx->prof.on = prof;
+        return rc;
+      }
+      res.trials += 1;
+      if (ms < best) {
+        best = ms;
+        pick = v;
+      }
+    }
+    *knob[k] = pick;
+  }
+  // grids fitted to the chosen configuration at the next run
+  p->gcap_fitted = false;
+  rc = ctm_plan_run(p);
+  ctx->prof.on = prof;
+  if (rc != CTM_OK) return rc;
+  res.window_add = p->L.win_add;
+  res.window_scale = p->L.win_scale;
+  res.micro_ratio = p->L.micro_ratio;
+  res.ms_chosen = best;
+  res.runs_per_trial = runs;
+  if (out) *out = res;
+  return CTM_OK;
 }
 
 int ctm_plan_info_get(ctm_plan *p, ctm_plan_info *info) {

@@ -48,16 +48,18 @@ hipError_t launch_assemble_pack4(const int32_t *order, int n_order, int m0,
                                  const int32_t *dn = nullptr, int grid = 0);
 // Micro-tile of a block of order nm: 4 (four blocks per workgroup on
 // kPackRows-row tiles) from kPackMin to kPackRows, 1 below, else 3 (48-row
-// tiles) where their padded pair count, at half the cost per pair of
-// 32-row tiles (measured: the 3 x 3 micro-tile reads fewer LDS operands per
-// pair), is smaller, else 2.  (m = 0 tiles are twice as wide.)
-__host__ __device__ inline int assemble_micro(int nm, bool m0) {
+// tiles) where their padded pair count, at `ratio` of the cost per pair of
+// 32-row tiles (measured 0.5: the 3 x 3 micro-tile reads fewer LDS operands
+// per pair), is smaller, else 2.  (m = 0 tiles are twice as wide.)
+constexpr double kMicroRatio = 0.5;
+__host__ __device__ inline int assemble_micro(int nm, bool m0,
+                                              double ratio = kMicroRatio) {
   if (nm >= kPackMin && nm <= kPackRows) return 4;
   if (nm <= kSmallBlock) return 1;
   const long r32 = (nm + 31) / 32, c32 = m0 ? (nm + 63) / 64 : r32;
   const long r48 = (nm + 47) / 48, c48 = m0 ? (nm + 95) / 96 : r48;
-  return 0.5 * (double)(r48 * c48 * 2304) < (double)(r32 * c32 * 1024) ? 3
-                                                                       : 2;
+  return ratio * (double)(r48 * c48 * 2304) < (double)(r32 * c32 * 1024) ? 3
+                                                                         : 2;
 }
 hipError_t launch_assemble(int micro, const int32_t *order,
                            const int32_t *tile_prefix, int n_order,

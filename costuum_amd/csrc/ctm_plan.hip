@@ -97,7 +97,8 @@ __global__ void k_plan_first(const PlanLists L, const GrainParams gp) {
   c.status = 1;
   L.state[g] = c;
   const int end = first_window_end(gp.radius[g], gp.axis_ratio[g],
-                                   gp.wavelength[g], gp.mr_re[g]);
+                                   gp.wavelength[g], gp.mr_re[g],
+                                   L.win_scale, L.win_add);
   const int lim = end > L.min_order + 1 ? end : L.min_order + 1;
   L.lo[g] = L.min_order;
   L.hi[g] = lim < L.last_order ? lim : L.last_order;
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(256) void k_plan_items(const PlanLists L,
       it.wig = L.wig_base[it.N] + it.m;
       L.items[i] = it;
       const int nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-      const int part = block_part(nm, it.m == 0);
+      const int part = block_part(nm, it.m == 0, L.micro_ratio);
       const int bp = part * (L.gmax + 1) + (L.gmax - it.G);
       const int bn = L.last_order - nm;
       if (in_lds) {
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(256) void k_plan_scatter(const PlanLists L,
     if (live) {
       it = L.items[i];
       nm = it.N + 1 - (it.m > 0 ? it.m : 1);
-      part = block_part(nm, it.m == 0);
+      part = block_part(nm, it.m == 0, L.micro_ratio);
       bp = part * (L.gmax + 1) + (L.gmax - it.G);
       bn = L.last_order - nm;
       if (in_lds) {

@@ -49,8 +49,13 @@ __host__ __device__ inline int order_estimate(double rv, double d,
 // 126), against -6 .. 5 for the estimate alone; the window ends 3 + 4 %
 // above the model.  Only the amount of work depends on it: the
 // convergence decision is made order by order as the reference makes it.
+// (scale, add: the margin above the model; the plan's autotuner may choose
+// another.)
+constexpr double kWindowScale = 1.04, kWindowAdd = 3.;
 __host__ __device__ inline int first_window_end(double rv, double d,
-                                                double lam, double mre) {
+                                                double lam, double mre,
+                                                double scale = kWindowScale,
+                                                double add = kWindowAdd) {
   const int est = order_estimate(rv, d, lam);
   if (est < 10) return est + 3 + est / 8;
   const double c = cbrt(d);
@@ -60,7 +65,7 @@ __host__ __device__ inline int first_window_end(double rv, double d,
   const double model = 0.2213 + 0.6409 * x + 2.4758 * cx +
                        0.4839 * (mre - 1.) * x + 0.7758 * ld * x -
                        0.1053 * ld * cx;
-  return (int)ceil(1.04 * model + 3.);
+  return (int)ceil(scale * model + add);
 }
 
 // Window of the next convergence round of a grain that has not converged
@@ -76,8 +81,9 @@ __host__ __device__ inline int next_window_end(int hi, int last_order,
 
 // Assembly part of a block (micro-tile 3, 2, 1 -> 0, 1, 2; four per
 // workgroup -> 3 for m > 0, 4 for m = 0).
-__host__ __device__ inline int block_part(int nm, bool m0) {
-  const int micro = assemble_micro(nm, m0);
+__host__ __device__ inline int block_part(int nm, bool m0,
+                                          double ratio = kMicroRatio) {
+  const int micro = assemble_micro(nm, m0, ratio);
   return micro == 4 ? (m0 ? 4 : 3) : 3 - micro;
 }
 
@@ -184,6 +190,8 @@ struct PlanLists {
   int n, glfac, min_order, last_order;
   int gmax;  // glfac last_order
   int n_in, n_pairs;  // observables (0 without)
+  // tunables: first-window margin and tile-shape cost ratio
+  double win_scale, win_add, micro_ratio;
 };
 
 // Ints of the histogram area: (part, G) bins with their cursors, order

@@ -296,6 +296,19 @@ class BatchPlan:
         self.ctx.check(self.ctx._lib.ctm_plan_run(self._h), "ctm_plan_run")
         return self
 
+    def tune(self, runs=3):
+        """Autotunes the plan's first-window margin and tile-shape ratio by
+        timing whole runs (ctm_plan_tune); returns the chosen values and
+        the per-run times before and after."""
+        t = _native.CtmPlanTuning()
+        self.ctx.check(self.ctx._lib.ctm_plan_tune(self._h, runs,
+                                                   ctypes.byref(t)),
+                       "ctm_plan_tune")
+        return {"window_scale": t.window_scale, "window_add": t.window_add,
+                "micro_ratio": t.micro_ratio, "ms_default": t.ms_default,
+                "ms_chosen": t.ms_chosen, "trials": t.trials,
+                "runs_per_trial": t.runs_per_trial}
+
     def info(self):
         i = _native.CtmPlanInfo()
         self.ctx.check(self.ctx._lib.ctm_plan_info_get(self._h,

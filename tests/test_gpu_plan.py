@@ -111,6 +111,35 @@ def test_plan_equals_host_path(precision):
         ctx.close()
 
 
+def test_plan_tuned_equals_host_path():
+    """The autotuner changes only the round and tile split of the work:
+    after tuning, the plan's results still equal the host path's."""
+    from costuum_amd.tmatrix import BatchPlan, Context
+    g = _spread()
+    ctx = Context(0)
+    try:
+        print("48-grain spread: host path", flush=True)
+        ref = _host(g, "fp64", ctx)
+        p = BatchPlan(*[g[k] for k in KEYS], min_order=4, max_order=100,
+                      glfac=2, tolerance=1e-4, theta=THETA, ngauss=20,
+                      ctx=ctx)
+        try:
+            print("48-grain spread: plan, tuning", flush=True)
+            t = p.tune(2)
+            print("tuning", t, flush=True)
+            assert t["trials"] >= 10 and t["ms_chosen"] <= t["ms_default"]
+            p.nmax.zero_()
+            p.qext.zero_()
+            import torch
+            torch.cuda.synchronize()
+            p.run()
+            _compare(ref, _plan_out(p), "tuned plan")
+        finally:
+            p.close()
+    finally:
+        ctx.close()
+
+
 def test_plan_pools_grow():
     from costuum_amd.tmatrix import BatchPlan, Context
     small = _sample(0, 2000, 24)     # orders up to about 8
