@@ -120,7 +120,9 @@ def dominant_kernel(stats):
     """The assembly kernel with the most algorithmic work in the timed
     steps.  (By work, not by event time: the smaller assembly parts run
     beside each other and beside solves, so their event durations include
-    time they shared the GPU; the 3 x 3 part runs alone.)"""
+    time they shared the GPU; the 3 x 3 part of the all-m phase runs alone.
+    With the plan driver its launches in the convergence windows run beside
+    the other parts and are counted apart, as "assemble_window".)"""
     return max(ASSEMBLY_KERNELS, key=lambda k: stats[k[1]][2])
 
 
@@ -657,6 +659,13 @@ def main():
     avg_ms = ms / max(launches, 1)
     achieved = flops / (ms / 1e3) / 1e12 if ms > 0 else 0.0
     roofline = {"kernel": symbol, "bound": "mfma",
+                "launches_counted": "the all-m phase's launches (the kernel "
+                                    "runs alone); with the plan driver its "
+                                    "convergence-window launches run beside "
+                                    "the other assembly parts and are listed "
+                                    "under families.assemble_window"
+                if pname == "assemble_wide" and a.driver == "plan" else
+                "every launch with work in the timed steps",
                 "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS,
                                                   4),
@@ -667,7 +676,7 @@ def main():
                 "algorithmic_bytes_per_launch": nbytes / max(launches, 1)}
     families = {}
     for fam in ("solve", "refine", "obs_x", "obs_sum", "assemble_small",
-                "assemble_pack", "assemble", "tables"):
+                "assemble_pack", "assemble", "assemble_window", "tables"):
         l, fm, ff, fb = stats.get(fam, (0, 0., 0., 0.))
         if l == 0:
             continue

@@ -50,7 +50,8 @@ QUERY_OUT = 40
 
 KERNELS = ("gauss_legendre", "wigner", "tables", "assemble", "solve",
            "converge", "sums", "obs_x", "obs_sum", "average",
-           "assemble_small", "assemble_wide", "assemble_pack", "refine")
+           "assemble_small", "assemble_wide", "assemble_pack", "refine",
+           "assemble_window")
 
 
 class CtmObservableSpec(ctypes.Structure):

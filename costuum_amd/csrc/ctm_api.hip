@@ -2456,14 +2456,19 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
               launch_tables(L.sets, 0, 0, 0, gp, p->d_gl.as<double>(),
                             p->d_tables.as<double>(), s, &dph->sets_nodes[0],
                             std::max(1, (gc[0] + 127) / 128)));
-  // assembly parts: the 3 x 3 part alone, then the others side by side
-  const int kid[3] = {CTM_KERNEL_ASSEMBLE_WIDE, CTM_KERNEL_ASSEMBLE,
-                      CTM_KERNEL_ASSEMBLE_SMALL};
+  // assembly parts.  All m: the 3 x 3 part alone (so that its launch
+  // measures the kernel), then the others side by side.  Windows: all parts
+  // side by side from the start (the 3 x 3 launches of the windows are
+  // credited to CTM_KERNEL_ASSEMBLE_WINDOW)
+  const int kid[3] = {want_t ? CTM_KERNEL_ASSEMBLE_WIDE
+                             : CTM_KERNEL_ASSEMBLE_WINDOW,
+                      CTM_KERNEL_ASSEMBLE, CTM_KERNEL_ASSEMBLE_SMALL};
   // (window lists hold only m = 0 blocks: the packed m = 0 part takes the
   // lane of the empty packed m > 0 part)
   const int lane_of_part[5] = {0, 1, 2, 3, want_t ? 2 : 3};
+  if (!want_t) PLAN_HIP(fork_lanes(ctx));
   for (int part = 0; part < kPlanParts; ++part) {
-    if (part == 1) PLAN_HIP(fork_lanes(ctx));
+    if (part == 1 && want_t) PLAN_HIP(fork_lanes(ctx));
     const hipStream_t ls = lane(ctx, lane_of_part[part]);
     const int32_t *ord = L.ord + part * cap;
     const int grid = std::max(1, gc[plan_list_part(part)]);

@@ -340,7 +340,11 @@ double ctm_grain_memory_estimate(double radius, double axis_ratio,
 #define CTM_KERNEL_ASSEMBLE_WIDE 11  /* blocks on 48-row tiles */
 #define CTM_KERNEL_ASSEMBLE_PACK 12  /* blocks of order 9-24, four per group */
 #define CTM_KERNEL_REFINE 13  /* FP64 refinement of the mixed solve */
-#define CTM_KERNEL_COUNT 14
+/* 48-row tiles of the plan's convergence windows, which run beside the
+ * other assembly parts (CTM_KERNEL_ASSEMBLE_WIDE: the launches that run
+ * alone) */
+#define CTM_KERNEL_ASSEMBLE_WINDOW 14
+#define CTM_KERNEL_COUNT 15
 
 typedef struct ctm_kernel_stats {
   uint64_t launches;
