@@ -106,7 +106,9 @@ class CtmPlanTuning(ctypes.Structure):
                 ("ms_default", ctypes.c_double),
                 ("ms_chosen", ctypes.c_double),
                 ("trials", ctypes.c_int32),
-                ("runs_per_trial", ctypes.c_int32)]
+                ("runs_per_trial", ctypes.c_int32),
+                ("big_first", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 _lib = None

@@ -2402,7 +2402,7 @@ void plan_solve_shape(int nm_max, bool m0_only, int64_t *store, int *rows) {
 // yet joined) a bucket starts on a lane once the solve buckets that hold
 // its block orders have finished, beside the remaining solves; otherwise
 // (all T blocks complete) the buckets run side by side.
-int plan_obs_x(ctm_plan *p, const bool *solved, const int *lane_of_bucket) {
+int plan_obs_x(ctm_plan *p, const bool *solved) {
   ctm_context *ctx = p->ctx;
   const PlanLists &L = p->L;
   const int ph = kPlanPhases - 1;
@@ -2415,19 +2415,17 @@ int plan_obs_x(ctm_plan *p, const bool *solved, const int *lane_of_bucket) {
     if (capn <= lo_b) continue;
     hipStream_t ls = lane(ctx, b % kLanes);
     if (solved) {
-      // the solve buckets holding orders lo_b + 1 .. capn: the bucket runs
-      // on the lane of the last of them and waits for the others
-      int last = -1;
+      // Lanes chosen so that no bucket queues behind a solve it does not
+      // need (a lane runs in order): orders above 64 follow the order
+      // 71-99 solves on lane 0, orders 33-64 the 49-64 solves on lane 1,
+      // 17-32 the 33-48 solves on lane 3, up to 16 the smallest solves on
+      // lane 2; each waits for the solve buckets holding its orders.
+      static const int obs_lane[kPlanObsBuckets] = {0, 0, 1, 3, 2};
+      ls = lane(ctx, obs_lane[b]);
       for (int sb = 0; sb < kPlanBuckets; ++sb) {
         const int shi = plan_bucket_hi(sb);
         const int slo = sb + 1 < kPlanBuckets ? plan_bucket_hi(sb + 1) : 0;
-        if (shi > lo_b && slo < capn && solved[sb]) last = sb;
-      }
-      if (last >= 0) ls = lane(ctx, lane_of_bucket[last]);
-      for (int sb = 0; sb < kPlanBuckets; ++sb) {
-        const int shi = plan_bucket_hi(sb);
-        const int slo = sb + 1 < kPlanBuckets ? plan_bucket_hi(sb + 1) : 0;
-        if (sb != last && shi > lo_b && slo < capn && solved[sb])
+        if (shi > lo_b && slo < capn && solved[sb])
           PLAN_HIP(hipStreamWaitEvent(ls, p->ev_solve[sb], 0));
       }
     }
@@ -2466,52 +2464,35 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
   // (window lists hold only m = 0 blocks: the packed m = 0 part takes the
   // lane of the empty packed m > 0 part)
   const int lane_of_part[5] = {0, 1, 2, 3, want_t ? 2 : 3};
-  if (!want_t) PLAN_HIP(fork_lanes(ctx));
-  for (int part = 0; part < kPlanParts; ++part) {
-    if (part == 1 && want_t) PLAN_HIP(fork_lanes(ctx));
-    const hipStream_t ls = lane(ctx, lane_of_part[part]);
-    const int32_t *ord = L.ord + part * cap;
-    const int grid = std::max(1, gc[plan_list_part(part)]);
-    if (part < 3) {
-      PLAN_LAUNCH(ls, kid[part], ph, 1, part,
-                  launch_assemble(3 - part, ord, L.pre + part * (cap + 1), 0,
-                                  0, L.items, L.sets,
-                                  p->d_wigkeys.as<WigKey>(), gp,
-                                  p->d_tables.as<double>(),
-                                  p->d_wig.as<double>(), p->d_q.as<cdbl>(),
-                                  ls, &dph->part[part][0], grid));
-    } else {
-      PLAN_LAUNCH(ls, CTM_KERNEL_ASSEMBLE_PACK, ph, 1, part,
-                  launch_assemble_pack4(ord, 0, part == 4 ? 1 : 0, L.items,
-                                        L.sets, p->d_wigkeys.as<WigKey>(), gp,
-                                        p->d_tables.as<double>(),
-                                        p->d_wig.as<double>(),
-                                        p->d_q.as<cdbl>(), ls,
-                                        &dph->part[part][0], grid));
-    }
-    PLAN_HIP(hipEventRecord(ctx->ev_part[part], ls));
-  }
   // solve buckets, placed and ordered as the host path places them
   const int lane_of_bucket[kPlanBuckets] = {0, 3, 1, 3, 2, 2};
   const bool m0_only = !want_t;
   const bool mix = p->mixed && want_t;
   bool after_big = false;
   bool solved[kPlanBuckets] = {};
-  for (int b = 0; b < kPlanBuckets; ++b) {
+  bool launched[kPlanBuckets] = {};
+  // the assembly parts that can hold blocks of bucket b's orders
+  auto bucket_need = [&](int b) {
     const int lo_b = b + 1 < kPlanBuckets ? plan_bucket_hi(b + 1) : 0;
-    if (p->last_order <= lo_b) continue;  // no block of these orders
+    const int nm_max = std::min(plan_bucket_hi(b), p->last_order);
+    unsigned need = 0;
+    for (int nm = lo_b + 1; nm <= nm_max; ++nm) {
+      need |= 1u << block_part(nm, true, p->L.micro_ratio);
+      if (!m0_only) need |= 1u << block_part(nm, false, p->L.micro_ratio);
+    }
+    return need;
+  };
+  auto launch_bucket = [&](int b) -> int {
+    launched[b] = true;
+    const int lo_b = b + 1 < kPlanBuckets ? plan_bucket_hi(b + 1) : 0;
+    if (p->last_order <= lo_b) return CTM_OK;  // no block of these orders
     const int nm_max = std::min(plan_bucket_hi(b), p->last_order);
     int64_t store;
     int rows;
     plan_solve_shape(nm_max, m0_only, &store, &rows);
     const hipStream_t ls = lane(ctx, lane_of_bucket[b]);
     if (after_big) PLAN_HIP(hipStreamWaitEvent(ls, ctx->ev_bucket, 0));
-    // the parts that can hold blocks of these orders
-    unsigned need = 0;
-    for (int nm = lo_b + 1; nm <= nm_max; ++nm) {
-      need |= 1u << block_part(nm, true, p->L.micro_ratio);
-      if (!m0_only) need |= 1u << block_part(nm, false, p->L.micro_ratio);
-    }
+    const unsigned need = bucket_need(b);
     for (int part = 0; part < kPlanParts; ++part)
       if (need >> part & 1u)
         PLAN_HIP(hipStreamWaitEvent(ls, ctx->ev_part[part], 0));
@@ -2550,9 +2531,51 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
       PLAN_HIP(hipEventRecord(ctx->ev_bucket, ls));
       after_big = true;
     }
+    return CTM_OK;
+  };
+  // big_first (tunable): in the all-m phase the largest blocks' solves
+  // (their longest elimination chains) are launched right after the 3 x 3
+  // part that assembles them, before the other parts take the CUs' LDS;
+  // only when that part holds all of them
+  const bool big_first = want_t && p->L.big_first && bucket_need(0) == 1u;
+  if (!want_t) PLAN_HIP(fork_lanes(ctx));
+  for (int part = 0; part < kPlanParts; ++part) {
+    if (part == 1 && want_t) {
+      PLAN_HIP(fork_lanes(ctx));
+      if (big_first) {
+        const int rc = launch_bucket(0);
+        if (rc != CTM_OK) return rc;
+      }
+    }
+    const hipStream_t ls = lane(ctx, lane_of_part[part]);
+    const int32_t *ord = L.ord + part * cap;
+    const int grid = std::max(1, gc[plan_list_part(part)]);
+    if (part < 3) {
+      PLAN_LAUNCH(ls, kid[part], ph, 1, part,
+                  launch_assemble(3 - part, ord, L.pre + part * (cap + 1), 0,
+                                  0, L.items, L.sets,
+                                  p->d_wigkeys.as<WigKey>(), gp,
+                                  p->d_tables.as<double>(),
+                                  p->d_wig.as<double>(), p->d_q.as<cdbl>(),
+                                  ls, &dph->part[part][0], grid));
+    } else {
+      PLAN_LAUNCH(ls, CTM_KERNEL_ASSEMBLE_PACK, ph, 1, part,
+                  launch_assemble_pack4(ord, 0, part == 4 ? 1 : 0, L.items,
+                                        L.sets, p->d_wigkeys.as<WigKey>(), gp,
+                                        p->d_tables.as<double>(),
+                                        p->d_wig.as<double>(),
+                                        p->d_q.as<cdbl>(), ls,
+                                        &dph->part[part][0], grid));
+    }
+    PLAN_HIP(hipEventRecord(ctx->ev_part[part], ls));
+  }
+  for (int b = 0; b < kPlanBuckets; ++b) {
+    if (launched[b]) continue;
+    const int rc = launch_bucket(b);
+    if (rc != CTM_OK) return rc;
   }
   if (want_t && !mix && p->want_x) {
-    const int rc = plan_obs_x(p, solved, lane_of_bucket);
+    const int rc = plan_obs_x(p, solved);
     if (rc != CTM_OK) return rc;
   }
   PLAN_HIP(join_lanes(ctx));
@@ -2617,7 +2640,7 @@ int plan_enqueue(ctm_plan *p) {
     if (p->mixed) {
       // after the FP64 re-solves of the mixed path: every T block is final
       PLAN_HIP(fork_lanes(ctx));
-      rc = plan_obs_x(p, nullptr, nullptr);
+      rc = plan_obs_x(p, nullptr);
       if (rc != CTM_OK) return rc;
       PLAN_HIP(join_lanes(ctx));
     }
@@ -2990,6 +3013,7 @@ int ctm_plan_create(ctm_context *ctx, int n, const double *radius,
   L.win_scale = kWindowScale;
   L.win_add = kWindowAdd;
   L.micro_ratio = kMicroRatio;
+  L.big_first = 0;
   plan_first_grids(pp, pp->hi1);
   if ((rc = plan_alloc(pp)) != CTM_OK) return rc;
   CTM_HIP(hipStreamSynchronize(s));
@@ -3078,10 +3102,13 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
     *ms = t[t.size() / 2];
     return CTM_OK;
   };
-  double *knob[3] = {&p->L.win_add, &p->L.win_scale, &p->L.micro_ratio};
-  const std::vector<double> cand[3] = {{1., 2., 3., 4., 5.},
+  double big = p->L.big_first;
+  double *knob[4] = {&p->L.win_add, &p->L.win_scale, &p->L.micro_ratio,
+                     &big};
+  const std::vector<double> cand[4] = {{1., 2., 3., 4., 5.},
                                        {1.0, 1.04, 1.08},
-                                       {0.35, 0.5, 0.65, 0.8}};
+                                       {0.35, 0.5, 0.65, 0.8},
+                                       {0., 1.}};
   ctm_plan_tuning res{};
   double best = 0.;
   int rc = measure(&best);
@@ -3091,16 +3118,18 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
   }
   res.ms_default = best;
   res.trials = 1;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     const double keep = *knob[k];
     double pick = keep;
     for (double v : cand[k]) {
       if (v == keep) continue;
       *knob[k] = v;
+      p->L.big_first = big != 0.;
       double ms = 0.;
       rc = measure(&ms);
       if (rc != CTM_OK) {
         *knob[k] = keep;
+        p->L.big_first = big != 0.;
         ctx->prof.on = prof;
         return rc;
       }
@@ -3111,6 +3140,7 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
       }
     }
     *knob[k] = pick;
+    p->L.big_first = big != 0.;
   }
   // grids fitted to the chosen configuration at the next run
   p->gcap_fitted = false;
@@ -3120,6 +3150,7 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
   res.window_add = p->L.win_add;
   res.window_scale = p->L.win_scale;
   res.micro_ratio = p->L.micro_ratio;
+  res.big_first = p->L.big_first;
   res.ms_chosen = best;
   res.runs_per_trial = runs;
   if (out) *out = res;

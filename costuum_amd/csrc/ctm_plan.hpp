@@ -190,8 +190,10 @@ struct PlanLists {
   int n, glfac, min_order, last_order;
   int gmax;  // glfac last_order
   int n_in, n_pairs;  // observables (0 without)
-  // tunables: first-window margin and tile-shape cost ratio
+  // tunables: first-window margin, tile-shape cost ratio, and whether the
+  // largest blocks' solves start before the smaller assembly parts
   double win_scale, win_add, micro_ratio;
+  int big_first;
 };
 
 // Ints of the histogram area: (part, G) bins with their cursors, order

@@ -305,7 +305,8 @@ class BatchPlan:
                                                    ctypes.byref(t)),
                        "ctm_plan_tune")
         return {"window_scale": t.window_scale, "window_add": t.window_add,
-                "micro_ratio": t.micro_ratio, "ms_default": t.ms_default,
+                "micro_ratio": t.micro_ratio, "big_first": t.big_first,
+                "ms_default": t.ms_default,
                 "ms_chosen": t.ms_chosen, "trials": t.trials,
                 "runs_per_trial": t.runs_per_trial}
 

@@ -419,10 +419,12 @@ int ctm_plan_info_get(ctm_plan *plan, ctm_plan_info *info);
 /* Autotuner (no reference counterpart; it replaces the fixed schedule of
  * QuickSched and the hand-set tile choice): times whole runs of the plan
  * (median of `runs` after one untimed run per candidate) under candidate
- * values of the first-window margin (end = ceil(scale * model + add)) and
- * of the cost ratio that sends blocks to 48-row or 32-row assembly tiles,
- * by coordinate descent, and keeps the fastest.  Results do not depend on
- * them.  `out` (may be NULL) receives the chosen values and times. */
+ * values of the first-window margin (end = ceil(scale * model + add)), of
+ * the cost ratio that sends blocks to 48-row or 32-row assembly tiles and
+ * of the launch order of the largest blocks' solves (big_first: right
+ * after the part that assembles them), by coordinate descent, and keeps the
+ * fastest.  Results do not depend on them.  `out` (may be NULL) receives
+ * the chosen values and times. */
 typedef struct ctm_plan_tuning {
   double window_scale;
   double window_add;
@@ -431,6 +433,8 @@ typedef struct ctm_plan_tuning {
   double ms_chosen;   /* per run, the chosen values */
   int32_t trials;
   int32_t runs_per_trial;
+  int32_t big_first;
+  int32_t pad;
 } ctm_plan_tuning;
 int ctm_plan_tune(ctm_plan *plan, int runs, ctm_plan_tuning *out);
 int ctm_plan_destroy(ctm_plan *plan);
