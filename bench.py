@@ -57,6 +57,10 @@ def parse():
                     help="plan driver: autotune the plan on the batch "
                          "before the warm-up (ctm_plan_tune; results do "
                          "not depend on it)")
+    ap.add_argument("--secondary", type=int, default=1,
+                    help="batch workload: also run the single-grain, "
+                         "gradient and grid workloads (one short "
+                         "measurement each) and attach their results")
     ap.add_argument("--compare-driver", type=int, default=1,
                     help="also time the other driver's FP64 step")
     ap.add_argument("--per-grain", type=int, default=64,
@@ -244,7 +248,7 @@ def verify(fx, nmax, status, qext, qabs):
             if "nmax_ulp" in fx else None}, ok6
 
 
-def run_grid(a):
+def run_grid(a, emit=True):
     """The (size, wavelength, shape) grid: 64 wavelengths x 48 radii x 20
     Draine-Hensley-weighted axis ratios in [0.5, 2], m(lambda) from a
     seeded Lorentz dielectric function, task-path convergence up to order
@@ -330,9 +334,11 @@ def run_grid(a):
     # in the reference, and carry meaningless values)
     good = (st.cpu().numpy().reshape(-1, nshape) == 0).all(axis=1)
     parity = None
+    cpu = None
     if a.cpu_baseline:
         import oracle
         errs = []
+        t_cpu = time.perf_counter()
         for i, (qg, nm) in sorted(per_grain.items()):
             rv, d, lam = g["radius"][i], g["axis_ratio"][i], \
                 g["wavelength"][i]
@@ -347,8 +353,16 @@ def run_grid(a):
             errs.append(max(np.abs(qq[:, 0] - qe).max() / np.abs(qe).max(),
                             np.abs(qq[:, 1] - qa).max() / np.abs(qa).max()))
             assert int(nm) == n, (i, int(nm), n)
+        dt = time.perf_counter() - t_cpu
         parity = {"grains": len(errs), "same_nmax": True,
                   "max_rel_err_q": float(max(errs))}
+        cpu = {"value": round(len(errs) / dt, 3),
+               "unit": "converged T-matrices/s", "cores": 1, "kind": "port",
+               "host": host_info(),
+               "sample": "the %d parity grains (x 0.5-12): convergence, "
+                         "T-matrix, Mishchenko average, Q_ext and Q_abs at "
+                         "8 angles (no Z), one thread, %.2f s"
+                         % (len(errs), dt)}
     out = {
         "metric": "converged T-matrices/s on 1 GPU, size x wavelength x "
                   "shape grid",
@@ -372,15 +386,54 @@ def run_grid(a):
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": None},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
         "kernels": {k: {"launches": int(v[0]), "ms": round(v[1], 3)}
                     for k, v in stats.items()},
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def run_gradients(a):
+def gradients_cpu_baseline(g, theta, k, threads):
+    """The CPU oracle on k grains stratified over the size parameter (the
+    smallest 95 %: one grain of order 99 alone would take minutes): the
+    task-path convergence, then central differences (relative step 1e-6)
+    of Q_ext and Q_abs (20 x 20 scattered light) at the angles along
+    (R_V, d, Re m, Im m) at the converged order -- eight forced-order
+    evaluations per grain, grains in parallel on the host cores."""
+    import numpy as np
+    import oracle
+    x = 2 * np.pi * g["radius"] / g["wavelength"]
+    sel = np.argsort(x)[np.linspace(0, int(0.95 * (len(x) - 1)),
+                                    k).astype(int)]
+    keys = ("radius", "axis_ratio", "wavelength", "mr_re", "mr_im")
+    base = [g[key][sel].copy() for key in keys]
+    w = np.full(len(theta), 1.0 / len(theta))
+    t0 = time.perf_counter()
+    conv = oracle.batch_task_obs_ext(*base, theta, w, ngauss=20,
+                                     nthreads=threads)
+    nmax = conv["nmax"]
+    for j in (0, 1, 3, 4):  # R_V, d, Re m, Im m
+        for sgn in (1.0, -1.0):
+            args = [b.copy() for b in base]
+            args[j] = args[j] * (1.0 + sgn * 1e-6)
+            oracle.batch_task_obs_ext(*args, theta, w, ngauss=20,
+                                      nthreads=threads, nmax_forced=nmax,
+                                      per_angle=True)
+    dt = time.perf_counter() - t0
+    return {"value": round(k / dt, 3),
+            "unit": "converged T-matrices with d/d(R_V, d, Re m, Im m) per s",
+            "cores": int(threads), "kind": "port", "host": host_info(),
+            "sample": "%d grains (stratified over the smallest 95 %% in x, "
+                      "largest order %d): "
+                      "convergence plus 8 central-difference evaluations of "
+                      "Q_ext, Q_abs at %d angles, %.2f s on %d threads"
+                      % (k, int(nmax.max()), len(theta), dt, threads)}
+
+
+def run_gradients(a, emit=True):
     """The 1024-grain seed-1 batch: converged T-matrices, Q_ext and Q_abs
     (with the 20 x 20 scattered-light correction; Q_sca = Q_ext - Q_abs) at
     16 Gauss-Legendre incidence angles, the full-T Q_sca/Q_ext sums, and
@@ -431,6 +484,10 @@ def run_gradients(a):
     _, ms_fwd = timed(forward)
     (r, q, dqs, dqe, dq), ms = timed(step)
     st = r.status.cpu().numpy()
+    cpu = None
+    if a.cpu_baseline:
+        threads = a.cpu_threads or len(os.sched_getaffinity(0)) or 1
+        cpu = gradients_cpu_baseline(g, theta, 32, threads)
     out = {
         "metric": "converged T-matrices with gradients/s on 1 GPU",
         "value": round(int((st == 0).sum()) / (ms / 1e3), 2),
@@ -449,12 +506,14 @@ def run_gradients(a):
                    "mean_dqext_dR_converged": float(
                        dq[0]["qext"][torch.as_tensor(st == 0, device=dev)]
                        .mean().item())},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": None, "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def run_single(a):
+def run_single(a, emit=True):
     """The single-grain case through the reference's module surface:
     R_V = 0.2 um, lambda = 0.55 um, m = 1.5 + 0.02i, for the prolate (d =
     0.5) and oblate (d = 2) reading of axis ratio 2; TMatrixCalculator
@@ -486,6 +545,32 @@ def run_single(a):
     for _ in range(a.steps):
         res = step()
     ms = (time.perf_counter() - t0) * 1e3 / max(a.steps, 1)
+    cpu = None
+    if a.cpu_baseline:
+        # the oracle: the calculator's two-phase convergence, the isotropic
+        # ensemble, Q_ext and Q_abs at 30 deg and Z at 91 angles, per grain
+        import oracle
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            for d in (0.5, 2.0):
+                stc, n, ng, rve = oracle.converge_calculator(1.0, d, rv, lam,
+                                                             mr)
+                T = oracle.tmatrix_all(rve, d, lam, mr, n, ng)
+                pN = np.zeros(2 * n + 1)
+                pN[0] = 1.0
+                Ta = oracle.orientation_average(n, T, pN)
+                oracle.extinction(n, Ta, lam, rve, th)
+                oracle.absorption(n, Ta, lam, rve, th, ngauss=100)
+                for t_out in Theta:
+                    oracle.scattering_query(n, Ta, lam, 0., 0., 0., 0.,
+                                            t_out, 0.)
+        dt = (time.perf_counter() - t0) / reps
+        cpu = {"value": round(2 / dt, 2), "unit": "converged T-matrices/s "
+               "(with Q and Z at 91 angles)", "cores": 1, "kind": "port",
+               "host": host_info(),
+               "sample": "both grains, %d repetitions, %.1f ms per step on "
+                         "one thread" % (reps, dt * 1e3)}
     out = {
         "metric": "single grains through the calculator path per second",
         "value": round(2 / (ms / 1e3), 2),
@@ -500,9 +585,11 @@ def run_single(a):
                                "qext_30deg": r[2], "qabs_30deg": r[3],
                                "z11_90deg": r[4]}
                               for d, r in zip((0.5, 2.0), res)]},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": None, "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -752,6 +839,24 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
+    if a.secondary:
+        # the other configurations of the workload, one short measurement
+        # each (their own modes: --workload single | gradients | grid)
+        import copy
+        sec = {}
+        for name, fn, steps, warm in (("single", run_single, 10, 1),
+                                      ("gradients", run_gradients, 2, 1),
+                                      ("grid", run_grid, 1, 1)):
+            aa = copy.copy(a)
+            aa.steps, aa.warmup, aa.workload = steps, warm, name
+            r = fn(aa, emit=False)
+            sec[name] = {k: r[k] for k in ("metric", "value", "unit",
+                                           "ms_per_step", "steps", "warmup",
+                                           "config", "roofline",
+                                           "cpu_baseline")}
+            if "parity_vs_oracle" in r:
+                sec[name]["parity_vs_oracle"] = r["parity_vs_oracle"]
+        out["secondary"] = sec
     print(json.dumps(out), flush=True)
 
 
