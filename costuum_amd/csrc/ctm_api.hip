@@ -72,6 +72,15 @@ struct DevBuf {
     p = nullptr;
     cap = 0;
   }
+  // Frees a buffer that holds more than twice `bytes` and allocates it
+  // again for `bytes` (contents are lost).
+  hipError_t fit(size_t bytes) {
+    if (cap <= 2 * std::max(bytes, (size_t)256)) return hipSuccess;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    release();
+    return ensure(bytes);
+  }
   template <typename T>
   T *as() const {
     return reinterpret_cast<T *>(p);
@@ -2338,6 +2347,10 @@ struct ctm_plan {
   // end of each all-m solve bucket (the observables of its block orders
   // start after it)
   hipEvent_t ev_solve[kPlanBuckets] = {};
+  // end of the k_obs_x bucket of orders up to 16 (all-m phase, FP64)
+  hipEvent_t ev_obs16 = nullptr;
+  // the sums of grains up to order 32 were launched on the lanes
+  bool obs_q_early = false;
   std::vector<int> hi1;  // host estimate of the first windows
   // after a run
   PlanStatus st{};
@@ -2435,7 +2448,37 @@ int plan_obs_x(ctm_plan *p, const bool *solved) {
                              nullptr, nullptr, p->d_pt.as<double>(),
                              p->last_order, p->d_x.as<cdbl>(), ls,
                              &dph->obs[b][0],
-                             std::max(1, p->gcap[ph][plan_list_obs(b)])));
+                             std::max(1, p->gcap[ph][plan_list_obs(b)]),
+                             p->L.obs_lean != 0));
+  }
+  p->obs_q_early = false;
+  if (solved && p->last_order > 32) {
+    // Sums of the grains whose blocks all lie in the last two buckets,
+    // while the larger blocks are still being contracted: orders up to 16
+    // (up to 17 m blocks) after the order-16 bucket on its lane, orders
+    // 17-32 after both buckets on the lane of the order-32 bucket; the
+    // larger grains are summed after the lanes join.
+    const bool ext = p->out.obs_qext || p->out.obs_qextpol ||
+                     p->out.obs_qextcpol;
+    const bool ab = p->out.obs_qabs || p->out.obs_qabspol;
+    if (ext || ab) {
+      const hipStream_t l16 = lane(ctx, 2), l32 = lane(ctx, 3);
+      PLAN_HIP(hipEventRecord(p->ev_obs16, l16));
+      PLAN_LAUNCH(l16, CTM_KERNEL_OBS_SUM, ph, 0, 0,
+                  launch_obs_q(p->lay.sp, p->n, L.gobs, p->gp,
+                               p->d_x.as<cdbl>(), p->out.obs_qext,
+                               p->out.obs_qextpol, p->out.obs_qextcpol,
+                               p->out.obs_qabs, p->out.obs_qabspol, l16, 0,
+                               17));
+      PLAN_HIP(hipStreamWaitEvent(l32, p->ev_obs16, 0));
+      PLAN_LAUNCH(l32, CTM_KERNEL_OBS_SUM, ph, 0, 0,
+                  launch_obs_q(p->lay.sp, p->n, L.gobs, p->gp,
+                               p->d_x.as<cdbl>(), p->out.obs_qext,
+                               p->out.obs_qextpol, p->out.obs_qextcpol,
+                               p->out.obs_qabs, p->out.obs_qabspol, l32, 18,
+                               33));
+      p->obs_q_early = true;
+    }
   }
   return CTM_OK;
 }
@@ -2652,7 +2695,8 @@ int plan_enqueue(ctm_plan *p) {
                   launch_obs_q(p->lay.sp, p->n, L.gobs, p->gp,
                                p->d_x.as<cdbl>(), p->out.obs_qext,
                                p->out.obs_qextpol, p->out.obs_qextcpol,
-                               p->out.obs_qabs, p->out.obs_qabspol, s));
+                               p->out.obs_qabs, p->out.obs_qabspol, s,
+                               p->obs_q_early ? 34 : 0));
   }
   return CTM_OK;
 }
@@ -2845,6 +2889,7 @@ int ctm_plan_create(ctm_context *ctx, int n, const double *radius,
   }
   for (hipEvent_t &e : p->ev_solve)
     CTM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  CTM_HIP(hipEventCreateWithFlags(&p->ev_obs16, hipEventDisableTiming));
   const int lo_n = p->min_order, hi_n = p->last_order, glfac = p->glfac;
   // grain-independent tables of every order the plan can reach
   std::vector<GlKey> glk;
@@ -3014,6 +3059,7 @@ int ctm_plan_create(ctm_context *ctx, int n, const double *radius,
   L.win_add = kWindowAdd;
   L.micro_ratio = kMicroRatio;
   L.big_first = 0;
+  L.obs_lean = 1;
   plan_first_grids(pp, pp->hi1);
   if ((rc = plan_alloc(pp)) != CTM_OK) return rc;
   CTM_HIP(hipStreamSynchronize(s));
@@ -3102,13 +3148,19 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
     *ms = t[t.size() / 2];
     return CTM_OK;
   };
-  double big = p->L.big_first;
-  double *knob[4] = {&p->L.win_add, &p->L.win_scale, &p->L.micro_ratio,
-                     &big};
-  const std::vector<double> cand[4] = {{1., 2., 3., 4., 5.},
-                                       {1.0, 1.04, 1.08},
-                                       {0.35, 0.5, 0.65, 0.8},
-                                       {0., 1.}};
+  double big = p->L.big_first, lean = p->L.obs_lean;
+  constexpr int kKnobs = 5;
+  double *knob[kKnobs] = {&p->L.win_add, &p->L.win_scale,
+                          &p->L.micro_ratio, &big, &lean};
+  const std::vector<double> cand[kKnobs] = {{1., 2., 3., 4., 5.},
+                                            {1.0, 1.04, 1.08},
+                                            {0.35, 0.5, 0.65, 0.8},
+                                            {0., 1.},
+                                            {0., 1.}};
+  auto flags = [&]() {
+    p->L.big_first = big != 0.;
+    p->L.obs_lean = lean != 0.;
+  };
   ctm_plan_tuning res{};
   double best = 0.;
   int rc = measure(&best);
@@ -3118,18 +3170,18 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
   }
   res.ms_default = best;
   res.trials = 1;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < kKnobs; ++k) {
     const double keep = *knob[k];
     double pick = keep;
     for (double v : cand[k]) {
       if (v == keep) continue;
       *knob[k] = v;
-      p->L.big_first = big != 0.;
+      flags();
       double ms = 0.;
       rc = measure(&ms);
       if (rc != CTM_OK) {
         *knob[k] = keep;
-        p->L.big_first = big != 0.;
+        flags();
         ctx->prof.on = prof;
         return rc;
       }
@@ -3140,17 +3192,44 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
       }
     }
     *knob[k] = pick;
-    p->L.big_first = big != 0.;
+    flags();
   }
   // grids fitted to the chosen configuration at the next run
   p->gcap_fitted = false;
   rc = ctm_plan_run(p);
+  if (rc == CTM_OK) {
+    // pools back to what the chosen configuration needs (candidates with
+    // smaller first windows send more grains to the later rounds and grew
+    // them); a later batch that needs more grows them again
+    auto fit = [](int64_t need) { return need + need / 4 + (1 << 20); };
+    PlanCaps &c = p->caps;
+    c.tables = std::min(c.tables, fit(p->st.need_tables));
+    c.q = std::min(c.q, fit(p->st.need_q));
+    c.t = std::min(c.t, fit(p->st.need_t));
+    c.x = std::min(c.x, fit(p->st.need_x));
+    DevBuf *bufs[] = {&p->d_tables, &p->d_q, &p->d_t, &p->d_x,
+                      &p->d_x32, &p->d_xexp, &p->d_rbuf};
+    const size_t want[] = {(size_t)c.tables * sizeof(double),
+                           (size_t)c.q * sizeof(cdbl),
+                           (size_t)c.t * sizeof(cdbl),
+                           (size_t)c.x * sizeof(cdbl),
+                           (size_t)c.t * sizeof(cflt),
+                           (size_t)c.t * sizeof(int32_t),
+                           (size_t)c.t * sizeof(cdbl)};
+    for (int k = 0; k < 7 && rc == CTM_OK; ++k)
+      if (bufs[k]->p && bufs[k]->fit(want[k]) != hipSuccess) {
+        ctx->err = "plan: pool reallocation failed";
+        rc = CTM_ERR_HIP;
+      }
+    if (rc == CTM_OK) rc = plan_alloc(p);
+  }
   ctx->prof.on = prof;
   if (rc != CTM_OK) return rc;
   res.window_add = p->L.win_add;
   res.window_scale = p->L.win_scale;
   res.micro_ratio = p->L.micro_ratio;
   res.big_first = p->L.big_first;
+  res.obs_lean = p->L.obs_lean;
   res.ms_chosen = best;
   res.runs_per_trial = runs;
   if (out) *out = res;
@@ -3190,6 +3269,7 @@ int ctm_plan_destroy(ctm_plan *p) {
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : p->ev_solve)
     if (e) (void)hipEventDestroy(e);
+  if (p->ev_obs16) (void)hipEventDestroy(p->ev_obs16);
   for (const ctm_plan::Rec &r : p->recs) {
     ctx->prof.pool.push_back(r.a);
     ctx->prof.pool.push_back(r.b);

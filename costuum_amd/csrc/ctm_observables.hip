@@ -299,15 +299,19 @@ constexpr int kObsQJ = 4;
 constexpr int kObsQPhi = 2;  // azimuths per thread (measured best of 2, 4, 5, 10)
 constexpr int kObsQM = 4;    // m values per batch of X loads
 
+// Only grains with nq_lo <= (number of m blocks) <= nq_hi are summed (the
+// others return at once): the sums of a class of grains can run as soon
+// as the X of its blocks exist.
 __global__ __launch_bounds__(256) void k_obs_q(
     const ObsSpec spec, const int32_t *gi_off, const GrainParams gp,
     const cdbl *X, double *qext, double *qextpol, double *qextcpol,
-    double *qabs, double *qabspol) {
+    double *qabs, double *qabspol, int nq_lo, int nq_hi) {
   const int njb = (spec.n_theta + kObsQJ - 1) / kObsQJ;
   const int g = blockIdx.x / njb;
   const int j0 = (blockIdx.x - g * njb) * kObsQJ;
   const int nj = min(kObsQJ, spec.n_theta - j0);
   const int i0 = gi_off[g], i1 = gi_off[g + 1];
+  if (i1 - i0 < nq_lo || i1 - i0 > nq_hi) return;
   const double lam = gp.wavelength[g];
   const double k = 2. * M_PI / lam;
   const double kinv = 1. / k;
@@ -554,8 +558,17 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
                         const cdbl *tbuf, const cdbl *abuf,
                         const int64_t *aoff, const double *PT, int nmax_all,
                         cdbl *X, hipStream_t s, const int32_t *dn,
-                        int grid) {
+                        int grid, bool lean) {
   if (!dn && n_items <= 0) return hipSuccess;
+  // lean: the smallest LDS footprint for the largest blocks (one chunk of
+  // eight directions, outgoing pi/tau from the table), for launches that
+  // share the GPU with other kernels: a workgroup needing most of a CU's
+  // LDS waits until the CU has drained
+  if (lean && ck == 8 &&
+      obs_x_lds_bytes(nm_cap, spec.n_out, 8, 256, 1, true) <= kObsXMaxLds)
+    return launch_obs_x_ck<8, 256, 1, true>(spec, item_list, n_items, nm_cap,
+                                            items, tbuf, abuf, aoff, PT,
+                                            nmax_all, X, s, dn, grid);
   // ck 8: the largest blocks, 512 threads (two 8-direction chunks side by
   // side) if the LDS allows, else 256; the outgoing pi/tau are staged in
   // LDS unless they do not fit, then read from the table (measured slower
@@ -597,12 +610,12 @@ hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,
                         const int32_t *gi_off, const GrainParams &gp,
                         const cdbl *X, double *qext, double *qextpol,
                         double *qextcpol, double *qabs, double *qabspol,
-                        hipStream_t s) {
+                        hipStream_t s, int nq_lo, int nq_hi) {
   if (n_grains <= 0 || spec.n_theta <= 0) return hipSuccess;
   const int njb = (spec.n_theta + kObsQJ - 1) / kObsQJ;
   hipLaunchKernelGGL(k_obs_q, dim3(n_grains * njb), dim3(256), 0, s,
                      spec, gi_off, gp, X, qext, qextpol, qextcpol, qabs,
-                     qabspol);
+                     qabspol, nq_lo, nq_hi);
   return hipGetLastError();
 }
 

@@ -64,12 +64,14 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
                         const cdbl *tbuf, const cdbl *abuf,
                         const int64_t *aoff, const double *PT, int nmax_all,
                         cdbl *X, hipStream_t s, const int32_t *dn = nullptr,
-                        int grid = 0);
+                        int grid = 0, bool lean = false);
+// nq_lo, nq_hi: only grains with that many m blocks (the rest untouched)
 hipError_t launch_obs_q(const ObsSpec &spec, int n_grains,
                         const int32_t *gi_off, const GrainParams &gp,
                         const cdbl *X, double *qext, double *qextpol,
                         double *qextcpol, double *qabs, double *qabspol,
-                        hipStream_t s);
+                        hipStream_t s, int nq_lo = 0,
+                        int nq_hi = 0x7fffffff);
 hipError_t launch_obs_z(const ObsSpec &spec, int n_grains,
                         const int32_t *gi_off, const GrainParams &gp,
                         const cdbl *X, double *Z, hipStream_t s);

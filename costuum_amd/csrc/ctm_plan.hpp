@@ -194,6 +194,7 @@ struct PlanLists {
   // largest blocks' solves start before the smaller assembly parts
   double win_scale, win_add, micro_ratio;
   int big_first;
+  int obs_lean;  // host-side knob: low-LDS k_obs_x for the largest blocks
 };
 
 // Ints of the histogram area: (part, G) bins with their cursors, order

@@ -306,6 +306,7 @@ class BatchPlan:
                        "ctm_plan_tune")
         return {"window_scale": t.window_scale, "window_add": t.window_add,
                 "micro_ratio": t.micro_ratio, "big_first": t.big_first,
+                "obs_lean": t.obs_lean,
                 "ms_default": t.ms_default,
                 "ms_chosen": t.ms_chosen, "trials": t.trials,
                 "runs_per_trial": t.runs_per_trial}

@@ -422,8 +422,10 @@ int ctm_plan_info_get(ctm_plan *plan, ctm_plan_info *info);
  * values of the first-window margin (end = ceil(scale * model + add)), of
  * the cost ratio that sends blocks to 48-row or 32-row assembly tiles and
  * of the launch order of the largest blocks' solves (big_first: right
- * after the part that assembles them), by coordinate descent, and keeps the
- * fastest.  Results do not depend on them.  `out` (may be NULL) receives
+ * after the part that assembles them) and of the kernel variant that forms
+ * the largest blocks' observable forms (obs_lean: the one with the smallest
+ * LDS footprint, which shares CUs with the kernels beside it), by
+ * coordinate descent, and keeps the fastest.  Results do not depend on them.  `out` (may be NULL) receives
  * the chosen values and times. */
 typedef struct ctm_plan_tuning {
   double window_scale;
@@ -434,7 +436,7 @@ typedef struct ctm_plan_tuning {
   int32_t trials;
   int32_t runs_per_trial;
   int32_t big_first;
-  int32_t pad;
+  int32_t obs_lean;   /* largest blocks' X on the low-LDS kernel variant */
 } ctm_plan_tuning;
 int ctm_plan_tune(ctm_plan *plan, int runs, ctm_plan_tuning *out);
 int ctm_plan_destroy(ctm_plan *plan);
