@@ -33,6 +33,7 @@ EXPORTS = (
     "ctm_tmatrix_set", "ctm_grain_memory_estimate", "ctm_precision_stats",
     "ctm_observables_derivative", "ctm_plan_create", "ctm_plan_run",
     "ctm_plan_info_get", "ctm_plan_destroy", "ctm_plan_tune",
+    "ctm_plan_tuning_set",
 )
 
 # double f(double x, void *arg): the host callback type (ctm_real_fn)
@@ -108,7 +109,9 @@ class CtmPlanTuning(ctypes.Structure):
                 ("trials", ctypes.c_int32),
                 ("runs_per_trial", ctypes.c_int32),
                 ("big_first", ctypes.c_int32),
-                ("obs_lean", ctypes.c_int32)]
+                ("obs_lean", ctypes.c_int32),
+                ("solve_lean", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 _lib = None
@@ -226,6 +229,8 @@ def lib():
     L.ctm_plan_tune.argtypes = [vp, ctypes.c_int,
                                 ctypes.POINTER(CtmPlanTuning)]
     L.ctm_plan_tune.restype = ctypes.c_int
+    L.ctm_plan_tuning_set.argtypes = [vp, ctypes.POINTER(CtmPlanTuning)]
+    L.ctm_plan_tuning_set.restype = ctypes.c_int
     L.ctm_plan_destroy.argtypes = [vp]
     L.ctm_plan_destroy.restype = ctypes.c_int
     _lib = L

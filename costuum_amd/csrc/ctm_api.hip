@@ -2330,6 +2330,8 @@ struct ctm_plan {
   // grain-independent tables: Gauss-Legendre nodes of every G = glfac N and
   // Wigner values of every (N, m) for N = min_order .. last_order
   DevBuf d_gl, d_glkeys, d_wig, d_wigkeys, d_gloff, d_wigbase, d_spec, d_pt;
+  // global scratch of the order 71-99 solves (L.solve_lean)
+  DevBuf d_gs;
   // lists and per-grain state
   DevBuf d_sets, d_items, d_ord, d_pre, d_sol, d_obsl, d_flags, d_fb, d_hist,
       d_cnt, d_goff, d_gitem, d_gobs, d_lo, d_hi, d_state, d_phase, d_status,
@@ -2556,13 +2558,15 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
                                 p->d_qparts.as<double>(),
                                 L.flags + b * 2 * cap, ls, dn, grid));
     } else {
+      const bool lean = b == 0 && p->L.solve_lean && p->d_gs.p;
       PLAN_LAUNCH(ls, CTM_KERNEL_SOLVE, ph, 2, b,
                   launch_invert_solve(tasks, 0, nm_max, store, rows, L.items,
                                       p->d_q.as<cdbl>(),
                                       want_t ? p->d_t.as<cdbl>() : nullptr,
                                       p->d_qparts.as<double>(),
-                                      want_t ? 0 : 1, nullptr, nullptr, ls,
-                                      dn, grid));
+                                      want_t ? 0 : 1,
+                                      lean ? p->d_gs.as<cdbl>() : nullptr,
+                                      nullptr, ls, dn, grid, lean));
     }
     if (want_t) {
       PLAN_HIP(hipEventRecord(p->ev_solve[b], ls));
@@ -2578,9 +2582,11 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
   };
   // big_first (tunable): in the all-m phase the largest blocks' solves
   // (their longest elimination chains) are launched right after the 3 x 3
-  // part that assembles them, before the other parts take the CUs' LDS;
-  // only when that part holds all of them
+  // part that assembles them, before the other parts take the CUs' LDS
+  // (1), or alone, the other parts waiting for them (2); only when that
+  // part holds all of them
   const bool big_first = want_t && p->L.big_first && bucket_need(0) == 1u;
+  const bool big_alone = big_first && p->L.big_first == 2;
   if (!want_t) PLAN_HIP(fork_lanes(ctx));
   for (int part = 0; part < kPlanParts; ++part) {
     if (part == 1 && want_t) {
@@ -2591,6 +2597,8 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
       }
     }
     const hipStream_t ls = lane(ctx, lane_of_part[part]);
+    if (part >= 1 && big_alone)
+      PLAN_HIP(hipStreamWaitEvent(ls, p->ev_solve[0], 0));
     const int32_t *ord = L.ord + part * cap;
     const int grid = std::max(1, gc[plan_list_part(part)]);
     if (part < 3) {
@@ -2714,6 +2722,14 @@ int plan_alloc(ctm_plan *p) {
     PLAN_HIP(p->d_x32.ensure(std::max<int64_t>(c.t, 1) * sizeof(cflt)));
     PLAN_HIP(p->d_xexp.ensure(std::max<int64_t>(c.t, 1) * sizeof(int32_t)));
     PLAN_HIP(p->d_rbuf.ensure(std::max<int64_t>(c.t, 1) * sizeof(cdbl)));
+  }
+  if (p->L.solve_lean && p->last_order > 64) {
+    // one matrix of the largest order per workgroup of the widest launch
+    const int64_t nm = std::min(p->last_order, kMaxLdsOrder);
+    int64_t g = 1;
+    for (int ph = 0; ph < kPlanPhases; ++ph)
+      g = std::max<int64_t>(g, p->gcap[ph][plan_list_bucket(0)]);
+    PLAN_HIP(p->d_gs.ensure((size_t)g * nm * nm * sizeof(cdbl)));
   }
   // the pools may have been reallocated: the capacities the device checks
   // against are the requested ones (DevBuf rounds up)
@@ -3060,6 +3076,7 @@ int ctm_plan_create(ctm_context *ctx, int n, const double *radius,
   L.micro_ratio = kMicroRatio;
   L.big_first = 0;
   L.obs_lean = 1;
+  L.solve_lean = 0;
   plan_first_grids(pp, pp->hi1);
   if ((rc = plan_alloc(pp)) != CTM_OK) return rc;
   CTM_HIP(hipStreamSynchronize(s));
@@ -3127,6 +3144,28 @@ int ctm_plan_run(ctm_plan *p) {
 // growth); the fastest values are kept.  The tunables change only how the
 // work is split into rounds and tiles: every candidate computes the same
 // orders and values.
+int ctm_plan_tuning_set(ctm_plan *p, const ctm_plan_tuning *t) {
+  if (!p || !t) return CTM_ERR_ARGUMENT;
+  ctm_context *ctx = p->ctx;
+  if (!(t->window_scale >= 0.5 && t->window_scale <= 2.) ||
+      !(t->window_add >= 0. && t->window_add <= 50.) ||
+      !(t->micro_ratio > 0. && t->micro_ratio <= 4.)) {
+    ctx->err = "plan tuning out of range (window_scale 0.5-2, window_add "
+               "0-50, micro_ratio 0-4)";
+    return CTM_ERR_ARGUMENT;
+  }
+  CTM_HIP(hipSetDevice(ctx->device));
+  p->L.win_scale = t->window_scale;
+  p->L.win_add = t->window_add;
+  p->L.micro_ratio = t->micro_ratio;
+  p->L.big_first = std::min(std::max(t->big_first, 0), 2);
+  p->L.obs_lean = t->obs_lean != 0;
+  p->L.solve_lean = t->solve_lean != 0;
+  // grids fitted again to the lists of the next run
+  p->gcap_fitted = false;
+  return plan_alloc(p);
+}
+
 int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
   if (!p || runs < 1) return CTM_ERR_ARGUMENT;
   ctm_context *ctx = p->ctx;
@@ -3148,18 +3187,23 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
     *ms = t[t.size() / 2];
     return CTM_OK;
   };
-  double big = p->L.big_first, lean = p->L.obs_lean;
-  constexpr int kKnobs = 5;
+  double big = p->L.big_first, lean = p->L.obs_lean,
+         slean = p->L.solve_lean;
+  constexpr int kKnobs = 6;
   double *knob[kKnobs] = {&p->L.win_add, &p->L.win_scale,
-                          &p->L.micro_ratio, &big, &lean};
+                          &p->L.micro_ratio, &big, &lean, &slean};
   const std::vector<double> cand[kKnobs] = {{1., 2., 3., 4., 5.},
                                             {1.0, 1.04, 1.08},
                                             {0.35, 0.5, 0.65, 0.8},
+                                            {0., 1., 2.},
                                             {0., 1.},
                                             {0., 1.}};
+  // (the solve scratch is allocated by the next run's pool check)
   auto flags = [&]() {
-    p->L.big_first = big != 0.;
+    p->L.big_first = (int)big;
     p->L.obs_lean = lean != 0.;
+    p->L.solve_lean = slean != 0.;
+    return plan_alloc(p);
   };
   ctm_plan_tuning res{};
   double best = 0.;
@@ -3176,12 +3220,12 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
     for (double v : cand[k]) {
       if (v == keep) continue;
       *knob[k] = v;
-      flags();
+      rc = flags();
       double ms = 0.;
-      rc = measure(&ms);
+      if (rc == CTM_OK) rc = measure(&ms);
       if (rc != CTM_OK) {
         *knob[k] = keep;
-        flags();
+        (void)flags();
         ctx->prof.on = prof;
         return rc;
       }
@@ -3192,7 +3236,11 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
       }
     }
     *knob[k] = pick;
-    flags();
+    rc = flags();
+    if (rc != CTM_OK) {
+      ctx->prof.on = prof;
+      return rc;
+    }
   }
   // grids fitted to the chosen configuration at the next run
   p->gcap_fitted = false;
@@ -3230,6 +3278,7 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
   res.micro_ratio = p->L.micro_ratio;
   res.big_first = p->L.big_first;
   res.obs_lean = p->L.obs_lean;
+  res.solve_lean = p->L.solve_lean;
   res.ms_chosen = best;
   res.runs_per_trial = runs;
   if (out) *out = res;
@@ -3265,7 +3314,7 @@ int ctm_plan_destroy(ctm_plan *p) {
                     &p->d_gobs, &p->d_lo, &p->d_hi, &p->d_state,
                     &p->d_phase, &p->d_status, &p->d_iota, &p->d_tables,
                     &p->d_q, &p->d_t, &p->d_x, &p->d_qparts, &p->d_x32,
-                    &p->d_xexp, &p->d_rbuf, &p->d_gcap};
+                    &p->d_xexp, &p->d_rbuf, &p->d_gcap, &p->d_gs};
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : p->ev_solve)
     if (e) (void)hipEventDestroy(e);

@@ -140,6 +140,45 @@ def test_plan_tuned_equals_host_path():
         ctx.close()
 
 
+@pytest.mark.parametrize("knobs", [
+    dict(solve_lean=1, obs_lean=1, big_first=1),
+    dict(solve_lean=0, obs_lean=0, big_first=0, window_add=1.0,
+         micro_ratio=0.8),
+    dict(big_first=2),
+])
+def test_plan_tuning_set_equals_host_path(knobs):
+    """Tunables applied by ctm_plan_tuning_set (the order 71-99 blocks
+    eliminated in global scratch or in LDS, the low-LDS or the staged
+    observable kernel for the largest blocks, the launch order of the
+    largest solves, the first-window margin and tile ratio) change where
+    the work runs, not its result: three runs in a row equal the host
+    path."""
+    from costuum_amd.tmatrix import BatchPlan, Context
+    g = _spread()
+    ctx = Context(0)
+    try:
+        print("48-grain spread: host path", flush=True)
+        ref = _host(g, "fp64", ctx)
+        p = BatchPlan(*[g[k] for k in KEYS], min_order=4, max_order=100,
+                      glfac=2, tolerance=1e-4, theta=THETA, ngauss=20,
+                      ctx=ctx)
+        try:
+            p.set_tuning(**knobs)
+            for rep in range(3):
+                print("plan with %s, run %d" % (knobs, rep), flush=True)
+                p.nmax.zero_()
+                p.qext.zero_()
+                p.obs["qabs"].zero_()
+                import torch
+                torch.cuda.synchronize()
+                p.run()
+                _compare(ref, _plan_out(p), "tuning %s run %d" % (knobs, rep))
+        finally:
+            p.close()
+    finally:
+        ctx.close()
+
+
 def test_plan_pools_grow():
     from costuum_amd.tmatrix import BatchPlan, Context
     small = _sample(0, 2000, 24)     # orders up to about 8
