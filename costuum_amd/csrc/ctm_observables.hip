@@ -569,6 +569,12 @@ hipError_t launch_obs_x(const ObsSpec &spec, const int32_t *item_list,
     return launch_obs_x_ck<8, 256, 1, true>(spec, item_list, n_items, nm_cap,
                                             items, tbuf, abuf, aoff, PT,
                                             nmax_all, X, s, dn, grid);
+  if (lean && ck == 4 && nm_cap > 32 &&
+      obs_x_lds_bytes(nm_cap, spec.n_out, 4, 256, 256, true) <= kObsXMaxLds)
+    return launch_obs_x_ck<4, 256, 256, true>(spec, item_list, n_items,
+                                              nm_cap, items, tbuf, abuf,
+                                              aoff, PT, nmax_all, X, s, dn,
+                                              grid);
   // ck 8: the largest blocks, 512 threads (two 8-direction chunks side by
   // side) if the LDS allows, else 256; the outgoing pi/tau are staged in
   // LDS unless they do not fit, then read from the table (measured slower
