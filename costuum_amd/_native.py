@@ -109,9 +109,7 @@ class CtmPlanTuning(ctypes.Structure):
                 ("trials", ctypes.c_int32),
                 ("runs_per_trial", ctypes.c_int32),
                 ("big_first", ctypes.c_int32),
-                ("obs_lean", ctypes.c_int32),
-                ("solve_lean", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("obs_lean", ctypes.c_int32)]
 
 
 _lib = None

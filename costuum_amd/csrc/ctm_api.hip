@@ -2326,12 +2326,13 @@ struct ctm_plan {
   bool mixed = false;
   ctm_plan_outputs out{};
   bool want_x = false;
+  // while tuning: a run whose pools would outgrow this many bytes stops
+  // with CTM_ERR_MEMORY instead of growing them (0: no limit)
+  size_t grow_limit = 0;
   ObsLayout lay;
   // grain-independent tables: Gauss-Legendre nodes of every G = glfac N and
   // Wigner values of every (N, m) for N = min_order .. last_order
   DevBuf d_gl, d_glkeys, d_wig, d_wigkeys, d_gloff, d_wigbase, d_spec, d_pt;
-  // global scratch of the order 71-99 solves (L.solve_lean)
-  DevBuf d_gs;
   // lists and per-grain state
   DevBuf d_sets, d_items, d_ord, d_pre, d_sol, d_obsl, d_flags, d_fb, d_hist,
       d_cnt, d_goff, d_gitem, d_gobs, d_lo, d_hi, d_state, d_phase, d_status,
@@ -2558,15 +2559,13 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
                                 p->d_qparts.as<double>(),
                                 L.flags + b * 2 * cap, ls, dn, grid));
     } else {
-      const bool lean = b == 0 && p->L.solve_lean && p->d_gs.p;
       PLAN_LAUNCH(ls, CTM_KERNEL_SOLVE, ph, 2, b,
                   launch_invert_solve(tasks, 0, nm_max, store, rows, L.items,
                                       p->d_q.as<cdbl>(),
                                       want_t ? p->d_t.as<cdbl>() : nullptr,
                                       p->d_qparts.as<double>(),
-                                      want_t ? 0 : 1,
-                                      lean ? p->d_gs.as<cdbl>() : nullptr,
-                                      nullptr, ls, dn, grid, lean));
+                                      want_t ? 0 : 1, nullptr, nullptr, ls,
+                                      dn, grid));
     }
     if (want_t) {
       PLAN_HIP(hipEventRecord(p->ev_solve[b], ls));
@@ -2722,14 +2721,6 @@ int plan_alloc(ctm_plan *p) {
     PLAN_HIP(p->d_x32.ensure(std::max<int64_t>(c.t, 1) * sizeof(cflt)));
     PLAN_HIP(p->d_xexp.ensure(std::max<int64_t>(c.t, 1) * sizeof(int32_t)));
     PLAN_HIP(p->d_rbuf.ensure(std::max<int64_t>(c.t, 1) * sizeof(cdbl)));
-  }
-  if (p->L.solve_lean && p->last_order > 64) {
-    // one matrix of the largest order per workgroup of the widest launch
-    const int64_t nm = std::min(p->last_order, kMaxLdsOrder);
-    int64_t g = 1;
-    for (int ph = 0; ph < kPlanPhases; ++ph)
-      g = std::max<int64_t>(g, p->gcap[ph][plan_list_bucket(0)]);
-    PLAN_HIP(p->d_gs.ensure((size_t)g * nm * nm * sizeof(cdbl)));
   }
   // the pools may have been reallocated: the capacities the device checks
   // against are the requested ones (DevBuf rounds up)
@@ -3076,7 +3067,6 @@ int ctm_plan_create(ctm_context *ctx, int n, const double *radius,
   L.micro_ratio = kMicroRatio;
   L.big_first = 0;
   L.obs_lean = 1;
-  L.solve_lean = 0;
   plan_first_grids(pp, pp->hi1);
   if ((rc = plan_alloc(pp)) != CTM_OK) return rc;
   CTM_HIP(hipStreamSynchronize(s));
@@ -3125,10 +3115,24 @@ int ctm_plan_run(ctm_plan *p) {
     auto grow = [](int64_t cur, int64_t need) {
       return need > cur ? need + need / 4 : cur;
     };
-    p->caps.tables = grow(p->caps.tables, p->st.need_tables);
-    p->caps.q = grow(p->caps.q, p->st.need_q);
-    p->caps.t = grow(p->caps.t, p->st.need_t);
-    p->caps.x = grow(p->caps.x, p->st.need_x);
+    PlanCaps c = p->caps;
+    c.tables = grow(c.tables, p->st.need_tables);
+    c.q = grow(c.q, p->st.need_q);
+    c.t = grow(c.t, p->st.need_t);
+    c.x = grow(c.x, p->st.need_x);
+    if (p->grow_limit) {
+      const size_t bytes =
+          (size_t)c.tables * sizeof(double) + (size_t)c.q * sizeof(cdbl) +
+          (size_t)c.t * sizeof(cdbl) + (size_t)c.x * sizeof(cdbl) +
+          (p->mixed ? (size_t)c.t * (sizeof(cflt) + sizeof(int32_t) +
+                                     sizeof(cdbl))
+                    : 0);
+      if (bytes > p->grow_limit) {
+        ctx->err = "plan: pools would outgrow the tuning limit";
+        return CTM_ERR_MEMORY;
+      }
+    }
+    p->caps = c;
     rc = plan_alloc(p);
     if (rc != CTM_OK) return rc;
     p->reruns += 1;
@@ -3160,7 +3164,6 @@ int ctm_plan_tuning_set(ctm_plan *p, const ctm_plan_tuning *t) {
   p->L.micro_ratio = t->micro_ratio;
   p->L.big_first = std::min(std::max(t->big_first, 0), 2);
   p->L.obs_lean = t->obs_lean != 0;
-  p->L.solve_lean = t->solve_lean != 0;
   // grids fitted again to the lists of the next run
   p->gcap_fitted = false;
   return plan_alloc(p);
@@ -3187,23 +3190,19 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
     *ms = t[t.size() / 2];
     return CTM_OK;
   };
-  double big = p->L.big_first, lean = p->L.obs_lean,
-         slean = p->L.solve_lean;
-  constexpr int kKnobs = 6;
+  double big = p->L.big_first, lean = p->L.obs_lean;
+  constexpr int kKnobs = 5;
   double *knob[kKnobs] = {&p->L.win_add, &p->L.win_scale,
-                          &p->L.micro_ratio, &big, &lean, &slean};
+                          &p->L.micro_ratio, &big, &lean};
   const std::vector<double> cand[kKnobs] = {{1., 2., 3., 4., 5.},
                                             {1.0, 1.04, 1.08},
                                             {0.35, 0.5, 0.65, 0.8},
                                             {0., 1., 2.},
-                                            {0., 1.},
                                             {0., 1.}};
-  // (the solve scratch is allocated by the next run's pool check)
   auto flags = [&]() {
     p->L.big_first = (int)big;
     p->L.obs_lean = lean != 0.;
-    p->L.solve_lean = slean != 0.;
-    return plan_alloc(p);
+    return CTM_OK;
   };
   ctm_plan_tuning res{};
   double best = 0.;
@@ -3211,6 +3210,18 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
   if (rc != CTM_OK) {
     ctx->prof.on = prof;
     return rc;
+  }
+  // a candidate whose pools would grow past four times those of the
+  // starting values (or past three quarters of the free device memory)
+  // counts as too slow instead of taking the device's memory
+  {
+    size_t free_b = 0, total_b = 0;
+    CTM_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t now = p->d_tables.cap + p->d_q.cap + p->d_t.cap +
+                       p->d_x.cap + p->d_x32.cap + p->d_xexp.cap +
+                       p->d_rbuf.cap;
+    p->grow_limit = std::max<size_t>(
+        now + 1, std::min<size_t>(4 * now, now + free_b / 4 * 3));
   }
   res.ms_default = best;
   res.trials = 1;
@@ -3223,9 +3234,15 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
       rc = flags();
       double ms = 0.;
       if (rc == CTM_OK) rc = measure(&ms);
+      if (rc == CTM_ERR_MEMORY && p->st.overflow != 0) {
+        // too much memory for this candidate: rejected
+        res.trials += 1;
+        continue;
+      }
       if (rc != CTM_OK) {
         *knob[k] = keep;
         (void)flags();
+        p->grow_limit = 0;
         ctx->prof.on = prof;
         return rc;
       }
@@ -3243,6 +3260,7 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
     }
   }
   // grids fitted to the chosen configuration at the next run
+  p->grow_limit = 0;
   p->gcap_fitted = false;
   rc = ctm_plan_run(p);
   if (rc == CTM_OK) {
@@ -3278,7 +3296,6 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
   res.micro_ratio = p->L.micro_ratio;
   res.big_first = p->L.big_first;
   res.obs_lean = p->L.obs_lean;
-  res.solve_lean = p->L.solve_lean;
   res.ms_chosen = best;
   res.runs_per_trial = runs;
   if (out) *out = res;
@@ -3314,7 +3331,7 @@ int ctm_plan_destroy(ctm_plan *p) {
                     &p->d_gobs, &p->d_lo, &p->d_hi, &p->d_state,
                     &p->d_phase, &p->d_status, &p->d_iota, &p->d_tables,
                     &p->d_q, &p->d_t, &p->d_x, &p->d_qparts, &p->d_x32,
-                    &p->d_xexp, &p->d_rbuf, &p->d_gcap, &p->d_gs};
+                    &p->d_xexp, &p->d_rbuf, &p->d_gcap};
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : p->ev_solve)
     if (e) (void)hipEventDestroy(e);

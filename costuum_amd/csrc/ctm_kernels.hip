@@ -1654,18 +1654,8 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
                                cdbl *ainv_out, hipStream_t s,
-                               const int32_t *dn, int grid, bool lean) {
+                               const int32_t *dn, int grid) {
   if (!dn && n <= 0) return hipSuccess;
-  if (lean && gscratch && nm_max > 64 && nm_max <= kMaxLdsOrder &&
-      rows_max <= 2 * 64) {
-    // blocks above order 64 with their sub-blocks in gscratch (nm_max^2
-    // complex values per workgroup) instead of a CU's LDS: they share CUs
-    // with the kernels beside them
-    return launch_solve_blk<512, 2, 4, true>(
-        tasks, n, 2 * (size_t)nm_max * sizeof(int), items, qbuf, tbuf, qparts,
-        diag_only, ainv_out, s, dn, grid, gscratch,
-        (int64_t)nm_max * nm_max);
-  }
   if (nm_max > kMaxLdsOrder) {
     // the matrix lives in gscratch (n * nm_max^2 complex values): blocked
     // elimination with four rows per lane (sub-blocks up to 256 rows), the

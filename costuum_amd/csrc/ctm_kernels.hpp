@@ -81,8 +81,7 @@ hipError_t launch_invert_solve(const int32_t *tasks, int n, int nm_max,
                                const Item *items, const cdbl *qbuf, cdbl *tbuf,
                                double *qparts, int diag_only, cdbl *gscratch,
                                cdbl *ainv_out, hipStream_t s,
-                               const int32_t *dn = nullptr, int grid = 0,
-                               bool lean = false);
+                               const int32_t *dn = nullptr, int grid = 0);
 hipError_t launch_converge(const int32_t *active, int n_active,
                            const int32_t *win_off, const int32_t *win_items,
                            const Item *items, const double *qparts, double tol,

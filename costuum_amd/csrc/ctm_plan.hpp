@@ -195,7 +195,6 @@ struct PlanLists {
   double win_scale, win_add, micro_ratio;
   int big_first;
   int obs_lean;  // host-side knob: low-LDS k_obs_x for the largest blocks
-  int solve_lean;  // host-side knob: orders 71-99 solved in global scratch
 };
 
 // Ints of the histogram area: (part, G) bins with their cursors, order

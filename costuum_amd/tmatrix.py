@@ -306,13 +306,13 @@ class BatchPlan:
                        "ctm_plan_tune")
         return {"window_scale": t.window_scale, "window_add": t.window_add,
                 "micro_ratio": t.micro_ratio, "big_first": t.big_first,
-                "obs_lean": t.obs_lean, "solve_lean": t.solve_lean,
+                "obs_lean": t.obs_lean,
                 "ms_default": t.ms_default,
                 "ms_chosen": t.ms_chosen, "trials": t.trials,
                 "runs_per_trial": t.runs_per_trial}
 
     def set_tuning(self, window_scale=1.04, window_add=3.0, micro_ratio=0.5,
-                   big_first=0, obs_lean=1, solve_lean=0):
+                   big_first=0, obs_lean=1):
         """Applies tunable values without timing (ctm_plan_tuning_set),
         e.g. those tune() returned for a similar batch."""
         t = _native.CtmPlanTuning()
@@ -321,7 +321,6 @@ class BatchPlan:
         t.micro_ratio = micro_ratio
         t.big_first = int(big_first)
         t.obs_lean = int(obs_lean)
-        t.solve_lean = int(solve_lean)
         self.ctx.check(self.ctx._lib.ctm_plan_tuning_set(self._h,
                                                          ctypes.byref(t)),
                        "ctm_plan_tuning_set")

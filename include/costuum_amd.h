@@ -425,10 +425,8 @@ int ctm_plan_info_get(ctm_plan *plan, ctm_plan_info *info);
  * after the part that assembles them; 2: alone, the other parts waiting
  * for them) and of the kernel variant that forms
  * the largest blocks' observable forms (obs_lean: the one with the smallest
- * LDS footprint, which shares CUs with the kernels beside it) and of the
- * storage of the order 71-99 blocks during elimination (solve_lean: global
- * scratch instead of a CU's LDS), by coordinate descent, and keeps the
- * fastest.  Results do not depend on them.  `out` (may be NULL) receives
+ * LDS footprint, which shares CUs with the kernels beside it), by
+ * coordinate descent, and keeps the fastest.  Results do not depend on them.  `out` (may be NULL) receives
  * the chosen values and times. */
 typedef struct ctm_plan_tuning {
   double window_scale;
@@ -440,8 +438,6 @@ typedef struct ctm_plan_tuning {
   int32_t runs_per_trial;
   int32_t big_first;
   int32_t obs_lean;   /* largest blocks' X on the low-LDS kernel variant */
-  int32_t solve_lean; /* orders 71-99 eliminated in global scratch */
-  int32_t pad;
 } ctm_plan_tuning;
 int ctm_plan_tune(ctm_plan *plan, int runs, ctm_plan_tuning *out);
 /* Applies tunable values (e.g. those ctm_plan_tune chose for a similar

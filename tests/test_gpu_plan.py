@@ -141,14 +141,12 @@ def test_plan_tuned_equals_host_path():
 
 
 @pytest.mark.parametrize("knobs", [
-    dict(solve_lean=1, obs_lean=1, big_first=1),
-    dict(solve_lean=0, obs_lean=0, big_first=0, window_add=1.0,
-         micro_ratio=0.8),
+    dict(obs_lean=1, big_first=1),
+    dict(obs_lean=0, big_first=0, window_add=1.0, micro_ratio=0.8),
     dict(big_first=2),
 ])
 def test_plan_tuning_set_equals_host_path(knobs):
-    """Tunables applied by ctm_plan_tuning_set (the order 71-99 blocks
-    eliminated in global scratch or in LDS, the low-LDS or the staged
+    """Tunables applied by ctm_plan_tuning_set (the low-LDS or the staged
     observable kernel for the largest blocks, the launch order of the
     largest solves, the first-window margin and tile ratio) change where
     the work runs, not its result: three runs in a row equal the host
