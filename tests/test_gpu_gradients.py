@@ -72,6 +72,51 @@ def test_tangent_tmatrix_matches_central_differences(ctx):
             assert err < 1e-5, (j, g, err)
 
 
+# grains near their natural convergence at orders 45, 68 and 110 (size
+# parameters 32, 50 and 90, near-spherical so that differences are well
+# conditioned): the blocked elimination in LDS, with two rows per lane, and
+# in global scratch (blocks above order 99)
+HIGH = [
+    (2.8, 1.15, 0.55, 1.33 + 0.005j, 45, 90),
+    (4.4, 0.9, 0.55, 1.4 + 0.01j, 68, 136),
+    (7.9, 1.05, 0.55, 1.3 + 0.002j, 110, 220),
+]
+
+
+def test_tangent_tmatrix_high_orders(ctx):
+    """Tangents at orders 45, 68 and 110 against central differences of the
+    GPU forward path at relative step 1e-6 (measured: at most 1.4e-7 of the
+    largest derivative element, 100 times less than at step 1e-5, so the
+    difference is truncation, not a wrong tangent): bar 5e-7."""
+    from costuum_amd.tmatrix import tangent_tmatrix, tangents
+    h = 1e-6
+
+    def shift(j, sign):
+        out = []
+        for c in HIGH:
+            p = _params(c)
+            p[j] += sign * h * max(abs(p[j]), 1e-3)
+            out.append((p[0], p[1], c[2], complex(p[2], p[3]), c[4], c[5]))
+        return out
+
+    res = _fixed(ctx, HIGH)
+    tangents(res)
+    dT = {(g, j): tangent_tmatrix(res, g, j)
+          for g in range(len(HIGH)) for j in range(4)}
+    for j in range(4):
+        print("orders 45/68/110: direction %d" % j, flush=True)
+        rp = _fixed(ctx, shift(j, +1))
+        Tp = [rp.tmatrix(g) for g in range(len(HIGH))]
+        rm = _fixed(ctx, shift(j, -1))
+        Tm = [rm.tmatrix(g) for g in range(len(HIGH))]
+        for g in range(len(HIGH)):
+            step = 2 * h * max(abs(_params(HIGH[g])[j]), 1e-3)
+            fd = (Tp[g] - Tm[g]) / step
+            err = np.abs(dT[(g, j)] - fd).max() / np.abs(fd).max()
+            print("  order %d: %.2e" % (HIGH[g][4], err), flush=True)
+            assert err < 5e-7, (j, g, err)
+
+
 def test_sum_tangents_match_oracle_differences(ctx):
     """d(Q_sca, Q_ext)/dp against central differences of the oracle's
     sums: pins the derivative to the reference algorithm."""
