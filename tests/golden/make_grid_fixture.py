@@ -11,8 +11,11 @@ incidence angles theta = arccos(x_i), and Z at 91 scattering angles 0, 2,
 ..., 180 deg for incidence along the symmetry axis (alpha = beta = 0,
 theta_in = phi_in = phi_out = 0).  Stored per cell: the converged orders and
 status of the shapes, their Q values and the same at R_V moved up by one
-ulp (the oracle's own sensitivity), the shape weights, and the
-shape-averaged Z.
+ulp (the oracle's own sensitivity), each shape's Z at 19 of the angles (0,
+10, ..., 180 deg) and the same at R_V + 1 ulp, the shape weights, and the
+shape-averaged Z.  Some shapes of the first cell reach order 159 without
+converging; their results change by orders of magnitude under one ulp of
+R_V (the reference's EBCM shares this), which the stored ulp values show.
 
     cell (lambda index 5, radius index 47): x = 82.5, orders above 100
     cell (lambda index 10, radius index 40): x = 22.2, orders about 35
@@ -40,7 +43,7 @@ import make_inputs  # noqa: E402
 CELLS = ((5, 47), (10, 40))
 OUT = os.path.join(HERE, "oracle_grid_cells.npz")
 PARTIAL = os.path.join(HERE, "oracle_grid_cells.partial.npz")
-FIELDS = ("n", "st", "qe", "qa", "Z", "qe2", "qa2")
+FIELDS = ("n", "st", "qe", "qa", "Z", "qe2", "qa2", "Z2")
 
 
 def one_shape(args):
@@ -63,7 +66,9 @@ def one_shape(args):
     Ta2 = O.orientation_average(n, O.tmatrix_all(rv2, d, lam, mr, n, ng), pN)
     qe2, _, _ = O.extinction(n, Ta2, lam, rv2, theta)
     qa2, _ = O.absorption(n, Ta2, lam, rv2, theta, ngauss=20)
-    return idx, (n, st, qe, qa, Z, qe2, qa2)
+    Z2 = np.array([O.scattering_query(n, Ta2, lam, 0., 0., 0., 0., th, 0.)[1]
+                   for th in Theta[::5]])
+    return idx, (n, st, qe, qa, Z, qe2, qa2, Z2)
 
 
 def main():
@@ -105,6 +110,8 @@ def main():
         out["qabs_ulp_%d" % c] = np.array([x[6] for x in r])
         Z = np.array([x[4] for x in r])
         out["z_avg_%d" % c] = np.tensordot(w, Z, axes=1) / w.sum()
+        out["z_shape_%d" % c] = Z[:, ::5]
+        out["z_shape_ulp_%d" % c] = np.array([x[7] for x in r])
         print("cell", CELLS[c], "orders", out["nmax_%d" % c].min(), "..",
               out["nmax_%d" % c].max(), "status", out["status_%d" % c].max(),
               flush=True)
