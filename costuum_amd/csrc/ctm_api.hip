@@ -3216,7 +3216,11 @@ int ctm_plan_tune(ctm_plan *p, int runs, ctm_plan_tuning *out) {
   // counts as too slow instead of taking the device's memory
   {
     size_t free_b = 0, total_b = 0;
-    CTM_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      ctx->err = "plan tuning: hipMemGetInfo failed";
+      ctx->prof.on = prof;
+      return CTM_ERR_HIP;
+    }
     const size_t now = p->d_tables.cap + p->d_q.cap + p->d_t.cap +
                        p->d_x.cap + p->d_x32.cap + p->d_xexp.cap +
                        p->d_rbuf.cap;
