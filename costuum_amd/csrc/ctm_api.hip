@@ -2418,6 +2418,10 @@ void plan_solve_shape(int nm_max, bool m0_only, int64_t *store, int *rows) {
 // yet joined) a bucket starts on a lane once the solve buckets that hold
 // its block orders have finished, beside the remaining solves; otherwise
 // (all T blocks complete) the buckets run side by side.
+// Lanes of the observable buckets in the all-m phase of the FP64 path (see
+// plan_obs_x); the early sums run on those of the last two buckets.
+constexpr int kPlanObsLane[kPlanObsBuckets] = {0, 0, 1, 3, 2};
+
 int plan_obs_x(ctm_plan *p, const bool *solved) {
   ctm_context *ctx = p->ctx;
   const PlanLists &L = p->L;
@@ -2436,8 +2440,7 @@ int plan_obs_x(ctm_plan *p, const bool *solved) {
       // 71-99 solves on lane 0, orders 33-64 the 49-64 solves on lane 1,
       // 17-32 the 33-48 solves on lane 3, up to 16 the smallest solves on
       // lane 2; each waits for the solve buckets holding its orders.
-      static const int obs_lane[kPlanObsBuckets] = {0, 0, 1, 3, 2};
-      ls = lane(ctx, obs_lane[b]);
+      ls = lane(ctx, kPlanObsLane[b]);
       for (int sb = 0; sb < kPlanBuckets; ++sb) {
         const int shi = plan_bucket_hi(sb);
         const int slo = sb + 1 < kPlanBuckets ? plan_bucket_hi(sb + 1) : 0;
@@ -2465,7 +2468,8 @@ int plan_obs_x(ctm_plan *p, const bool *solved) {
                      p->out.obs_qextcpol;
     const bool ab = p->out.obs_qabs || p->out.obs_qabspol;
     if (ext || ab) {
-      const hipStream_t l16 = lane(ctx, 2), l32 = lane(ctx, 3);
+      const hipStream_t l16 = lane(ctx, kPlanObsLane[kPlanObsBuckets - 1]);
+      const hipStream_t l32 = lane(ctx, kPlanObsLane[kPlanObsBuckets - 2]);
       PLAN_HIP(hipEventRecord(p->ev_obs16, l16));
       PLAN_LAUNCH(l16, CTM_KERNEL_OBS_SUM, ph, 0, 0,
                   launch_obs_q(p->lay.sp, p->n, L.gobs, p->gp,
