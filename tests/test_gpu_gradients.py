@@ -117,6 +117,26 @@ def test_tangent_tmatrix_high_orders(ctx):
             assert err < 5e-7, (j, g, err)
 
 
+def test_tangent_tmatrix_matches_dual_oracle(ctx):
+    """Tangent T-matrices against the dual-number oracle (oracle/dual.py:
+    the oracle's chain carried as value and exact derivative, itself pinned
+    to the C oracle's values and differences by tests/test_oracle_dual.py)
+    at the CASES' forced orders, with no difference step in between:
+    measured at most 9e-13 of the largest derivative element, bar 1e-11."""
+    from oracle import dual
+    from costuum_amd.tmatrix import tangent_tmatrix, tangents
+    res = _fixed(ctx, CASES)
+    tangents(res)
+    for g, c in enumerate(CASES):
+        for j in range(4):
+            _, ref = dual.tmatrix_all_dual(*c, direction=j)
+            dT = tangent_tmatrix(res, g, j)
+            err = np.abs(dT - ref).max() / np.abs(ref).max()
+            print("grain %d (order %d) direction %d: %.2e" % (g, c[4], j, err),
+                  flush=True)
+            assert err < 1e-11, (g, j, err)
+
+
 def test_sum_tangents_match_oracle_differences(ctx):
     """d(Q_sca, Q_ext)/dp against central differences of the oracle's
     sums: pins the derivative to the reference algorithm."""
@@ -206,6 +226,10 @@ def test_autograd_of_sums(ctx):
 # 47) are checked against the oracle
 SEED1 = [458, 825, 805, 765, 811, 79, 302, 679, 463, 785, 602, 590, 906, 401]
 SEED1_CHECK = (785, 906, 401)
+# tangent T-matrices of the checked grains against the dual-number oracle:
+# measured 7e-15 (order 11), 5e-14 (order 22) and 3.6e-9 (order 47, whose
+# Q blocks are the worst conditioned) of the largest derivative element
+DUAL_BAR_SEED1 = 1e-7
 
 
 def test_seed1_batch_gradients_match_oracle(ctx):
@@ -232,6 +256,23 @@ def test_seed1_batch_gradients_match_oracle(ctx):
                       max_order=100, glfac=2, tolerance=1e-4)
     nmax, ngs, status = r.orders()
     tangents(r)
+    # the tangent T-matrices of the checked grains against the dual-number
+    # oracle (exact derivatives, no difference step)
+    from oracle import dual
+    from costuum_amd.tmatrix import tangent_tmatrix
+    for gi in SEED1_CHECK:
+        k = SEED1.index(gi)
+        n, ng = int(nmax[k]), int(ngs[k])
+        mr = complex(sub["mr_re"][k], sub["mr_im"][k])
+        for j in range(4):
+            _, ref = dual.tmatrix_all_dual(sub["radius"][k],
+                                           sub["axis_ratio"][k],
+                                           sub["wavelength"][k], mr, n, ng, j)
+            dT = tangent_tmatrix(r, k, j)
+            err = np.abs(dT - ref).max() / np.abs(ref).max()
+            print("seed-1 grain %d (order %d) direction %d: dT vs dual "
+                  "oracle %.2e" % (gi, n, j, err), flush=True)
+            assert err < DUAL_BAR_SEED1, (gi, n, j, err)
     d = [observables_tangent(theta, j, ctx, len(SEED1),
                              radius=sub["radius"]) for j in range(4)]
     def central(p0, lam, n, ng, j, h):
