@@ -1142,6 +1142,9 @@ __device__ __forceinline__ void solve_tail(
 // (3 nm values); the pivot value of the next step travels with the
 // (magnitude, row) reduction.  The inverse is stored
 // to LDS once at the end for the product.
+// Only the one-wave shape <64, 8, 2, 2> is launched and tested; a two-wave
+// shape (<128, 16, 4, 2>, orders up to 32) returned wrong values in a timing
+// experiment, cause not found, so another shape needs its own parity test.
 // One workgroup per task.  dn (may be NULL): the task count is read from
 // dn[0]; workgroups beyond it return at once.
 template <int NT, int TCC, int RPT, int CPT>
