@@ -649,11 +649,25 @@ __device__ __forceinline__ void store_pairs(
   }
 }
 
+// Sum of a partial sum over the lanes that differ in bit 2 (columns split)
+// and bit 5 (rows split) of the lane index; every lane of a group gets it.
+__device__ __forceinline__ void add_lane_halves(cdbl &v, bool split_c,
+                                                bool split_r) {
+  if (split_c) {
+    v.x += __shfl_xor(v.x, 4);
+    v.y += __shfl_xor(v.y, 4);
+  }
+  if (split_r) {
+    v.x += __shfl_xor(v.x, 32);
+    v.y += __shfl_xor(v.y, 32);
+  }
+}
+
 // One workgroup per tile.  dn (device list counts, may be NULL): the item
 // and tile counts are read from dn[0], dn[1]; workgroups beyond the tiles
 // return at once.
 template <int MT>
-__global__ __launch_bounds__(256) void k_assemble(
+__global__ __launch_bounds__(256, 2) void k_assemble(
     const int32_t *order, const int32_t *tile_prefix, int n_order,
     const Item *items, const TableSet *sets, const WigKey *wkeys,
     const GrainParams gp, const double *tables, const double *wig,
@@ -706,6 +720,12 @@ __global__ __launch_bounds__(256) void k_assemble(
   // node groups as well.  Node group sg of ns sums the stage nodes
   // kk = sg, sg + ns, ...; the partial sums are added at the end.
   int sg = 0, ns = 1;
+  // MT = 2, 3: a tile whose rows (or columns) within the block all lie in
+  // its first half maps its lanes onto that half, and the lanes freed take
+  // every other node of a stage (lane bit 5 for rows, lane bit 2 for
+  // columns; for m = 0 waves 2 and 3 when the block ends within the tile's
+  // first TR columns), at most CH node groups
+  bool split_w = false, split_r = false, split_c = false;
   if (MT == 1) {
     if (m0) {
       cbase = 0;
@@ -718,12 +738,44 @@ __global__ __launch_bounds__(256) void k_assemble(
       lr &= 3;
       lc &= 3;
     }
+  } else {
+    const int vr = nm - tr * TR, vc = nm - tc * cw;
+    if (m0 && vc <= TR) {
+      cbase = 0;
+      sg = wv >> 1;
+      ns = 2;
+      split_w = true;
+    }
+    if (!m0 && vc <= TR / 2 && 2 * ns <= CH) {
+      sg = 2 * sg + (lc >> 2);
+      ns *= 2;
+      lc &= 3;
+      split_c = true;
+    }
+    if (vr <= TR / 2 && 2 * ns <= CH) {
+      sg = 2 * sg + (lr >> 2);
+      ns *= 2;
+      lr &= 3;
+      split_r = true;
+    }
   }
+  // pair row of micro-tile row a: (lr mod 4) + 4 a in the tile's first half
+  // for lr < 4, the same in its second half for lr >= 4 (MT = 1: lr + 8 a);
+  // a split keeps every lane in the first half.  Columns alike.
+  int r0, c0;
+  if (MT == 1) {
+    r0 = lr;
+    c0 = lc;
+  } else {
+    r0 = (lr & 3) + (split_r ? 0 : 4 * MT * (lr >> 2));
+    c0 = (lc & 3) + (split_c ? 0 : 4 * MT * (lc >> 2));
+  }
+  constexpr int kStep = MT == 1 ? 8 : 4;
   int rr[MT], cc[MT];
 #pragma unroll
   for (int a = 0; a < MT; ++a) {
-    rr[a] = 2 * (lr + 8 * a) + prow;
-    cc[a] = cbase + 2 * (lc + 8 * a) + pcol;
+    rr[a] = 2 * (r0 + kStep * a) + prow;
+    cc[a] = cbase + 2 * (c0 + kStep * a) + pcol;
   }
   const bool odd = !m0 && ((prow ^ pcol) != 0);
 
@@ -779,6 +831,55 @@ __global__ __launch_bounds__(256) void k_assemble(
     Jb[0][0] = cmk(v[2], v[3]);
     RJa[0][0] = cmk(v[4], v[5]);
     RJb[0][0] = cmk(v[6], v[7]);
+    if (sg != 0) return;
+  }
+  if (MT > 1 && ns > 1) {
+    // add the node groups: lane halves by shuffles (both lanes of a pair
+    // then hold the same sums), then waves 2 and 3 into waves 0 and 1
+    // through the row-factor stage buffer, EPR pairs of the micro-tile a
+    // round
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+      for (int b = 0; b < MT; ++b) {
+        add_lane_halves(Ja[a][b], split_c, split_r);
+        add_lane_halves(Jb[a][b], split_c, split_r);
+        add_lane_halves(RJa[a][b], split_c, split_r);
+        add_lane_halves(RJb[a][b], split_c, split_r);
+      }
+    if (split_w) {
+      constexpr int EPR = CH * TR * kLeftStride / 128 / 8;
+      static_assert(EPR >= 1, "stage buffer too small for the reduction");
+      double *red = &Lsh[0][0][0];
+      const int tl = t & 127;
+#pragma unroll
+      for (int e0 = 0; e0 < MT * MT; e0 += EPR) {
+        __syncthreads();
+#pragma unroll
+        for (int e = e0; e < e0 + EPR && e < MT * MT; ++e) {
+          const int a = e / MT, b = e % MT;
+          double *r = red + (int64_t)(e - e0) * 8 * 128 + tl;
+          if (wv >= 2) {
+            r[0] = Ja[a][b].x;    r[128] = Ja[a][b].y;
+            r[256] = Jb[a][b].x;  r[384] = Jb[a][b].y;
+            r[512] = RJa[a][b].x; r[640] = RJa[a][b].y;
+            r[768] = RJb[a][b].x; r[896] = RJb[a][b].y;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = e0; e < e0 + EPR && e < MT * MT; ++e) {
+          const int a = e / MT, b = e % MT;
+          const double *r = red + (int64_t)(e - e0) * 8 * 128 + tl;
+          if (wv < 2) {
+            Ja[a][b].x += r[0];    Ja[a][b].y += r[128];
+            Jb[a][b].x += r[256];  Jb[a][b].y += r[384];
+            RJa[a][b].x += r[512]; RJa[a][b].y += r[640];
+            RJb[a][b].x += r[768]; RJb[a][b].y += r[896];
+          }
+        }
+      }
+    }
     if (sg != 0) return;
   }
   int pr[MT], pc[MT];

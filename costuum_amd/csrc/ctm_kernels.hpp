@@ -50,7 +50,10 @@ hipError_t launch_assemble_pack4(const int32_t *order, int n_order, int m0,
 // kPackRows-row tiles) from kPackMin to kPackRows, 1 below, else 3 (48-row
 // tiles) where their padded pair count, at `ratio` of the cost per pair of
 // 32-row tiles (measured 0.5: the 3 x 3 micro-tile reads fewer LDS operands
-// per pair), is smaller, else 2.  (m = 0 tiles are twice as wide.)
+// per pair), is smaller, else 2.  (m = 0 tiles are twice as wide.)  The
+// count leaves out that edge tiles split their nodes: counting them as half
+// moves orders 49-64 to 48-row tiles, measured slower (15.63 -> 16.56 ms
+// per batch step).
 constexpr double kMicroRatio = 0.5;
 __host__ __device__ inline int assemble_micro(int nm, bool m0,
                                               double ratio = kMicroRatio) {
