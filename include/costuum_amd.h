@@ -426,8 +426,11 @@ int ctm_plan_info_get(ctm_plan *plan, ctm_plan_info *info);
  * for them) and of the kernel variant that forms
  * the largest blocks' observable forms (obs_lean: the one with the smallest
  * LDS footprint, which shares CUs with the kernels beside it), by
- * coordinate descent, and keeps the fastest.  Results do not depend on them.  `out` (may be NULL) receives
- * the chosen values and times. */
+ * coordinate descent, and keeps the fastest.  Orders and values do not
+ * depend on them, except that the tile ratio can move a block to tiles of
+ * the other height, whose edge tiles may sum the quadrature nodes in
+ * another order: values then change in rounding only.  `out` (may be NULL)
+ * receives the chosen values and times. */
 typedef struct ctm_plan_tuning {
   double window_scale;
   double window_add;
