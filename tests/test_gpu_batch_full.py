@@ -22,7 +22,13 @@ Q_ext; one probe can miss a value's sensitivity by orders of magnitude) and,
 for Q_abs, its own.  Two FP64 implementations that round differently differ
 by about cond(Q) 1e-16 on ill-conditioned grains.  Grains that converge nowhere (the
 oracle's status 1) keep the last order in both and are not compared, as
-their EBCM systems are numerically meaningless."""
+their EBCM systems are numerically meaningless.
+
+The converged grains' blocks take every split edge-tile path of the
+assembly at the default tile ratio: 48-row tiles of m = 0 blocks split by
+waves (73 tiles) and by rows (10), of m > 0 blocks by rows (91) and by
+columns (182, corners included); 32-row tiles of m = 0 blocks by waves
+(88)."""
 import os
 
 import numpy as np
