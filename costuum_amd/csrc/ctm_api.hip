@@ -2603,6 +2603,12 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
         if (rc != CTM_OK) return rc;
       }
     }
+    // window lists hold only m = 0 blocks, so the packed m > 0 part is
+    // empty: not launched (an empty launch still waits for free workgroup
+    // slots behind the other parts, 0.4 ms in the first window, holding
+    // back the packed m = 0 part on its lane; the step moved by less than
+    // its spread); no window solve bucket waits for its event
+    if (part == 3 && !want_t) continue;
     const hipStream_t ls = lane(ctx, lane_of_part[part]);
     if (part >= 1 && big_alone)
       PLAN_HIP(hipStreamWaitEvent(ls, p->ev_solve[0], 0));
