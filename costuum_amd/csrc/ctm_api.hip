@@ -2512,8 +2512,10 @@ int plan_phase(ctm_plan *p, int ph, bool want_t) {
                              : CTM_KERNEL_ASSEMBLE_WINDOW,
                       CTM_KERNEL_ASSEMBLE, CTM_KERNEL_ASSEMBLE_SMALL};
   // (window lists hold only m = 0 blocks: the packed m = 0 part takes the
-  // lane of the empty packed m > 0 part)
-  const int lane_of_part[5] = {0, 1, 2, 3, want_t ? 2 : 3};
+  // lane of the empty packed m > 0 part; in the all-m phase it follows the
+  // 2 x 2 part on lane 1: 15.51 -> 15.47 ms per batch step against lane 2,
+  // lanes 0 and 3 of the windows' parts measured slower)
+  const int lane_of_part[5] = {0, 1, 2, 3, want_t ? 1 : 3};
   // solve buckets, placed and ordered as the host path places them; in the
   // window rounds the order 33-48 bucket follows the order 49-64 one on
   // lane 1 instead of the order 65-70 one on lane 3 (the first window's
