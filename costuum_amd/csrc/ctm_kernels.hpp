@@ -39,7 +39,10 @@ constexpr int kSmallBlock = 16;
 // Blocks of order up to kPackRows, four per workgroup (order: their item
 // indices, all with m = 0 when m0 is set, else all with m > 0).
 constexpr int kPackRows = 24;
-constexpr int kPackMin = 9;  // smallest order assembled four per workgroup
+// smallest order assembled four per workgroup (orders 9-16 on the one-pair
+// kernel's 16-row tiles measured 15.33 -> 15.24 ms per batch step against
+// 9; 13, 15, 21 and 25 between or slower)
+constexpr int kPackMin = 17;
 hipError_t launch_assemble_pack4(const int32_t *order, int n_order, int m0,
                                  const Item *items, const TableSet *sets,
                                  const WigKey *wkeys, const GrainParams &gp,
